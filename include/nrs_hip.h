@@ -1,0 +1,266 @@
+/*
+ * nrs_hip.h -- C ABI of libnekrs_hip, an MI355X (gfx950) implementation of
+ * nekRS's pressure-Poisson hot path: the element stiffness apply (Ax), the
+ * ogs gather-scatter, the p-multigrid preconditioner and PCG / flexible CG.
+ *
+ * Conventions (same as the reference):
+ *   - E-vectors are laid out [field][e][k][j][i] (i fastest), Np = (N+1)^3.
+ *   - ggeo is [e][7][Np] with ids G00,G01,G11,G12,G02,G22,GWJ = 0..6
+ *     (src/mesh/mesh3D.h:96-102).
+ *   - D is row-major, D[i*(N+1)+j] = d(l_j)/dr at r_i (GLL nodes).
+ *   - dlong = int32 (element and node indices), hlong = int64 (global ids)
+ *     (3rd_party/gslib/ogs/include/ogstypes.h:21-45).
+ *   - Every array argument documented "device" is a device pointer owned by
+ *     the caller; the library never frees caller memory.  "host" arrays are
+ *     read during the call only.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  All work of a
+ *     call is enqueued on that stream; calls that return scalars synchronise it.
+ *   - Every function returns NRS_OK (0) or a negative nrs_status; the
+ *     reference aborts instead (nrsCheck -> MPI_Abort).  nrs_last_error()
+ *     gives the message of the last failure on the calling thread.
+ *   - A handle is used by one host thread at a time.
+ */
+#ifndef NRS_HIP_H
+#define NRS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nrs_status {
+  NRS_OK = 0,
+  NRS_ERR_INVALID = -1,      /* bad argument or unsupported configuration      */
+  NRS_ERR_HIP = -2,          /* HIP runtime error                              */
+  NRS_ERR_NAN = -3,          /* NaN residual norm (PCG.cpp:193-195)            */
+  NRS_ERR_UNSUPPORTED = -4,  /* valid for the reference, not built here        */
+  NRS_ERR_NO_DEVICE = -5
+};
+
+enum nrs_dtype { NRS_F64 = 0, NRS_F32 = 1, NRS_I32 = 2 };
+/* gather-scatter reductions: ogsAdd / ogsMul / ogsMin / ogsMax
+ * (3rd_party/gslib/ogs/ogs.hpp:124-136) */
+enum nrs_op { NRS_ADD = 0, NRS_MUL = 1, NRS_MIN = 2, NRS_MAX = 3 };
+
+typedef struct nrs_ogs nrs_ogs;
+typedef struct nrs_elliptic nrs_elliptic;
+
+const char *nrs_last_error(void);
+/* Select the HIP device for the calling thread and check it is gfx950. */
+int nrs_init(int device);
+const char *nrs_version(void);
+
+/* ------------------------------------------------------------------------- */
+/* Setup helpers                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* Structured box [0,1]^3, nx*ny*nz hexes in lexicographic order, GLL nodes of
+ * order N.  map: 0 affine, 1 sine bump x_i += 0.05 sin(pi x)sin(pi y)sin(pi z)
+ * at every node, 2 Kershaw(eps) applied to element vertices with trilinear
+ * elements (examples/kershaw/kershaw.usr:107-226).  All outputs host arrays:
+ * x,y,z[E*Np], globalIds[E*Np] (1-based), EToB[E*6] (1 = Dirichlet, 0 interior,
+ * faces ordered as FaceNodesHex3D, src/mesh/meshBasisHex3D.cpp:53-82).
+ * Replaces the Nek5000 genbox/.re2 + usrdat path, which this build omits. */
+int nrs_box_mesh(int nx, int ny, int nz, int N, int map, double eps,
+                 double *x, double *y, double *z, long long *globalIds, int *EToB);
+
+/* GLL nodes/weights and differentiation matrix (host arrays of N+1, (N+1)^2).
+ * Replaces JacobiGLL / Dmatrix1D (src/mesh/meshBasis1D.cpp:99-119,249-281). */
+int nrs_gll(int N, double *z, double *w, double *D);
+
+/* Geometric factors from node coordinates, on the GPU.
+ * Replaces geometricFactorsHex3D (kernels/mesh/geometricFactorsHex3D.okl:27-143).
+ * x,y,z: device [E*Np]; ggeo: device [E*7*Np]. */
+int nrs_geometric_factors(int N, int Nelements, const double *x, const double *y,
+                          const double *z, double *ggeo, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Kernel-level seams                                                         */
+/* ------------------------------------------------------------------------- */
+
+/* Element stiffness apply, Aq_e = D^T (G . (D q_e)) [+ GwJ lambda1 q_e].
+ * Replaces ellipticPartialAxCoeffHex3D (ABI kernels/elliptic/
+ * ellipticPartialAxCoeffHex3D.c:2-12, registered as
+ * "poisson-ellipticPartialAxCoeffHex3D", src/core/registerEllipticKernels.cpp:183).
+ *   dtype      NRS_F64 or NRS_F32 (every floating array in that type)
+ *   elementList device int32 [Nelements] or NULL for 0..Nelements-1
+ *   lambda0/1  device; lambda_field=0: lambda0[0] scalar (p_lambda=0),
+ *              lambda_field=1: per-node fields indexed like q (p_lambda=1);
+ *              lambda1 ignored when poisson=1.
+ *   q, Aq      device E-vectors (Aq written only for listed elements). */
+int nrs_ax(int N, int dtype, int Nelements, const int *elementList, const void *ggeo,
+           const void *D, const void *lambda0, const void *lambda1, int poisson,
+           int lambda_field, const void *q, void *Aq, void *stream);
+
+/* Three-field Helmholtz apply with coefficient fields.
+ * Replaces ellipticBlockPartialAxCoeffHex3D (kernels/elliptic/
+ * ellipticBlockPartialAxCoeffHex3D.c:2-147). q,Aq: [3][offset]; lambda0/1:
+ * [3][loffset]. FP64. */
+int nrs_block_ax(int N, int Nelements, int64_t offset, int64_t loffset, const double *ggeo,
+                 const double *D, const double *lambda0, const double *lambda1,
+                 const double *q, double *Aq, void *stream);
+
+/* Element diagonal of A (before assembly).
+ * Replaces ellipticBlockBuildDiagonalHex3D (kernels/elliptic/
+ * ellipticBlockBuildDiagonalHex3D.okl:27-117), single field. */
+int nrs_build_diagonal(int N, int dtype, int Nelements, const void *ggeo, const void *D,
+                       const void *lambda0, const void *lambda1, int poisson, int lambda_field,
+                       void *diag, void *stream);
+
+/* v[maskIds[n]] = 0.  Replaces mask (kernels/core/mask.okl:27-35) behind
+ * ellipticApplyMask (src/solvers/elliptic/ellipticApplyMask.cpp:8-27). */
+int nrs_mask(int dtype, int64_t Nmasked, const int *maskIds, void *v, void *stream);
+
+/* Tensor-product transfers between orders Nf > Nc.  R: device [(Nc+1)*(Nf+1)],
+ * R = transpose(InterpolationMatrix1D(Nc -> Nf))
+ * (MG/ellipticMultiGridLevelSetup.cpp:238-258).
+ * coarsen:    qc  = (R x R x R) qf        (ellipticPreconCoarsenHex3D.c:26-100)
+ * prolongate: qf += (R^T x R^T x R^T) qc  (ellipticPreconProlongateHex3D.c:27-114) */
+int nrs_coarsen(int Nf, int Nc, int dtype, int Nelements, const void *R, const void *qf,
+                void *qc, void *stream);
+int nrs_prolongate(int Nf, int Nc, int dtype, int Nelements, const void *R, const void *qc,
+                   void *qf, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Gather-scatter (ogs)                                                       */
+/* ------------------------------------------------------------------------- */
+
+/* Build a gather-scatter handle from global ids (host int64 [N], 0 = node
+ * excluded).  Replaces ogsSetup (3rd_party/gslib/ogs/ogs.hpp:183,
+ * src/ogsSetup.cpp:111-397) on one rank: rows are the groups of equal ids,
+ * ordered by first local appearance, members ascending; singleton rows are
+ * dropped from the device structure (the reference kernel skips them,
+ * gatherScatterMany.okl:71). */
+int nrs_ogs_setup(int64_t N, const long long *globalIds, nrs_ogs **ogs);
+int nrs_ogs_free(nrs_ogs *ogs);
+
+/* In-place QQ^T reduction on nvec vectors spaced `stride` elements apart.
+ * Replaces oogs::startFinish(o_v, k, stride, type, op, h)
+ * (3rd_party/gslib/ogs/ogs.hpp:282-285, src/oogs.cpp:682-837) and
+ * ogsGatherScatterMany. v: device. */
+int nrs_ogs_gather_scatter(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int dtype, int op,
+                           void *stream);
+
+/* 1/multiplicity per node (1 at excluded nodes), FP64 device [N]
+ * (ogs->o_invDegree, ogsSetup.cpp:364-392). */
+int nrs_ogs_inv_degree(nrs_ogs *ogs, double *invDegree, void *stream);
+
+/* Row statistics: rows with more than one member, their members, and the
+ * number of singleton rows. */
+int nrs_ogs_info(nrs_ogs *ogs, int64_t *Nrows, int64_t *Nmembers, int64_t *Nsingletons);
+
+/* ------------------------------------------------------------------------- */
+/* Elliptic solver (elliptic_t + ellipticSolve)                               */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+  int N;                         /* polynomial order                            */
+  int Nelements;
+  const double *x, *y, *z;       /* host [E*Np] node coordinates                */
+  const long long *globalIds;    /* host [E*Np], 1-based                        */
+  const int *EToB;               /* host [E*6], 1 = DIRICHLET (elliptic.h:45)   */
+  int poisson;                   /* 1: Poisson (lambda1 ignored)                */
+  double lambda0, lambda1;       /* constant coefficients (p_lambda = 0)        */
+} nrs_elliptic_desc;
+
+enum nrs_solver { NRS_SOLVER_PCG = 0, NRS_SOLVER_FCG = 1 };  /* SOLVER=PCG[+FLEXIBLE] */
+enum nrs_precon { NRS_PRECON_NONE = 0, NRS_PRECON_JACOBI = 1, NRS_PRECON_MULTIGRID = 2 };
+enum nrs_smoother {
+  NRS_SMOOTHER_CHEBYSHEV_JACOBI = 0,  /* DAMPEDJACOBI,CHEBYSHEV (1st kind)       */
+  NRS_SMOOTHER_OPT4_CHEBYSHEV_JACOBI = 1,
+  NRS_SMOOTHER_CHEBYSHEV_ASM = 2,     /* CHEBYSHEV+ASM (FDM/Schwarz)             */
+  NRS_SMOOTHER_OPT4_CHEBYSHEV_ASM = 3,
+  NRS_SMOOTHER_ASM = 4,
+  NRS_SMOOTHER_RAS = 5
+};
+enum nrs_coarse { NRS_COARSE_PCG_F64 = 0, NRS_COARSE_SMOOTH = 1 };
+enum nrs_mg_precision { NRS_MG_F32 = 0, NRS_MG_F64 = 1 };
+enum nrs_fdm_precision { NRS_FDM_F32 = 0, NRS_FDM_BF16 = 1, NRS_FDM_F16 = 2 };
+
+typedef struct {
+  int solver;                    /* nrs_solver                                  */
+  int preconditioner;            /* nrs_precon                                  */
+  double tol;                    /* SOLVER TOLERANCE                            */
+  int relative;                  /* LINEAR SOLVER STOPPING CRITERION=RELATIVE   */
+  int maxIter;                   /* MAXIMUM ITERATIONS                          */
+  /* multigrid */
+  int nLevels;                   /* 0: default schedule (determineMGLevels)     */
+  int levels[8];                 /* orders, finest first, e.g. {7,3,1}          */
+  int smoother;                  /* nrs_smoother                                */
+  int chebyDegree;               /* MULTIGRID CHEBYSHEV DEGREE                  */
+  double chebyMinFactor;         /* default 0.1                                 */
+  double chebyMaxFactor;         /* default 1.1                                 */
+  int coarseSolver;              /* nrs_coarse                                  */
+  double coarseTol;              /* relative tolerance of the coarse PCG        */
+  int coarseMaxIter;
+  int mgPrecision;               /* nrs_mg_precision                            */
+  int fdmPrecision;              /* nrs_fdm_precision                           */
+  unsigned long long seed;       /* Arnoldi start vector seed                   */
+} nrs_elliptic_opts;
+
+typedef struct {
+  int iterations;
+  int converged;
+  double res0;                   /* initial normalised residual norm            */
+  double res;                    /* final normalised residual norm              */
+  double seconds;                /* wall time of the solve                      */
+  double precon_seconds;
+} nrs_solve_stats;
+
+/* Fill opts with the reference's defaults for this path (PCG, Jacobi,
+ * tol 1e-8 relative, maxIter 500, Chebyshev factors 0.1/1.1, degree 2). */
+void nrs_elliptic_default_opts(nrs_elliptic_opts *opts);
+
+/* Replaces ellipticSolveSetup (src/solvers/elliptic/ellipticSetup.cpp:116-327):
+ * geometric factors, unmasked and masked ogs, Dirichlet mask
+ * (ellipticOgs.cpp:4-134), invDegree, Jacobi diagonal, p-multigrid levels. */
+int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts,
+                       nrs_elliptic **out);
+int nrs_elliptic_free(nrs_elliptic *el);
+
+/* Sizes: Nlocal = E*Np, Nmasked, volume = sum(GwJ). */
+int nrs_elliptic_info(nrs_elliptic *el, int64_t *Nlocal, int64_t *Nmasked, double *volume);
+
+/* Device buffers owned by the handle: 0 ggeo (f64), 1 D (f64), 2 invDegree
+ * (f64), 3 maskIds (int32), 4 invDiagA (f64). */
+int nrs_elliptic_buffer(nrs_elliptic *el, int which, void **ptr, int64_t *count);
+/* Copy `count` entries of buffer `which` into the caller's device array dst. */
+int nrs_elliptic_copy_buffer(nrs_elliptic *el, int which, void *dst, int64_t count, void *stream);
+
+/* Aq = gs(mask(Ax(q))) in FP64 on device E-vectors.
+ * Replaces ellipticOperator(elliptic, o_q, o_Aq, "double", masked)
+ * (src/solvers/elliptic/ellipticOperator.cpp:117-172). */
+int nrs_elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, int masked,
+                          void *stream);
+
+/* z = M^{-1} r.  Replaces ellipticPreconditioner
+ * (src/solvers/elliptic/ellipticPreconditioner.cpp:33-84). */
+int nrs_elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, void *stream);
+
+/* Solve A x = rhs.  rhs is the unassembled right-hand side (the solve applies
+ * mask and gather-scatter to rhs - A x0 itself); x holds the initial guess on
+ * entry and the solution on exit.  rhs is overwritten with the final residual.
+ * Replaces ellipticSolve (src/solvers/elliptic/ellipticSolve.cpp:32-190) with
+ * pcg (linearSolver/PCG.cpp:85-203).  Returns NRS_OK also when maxIter is hit
+ * (stats->converged = 0), NRS_ERR_NAN on a NaN residual. */
+int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *stats,
+                       void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Device timers (replaces platform->timer device tags, src/core/timer.cpp)   */
+/* ------------------------------------------------------------------------- */
+enum nrs_phase { NRS_PHASE_AX = 0, NRS_PHASE_GS = 1, NRS_PHASE_COUNT = 2 };
+
+/* Enable (1) or disable (0) per-launch event timing of the operator's Ax and
+ * gather-scatter launches; enabling clears previous records. */
+int nrs_elliptic_profile_enable(nrs_elliptic *el, int on);
+/* Synchronise on the recorded events and return, per phase, the total device
+ * milliseconds and the number of launches recorded since enabling. */
+int nrs_elliptic_profile_read(nrs_elliptic *el, double ms[NRS_PHASE_COUNT],
+                              int64_t launches[NRS_PHASE_COUNT]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,285 @@
+"""nekrs_amd -- MI355X (gfx950) pressure-Poisson hot path of nekRS.
+
+Python mirror of the reference's solver interface over the C ABI of
+libnekrs_hip.so (include/nrs_hip.h).  Device memory is torch tensors on
+``cuda`` (ROCm); their data pointers and the current HIP stream are handed to
+the library.  All arithmetic runs in the library's HIP kernels.
+
+  ogs = Ogs(global_ids)                  # ogsSetup
+  ogs.gather_scatter(v, "add")           # oogs::startFinish
+  ax(N, ggeo, D, q, Aq)                  # ellipticPartialAxCoeffHex3D
+  el = Elliptic(mesh, precon="jacobi")   # ellipticSolveSetup
+  el.operator(q, Aq)                     # ellipticOperator
+  stats = el.solve(rhs, x)               # ellipticSolve
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import NrsError
+
+__all__ = ["NrsError", "init", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
+           "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh"]
+
+OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
+MAPS = {"none": 0, "sine": 1, "kershaw": 2}
+SOLVERS = {"pcg": 0, "fcg": 1, "pcg+flexible": 1}
+PRECONS = {"none": 0, "jacobi": 1, "multigrid": 2}
+SMOOTHERS = {"chebyshev+jacobi": 0, "opt4chebyshev+jacobi": 1, "chebyshev+asm": 2,
+             "opt4chebyshev+asm": 3, "asm": 4, "ras": 5}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dtype_code(t):
+    torch = _torch()
+    if t.dtype == torch.float64:
+        return _lib.F64
+    if t.dtype == torch.float32:
+        return _lib.F32
+    if t.dtype == torch.int32:
+        return _lib.I32
+    raise TypeError("unsupported dtype %s" % t.dtype)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+def init(device=0):
+    _lib.call("nrs_init", device)
+
+
+# ------------------------------------------------------------------ setup helpers
+def gll(N):
+    z = np.zeros(N + 1); w = np.zeros(N + 1); D = np.zeros((N + 1) * (N + 1))
+    _lib.call("nrs_gll", N, z.ctypes.data, w.ctypes.data, D.ctypes.data)
+    return z, w, D.reshape(N + 1, N + 1)
+
+
+def box_mesh(nx, ny, nz, N, map="none", eps=0.0):
+    """Host arrays (x, y, z, globalIds, EToB) of a structured box [0,1]^3."""
+    E = nx * ny * nz
+    L = E * (N + 1) ** 3
+    x = np.zeros(L); y = np.zeros(L); z = np.zeros(L)
+    gid = np.zeros(L, dtype=np.int64)
+    EToB = np.zeros(E * 6, dtype=np.int32)
+    _lib.call("nrs_box_mesh", nx, ny, nz, N, MAPS[map], float(eps), x.ctypes.data, y.ctypes.data,
+              z.ctypes.data, gid.ctypes.data, EToB.ctypes.data)
+    return x, y, z, gid, EToB
+
+
+class BoxMesh:
+    """Host-side description of a box mesh (coordinates, global ids, BCs)."""
+
+    def __init__(self, nx, ny, nz, N, map="none", eps=0.0):
+        self.nx, self.ny, self.nz, self.N = nx, ny, nz, N
+        self.Nq = N + 1
+        self.Np = self.Nq ** 3
+        self.E = nx * ny * nz
+        self.Nlocal = self.E * self.Np
+        self.map, self.eps = map, eps
+        self.x, self.y, self.z, self.gid, self.EToB = box_mesh(nx, ny, nz, N, map, eps)
+
+
+def geometric_factors(N, x, y, z, ggeo, stream=None):
+    E = x.numel() // (N + 1) ** 3
+    _lib.call("nrs_geometric_factors", N, E, _ptr(x), _ptr(y), _ptr(z), _ptr(ggeo), _stream(stream))
+
+
+# ------------------------------------------------------------------ kernel seams
+def ax(N, ggeo, D, q, Aq, lambda0=None, lambda1=None, poisson=True, lambda_field=False,
+       element_list=None, stream=None):
+    """Aq = Ax(q) per element (ellipticPartialAxCoeffHex3D)."""
+    torch = _torch()
+    E = len(element_list) if element_list is not None else q.numel() // (N + 1) ** 3
+    if lambda0 is None:
+        lambda0 = torch.ones(1, dtype=q.dtype, device=q.device)
+    _lib.call("nrs_ax", N, _dtype_code(q), E, _ptr(element_list), _ptr(ggeo), _ptr(D),
+              _ptr(lambda0), _ptr(lambda1), int(poisson), int(lambda_field), _ptr(q), _ptr(Aq),
+              _stream(stream))
+
+
+def block_ax(N, ggeo, D, lambda0, lambda1, q, Aq, offset, loffset, E, stream=None):
+    _lib.call("nrs_block_ax", N, E, offset, loffset, _ptr(ggeo), _ptr(D), _ptr(lambda0),
+              _ptr(lambda1), _ptr(q), _ptr(Aq), _stream(stream))
+
+
+def build_diagonal(N, ggeo, D, diag, lambda0, lambda1=None, poisson=True, lambda_field=False,
+                   stream=None):
+    E = diag.numel() // (N + 1) ** 3
+    _lib.call("nrs_build_diagonal", N, _dtype_code(diag), E, _ptr(ggeo), _ptr(D), _ptr(lambda0),
+              _ptr(lambda1), int(poisson), int(lambda_field), _ptr(diag), _stream(stream))
+
+
+def mask(mask_ids, v, stream=None):
+    _lib.call("nrs_mask", _dtype_code(v), mask_ids.numel(), _ptr(mask_ids), _ptr(v), _stream(stream))
+
+
+def coarsen(Nf, Nc, R, qf, qc, stream=None):
+    E = qf.numel() // (Nf + 1) ** 3
+    _lib.call("nrs_coarsen", Nf, Nc, _dtype_code(qf), E, _ptr(R), _ptr(qf), _ptr(qc), _stream(stream))
+
+
+def prolongate(Nf, Nc, R, qc, qf, stream=None):
+    E = qc.numel() // (Nc + 1) ** 3
+    _lib.call("nrs_prolongate", Nf, Nc, _dtype_code(qf), E, _ptr(R), _ptr(qc), _ptr(qf),
+              _stream(stream))
+
+
+class Ogs:
+    """Gather-scatter handle (ogsSetup / oogs::startFinish)."""
+
+    def __init__(self, global_ids):
+        ids = np.ascontiguousarray(global_ids, dtype=np.int64)
+        self.N = ids.size
+        h = C.c_void_p()
+        _lib.call("nrs_ogs_setup", self.N, ids.ctypes.data, C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().nrs_ogs_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def gather_scatter(self, v, op="add", nvec=1, stride=None, stream=None):
+        stride = self.N if stride is None else stride
+        _lib.call("nrs_ogs_gather_scatter", self._h, _ptr(v), nvec, stride, _dtype_code(v),
+                  OPS[op], _stream(stream))
+
+    def inv_degree(self, out=None, stream=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(self.N, dtype=torch.float64, device="cuda")
+        _lib.call("nrs_ogs_inv_degree", self._h, _ptr(out), _stream(stream))
+        return out
+
+    def info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.call("nrs_ogs_info", self._h, C.byref(a), C.byref(b), C.byref(c))
+        return {"rows": a.value, "members": b.value, "singletons": c.value}
+
+
+def default_opts():
+    o = _lib.EllipticOpts()
+    _lib.load().nrs_elliptic_default_opts(C.byref(o))
+    return o
+
+
+class Elliptic:
+    """elliptic_t for one scalar field (ellipticSolveSetup / ellipticSolve)."""
+
+    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, solver="pcg",
+                 precon="jacobi", tol=1e-8, relative=True, max_iter=500, levels=None,
+                 smoother="chebyshev+jacobi", cheby_degree=2, cheby_min=0.1, cheby_max=1.1,
+                 coarse_tol=1e-2, coarse_max_iter=200, seed=0, **extra):
+        self.mesh = mesh
+        d = _lib.EllipticDesc(mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
+                              mesh.z.ctypes.data, mesh.gid.ctypes.data, mesh.EToB.ctypes.data,
+                              int(poisson), float(lambda0), float(lambda1))
+        o = default_opts()
+        o.solver = SOLVERS[solver]
+        o.preconditioner = PRECONS[precon]
+        o.tol = tol
+        o.relative = int(relative)
+        o.maxIter = max_iter
+        if levels:
+            o.nLevels = len(levels)
+            for i, l in enumerate(levels):
+                o.levels[i] = l
+        o.smoother = SMOOTHERS[smoother]
+        o.chebyDegree = cheby_degree
+        o.chebyMinFactor = cheby_min
+        o.chebyMaxFactor = cheby_max
+        o.coarseTol = coarse_tol
+        o.coarseMaxIter = coarse_max_iter
+        o.seed = seed
+        for k, v in extra.items():
+            setattr(o, k, v)
+        self.opts = o
+        h = C.c_void_p()
+        _lib.call("nrs_elliptic_setup", C.byref(d), C.byref(o), C.byref(h))
+        self._h = h
+        n, m, vol = C.c_int64(), C.c_int64(), C.c_double()
+        _lib.call("nrs_elliptic_info", h, C.byref(n), C.byref(m), C.byref(vol))
+        self.Nlocal, self.Nmasked, self.volume = n.value, m.value, vol.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().nrs_elliptic_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def buffer(self, which):
+        """Device tensor view of an internal buffer (copy)."""
+        torch = _torch()
+        p, n = C.c_void_p(), C.c_int64()
+        _lib.call("nrs_elliptic_buffer", self._h, which, C.byref(p), C.byref(n))
+        dt = torch.int32 if which == 3 else torch.float64
+        out = torch.empty(n.value, dtype=dt, device="cuda")
+        _lib.call("nrs_elliptic_copy_buffer", self._h, which, _ptr(out), n.value, _stream())
+        return out
+
+    @property
+    def ggeo(self):
+        return self.buffer(0)
+
+    @property
+    def D(self):
+        return self.buffer(1)
+
+    @property
+    def inv_degree(self):
+        return self.buffer(2)
+
+    @property
+    def mask_ids(self):
+        return self.buffer(3)
+
+    def operator(self, q, Aq, masked=True, stream=None):
+        _lib.call("nrs_elliptic_operator", self._h, _ptr(q), _ptr(Aq), int(masked), _stream(stream))
+
+    def preconditioner(self, r, z, stream=None):
+        _lib.call("nrs_elliptic_preconditioner", self._h, _ptr(r), _ptr(z), _stream(stream))
+
+    def profile(self, on=True):
+        """Enable/disable per-launch device timing of the operator's kernels."""
+        _lib.call("nrs_elliptic_profile_enable", self._h, int(on))
+
+    def profile_read(self):
+        """{phase: (total_ms, launches)} since profiling was enabled."""
+        n = len(_lib.PHASES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        _lib.call("nrs_elliptic_profile_read", self._h, ms, cnt)
+        return {p: (ms[i], cnt[i]) for i, p in enumerate(_lib.PHASES)}
+
+    def solve(self, rhs, x, stream=None):
+        """In place: x <- solution, rhs <- final residual.  Returns a stats dict."""
+        st = _lib.SolveStats()
+        _lib.call("nrs_elliptic_solve", self._h, _ptr(rhs), _ptr(x), C.byref(st), _stream(stream))
+        return {"iterations": st.iterations, "converged": bool(st.converged), "res0": st.res0,
+                "res": st.res, "seconds": st.seconds, "precon_seconds": st.precon_seconds}

@@ -1,0 +1,381 @@
+// ax.hip -- element stiffness apply (Poisson / Helmholtz), its diagonal, the
+// three-field block apply, and the Dirichlet mask, for gfx950.
+//
+// Design (one element = one (N+1)^2 slab of lanes, N=7: exactly one wave64):
+//   * lane (i,j) keeps the k-pencil q(i,j,0..N) and the output pencil Aq(i,j,:)
+//     in registers, so the t-direction contractions never touch LDS;
+//   * the r/s contractions of plane k read the plane from LDS (one 8x8 slab
+//     of doubles = 512 B), D rows/columns live in registers;
+//   * ggeo streams as 6 coalesced runs of (N+1)^2 values per plane;
+//   * the Dirichlet mask is fused into the store epilogue via a per-element
+//     bit mask (Np bits), so the operator needs no separate mask launch.
+// Small orders pack several elements per block so that a block is >= 64 lanes.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace nrs {
+
+namespace {
+constexpr int kG00 = 0, kG01 = 1, kG11 = 2, kG12 = 3, kG02 = 4, kG22 = 5, kGWJ = 6;
+}
+
+#ifndef AX_WAVES_PER_EU
+#define AX_WAVES_PER_EU 3
+#endif
+
+template <int NQ> struct AxShape {
+  // elements per block: fill at least one wave64
+  static constexpr int NE = (NQ * NQ >= 64) ? 1 : (64 + NQ * NQ - 1) / (NQ * NQ);
+  static constexpr int THREADS = NQ * NQ * NE;
+};
+
+// -------------------------------------------------------------------------------------
+// Single-field apply.  Restates ellipticPartialAxCoeffHex3D
+// (kernels/elliptic/ellipticPartialAxCoeffHex3D.c:2-97) with the same arithmetic:
+//   qr = sum_m D[i][m] q(m,j,k), qs = sum_m D[j][m] q(i,m,k), qt = sum_m D[k][m] q(i,j,m)
+//   [Gqr,Gqs,Gqt] = lambda0 * G * [qr,qs,qt]
+//   Aq = sum_m D[m][i] Gqr(m,j,k) + D[m][j] Gqs(i,m,k) + D[m][k] Gqt(i,j,m) (+ GwJ lambda1 q)
+// -------------------------------------------------------------------------------------
+template <typename T, int NQ, bool POISSON, bool LFIELD>
+__global__ void __launch_bounds__(AxShape<NQ>::THREADS, AX_WAVES_PER_EU)
+ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
+          const T *__restrict__ Dg, const T *__restrict__ lam0, const T *__restrict__ lam1,
+          const T *__restrict__ q, T *__restrict__ Aq, const uint32_t *__restrict__ maskBits)
+{
+  constexpr int NE = AxShape<NQ>::NE;
+  constexpr int NP = NQ * NQ * NQ;
+  constexpr int NW = (NP + 31) / 32;
+  __shared__ T s_D[NQ][NQ];
+  __shared__ T s_q[NE][NQ][NQ];
+  __shared__ T s_Gr[NE][NQ][NQ];
+  __shared__ T s_Gs[NE][NQ][NQ];
+
+  const int t = threadIdx.x;
+  const int i = t % NQ;
+  const int j = (t / NQ) % NQ;
+  const int le = (NE == 1) ? 0 : t / (NQ * NQ);
+  const int eidx = blockIdx.x * NE + le;
+  const bool active = eidx < E;
+  int64_t e = active ? (elist ? elist[eidx] : eidx) : 0;
+  if (NE == 1) e = __builtin_amdgcn_readfirstlane((int)e);   // uniform element -> scalar base
+
+  if (t < NQ * NQ) s_D[t / NQ][t % NQ] = Dg[t];
+  __syncthreads();
+
+  T Di[NQ], Dj[NQ], DTi[NQ], DTj[NQ];
+#pragma unroll
+  for (int m = 0; m < NQ; ++m) {
+    Di[m] = s_D[i][m];
+    Dj[m] = s_D[j][m];
+    DTi[m] = s_D[m][i];
+    DTj[m] = s_D[m][j];
+  }
+
+  const int lid = j * NQ + i;          // lane offset inside a plane (32-bit)
+  const T *qe = q + e * NP;
+  const T *G = ggeo + e * 7 * NP;
+  T r_q[NQ], r_Aq[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    r_q[k] = active ? qe[k * NQ * NQ + lid] : T(0);
+    r_Aq[k] = T(0);
+  }
+
+  const T l0c = LFIELD ? T(0) : lam0[0];
+  const T l1c = (LFIELD || POISSON) ? T(0) : lam1[0];
+
+  // geometric factors of plane 0; later planes are prefetched one plane ahead
+  T g[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) g[c] = (active && (c < 6 || !POISSON)) ? G[c * NP + lid] : T(0);
+
+#pragma unroll 1
+  for (int k = 0; k < NQ; ++k) {
+    const int n = k * NQ * NQ;
+    T qk = r_q[0];
+#pragma unroll
+    for (int m = 1; m < NQ; ++m) qk = (k == m) ? r_q[m] : qk;
+    T gn[7];
+    const bool more = active && (k + 1 < NQ);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) gn[c] = (more && (c < 6 || !POISSON)) ? G[c * NP + n + NQ * NQ + lid] : T(0);
+    T l0 = l0c, l1 = l1c;
+    if (LFIELD && active) {
+      l0 = lam0[e * NP + n + lid];
+      if (!POISSON) l1 = lam1[e * NP + n + lid];
+    }
+    const T G00 = g[kG00], G01 = g[kG01], G02 = g[kG02], G11 = g[kG11], G12 = g[kG12], G22 = g[kG22];
+
+    s_q[le][j][i] = qk;
+    T qt = 0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) qt += Dg[k * NQ + m] * r_q[m];
+    __syncthreads();
+
+    T qr = 0, qs = 0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      qr += Di[m] * s_q[le][j][m];
+      qs += Dj[m] * s_q[le][m][i];
+    }
+    T Gqr = G00 * qr; Gqr += G01 * qs; Gqr += G02 * qt;
+    T Gqs = G01 * qr; Gqs += G11 * qs; Gqs += G12 * qt;
+    T Gqt = G02 * qr; Gqt += G12 * qs; Gqt += G22 * qt;
+    s_Gr[le][j][i] = l0 * Gqr;
+    s_Gs[le][j][i] = l0 * Gqs;
+    Gqt *= l0;
+    __syncthreads();
+
+    T a = 0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) a += DTi[m] * s_Gr[le][j][m];
+    T b = 0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) b += DTj[m] * s_Gs[le][m][i];
+    T ab = a + b;
+    if (!POISSON) ab += g[kGWJ] * l1 * qk;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) r_Aq[m] += Dg[k * NQ + m] * Gqt;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) r_Aq[m] += (k == m) ? ab : T(0);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) g[c] = gn[c];
+  }
+
+  if (!active) return;
+  T *Ae = Aq + e * NP + lid;
+  if (maskBits) {
+    const uint32_t *mb = maskBits + e * NW;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int n = k * NQ * NQ + j * NQ + i;
+      const bool masked = (mb[n >> 5] >> (n & 31)) & 1u;
+      Ae[k * NQ * NQ] = masked ? T(0) : r_Aq[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) Ae[k * NQ * NQ] = r_Aq[k];
+  }
+}
+
+template <typename T, int NQ>
+static void launch_ax_nq(int E, const int *elist, const T *ggeo, const T *D, const T *l0,
+                         const T *l1, bool poisson, bool lfield, const T *q, T *Aq,
+                         const uint32_t *maskBits, hipStream_t s)
+{
+  using S = AxShape<NQ>;
+  const unsigned grid = (unsigned)((E + S::NE - 1) / S::NE);
+  if (grid == 0) return;
+  if (poisson && !lfield)
+    ax_kernel<T, NQ, true, false><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+  else if (poisson && lfield)
+    ax_kernel<T, NQ, true, true><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+  else if (!poisson && !lfield)
+    ax_kernel<T, NQ, false, false><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+  else
+    ax_kernel<T, NQ, false, true><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+  NRS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_ax(int N, int E, const int *elist, const T *ggeo, const T *D, const T *l0, const T *l1,
+               bool poisson, bool lfield, const T *q, T *Aq, const uint32_t *maskBits,
+               hipStream_t s)
+{
+  switch (N + 1) {
+    case 2: launch_ax_nq<T, 2>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 3: launch_ax_nq<T, 3>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 4: launch_ax_nq<T, 4>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 5: launch_ax_nq<T, 5>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 6: launch_ax_nq<T, 6>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 7: launch_ax_nq<T, 7>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 8: launch_ax_nq<T, 8>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 9: launch_ax_nq<T, 9>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 10: launch_ax_nq<T, 10>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    default: throw Error(NRS_ERR_UNSUPPORTED, "Ax: order N must be 1..9");
+  }
+}
+
+template void launch_ax<double>(int, int, const int *, const double *, const double *, const double *,
+                                const double *, bool, bool, const double *, double *,
+                                const uint32_t *, hipStream_t);
+template void launch_ax<float>(int, int, const int *, const float *, const float *, const float *,
+                               const float *, bool, bool, const float *, float *, const uint32_t *,
+                               hipStream_t);
+
+// -------------------------------------------------------------------------------------
+// Three-field Helmholtz apply with coefficient fields.  Restates
+// ellipticBlockPartialAxCoeffHex3D (kernels/elliptic/ellipticBlockPartialAxCoeffHex3D.c:2-147);
+// the geometric factors are read once for the three fields.
+// -------------------------------------------------------------------------------------
+template <int NQ>
+__global__ void __launch_bounds__(AxShape<NQ>::THREADS)
+block_ax_kernel(int E, int64_t offset, int64_t loffset, const double *__restrict__ ggeo,
+                const double *__restrict__ Dg, const double *__restrict__ lam0,
+                const double *__restrict__ lam1, const double *__restrict__ q,
+                double *__restrict__ Aq)
+{
+  constexpr int NE = AxShape<NQ>::NE;
+  constexpr int NP = NQ * NQ * NQ;
+  constexpr int NF = 3;
+  __shared__ double s_D[NQ][NQ];
+  __shared__ double s_q[NF][NE][NQ][NQ];
+  __shared__ double s_Gr[NF][NE][NQ][NQ];
+  __shared__ double s_Gs[NF][NE][NQ][NQ];
+  const int t = threadIdx.x;
+  const int i = t % NQ, j = (t / NQ) % NQ, le = t / (NQ * NQ);
+  const int eidx = blockIdx.x * NE + le;
+  const bool active = eidx < E;
+  const int64_t e = active ? eidx : 0;
+  if (t < NQ * NQ) s_D[t / NQ][t % NQ] = Dg[t];
+  __syncthreads();
+  double r_q[NF][NQ], r_Aq[NF][NQ];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      r_q[f][k] = active ? q[f * offset + e * NP + k * NQ * NQ + j * NQ + i] : 0.0;
+      r_Aq[f][k] = 0.0;
+    }
+  const double *G = ggeo + e * 7 * NP + j * NQ + i;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int n = k * NQ * NQ;
+    const int64_t id = e * NP + n + j * NQ + i;
+    double G00 = 0, G01 = 0, G02 = 0, G11 = 0, G12 = 0, G22 = 0, GWJ = 0;
+    if (active) {
+      G00 = G[kG00 * NP + n]; G01 = G[kG01 * NP + n]; G02 = G[kG02 * NP + n];
+      G11 = G[kG11 * NP + n]; G12 = G[kG12 * NP + n]; G22 = G[kG22 * NP + n];
+      GWJ = G[kGWJ * NP + n];
+    }
+    double Gqt[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s_q[f][le][j][i] = r_q[f][k];
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double qr = 0, qs = 0, qt = 0;
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) {
+        qr += s_D[i][m] * s_q[f][le][j][m];
+        qs += s_D[j][m] * s_q[f][le][m][i];
+        qt += s_D[k][m] * r_q[f][m];
+      }
+      const double l0 = active ? lam0[f * loffset + id] : 0.0;
+      s_Gr[f][le][j][i] = l0 * (G00 * qr + G01 * qs + G02 * qt);
+      s_Gs[f][le][j][i] = l0 * (G01 * qr + G11 * qs + G12 * qt);
+      Gqt[f] = l0 * (G02 * qr + G12 * qs + G22 * qt);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const double l1 = active ? lam1[f * loffset + id] : 0.0;
+      double a = 0, b = 0;
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) {
+        a += s_D[m][i] * s_Gr[f][le][j][m];
+        b += s_D[m][j] * s_Gs[f][le][m][i];
+      }
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) r_Aq[f][m] += s_D[k][m] * Gqt[f];
+      r_Aq[f][k] += a + b + GWJ * l1 * r_q[f][k];
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) Aq[f * offset + e * NP + k * NQ * NQ + j * NQ + i] = r_Aq[f][k];
+}
+
+void launch_block_ax(int N, int E, int64_t offset, int64_t loffset, const double *ggeo,
+                     const double *D, const double *l0, const double *l1, const double *q,
+                     double *Aq, hipStream_t s)
+{
+#define BLK(NQ)                                                                              \
+  case NQ: {                                                                                 \
+    const unsigned g = (unsigned)((E + AxShape<NQ>::NE - 1) / AxShape<NQ>::NE);              \
+    if (g) block_ax_kernel<NQ><<<g, AxShape<NQ>::THREADS, 0, s>>>(E, offset, loffset, ggeo, D, l0, l1, q, Aq); \
+    break;                                                                                   \
+  }
+  switch (N + 1) {
+    BLK(2) BLK(3) BLK(4) BLK(5) BLK(6) BLK(7) BLK(8) BLK(9) BLK(10)
+    default: throw Error(NRS_ERR_UNSUPPORTED, "block Ax: order N must be 1..9");
+  }
+#undef BLK
+  NRS_LAUNCH_CHECK();
+}
+
+// -------------------------------------------------------------------------------------
+// Element diagonal.  Restates ellipticBlockBuildDiagonalHex3D
+// (kernels/elliptic/ellipticBlockBuildDiagonalHex3D.okl:27-117) for one field:
+//   diag = 2 l0 (G01 D_ii D_jj + G02 D_ii D_kk + G12 D_jj D_kk)
+//        + sum_m [G00 l0](m,j,k) D_mi^2 + [G11 l0](i,m,k) D_mj^2 + [G22 l0](i,j,m) D_mk^2
+//        (+ GwJ l1).  Setup-time kernel: one thread per node.
+// -------------------------------------------------------------------------------------
+template <typename T>
+__global__ void build_diag_kernel(int N, int E, const T *__restrict__ ggeo, const T *__restrict__ D,
+                                  const T *__restrict__ lam0, const T *__restrict__ lam1,
+                                  int poisson, int lfield, T *__restrict__ diag)
+{
+  const int Nq = N + 1, Np = Nq * Nq * Nq;
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)E * Np) return;
+  const int64_t e = gid / Np;
+  const int n = (int)(gid - e * Np);
+  const int i = n % Nq, j = (n / Nq) % Nq, k = n / (Nq * Nq);
+  const T *G = ggeo + e * 7 * Np;
+  auto L0 = [&](int nn) { return lfield ? lam0[e * Np + nn] : lam0[0]; };
+  const T l0 = L0(n);
+  T r = 0;
+  r += T(2) * G[kG01 * Np + n] * l0 * D[i * Nq + i] * D[j * Nq + j];
+  r += T(2) * G[kG02 * Np + n] * l0 * D[i * Nq + i] * D[k * Nq + k];
+  r += T(2) * G[kG12 * Np + n] * l0 * D[j * Nq + j] * D[k * Nq + k];
+  for (int m = 0; m < Nq; ++m) {
+    const int nr = m + Nq * (j + Nq * k), ns = i + Nq * (m + Nq * k), nt = i + Nq * (j + Nq * m);
+    r += G[kG00 * Np + nr] * L0(nr) * D[m * Nq + i] * D[m * Nq + i];
+    r += G[kG11 * Np + ns] * L0(ns) * D[m * Nq + j] * D[m * Nq + j];
+    r += G[kG22 * Np + nt] * L0(nt) * D[m * Nq + k] * D[m * Nq + k];
+  }
+  if (!poisson) r += G[kGWJ * Np + n] * (lfield ? lam1[gid] : lam1[0]);
+  diag[gid] = r;
+}
+
+template <typename T>
+void launch_build_diag(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1,
+                       bool poisson, bool lfield, T *diag, hipStream_t s)
+{
+  const int64_t n = (int64_t)E * (N + 1) * (N + 1) * (N + 1);
+  if (n == 0) return;
+  build_diag_kernel<T><<<grid_for(n, 256, 1ll << 30), 256, 0, s>>>(N, E, ggeo, D, l0, l1, poisson,
+                                                                   lfield, diag);
+  NRS_LAUNCH_CHECK();
+}
+template void launch_build_diag<double>(int, int, const double *, const double *, const double *,
+                                        const double *, bool, bool, double *, hipStream_t);
+template void launch_build_diag<float>(int, int, const float *, const float *, const float *,
+                                       const float *, bool, bool, float *, hipStream_t);
+
+// -------------------------------------------------------------------------------------
+// mask (kernels/core/mask.okl:27-35)
+// -------------------------------------------------------------------------------------
+template <typename T>
+__global__ void mask_kernel(int64_t Nm, const int *__restrict__ ids, T *__restrict__ v)
+{
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < Nm;
+       n += (int64_t)gridDim.x * blockDim.x)
+    v[ids[n]] = T(0);
+}
+
+template <typename T>
+void launch_mask(int64_t Nm, const int *ids, T *v, hipStream_t s)
+{
+  if (Nm <= 0) return;
+  mask_kernel<T><<<grid_for(Nm, 256, 8192), 256, 0, s>>>(Nm, ids, v);
+  NRS_LAUNCH_CHECK();
+}
+template void launch_mask<double>(int64_t, const int *, double *, hipStream_t);
+template void launch_mask<float>(int64_t, const int *, float *, hipStream_t);
+
+}  // namespace nrs

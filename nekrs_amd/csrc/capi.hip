@@ -1,0 +1,385 @@
+// capi.hip -- the extern "C" boundary of libnekrs_hip (declared in include/nrs_hip.h).
+// Every entry point validates its arguments, runs, and converts exceptions into
+// status codes; nothing here aborts the process.
+#include <cstring>
+#include <string>
+
+#include "elliptic.hpp"
+#include "kernels.hpp"
+#include "mesh.hpp"
+#include "mg.hpp"
+
+namespace nrs {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &m) { g_last_error = m; }
+}  // namespace nrs
+
+#define NRS_API_BEGIN try {
+#define NRS_API_END                                   \
+  return NRS_OK;                                      \
+  }                                                   \
+  catch (const nrs::Error &e) {                       \
+    nrs::set_last_error(e.what());                    \
+    return e.code;                                    \
+  }                                                   \
+  catch (const std::exception &e) {                   \
+    nrs::set_last_error(e.what());                    \
+    return NRS_ERR_INVALID;                           \
+  }
+
+using nrs::as_stream;
+
+extern "C" {
+
+const char *nrs_last_error(void) { return nrs::g_last_error.c_str(); }
+const char *nrs_version(void) { return "nekrs_amd 0.1 (gfx950)"; }
+
+int nrs_init(int device)
+{
+  NRS_API_BEGIN
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    throw nrs::Error(NRS_ERR_NO_DEVICE, "no HIP device");
+  NRS_CHECK(device >= 0 && device < n, "device index out of range");
+  NRS_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  NRS_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    throw nrs::Error(NRS_ERR_UNSUPPORTED, std::string("library built for gfx950, device is ") + prop.gcnArchName);
+  NRS_API_END
+}
+
+int nrs_box_mesh(int nx, int ny, int nz, int N, int map, double eps, double *x, double *y, double *z,
+                 long long *globalIds, int *EToB)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(x && y && z, "box mesh: null coordinate arrays");
+  NRS_CHECK(map >= 0 && map <= 2, "box mesh: map must be 0 (affine), 1 (sine) or 2 (Kershaw)");
+  nrs::box_mesh(nx, ny, nz, N, map, eps, x, y, z, globalIds, EToB);
+  NRS_API_END
+}
+
+int nrs_gll(int N, double *z, double *w, double *D)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(N >= 1 && N <= 32, "gll: N must be 1..32");
+  std::vector<double> zz, ww;
+  nrs::gll_nodes(N, zz, ww);
+  if (z) std::memcpy(z, zz.data(), zz.size() * sizeof(double));
+  if (w) std::memcpy(w, ww.data(), ww.size() * sizeof(double));
+  if (D) {
+    const auto DD = nrs::dmatrix(zz);
+    std::memcpy(D, DD.data(), DD.size() * sizeof(double));
+  }
+  NRS_API_END
+}
+
+int nrs_geometric_factors(int N, int Nelements, const double *x, const double *y, const double *z,
+                          double *ggeo, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(N >= 1 && N <= 9, "geometric factors: N must be 1..9");
+  NRS_CHECK(Nelements >= 0, "geometric factors: negative element count");
+  NRS_CHECK(Nelements == 0 || (x && y && z && ggeo), "geometric factors: null array");
+  std::vector<double> zz, ww;
+  nrs::gll_nodes(N, zz, ww);
+  nrs::DevBuf<double> D, w;
+  D.upload(nrs::dmatrix(zz));
+  w.upload(ww);
+  nrs::launch_geofacs(N, Nelements, D.get(), w.get(), x, y, z, ggeo, as_stream(stream));
+  NRS_HIP(hipStreamSynchronize(as_stream(stream)));
+  NRS_API_END
+}
+
+int nrs_ax(int N, int dtype, int Nelements, const int *elementList, const void *ggeo, const void *D,
+           const void *lambda0, const void *lambda1, int poisson, int lambda_field, const void *q,
+           void *Aq, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(N >= 1 && N <= 9, "Ax: N must be 1..9");
+  NRS_CHECK(Nelements >= 0, "Ax: negative element count");
+  if (Nelements == 0) return NRS_OK;
+  NRS_CHECK(ggeo && D && lambda0 && q && Aq, "Ax: null array");
+  NRS_CHECK(poisson || lambda1, "Ax: Helmholtz needs lambda1");
+  const hipStream_t s = as_stream(stream);
+  if (dtype == NRS_F64)
+    nrs::launch_ax<double>(N, Nelements, elementList, (const double *)ggeo, (const double *)D,
+                           (const double *)lambda0, (const double *)lambda1, poisson != 0,
+                           lambda_field != 0, (const double *)q, (double *)Aq, nullptr, s);
+  else if (dtype == NRS_F32)
+    nrs::launch_ax<float>(N, Nelements, elementList, (const float *)ggeo, (const float *)D,
+                          (const float *)lambda0, (const float *)lambda1, poisson != 0,
+                          lambda_field != 0, (const float *)q, (float *)Aq, nullptr, s);
+  else
+    throw nrs::Error(NRS_ERR_INVALID, "Ax: dtype must be NRS_F64 or NRS_F32");
+  NRS_API_END
+}
+
+int nrs_block_ax(int N, int Nelements, int64_t offset, int64_t loffset, const double *ggeo,
+                 const double *D, const double *lambda0, const double *lambda1, const double *q,
+                 double *Aq, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(N >= 1 && N <= 9, "block Ax: N must be 1..9");
+  NRS_CHECK(Nelements >= 0, "block Ax: negative element count");
+  if (Nelements == 0) return NRS_OK;
+  const int64_t L = (int64_t)Nelements * (N + 1) * (N + 1) * (N + 1);
+  NRS_CHECK(offset >= L && loffset >= L, "block Ax: field offsets smaller than E*Np");
+  NRS_CHECK(ggeo && D && lambda0 && lambda1 && q && Aq, "block Ax: null array");
+  nrs::launch_block_ax(N, Nelements, offset, loffset, ggeo, D, lambda0, lambda1, q, Aq, as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_build_diagonal(int N, int dtype, int Nelements, const void *ggeo, const void *D,
+                       const void *lambda0, const void *lambda1, int poisson, int lambda_field,
+                       void *diag, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(N >= 1 && N <= 9, "diagonal: N must be 1..9");
+  NRS_CHECK(Nelements >= 0, "diagonal: negative element count");
+  if (Nelements == 0) return NRS_OK;
+  NRS_CHECK(ggeo && D && lambda0 && diag && (poisson || lambda1), "diagonal: null array");
+  const hipStream_t s = as_stream(stream);
+  if (dtype == NRS_F64)
+    nrs::launch_build_diag<double>(N, Nelements, (const double *)ggeo, (const double *)D,
+                                   (const double *)lambda0, (const double *)lambda1, poisson != 0,
+                                   lambda_field != 0, (double *)diag, s);
+  else if (dtype == NRS_F32)
+    nrs::launch_build_diag<float>(N, Nelements, (const float *)ggeo, (const float *)D,
+                                  (const float *)lambda0, (const float *)lambda1, poisson != 0,
+                                  lambda_field != 0, (float *)diag, s);
+  else
+    throw nrs::Error(NRS_ERR_INVALID, "diagonal: dtype must be NRS_F64 or NRS_F32");
+  NRS_API_END
+}
+
+int nrs_mask(int dtype, int64_t Nmasked, const int *maskIds, void *v, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(Nmasked >= 0, "mask: negative count");
+  if (Nmasked == 0) return NRS_OK;
+  NRS_CHECK(maskIds && v, "mask: null array");
+  const hipStream_t s = as_stream(stream);
+  if (dtype == NRS_F64) nrs::launch_mask<double>(Nmasked, maskIds, (double *)v, s);
+  else if (dtype == NRS_F32) nrs::launch_mask<float>(Nmasked, maskIds, (float *)v, s);
+  else throw nrs::Error(NRS_ERR_INVALID, "mask: dtype must be NRS_F64 or NRS_F32");
+  NRS_API_END
+}
+
+int nrs_coarsen(int Nf, int Nc, int dtype, int Nelements, const void *R, const void *qf, void *qc,
+                void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(Nelements >= 0, "coarsen: negative element count");
+  if (Nelements == 0) return NRS_OK;
+  NRS_CHECK(R && qf && qc, "coarsen: null array");
+  const hipStream_t s = as_stream(stream);
+  if (dtype == NRS_F64) nrs::launch_coarsen<double>(Nf, Nc, Nelements, (const double *)R, (const double *)qf, (double *)qc, s);
+  else if (dtype == NRS_F32) nrs::launch_coarsen<float>(Nf, Nc, Nelements, (const float *)R, (const float *)qf, (float *)qc, s);
+  else throw nrs::Error(NRS_ERR_INVALID, "coarsen: dtype must be NRS_F64 or NRS_F32");
+  NRS_API_END
+}
+
+int nrs_prolongate(int Nf, int Nc, int dtype, int Nelements, const void *R, const void *qc, void *qf,
+                   void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(Nelements >= 0, "prolongate: negative element count");
+  if (Nelements == 0) return NRS_OK;
+  NRS_CHECK(R && qf && qc, "prolongate: null array");
+  const hipStream_t s = as_stream(stream);
+  if (dtype == NRS_F64) nrs::launch_prolongate<double>(Nf, Nc, Nelements, (const double *)R, (const double *)qc, (double *)qf, s);
+  else if (dtype == NRS_F32) nrs::launch_prolongate<float>(Nf, Nc, Nelements, (const float *)R, (const float *)qc, (float *)qf, s);
+  else throw nrs::Error(NRS_ERR_INVALID, "prolongate: dtype must be NRS_F64 or NRS_F32");
+  NRS_API_END
+}
+
+int nrs_ogs_setup(int64_t N, const long long *globalIds, nrs_ogs **ogs)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(ogs != nullptr, "ogs setup: null output handle");
+  *ogs = nullptr;
+  *ogs = nrs::ogs_setup(N, globalIds);
+  NRS_API_END
+}
+
+int nrs_ogs_free(nrs_ogs *ogs)
+{
+  NRS_API_BEGIN
+  delete ogs;
+  NRS_API_END
+}
+
+int nrs_ogs_gather_scatter(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int dtype, int op, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(ogs && (v || ogs->N == 0), "gather-scatter: null argument");
+  NRS_CHECK(op >= NRS_ADD && op <= NRS_MAX, "gather-scatter: unknown op");
+  const hipStream_t s = as_stream(stream);
+  if (dtype == NRS_F64) nrs::ogs_gather_scatter<double>(ogs, (double *)v, nvec, stride, op, s);
+  else if (dtype == NRS_F32) nrs::ogs_gather_scatter<float>(ogs, (float *)v, nvec, stride, op, s);
+  else if (dtype == NRS_I32) nrs::ogs_gather_scatter<int>(ogs, (int *)v, nvec, stride, op, s);
+  else throw nrs::Error(NRS_ERR_INVALID, "gather-scatter: unknown dtype");
+  NRS_API_END
+}
+
+int nrs_ogs_inv_degree(nrs_ogs *ogs, double *invDegree, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(ogs && (invDegree || ogs->N == 0), "invDegree: null argument");
+  if (ogs->N)
+    NRS_HIP(hipMemcpyAsync(invDegree, ogs->invDegree.get(), ogs->N * sizeof(double),
+                           hipMemcpyDeviceToDevice, as_stream(stream)));
+  NRS_API_END
+}
+
+int nrs_ogs_info(nrs_ogs *ogs, int64_t *Nrows, int64_t *Nmembers, int64_t *Nsingletons)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(ogs, "ogs info: null handle");
+  if (Nrows) *Nrows = ogs->Nrows;
+  if (Nmembers) *Nmembers = ogs->Nmembers;
+  if (Nsingletons) *Nsingletons = ogs->Nsingletons;
+  NRS_API_END
+}
+
+void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
+{
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->solver = NRS_SOLVER_PCG;
+  o->preconditioner = NRS_PRECON_JACOBI;
+  o->tol = 1e-8;
+  o->relative = 1;
+  o->maxIter = 500;
+  o->nLevels = 0;
+  o->smoother = NRS_SMOOTHER_CHEBYSHEV_JACOBI;
+  o->chebyDegree = 2;
+  o->chebyMinFactor = 0.1;
+  o->chebyMaxFactor = 1.1;
+  o->coarseSolver = NRS_COARSE_PCG_F64;
+  o->coarseTol = 1e-2;
+  o->coarseMaxIter = 200;
+  o->mgPrecision = NRS_MG_F32;
+  o->fdmPrecision = NRS_FDM_F32;
+  o->seed = 0;
+}
+
+int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts, nrs_elliptic **out)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(out != nullptr, "elliptic setup: null output handle");
+  *out = nullptr;
+  *out = nrs::elliptic_setup(desc, opts);
+  NRS_API_END
+}
+
+int nrs_elliptic_free(nrs_elliptic *el)
+{
+  NRS_API_BEGIN
+  delete el;
+  NRS_API_END
+}
+
+int nrs_elliptic_info(nrs_elliptic *el, int64_t *Nlocal, int64_t *Nmasked, double *volume)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el, "elliptic info: null handle");
+  if (Nlocal) *Nlocal = el->Nlocal;
+  if (Nmasked) *Nmasked = el->Nmasked;
+  if (volume) *volume = el->volume;
+  NRS_API_END
+}
+
+int nrs_elliptic_buffer(nrs_elliptic *el, int which, void **ptr, int64_t *count)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && ptr, "elliptic buffer: null argument");
+  void *p = nullptr;
+  int64_t n = 0;
+  switch (which) {
+    case 0: p = el->ggeo.get(); n = (int64_t)el->ggeo.size(); break;
+    case 1: p = el->D.get(); n = (int64_t)el->D.size(); break;
+    case 2: p = el->ogs->invDegree.get(); n = (int64_t)el->ogs->invDegree.size(); break;
+    case 3: p = el->maskIds.get(); n = el->Nmasked; break;
+    case 4: p = el->invDiagA.get(); n = (int64_t)el->invDiagA.size(); break;
+    default: throw nrs::Error(NRS_ERR_INVALID, "elliptic buffer: unknown buffer id");
+  }
+  *ptr = p;
+  if (count) *count = n;
+  NRS_API_END
+}
+
+int nrs_elliptic_copy_buffer(nrs_elliptic *el, int which, void *dst, int64_t count, void *stream)
+{
+  NRS_API_BEGIN
+  void *p = nullptr;
+  int64_t n = 0;
+  const int rc = nrs_elliptic_buffer(el, which, &p, &n);
+  if (rc != NRS_OK) return rc;
+  NRS_CHECK(count >= 0 && count <= n, "copy buffer: count exceeds the buffer");
+  NRS_CHECK(dst || count == 0, "copy buffer: null destination");
+  const size_t esz = (which == 3) ? sizeof(int) : sizeof(double);
+  if (count)
+    NRS_HIP(hipMemcpyAsync(dst, p, count * esz, hipMemcpyDeviceToDevice, as_stream(stream)));
+  NRS_API_END
+}
+
+int nrs_elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, int masked, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && q && Aq, "operator: null argument");
+  nrs::elliptic_operator(el, q, Aq, masked != 0, as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && r && z, "preconditioner: null argument");
+  nrs::elliptic_preconditioner(el, r, z, as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_elliptic_profile_enable(nrs_elliptic *el, int on)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el, "profile: null handle");
+  el->profiling = on != 0;
+  el->spans.clear();
+  el->eventNext = 0;
+  NRS_API_END
+}
+
+int nrs_elliptic_profile_read(nrs_elliptic *el, double ms[NRS_PHASE_COUNT], int64_t launches[NRS_PHASE_COUNT])
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && ms && launches, "profile: null argument");
+  for (int p = 0; p < NRS_PHASE_COUNT; ++p) { ms[p] = 0.0; launches[p] = 0; }
+  for (const auto &sp : el->spans) {
+    NRS_HIP(hipEventSynchronize(sp.b));
+    float t = 0.0f;
+    NRS_HIP(hipEventElapsedTime(&t, sp.a, sp.b));
+    ms[sp.phase] += t;
+    launches[sp.phase] += 1;
+  }
+  NRS_API_END
+}
+
+int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *stats, void *stream)
+{
+  try {
+    NRS_CHECK(el && rhs && x, "solve: null argument");
+    const int rc = nrs::elliptic_solve(el, rhs, x, stats, as_stream(stream));
+    if (rc != NRS_OK) nrs::set_last_error("solve: NaN residual norm");
+    return rc;
+  } catch (const nrs::Error &e) {
+    nrs::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    nrs::set_last_error(e.what());
+    return NRS_ERR_INVALID;
+  }
+}
+
+}  // extern "C"

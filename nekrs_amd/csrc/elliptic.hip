@@ -1,0 +1,270 @@
+// elliptic.hip -- elliptic_t setup, operator, preconditioner dispatch and the
+// ellipticSolve/pcg driver (host C++ enqueueing HIP kernels on the caller's stream).
+#include <chrono>
+#include <cmath>
+
+#include "elliptic.hpp"
+#include "kernels.hpp"
+#include "mesh.hpp"
+#include "mg.hpp"
+
+nrs_elliptic::~nrs_elliptic()
+{
+  if (ogs) delete ogs;
+  if (ogsFull) delete ogsFull;
+  if (scalHost) (void)hipHostFree(scalHost);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  for (auto e : eventPool) (void)hipEventDestroy(e);
+}
+
+hipEvent_t nrs_elliptic::next_event()
+{
+  if (eventNext == eventPool.size()) {
+    hipEvent_t e;
+    NRS_HIP(hipEventCreate(&e));
+    eventPool.push_back(e);
+  }
+  return eventPool[eventNext++];
+}
+
+namespace nrs {
+
+std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds)
+{
+  const int NW = (Np + 31) / 32;
+  std::vector<uint32_t> bits((size_t)E * NW, 0u);
+  for (int id : maskIds) {
+    const int64_t e = id / Np;
+    const int n = (int)(id - e * Np);
+    bits[e * NW + (n >> 5)] |= (1u << (n & 31));
+  }
+  return bits;
+}
+
+// Dirichlet mask: restates ellipticOgs (src/solvers/elliptic/ellipticOgs.cpp:17-60):
+// per node the smallest BC id of the faces it lies on (DIRICHLET=1 wins), reduced
+// with gs-min over the unmasked handle, then every node equal to DIRICHLET.
+std::vector<int> dirichlet_mask(int N, int E, const int *EToB, nrs_ogs *ogsFull, hipStream_t s)
+{
+  const int Nq = N + 1, Np = Nq * Nq * Nq;
+  const int64_t L = (int64_t)E * Np;
+  const int large = 1 << 20;
+  std::vector<int> mapB(L, large);
+  for (int64_t e = 0; e < E; ++e)
+    for (int f = 0; f < 6; ++f) {
+      const int bc = EToB[e * 6 + f];
+      if (bc <= 0) continue;
+      for (int b = 0; b < Nq; ++b)
+        for (int a = 0; a < Nq; ++a) {
+          int i, j, k;
+          switch (f) {  // FaceNodesHex3D (src/mesh/meshBasisHex3D.cpp:53-82)
+            case 0: i = a; j = b; k = 0; break;
+            case 1: i = a; j = 0; k = b; break;
+            case 2: i = N; j = a; k = b; break;
+            case 3: i = a; j = N; k = b; break;
+            case 4: i = 0; j = a; k = b; break;
+            default: i = a; j = b; k = N; break;
+          }
+          int &m = mapB[e * Np + i + Nq * (j + Nq * k)];
+          if (bc < m) m = bc;
+        }
+    }
+  DevBuf<int> d;
+  d.upload(mapB);
+  ogs_gather_scatter<int>(ogsFull, d.get(), 1, L, NRS_MIN, s);
+  NRS_HIP(hipStreamSynchronize(s));
+  mapB = d.download();
+  std::vector<int> ids;
+  for (int64_t n = 0; n < L; ++n)
+    if (mapB[n] == 1) ids.push_back((int)n);
+  return ids;
+}
+
+nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts *o)
+{
+  NRS_CHECK(d && o, "elliptic setup: null descriptor/options");
+  NRS_CHECK(d->N >= 1 && d->N <= 9, "elliptic setup: N must be 1..9");
+  NRS_CHECK(d->Nelements > 0, "elliptic setup: no elements");
+  NRS_CHECK(d->x && d->y && d->z && d->globalIds && d->EToB, "elliptic setup: null mesh arrays");
+  NRS_CHECK(o->solver == NRS_SOLVER_PCG || o->solver == NRS_SOLVER_FCG, "unknown solver");
+  NRS_CHECK(o->preconditioner >= NRS_PRECON_NONE && o->preconditioner <= NRS_PRECON_MULTIGRID,
+            "unknown preconditioner");
+  NRS_CHECK(o->maxIter >= 1, "maxIter must be >= 1");
+  // ellipticSetup.cpp:59-73: multigrid with non-Jacobi smoothers and its coarse solve
+  // are Poisson-only in the reference; kept as in the reference.
+  auto el = std::make_unique<nrs_elliptic>();
+  el->N = d->N; el->Nq = d->N + 1; el->Np = el->Nq * el->Nq * el->Nq; el->E = d->Nelements;
+  el->Nlocal = (int64_t)el->E * el->Np;
+  NRS_CHECK(el->Nlocal < (int64_t)INT32_MAX, "E*Np must fit in int32 (dlong)");
+  el->poisson = d->poisson != 0;
+  el->lambda0 = d->lambda0;
+  el->lambda1 = d->poisson ? 0.0 : d->lambda1;
+  el->opts = *o;
+  hipStream_t s = nullptr;
+
+  gll_nodes(el->N, el->z, el->w);
+  el->Dh = dmatrix(el->z);
+  el->D.upload(el->Dh);
+  el->lam0.upload(std::vector<double>{el->lambda0});
+  el->lam1.upload(std::vector<double>{el->lambda1});
+
+  // geometric factors on the device (meshGeometricFactorsHex3D)
+  {
+    DevBuf<double> x, y, zc, wd;
+    x.upload(d->x, el->Nlocal); y.upload(d->y, el->Nlocal); zc.upload(d->z, el->Nlocal);
+    wd.upload(el->w);
+    el->ggeo.alloc((size_t)el->Nlocal * 7);
+    launch_geofacs(el->N, el->E, el->D.get(), wd.get(), x.get(), y.get(), zc.get(), el->ggeo.get(), s);
+    NRS_HIP(hipStreamSynchronize(s));
+  }
+  el->red = std::make_unique<Reducer>();
+  el->scal.alloc(S_COUNT);
+  NRS_HIP(hipMemset(el->scal.get(), 0, S_COUNT * sizeof(double)));
+  NRS_HIP(hipHostMalloc((void **)&el->scalHost, S_COUNT * sizeof(double), hipHostMallocDefault));
+  plane_sum(*el->red, el->E, el->Np, 7 * (int64_t)el->Np, 6 * (int64_t)el->Np, el->ggeo.get(),
+            el->scal.get(), S_TMP0, s);
+  NRS_HIP(hipMemcpy(el->scalHost, el->scal.get(), S_COUNT * sizeof(double), hipMemcpyDeviceToHost));
+  el->volume = el->scalHost[S_TMP0];
+
+  // gather-scatter handles and the Dirichlet mask
+  el->ogsFull = ogs_setup(el->Nlocal, d->globalIds);
+  const std::vector<int> mids = dirichlet_mask(el->N, el->E, d->EToB, el->ogsFull, s);
+  el->Nmasked = (int64_t)mids.size();
+  el->maskIds.upload(mids);
+  el->maskBits.upload(mask_bits(el->Np, el->E, mids));
+  {
+    std::vector<long long> mg(d->globalIds, d->globalIds + el->Nlocal);
+    for (int id : mids) mg[id] = 0;
+    el->ogs = ogs_setup(el->Nlocal, mg.data());
+  }
+
+  // workspace
+  el->p.alloc(el->Nlocal); el->zv.alloc(el->Nlocal); el->Ap.alloc(el->Nlocal); el->x0.alloc(el->Nlocal);
+  NRS_HIP(hipEventCreate(&el->ev0));
+  NRS_HIP(hipEventCreate(&el->ev1));
+
+  // Jacobi: ellipticUpdateJacobi (src/solvers/elliptic/ellipticUpdateJacobi.cpp:62-85)
+  if (o->preconditioner == NRS_PRECON_JACOBI) {
+    el->invDiagA.alloc(el->Nlocal);
+    launch_build_diag<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                              el->lam1.get(), el->poisson, false, el->invDiagA.get(), s);
+    ogs_gather_scatter<double>(el->ogs, el->invDiagA.get(), 1, el->Nlocal, NRS_ADD, s);
+    reciprocal<double>(el->Nlocal, el->invDiagA.get(), s);
+  }
+  if (o->preconditioner == NRS_PRECON_MULTIGRID) {
+    el->mg = std::make_unique<MGSolver>();
+    mg_setup(*el->mg, el.get(), d, s);
+  }
+  NRS_HIP(hipStreamSynchronize(s));
+  return el.release();
+}
+
+void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s)
+{
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  if (el->profiling) {
+    e0 = el->next_event(); e1 = el->next_event(); e2 = el->next_event();
+    NRS_HIP(hipEventRecord(e0, s));
+  }
+  launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                    el->lam1.get(), el->poisson, false, q, Aq, masked ? el->maskBits.get() : nullptr, s);
+  if (el->profiling) NRS_HIP(hipEventRecord(e1, s));
+  ogs_gather_scatter<double>(el->ogs, Aq, 1, el->Nlocal, NRS_ADD, s);
+  if (el->profiling) {
+    NRS_HIP(hipEventRecord(e2, s));
+    el->spans.push_back({NRS_PHASE_AX, e0, e1});
+    el->spans.push_back({NRS_PHASE_GS, e1, e2});
+  }
+}
+
+void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s)
+{
+  switch (el->opts.preconditioner) {
+    case NRS_PRECON_JACOBI: axmyz<double>(el->Nlocal, el->invDiagA.get(), r, z, s); break;
+    case NRS_PRECON_MULTIGRID: mg_apply(*el->mg, r, z, s); break;
+    default:
+      NRS_HIP(hipMemcpyAsync(z, r, el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
+      break;
+  }
+}
+
+// ellipticSolve (src/solvers/elliptic/ellipticSolve.cpp:32-190) + pcg (PCG.cpp:85-203).
+int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s)
+{
+  const int64_t L = el->Nlocal;
+  const double *w = el->ogs->invDegree.get();
+  double *r = rhs;
+  double *scal = el->scal.get();
+  Reducer &R = *el->red;
+  const auto &o = el->opts;
+  const bool flexible = o.solver == NRS_SOLVER_FCG;
+  const bool none = o.preconditioner == NRS_PRECON_NONE;
+  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI;
+  const double resNormFactor = 1.0 / el->volume;
+  auto t0 = std::chrono::steady_clock::now();
+  NRS_HIP(hipEventRecord(el->ev0, s));
+
+  // r = rhs - A x0 (element apply only), mask, gs   (ellipticSolve.cpp:91-103)
+  launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                    el->lam1.get(), el->poisson, false, x, el->Ap.get(), nullptr, s);
+  axpby<double>(L, -1.0, el->Ap.get(), 1.0, r, s);
+  launch_mask<double>(el->Nmasked, el->maskIds.get(), r, s);
+  ogs_gather_scatter<double>(el->ogs, r, 1, L, NRS_ADD, s);
+  // x0 = x, x = 0   (:105-106)
+  NRS_HIP(hipMemcpyAsync(el->x0.get(), x, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+  fill<double>(L, 0.0, x, s);
+  // res0 = ||r||_w / sqrt(volume)   (:130-139)
+  wnorm(R, L, w, r, resNormFactor, scal, S_RDOTR, s);
+  NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+  NRS_HIP(hipStreamSynchronize(s));
+  const double res0 = el->scalHost[S_RDOTR];
+  if (std::isnan(res0)) return NRS_ERR_NAN;
+  double tol = o.tol;
+  if (o.relative) tol *= res0;
+
+  // pcg
+  double *p = el->p.get(), *Ap = el->Ap.get();
+  double *z = none ? r : el->zv.get();
+  fill<double>(L, 0.0, p, s);
+  double rdotr = res0;
+  int iter = 0;
+  double precon_s = 0.0;
+  int status = NRS_OK;
+  do {
+    iter++;
+    if (!none && !jacobi) {
+      auto p0 = std::chrono::steady_clock::now();
+      elliptic_preconditioner(el, r, z, s);
+      if (st) {  // per-phase timing only when stats are requested
+        NRS_HIP(hipStreamSynchronize(s));
+        precon_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
+      }
+    }
+    pcg_precon_dots(R, L, w, el->invDiagA.get(), r, z, Ap, jacobi, flexible, none, iter, scal, s);
+    pcg_update_p(L, z, p, scal, s);
+    elliptic_operator(el, p, Ap, true, s);
+    pcg_alpha(R, L, w, p, Ap, scal, s);
+    pcg_update_rx(R, L, w, Ap, p, r, x, resNormFactor, scal, s);
+    NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    rdotr = el->scalHost[S_RDOTR];
+    if (std::isnan(rdotr)) { status = NRS_ERR_NAN; break; }
+  } while (rdotr > tol && iter < o.maxIter);
+
+  // x += x0   (:178-186)
+  axpby<double>(L, 1.0, el->x0.get(), 1.0, x, s);
+  NRS_HIP(hipEventRecord(el->ev1, s));
+  NRS_HIP(hipStreamSynchronize(s));
+  if (st) {
+    st->iterations = iter;
+    st->converged = (rdotr <= tol) ? 1 : 0;
+    st->res0 = res0;
+    st->res = rdotr;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->precon_seconds = precon_s;
+  }
+  return status;
+}
+
+}  // namespace nrs

@@ -1,0 +1,63 @@
+// elliptic.hpp -- solver state (internal layout of nrs_elliptic).
+#pragma once
+#include <memory>
+
+#include "common.hpp"
+#include "linalg.hpp"
+#include "ogs.hpp"
+
+namespace nrs {
+struct MGSolver;
+}
+
+// Mirrors elliptic_t (src/solvers/elliptic/elliptic.h:73-181) for one field.
+struct nrs_elliptic {
+  int N = 0, Nq = 0, Np = 0, E = 0;
+  int64_t Nlocal = 0;
+  bool poisson = true;
+  double lambda0 = 1.0, lambda1 = 0.0;
+  double volume = 0.0;
+  nrs_elliptic_opts opts{};
+
+  std::vector<double> z, w, Dh;      // GLL nodes/weights, D (host)
+  nrs::DevBuf<double> ggeo, D, lam0, lam1;
+  nrs::DevBuf<float> ggeo32, D32, lam032, lam132;   // FP32 copies (multigrid levels)
+
+  nrs_ogs *ogsFull = nullptr;        // unmasked (mesh->ogs)
+  nrs_ogs *ogs = nullptr;            // masked   (elliptic->ogs)
+  int64_t Nmasked = 0;
+  nrs::DevBuf<int> maskIds;
+  nrs::DevBuf<uint32_t> maskBits;    // [E][ceil(Np/32)]
+  nrs::DevBuf<double> invDiagA;      // Jacobi (assembled), fine level
+
+  // workspace (ellipticUpdateWorkspace, elliptic.h:240-262)
+  nrs::DevBuf<double> p, zv, Ap, x0;
+  nrs::DevBuf<double> scal;          // nrs::S_COUNT device scalars
+  double *scalHost = nullptr;        // pinned mirror
+  std::unique_ptr<nrs::Reducer> red;
+  std::unique_ptr<nrs::MGSolver> mg;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // Per-phase device timing (platform->timer analogue, src/core/timer.cpp:131-230):
+  // when enabled, events bracket each Ax and gather-scatter launch of the
+  // operator on the caller's stream; totals are read after a synchronisation.
+  struct Span { int phase; hipEvent_t a, b; };
+  bool profiling = false;
+  std::vector<Span> spans;
+  std::vector<hipEvent_t> eventPool;
+  size_t eventNext = 0;
+  hipEvent_t next_event();
+
+  ~nrs_elliptic();
+};
+
+namespace nrs {
+nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts *o);
+void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s);
+void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s);
+int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s);
+
+// Shared helpers for the multigrid levels.
+std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds);
+std::vector<int> dirichlet_mask(int N, int E, const int *EToB, nrs_ogs *ogsFull, hipStream_t s);
+}  // namespace nrs

@@ -1,0 +1,33 @@
+// kernels.hpp -- host-side launchers of the HIP kernels (internal to libnekrs_hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nrs {
+
+// ax.hip
+template <typename T>
+void launch_ax(int N, int E, const int *elist, const T *ggeo, const T *D, const T *l0, const T *l1,
+               bool poisson, bool lfield, const T *q, T *Aq, const uint32_t *maskBits,
+               hipStream_t s);
+void launch_block_ax(int N, int E, int64_t offset, int64_t loffset, const double *ggeo,
+                     const double *D, const double *l0, const double *l1, const double *q,
+                     double *Aq, hipStream_t s);
+template <typename T>
+void launch_build_diag(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1,
+                       bool poisson, bool lfield, T *diag, hipStream_t s);
+template <typename T>
+void launch_mask(int64_t Nm, const int *ids, T *v, hipStream_t s);
+
+// mesh.hip
+void launch_geofacs(int N, int E, const double *D, const double *w, const double *x,
+                    const double *y, const double *z, double *ggeo, hipStream_t s);
+
+// transfer.hip
+template <typename T>
+void launch_coarsen(int Nf, int Nc, int E, const T *R, const T *qf, T *qc, hipStream_t s);
+template <typename T>
+void launch_prolongate(int Nf, int Nc, int E, const T *R, const T *qc, T *qf, hipStream_t s);
+
+}  // namespace nrs

@@ -1,0 +1,28 @@
+// ogs.hpp -- gather-scatter handle (internal layout of nrs_ogs).
+#pragma once
+#include "common.hpp"
+
+struct nrs_ogs {
+  int64_t N = 0;              // local E-vector length
+  int64_t Nrows = 0;          // rows with > 1 member
+  int64_t Nmembers = 0;       // members of those rows
+  int64_t Nsingletons = 0;    // rows with exactly one member (never touched)
+  // Rows bucketed by multiplicity; inside a bucket rows keep the reference's
+  // first-appearance order and members ascend, so sums are formed in the same
+  // order as gatherScatterMany (left to right from the op identity).
+  int64_t n2 = 0, n4 = 0, n8 = 0, nx = 0;   // rows per bucket (x = any other size)
+  nrs::DevBuf<int> pairs;     // [n2][2]
+  nrs::DevBuf<int> quads;     // [n4][4]
+  nrs::DevBuf<int> octs;      // [n8][8]
+  nrs::DevBuf<int> xoff;      // [nx+1] CSR offsets into xids
+  nrs::DevBuf<int> xids;
+  nrs::DevBuf<double> invDegree;  // [N]
+  std::vector<double> invDegreeHost;
+};
+
+namespace nrs {
+// In-place gather-scatter of nvec vectors (stride apart) of type T with op.
+template <typename T>
+void ogs_gather_scatter(const nrs_ogs *o, T *v, int nvec, int64_t stride, int op, hipStream_t s);
+nrs_ogs *ogs_setup(int64_t N, const long long *ids);
+}  // namespace nrs

@@ -1,0 +1,111 @@
+/*
+ * nrs_oracle.h -- CPU restatement of nekRS's pressure-Poisson hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in nekrs_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the nekRS v23 source tree).  Layouts follow the reference exactly:
+ *   E-vectors   [field][e][k][j][i]           (i fastest)
+ *   ggeo        [e][7][Np]  ids G00,G01,G11,G12,G02,G22,GWJ = 0..6
+ *   D           row-major, D[i*Nq+j] = d(l_j)/dr at r_i
+ */
+#ifndef NRS_ORACLE_H
+#define NRS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_G00 = 0, ORC_G01 = 1, ORC_G11 = 2, ORC_G12 = 3, ORC_G02 = 4, ORC_G22 = 5, ORC_GWJ = 6 };
+enum { ORC_OP_ADD = 0, ORC_OP_MUL = 1, ORC_OP_MIN = 2, ORC_OP_MAX = 3 };
+enum { ORC_MAP_NONE = 0, ORC_MAP_SINE = 1, ORC_MAP_KERSHAW = 2 };
+#define ORC_DIRICHLET 1
+#define ORC_NEUMANN 4
+
+/* ---- 1D basis (src/mesh/meshBasis1D.cpp) ---------------------------------- */
+void orc_gll(int N, double *z, double *w);
+void orc_dmatrix(int N, const double *z, double *D);
+void orc_interp(int Nin, const double *zin, int Nout, const double *zout, double *I);
+
+/* ---- structured box meshes (replaces Nek5000 genbox + usrdat maps) -------- */
+void orc_kershaw_map(double eps_y, double eps_z, double x, double y, double z,
+                     double *xx, double *yy, double *zz);
+void orc_box_mesh(int nx, int ny, int nz, int N, int map, double eps,
+                  double *x, double *y, double *z, long long *gid, int *EToB);
+
+/* ---- geometric factors (kernels/mesh/geometricFactorsHex3D.okl) ---------- */
+void orc_geofacs(int N, int E, const double *D, const double *w,
+                 const double *x, const double *y, const double *z,
+                 double *ggeo, double *J);
+
+/* ---- gather-scatter (3rd_party/gslib/ogs) --------------------------------- */
+typedef struct {
+  int64_t N;          /* local E-vector length                                */
+  int64_t Nrows;      /* local gather rows (including singletons)             */
+  int64_t Nids;       /* members of all rows (= #ids != 0)                    */
+  int64_t *offsets;   /* [Nrows+1]                                            */
+  int64_t *ids;       /* [Nids] local indices, ascending within a row         */
+  double *invDegree;  /* [N]  1/multiplicity, 1 at excluded (id==0) nodes      */
+} orc_ogs;
+
+orc_ogs *orc_ogs_setup(int64_t N, const long long *gids);
+void orc_ogs_free(orc_ogs *o);
+int64_t orc_ogs_nrows(const orc_ogs *o);
+void orc_ogs_get(const orc_ogs *o, int64_t *offsets, int64_t *ids, double *invDegree);
+void orc_gs_f64(const orc_ogs *o, double *v, int nvec, int64_t stride, int op);
+void orc_gs_f32(const orc_ogs *o, float *v, int nvec, int64_t stride, int op);
+void orc_gs_i32(const orc_ogs *o, int *v, int nvec, int64_t stride, int op);
+
+/* ---- Dirichlet mask (src/solvers/elliptic/ellipticOgs.cpp:4-134) ---------- */
+int64_t orc_mask_ids(int N, int E, const orc_ogs *unmasked, const int *EToB, int64_t *maskIds);
+void orc_mask_f64(int64_t Nmasked, const int64_t *maskIds, double *v);
+void orc_mask_f32(int64_t Nmasked, const int64_t *maskIds, float *v);
+
+/* ---- element operators ---------------------------------------------------- */
+void orc_ax_f64(int N, int E, const int *elist, const double *ggeo, const double *D,
+                const double *lambda0, const double *lambda1, int poisson, int lambda_field,
+                const double *q, double *Aq);
+void orc_ax_f32(int N, int E, const int *elist, const float *ggeo, const float *D,
+                const float *lambda0, const float *lambda1, int poisson, int lambda_field,
+                const float *q, float *Aq);
+void orc_block_ax_f64(int N, int E, int nfld, int64_t offset, int64_t loffset,
+                      const double *ggeo, const double *D, const double *lambda0,
+                      const double *lambda1, const double *q, double *Aq);
+void orc_build_diag_f64(int N, int E, const double *ggeo, const double *D,
+                        const double *lambda0, const double *lambda1, int poisson,
+                        int lambda_field, double *diag);
+void orc_coarsen(int Nf, int Nc, int E, const double *R, const double *qf, double *qc);
+void orc_prolongate(int Nf, int Nc, int E, const double *R, const double *qc, double *qf);
+
+/* ---- BLAS-1 on E-vectors (src/linAlg, kernels/linAlg) --------------------- */
+double orc_wdot(int64_t n, const double *w, const double *a, const double *b);
+double orc_update_pcg(int64_t n, const double *w, const double *Ap, double alpha, double *r);
+
+/* ---- solvers --------------------------------------------------------------- */
+typedef struct {
+  int N, E;
+  const double *ggeo, *D;
+  const orc_ogs *ogs;          /* masked ogs                                     */
+  int64_t Nmasked;
+  const int64_t *maskIds;
+  double lambda0, lambda1;     /* constant coefficients                          */
+  int poisson;
+  double volume;               /* resNormFactor = 1/volume                       */
+} orc_elliptic;
+
+void orc_operator(const orc_elliptic *el, const double *q, double *Aq);
+/* precon: 0 = NONE, 1 = JACOBI (invDiag given) */
+int orc_pcg(const orc_elliptic *el, int precon, const double *invDiag, int flexible,
+            double *r, double *x, double tol, int maxit, double *rdotr, double *reshist);
+int orc_solve(const orc_elliptic *el, int precon, const double *invDiag, int flexible,
+              double tol, int relative, int maxit, const double *rhs, double *x,
+              double *res0, double *res, double *reshist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
