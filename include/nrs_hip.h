@@ -197,6 +197,7 @@ typedef struct {
   int mgPrecision;               /* nrs_mg_precision                            */
   int fdmPrecision;              /* nrs_fdm_precision                           */
   unsigned long long seed;       /* Arnoldi start vector seed                   */
+  int useGraphs;                 /* replay captured HIP graphs for the coarse PCG */
 } nrs_elliptic_opts;
 
 typedef struct {
@@ -246,6 +247,18 @@ int nrs_elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, vo
  * (stats->converged = 0), NRS_ERR_NAN on a NaN residual. */
 int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *stats,
                        void *stream);
+
+/* p-multigrid level report (MGSolver_t::Report analogue,
+ * MG/ellipticMultiGridSetup.cpp:326-342): for each level i < *nLevels (<= max)
+ * its order, E-vector length, estimated lambda_max(S A) and Chebyshev degree;
+ * coarseIterations = PCG iterations of the last coarse solve. */
+int nrs_elliptic_mg_info(nrs_elliptic *el, int max, int *nLevels, int *orders, int64_t *Nlocal,
+                         double *rho, int *degree, int *coarseIterations);
+
+/* Global ids of the order-Nc nodes of a conforming hexahedral mesh derived from
+ * its order-Nf ids (host arrays [E*(Nf+1)^3] -> [E*(Nc+1)^3]); replaces Nek5000's
+ * numbering of the multigrid meshes (createMeshMG -> meshGlobalIds). */
+int nrs_coarse_global_ids(int Nf, int Nelements, const long long *idsFine, int Nc, long long *idsCoarse);
 
 /* ------------------------------------------------------------------------- */
 /* Device timers (replaces platform->timer device tags, src/core/timer.cpp)   */

@@ -98,6 +98,14 @@ class BoxMesh:
         self.x, self.y, self.z, self.gid, self.EToB = box_mesh(nx, ny, nz, N, map, eps)
 
 
+def coarse_global_ids(Nf, E, ids_fine, Nc):
+    """Order-Nc global ids derived from order-Nf ones (host arrays)."""
+    ids_fine = np.ascontiguousarray(ids_fine, dtype=np.int64)
+    out = np.zeros(E * (Nc + 1) ** 3, dtype=np.int64)
+    _lib.call("nrs_coarse_global_ids", Nf, E, ids_fine.ctypes.data, Nc, out.ctypes.data)
+    return out
+
+
 def geometric_factors(N, x, y, z, ggeo, stream=None):
     E = x.numel() // (N + 1) ** 3
     _lib.call("nrs_geometric_factors", N, E, _ptr(x), _ptr(y), _ptr(z), _ptr(ggeo), _stream(stream))
@@ -264,6 +272,16 @@ class Elliptic:
 
     def preconditioner(self, r, z, stream=None):
         _lib.call("nrs_elliptic_preconditioner", self._h, _ptr(r), _ptr(z), _stream(stream))
+
+    def mg_info(self):
+        """Multigrid levels: [{order, Nlocal, rho, degree}], coarse PCG iterations."""
+        n = C.c_int()
+        orders = (C.c_int * 8)(); nl = (C.c_int64 * 8)(); rho = (C.c_double * 8)()
+        deg = (C.c_int * 8)(); cit = C.c_int()
+        _lib.call("nrs_elliptic_mg_info", self._h, 8, C.byref(n), orders, nl, rho, deg, C.byref(cit))
+        lv = [{"order": orders[i], "Nlocal": nl[i], "rho": rho[i], "degree": deg[i]}
+              for i in range(min(n.value, 8))]
+        return lv, cit.value
 
     def profile(self, on=True):
         """Enable/disable per-launch device timing of the operator's kernels."""

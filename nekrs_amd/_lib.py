@@ -35,7 +35,8 @@ class EllipticOpts(C.Structure):
                 ("levels", C.c_int * 8), ("smoother", C.c_int), ("chebyDegree", C.c_int),
                 ("chebyMinFactor", C.c_double), ("chebyMaxFactor", C.c_double),
                 ("coarseSolver", C.c_int), ("coarseTol", C.c_double), ("coarseMaxIter", C.c_int),
-                ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong)]
+                ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong),
+                ("useGraphs", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -72,6 +73,8 @@ SIGNATURES = {
     "nrs_elliptic_operator": ([_vp, _vp, _vp, _i, _vp], _i),
     "nrs_elliptic_preconditioner": ([_vp, _vp, _vp, _vp], _i),
     "nrs_elliptic_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats), _vp], _i),
+    "nrs_elliptic_mg_info": ([_vp, _i, C.POINTER(_i), _vp, _vp, _vp, _vp, C.POINTER(_i)], _i),
+    "nrs_coarse_global_ids": ([_i, _i, _vp, _i, _vp], _i),
     "nrs_elliptic_profile_enable": ([_vp, _i], _i),
     "nrs_elliptic_profile_read": ([_vp, C.POINTER(_d), C.POINTER(_i64)], _i),
 }

@@ -23,6 +23,13 @@ constexpr int kG00 = 0, kG01 = 1, kG11 = 2, kG12 = 3, kG02 = 4, kG22 = 5, kGWJ =
 #define AX_WAVES_PER_EU 3
 #endif
 
+// Geometric factors are read exactly once per apply: load them non-temporally so
+// they do not evict q/Aq lines that the gather-scatter reuses.
+template <typename T> __device__ __forceinline__ T stream_load(const T *p)
+{
+  return __builtin_nontemporal_load(p);
+}
+
 template <int NQ> struct AxShape {
   // elements per block: fill at least one wave64
   static constexpr int NE = (NQ * NQ >= 64) ? 1 : (64 + NQ * NQ - 1) / (NQ * NQ);
@@ -87,7 +94,7 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
   // geometric factors of plane 0; later planes are prefetched one plane ahead
   T g[7];
 #pragma unroll
-  for (int c = 0; c < 7; ++c) g[c] = (active && (c < 6 || !POISSON)) ? G[c * NP + lid] : T(0);
+  for (int c = 0; c < 7; ++c) g[c] = (active && (c < 6 || !POISSON)) ? stream_load(G + c * NP + lid) : T(0);
 
 #pragma unroll 1
   for (int k = 0; k < NQ; ++k) {
@@ -98,7 +105,8 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
     T gn[7];
     const bool more = active && (k + 1 < NQ);
 #pragma unroll
-    for (int c = 0; c < 7; ++c) gn[c] = (more && (c < 6 || !POISSON)) ? G[c * NP + n + NQ * NQ + lid] : T(0);
+    for (int c = 0; c < 7; ++c)
+      gn[c] = (more && (c < 6 || !POISSON)) ? stream_load(G + c * NP + n + NQ * NQ + lid) : T(0);
     T l0 = l0c, l1 = l1c;
     if (LFIELD && active) {
       l0 = lam0[e * NP + n + lid];

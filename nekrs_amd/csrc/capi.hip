@@ -263,6 +263,7 @@ void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
   o->mgPrecision = NRS_MG_F32;
   o->fdmPrecision = NRS_FDM_F32;
   o->seed = 0;
+  o->useGraphs = 1;
 }
 
 int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts, nrs_elliptic **out)
@@ -338,6 +339,35 @@ int nrs_elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, vo
   NRS_API_BEGIN
   NRS_CHECK(el && r && z, "preconditioner: null argument");
   nrs::elliptic_preconditioner(el, r, z, as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_elliptic_mg_info(nrs_elliptic *el, int max, int *nLevels, int *orders, int64_t *Nlocal,
+                         double *rho, int *degree, int *coarseIterations)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && nLevels, "mg info: null argument");
+  NRS_CHECK(el->mg != nullptr, "mg info: the preconditioner is not multigrid");
+  const auto info = el->mg->info();
+  *nLevels = (int)info.size();
+  for (int i = 0; i < (int)info.size() && i < max; ++i) {
+    if (orders) orders[i] = info[i].N;
+    if (Nlocal) Nlocal[i] = info[i].Nlocal;
+    if (rho) rho[i] = info[i].rho;
+    if (degree) degree[i] = info[i].degree;
+  }
+  if (coarseIterations) *coarseIterations = el->mg->coarse_iterations();
+  NRS_API_END
+}
+
+int nrs_coarse_global_ids(int Nf, int Nelements, const long long *idsFine, int Nc, long long *idsCoarse)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(Nf >= 1 && Nc >= 1 && Nc <= Nf, "coarse ids: need 1 <= Nc <= Nf");
+  NRS_CHECK(Nelements >= 0, "coarse ids: negative element count");
+  NRS_CHECK(Nelements == 0 || (idsFine && idsCoarse), "coarse ids: null array");
+  const auto ids = nrs::coarse_global_ids(Nf, Nelements, idsFine, Nc);
+  std::memcpy(idsCoarse, ids.data(), ids.size() * sizeof(long long));
   NRS_API_END
 }
 

@@ -153,8 +153,7 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     reciprocal<double>(el->Nlocal, el->invDiagA.get(), s);
   }
   if (o->preconditioner == NRS_PRECON_MULTIGRID) {
-    el->mg = std::make_unique<MGSolver>();
-    mg_setup(*el->mg, el.get(), d, s);
+    el->mg = mg_create(el.get(), d, s);
   }
   NRS_HIP(hipStreamSynchronize(s));
   return el.release();
@@ -182,7 +181,7 @@ void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipSt
 {
   switch (el->opts.preconditioner) {
     case NRS_PRECON_JACOBI: axmyz<double>(el->Nlocal, el->invDiagA.get(), r, z, s); break;
-    case NRS_PRECON_MULTIGRID: mg_apply(*el->mg, r, z, s); break;
+    case NRS_PRECON_MULTIGRID: el->mg->apply(r, z, s); break;
     default:
       NRS_HIP(hipMemcpyAsync(z, r, el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
       break;
@@ -227,12 +226,12 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   double *p = el->p.get(), *Ap = el->Ap.get();
   double *z = none ? r : el->zv.get();
   fill<double>(L, 0.0, p, s);
+  pcg_init(scal, res0, tol, o.maxIter, s);
   double rdotr = res0;
   int iter = 0;
   double precon_s = 0.0;
   int status = NRS_OK;
-  do {
-    iter++;
+  for (;;) {
     if (!none && !jacobi) {
       auto p0 = std::chrono::steady_clock::now();
       elliptic_preconditioner(el, r, z, s);
@@ -241,7 +240,7 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
         precon_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
       }
     }
-    pcg_precon_dots(R, L, w, el->invDiagA.get(), r, z, Ap, jacobi, flexible, none, iter, scal, s);
+    pcg_precon_dots(R, L, w, el->invDiagA.get(), r, z, Ap, jacobi, flexible, none, scal, s);
     pcg_update_p(L, z, p, scal, s);
     elliptic_operator(el, p, Ap, true, s);
     pcg_alpha(R, L, w, p, Ap, scal, s);
@@ -249,8 +248,10 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
     NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
     NRS_HIP(hipStreamSynchronize(s));
     rdotr = el->scalHost[S_RDOTR];
+    iter = (int)el->scalHost[S_ITER];
     if (std::isnan(rdotr)) { status = NRS_ERR_NAN; break; }
-  } while (rdotr > tol && iter < o.maxIter);
+    if (el->scalHost[S_ACTIVE] == 0.0) break;     // rdotr <= tol or iter == maxIter (PCG.cpp:200)
+  }
 
   // x += x0   (:178-186)
   axpby<double>(L, 1.0, el->x0.get(), 1.0, x, s);

@@ -1,24 +1,31 @@
 // linalg.hip -- BLAS-1 on E-vectors and the fused PCG vector kernels.
 //
 // Reductions replace weightedInnerProdMany / ellipticBlockUpdatePCG + host sums
-// (src/linAlg/linAlg.cpp:986-1033, linearSolver/PCG.cpp:33-83): a fixed grid of
-// 1024 blocks writes one partial per block and the last block to arrive (atomic
-// ticket, acq_rel at agent scope) adds the partials in a fixed tree order, so the
-// result is deterministic and stays in device memory.  alpha and beta are formed
-// on the device by that last block.
+// (src/linAlg/linAlg.cpp:986-1033, linearSolver/PCG.cpp:33-83).  Each reduction
+// is two launches: the streaming kernel writes one partial per block (grid
+// sized to the vector, at most 1024 blocks), a one-block kernel adds them in a
+// fixed order and performs the scalar step of the algorithm (alpha, beta, the
+// residual norm, the convergence flag), so the result is deterministic and
+// stays in device memory.
 #include "linalg.hpp"
 
 namespace nrs {
 
-Reducer::Reducer() : partials(kBlocks * 4), ticket(1)
-{
-  NRS_HIP(hipMemset(ticket.get(), 0, sizeof(unsigned)));
-}
+Reducer::Reducer() : partials(kBlocks * 2) {}
 
 namespace {
 
 constexpr int RB = Reducer::kBlocks;
 constexpr int RT = Reducer::kThreads;
+
+inline int rgrid(int64_t n)
+{
+  const int64_t per = (int64_t)RT * 8;    // at least 8 entries per thread
+  int64_t g = (n + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > RB) g = RB;
+  return (int)g;
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -27,7 +34,7 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
-// Block-reduce NV values; the result is valid in thread 0.
+// Block-reduce NV values in a fixed order; the result is valid in thread 0.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV])
 {
@@ -50,65 +57,45 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
   }
 }
 
-// Publish this block's partials; returns true (in every thread) for the last
-// block, whose thread 0 then holds the grid totals in tot[].
+// partials laid out [c][block]
 template <int NV>
-__device__ bool grid_finish(double (&v)[NV], double *partials, unsigned *ticket, double (&tot)[NV])
+__device__ __forceinline__ void write_partials(double (&v)[NV], double *partials)
 {
-  __shared__ int s_last;
   block_sum<NV>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int c = 0; c < NV; ++c) partials[blockIdx.x * NV + c] = v[c];
-    const unsigned t =
-        __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (t == gridDim.x - 1);
+    for (int c = 0; c < NV; ++c) partials[c * gridDim.x + blockIdx.x] = v[c];
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double a[NV];
+}
+
+// sum of nb partials per component, valid in thread 0
+template <int NV>
+__device__ __forceinline__ void sum_partials(int nb, const double *partials, double (&tot)[NV])
+{
 #pragma unroll
   for (int c = 0; c < NV; ++c) {
-    a[c] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += RT)
-      a[c] += __hip_atomic_load(&partials[b * NV + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tot[c] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RT) tot[c] += partials[c * nb + b];
   }
-  block_sum<NV>(a);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int c = 0; c < NV; ++c) tot[c] = a[c];
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return true;
+  block_sum<NV>(tot);
 }
+
+// ------------------------------------------------------------------ stage-1 kernels
 
 template <typename T>
 __global__ void __launch_bounds__(RT)
 wdot_kernel(int64_t n, const T *__restrict__ w, const T *__restrict__ a, const T *__restrict__ b,
-            double *partials, unsigned *ticket, double *scal, int slot)
+            double *partials)
 {
   double v[1] = {0.0};
   for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
     v[0] += (double)w[i] * (double)a[i] * (double)b[i];
-  double tot[1];
-  if (grid_finish<1>(v, partials, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
-}
-
-__global__ void __launch_bounds__(RT)
-wnorm_kernel(int64_t n, const double *__restrict__ w, const double *__restrict__ a, double factor,
-             double *partials, unsigned *ticket, double *scal, int slot)
-{
-  double v[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
-    v[0] += w[i] * a[i] * a[i];
-  double tot[1];
-  if (grid_finish<1>(v, partials, ticket, tot) && threadIdx.x == 0) scal[slot] = sqrt(tot[0] * factor);
+  write_partials<1>(v, partials);
 }
 
 __global__ void __launch_bounds__(RT)
 plane_sum_kernel(int64_t E, int Np, int64_t estride, int64_t off, const double *__restrict__ x,
-                 double *partials, unsigned *ticket, double *scal, int slot)
+                 double *partials)
 {
   double v[1] = {0.0};
   const int64_t n = E * Np;
@@ -116,17 +103,144 @@ plane_sum_kernel(int64_t E, int Np, int64_t estride, int64_t off, const double *
     const int64_t e = i / Np;
     v[0] += x[e * estride + off + (i - e * Np)];
   }
-  double tot[1];
-  if (grid_finish<1>(v, partials, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
+  write_partials<1>(v, partials);
 }
+
+enum { PM_JACOBI = 0, PM_GIVEN = 1, PM_NONE = 2 };
+
+// Preconditioned dots (PCG.cpp:115-153): rdotz = <r,z>_w, zdotAp = <z,Ap>_w.
+template <int MODE, bool FLEX>
+__global__ void __launch_bounds__(RT)
+precon_dots_kernel(int64_t n, const double *__restrict__ w, const double *__restrict__ invDiag,
+                   const double *__restrict__ r, double *__restrict__ z,
+                   const double *__restrict__ Ap, const double *__restrict__ scal, double *partials)
+{
+  if (scal[S_ACTIVE] == 0.0) return;
+  const bool needZAp = FLEX && scal[S_ITER] >= 1.0;
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    const double ri = r[i];
+    double zi;
+    if (MODE == PM_JACOBI) { zi = invDiag[i] * ri; z[i] = zi; }
+    else if (MODE == PM_GIVEN) zi = z[i];
+    else zi = ri;
+    const double wi = w[i];
+    if (MODE != PM_NONE) v[0] += wi * ri * zi;
+    if (needZAp) v[1] += wi * zi * Ap[i];
+  }
+  write_partials<2>(v, partials);
+}
+
+__global__ void update_p_kernel(int64_t n, const double *__restrict__ z, double *__restrict__ p,
+                                const double *__restrict__ scal)
+{
+  if (scal[S_ACTIVE] == 0.0) return;
+  const double beta = scal[S_BETA];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+__global__ void __launch_bounds__(RT)
+pap_kernel(int64_t n, const double *__restrict__ w, const double *__restrict__ p,
+           const double *__restrict__ Ap, const double *__restrict__ scal, double *partials)
+{
+  if (scal[S_ACTIVE] == 0.0) return;
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
+    v[0] += w[i] * p[i] * Ap[i];
+  write_partials<1>(v, partials);
+}
+
+// ellipticBlockUpdatePCG (kernels/elliptic/ellipticBlockUpdatePCG.c:27-50) fused
+// with x += alpha p (PCG.cpp:67-74).
+__global__ void __launch_bounds__(RT)
+update_rx_kernel(int64_t n, const double *__restrict__ w, const double *__restrict__ Ap,
+                 const double *__restrict__ p, double *__restrict__ r, double *__restrict__ x,
+                 const double *__restrict__ scal, double *partials)
+{
+  if (scal[S_ACTIVE] == 0.0) return;
+  const double alpha = scal[S_ALPHA];
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    const double rn = r[i] - alpha * Ap[i];
+    v[0] += rn * rn * w[i];
+    r[i] = rn;
+    x[i] += alpha * p[i];
+  }
+  write_partials<1>(v, partials);
+}
+
+// ------------------------------------------------------------------ stage-2 kernels
+
+enum { FIN_STORE = 0, FIN_SQRT = 1, FIN_DOTS = 2, FIN_ALPHA = 3, FIN_RDOTR = 4 };
+
+template <int NV, int KIND, int MODE = 0, bool FLEX = false>
+__global__ void __launch_bounds__(RT)
+finish_kernel(int nb, const double *__restrict__ partials, double *scal, int slot, double factor)
+{
+  if (KIND >= FIN_DOTS && scal[S_ACTIVE] == 0.0) return;
+  double tot[NV];
+  sum_partials<NV>(nb, partials, tot);
+  if (threadIdx.x != 0) return;
+  if (KIND == FIN_STORE) {
+    scal[slot] = tot[0];
+  } else if (KIND == FIN_SQRT) {
+    scal[slot] = sqrt(tot[0] * factor);
+  } else if (KIND == FIN_DOTS) {
+    const int iter = (int)scal[S_ITER] + 1;
+    const double rdotz2 = scal[S_RDOTZ1];
+    // PCG.cpp:128-130: with no preconditioner rdotz1 is the running residual norm
+    const double rdotz1 = (MODE == PM_NONE) ? scal[S_RDOTR] : tot[0];
+    double beta = 0.0;
+    if (iter > 1) {
+      beta = rdotz1 / rdotz2;
+      if (FLEX) beta = -scal[S_ALPHA] * tot[1] / rdotz2;
+    }
+    scal[S_RDOTZ2] = rdotz2;
+    scal[S_RDOTZ1] = rdotz1;
+    scal[S_ZDOTAP] = NV > 1 ? tot[NV - 1] : 0.0;
+    scal[S_BETA] = beta;
+  } else if (KIND == FIN_ALPHA) {
+    scal[S_PAP] = tot[0];
+    scal[S_ALPHA] = scal[S_RDOTZ1] / (tot[0] + 1e-300);
+  } else {   // FIN_RDOTR
+    const double rdotr = sqrt(tot[0] * factor);
+    const double iter = scal[S_ITER] + 1.0;
+    scal[S_RDOTR] = rdotr;
+    scal[S_ITER] = iter;
+    scal[S_ACTIVE] = (rdotr > scal[S_TOL] && iter < scal[S_MAXIT]) ? 1.0 : 0.0;   // NaN stops too
+  }
+}
+
+__global__ void pcg_init_kernel(double *scal, double rdotr0, double tol, int maxit)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    scal[S_RDOTR] = rdotr0;
+    scal[S_RDOTZ1] = 0.0;
+    scal[S_ALPHA] = 0.0;
+    scal[S_TOL] = tol;
+    scal[S_ITER] = 0.0;
+    scal[S_MAXIT] = (double)maxit;
+    scal[S_ACTIVE] = 1.0;
+  }
+}
+
+// ------------------------------------------------------------------ element-wise kernels
 
 template <typename T> __global__ void fill_kernel(int64_t n, T v, T *x)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] = v;
 }
+// y = a x + b y; with b == 0 the old y is not read, so a freshly allocated
+// (uninitialised, possibly non-finite) y is simply overwritten.
 template <typename T> __global__ void axpby_kernel(int64_t n, T a, const T *__restrict__ x, T b, T *__restrict__ y)
 {
+  if (b == T(0)) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+      y[i] = a * x[i];
+    return;
+  }
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = a * x[i] + b * y[i];
 }
@@ -146,115 +260,43 @@ template <typename A, typename B> __global__ void convert_kernel(int64_t n, cons
     y[i] = (B)x[i];
 }
 
-enum { PM_JACOBI = 0, PM_GIVEN = 1, PM_NONE = 2 };
-
-// Preconditioned dots and beta (PCG.cpp:115-159).
-template <int MODE, bool FLEX>
-__global__ void __launch_bounds__(RT)
-precon_dots_kernel(int64_t n, const double *__restrict__ w, const double *__restrict__ invDiag,
-                   const double *__restrict__ r, double *__restrict__ z,
-                   const double *__restrict__ Ap, int iter, double *partials, unsigned *ticket,
-                   double *scal)
-{
-  double v[2] = {0.0, 0.0};
-  const bool needZAp = FLEX && iter > 1;
-  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
-    const double ri = r[i];
-    double zi;
-    if (MODE == PM_JACOBI) { zi = invDiag[i] * ri; z[i] = zi; }
-    else if (MODE == PM_GIVEN) zi = z[i];
-    else zi = ri;
-    const double wi = w[i];
-    if (MODE != PM_NONE) v[0] += wi * ri * zi;
-    if (needZAp) v[1] += wi * zi * Ap[i];
-  }
-  double tot[2];
-  if (grid_finish<2>(v, partials, ticket, tot) && threadIdx.x == 0) {
-    const double rdotz2 = scal[S_RDOTZ1];
-    // PCG.cpp:128-130: with no preconditioner rdotz1 is the running residual norm
-    const double rdotz1 = (MODE == PM_NONE) ? scal[S_RDOTR] : tot[0];
-    double beta = 0.0;
-    if (iter > 1) {
-      beta = rdotz1 / rdotz2;
-      if (FLEX) beta = -scal[S_ALPHA] * tot[1] / rdotz2;
-    }
-    scal[S_RDOTZ2] = rdotz2;
-    scal[S_RDOTZ1] = rdotz1;
-    scal[S_ZDOTAP] = tot[1];
-    scal[S_BETA] = beta;
-  }
-}
-
-__global__ void update_p_kernel(int64_t n, const double *__restrict__ z, double *__restrict__ p,
-                                const double *__restrict__ scal)
-{
-  const double beta = scal[S_BETA];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = z[i] + beta * p[i];
-}
-
-__global__ void __launch_bounds__(RT)
-alpha_kernel(int64_t n, const double *__restrict__ w, const double *__restrict__ p,
-             const double *__restrict__ Ap, double *partials, unsigned *ticket, double *scal)
-{
-  double v[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
-    v[0] += w[i] * p[i] * Ap[i];
-  double tot[1];
-  if (grid_finish<1>(v, partials, ticket, tot) && threadIdx.x == 0) {
-    scal[S_PAP] = tot[0];
-    scal[S_ALPHA] = scal[S_RDOTZ1] / (tot[0] + 1e-300);
-  }
-}
-
-// ellipticBlockUpdatePCG (kernels/elliptic/ellipticBlockUpdatePCG.c:27-50) fused
-// with x += alpha p (PCG.cpp:67-74).
-__global__ void __launch_bounds__(RT)
-update_rx_kernel(int64_t n, const double *__restrict__ w, const double *__restrict__ Ap,
-                 const double *__restrict__ p, double *__restrict__ r, double *__restrict__ x,
-                 double resNormFactor, double *partials, unsigned *ticket, double *scal)
-{
-  const double alpha = scal[S_ALPHA];
-  double v[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
-    const double rn = r[i] - alpha * Ap[i];
-    v[0] += rn * rn * w[i];
-    r[i] = rn;
-    x[i] += alpha * p[i];
-  }
-  double tot[1];
-  if (grid_finish<1>(v, partials, ticket, tot) && threadIdx.x == 0)
-    scal[S_RDOTR] = sqrt(tot[0] * resNormFactor);
-}
-
 constexpr int EB = 256;
 inline unsigned egrid(int64_t n) { return grid_for(n, EB, 8192); }
 
 }  // namespace
 
+// ------------------------------------------------------------------ host side
+
 void wdot(Reducer &R, int64_t n, const double *w, const double *a, const double *b, double *scal,
           int slot, hipStream_t s)
 {
-  wdot_kernel<double><<<RB, RT, 0, s>>>(n, w, a, b, R.partials.get(), R.ticket.get(), scal, slot);
+  const int g = rgrid(n);
+  wdot_kernel<double><<<g, RT, 0, s>>>(n, w, a, b, R.partials.get());
+  finish_kernel<1, FIN_STORE><<<1, RT, 0, s>>>(g, R.partials.get(), scal, slot, 1.0);
   NRS_LAUNCH_CHECK();
 }
 void wdot_f32(Reducer &R, int64_t n, const float *w, const float *a, const float *b, double *scal,
               int slot, hipStream_t s)
 {
-  wdot_kernel<float><<<RB, RT, 0, s>>>(n, w, a, b, R.partials.get(), R.ticket.get(), scal, slot);
+  const int g = rgrid(n);
+  wdot_kernel<float><<<g, RT, 0, s>>>(n, w, a, b, R.partials.get());
+  finish_kernel<1, FIN_STORE><<<1, RT, 0, s>>>(g, R.partials.get(), scal, slot, 1.0);
   NRS_LAUNCH_CHECK();
 }
 void wnorm(Reducer &R, int64_t n, const double *w, const double *a, double factor, double *scal,
            int slot, hipStream_t s)
 {
-  wnorm_kernel<<<RB, RT, 0, s>>>(n, w, a, factor, R.partials.get(), R.ticket.get(), scal, slot);
+  const int g = rgrid(n);
+  wdot_kernel<double><<<g, RT, 0, s>>>(n, w, a, a, R.partials.get());
+  finish_kernel<1, FIN_SQRT><<<1, RT, 0, s>>>(g, R.partials.get(), scal, slot, factor);
   NRS_LAUNCH_CHECK();
 }
-
 void plane_sum(Reducer &R, int64_t E, int Np, int64_t estride, int64_t off, const double *x,
                double *scal, int slot, hipStream_t s)
 {
-  plane_sum_kernel<<<RB, RT, 0, s>>>(E, Np, estride, off, x, R.partials.get(), R.ticket.get(), scal, slot);
+  const int g = rgrid(E * Np);
+  plane_sum_kernel<<<g, RT, 0, s>>>(E, Np, estride, off, x, R.partials.get());
+  finish_kernel<1, FIN_STORE><<<1, RT, 0, s>>>(g, R.partials.get(), scal, slot, 1.0);
   NRS_LAUNCH_CHECK();
 }
 
@@ -301,12 +343,23 @@ template void reciprocal<float>(int64_t, float *, hipStream_t);
 template void convert<double, float>(int64_t, const double *, float *, hipStream_t);
 template void convert<float, double>(int64_t, const float *, double *, hipStream_t);
 
+void pcg_init(double *scal, double rdotr0, double tol, int maxit, hipStream_t s)
+{
+  pcg_init_kernel<<<1, 64, 0, s>>>(scal, rdotr0, tol, maxit);
+  NRS_LAUNCH_CHECK();
+}
+
 void pcg_precon_dots(Reducer &R, int64_t n, const double *w, const double *invDiag, const double *r,
                      double *z, const double *Ap, bool jacobi, bool flexible, bool none_precon,
-                     int iter, double *scal, hipStream_t s)
+                     double *scal, hipStream_t s)
 {
-#define PD(MODE, FLEX) \
-  precon_dots_kernel<MODE, FLEX><<<RB, RT, 0, s>>>(n, w, invDiag, r, z, Ap, iter, R.partials.get(), R.ticket.get(), scal)
+  const int g = rgrid(n);
+  double *P = R.partials.get();
+#define PD(MODE, FLEX)                                                                            \
+  do {                                                                                            \
+    precon_dots_kernel<MODE, FLEX><<<g, RT, 0, s>>>(n, w, invDiag, r, z, Ap, scal, P);            \
+    finish_kernel<2, FIN_DOTS, MODE, FLEX><<<1, RT, 0, s>>>(g, P, scal, 0, 1.0);                  \
+  } while (0)
   const int mode = none_precon ? PM_NONE : (jacobi ? PM_JACOBI : PM_GIVEN);
   if (mode == PM_JACOBI) { if (flexible) PD(PM_JACOBI, true); else PD(PM_JACOBI, false); }
   else if (mode == PM_GIVEN) { if (flexible) PD(PM_GIVEN, true); else PD(PM_GIVEN, false); }
@@ -324,15 +377,18 @@ void pcg_update_p(int64_t n, const double *z, double *p, const double *scal, hip
 void pcg_alpha(Reducer &R, int64_t n, const double *w, const double *p, const double *Ap,
                double *scal, hipStream_t s)
 {
-  alpha_kernel<<<RB, RT, 0, s>>>(n, w, p, Ap, R.partials.get(), R.ticket.get(), scal);
+  const int g = rgrid(n);
+  pap_kernel<<<g, RT, 0, s>>>(n, w, p, Ap, scal, R.partials.get());
+  finish_kernel<1, FIN_ALPHA><<<1, RT, 0, s>>>(g, R.partials.get(), scal, 0, 1.0);
   NRS_LAUNCH_CHECK();
 }
 
 void pcg_update_rx(Reducer &R, int64_t n, const double *w, const double *Ap, const double *p,
                    double *r, double *x, double resNormFactor, double *scal, hipStream_t s)
 {
-  update_rx_kernel<<<RB, RT, 0, s>>>(n, w, Ap, p, r, x, resNormFactor, R.partials.get(),
-                                     R.ticket.get(), scal);
+  const int g = rgrid(n);
+  update_rx_kernel<<<g, RT, 0, s>>>(n, w, Ap, p, r, x, scal, R.partials.get());
+  finish_kernel<1, FIN_RDOTR><<<1, RT, 0, s>>>(g, R.partials.get(), scal, 0, resNormFactor);
   NRS_LAUNCH_CHECK();
 }
 

@@ -9,16 +9,20 @@ namespace nrs {
 // iteration needs no host round trip except the convergence check.
 enum Scalar : int {
   S_RDOTZ1 = 0, S_RDOTZ2, S_ZDOTAP, S_ALPHA, S_BETA, S_PAP, S_RDOTR, S_TOL, S_RESNORM,
-  S_ITER, S_TMP0, S_TMP1, S_TMP2, S_TMP3, S_COUNT = 16
+  S_ITER,     // completed iterations
+  S_ACTIVE,   // 1 while the iteration runs; kernels of a finished solve return at once
+  S_MAXIT, S_TMP0, S_TMP1, S_TMP2, S_TMP3, S_COUNT = 16
 };
 
-// Deterministic two-level reduction: a fixed grid writes one partial per
-// block, the last block to finish (atomic ticket) sums them in block order.
+// Deterministic two-launch reduction: a grid sized to the vector writes one
+// partial per block, then a one-block kernel adds the partials in a fixed tree
+// order and applies the scalar step (alpha, beta, norms).  No inter-block
+// atomics or device-scope fences are needed (a device-scope release per block
+// would write back the per-XCD L2 on every block).
 struct Reducer {
-  static constexpr int kBlocks = 1024;
+  static constexpr int kBlocks = 1024;   // upper bound on the partials grid
   static constexpr int kThreads = 256;
-  DevBuf<double> partials;     // [kBlocks * 4]
-  DevBuf<unsigned> ticket;     // [1], returns to 0 after every reduction
+  DevBuf<double> partials;               // [kBlocks * 2]
   Reducer();
 };
 
@@ -42,11 +46,16 @@ template <typename T> void reciprocal(int64_t n, T *y, hipStream_t s);
 template <typename A, typename B> void convert(int64_t n, const A *x, B *y, hipStream_t s);
 
 // ---- PCG pieces (src/solvers/elliptic/linearSolver/PCG.cpp:33-203) ----
+// The iteration state lives in device scalars: S_ITER counts completed
+// iterations, S_ACTIVE turns every PCG kernel into a no-op once the residual
+// reached S_TOL or S_ITER reached S_MAXIT, so any number of iterations can be
+// enqueued ahead of the host's convergence check without changing the result.
+void pcg_init(double *scal, double rdotr0, double tol, int maxit, hipStream_t s);
 // Jacobi preconditioner fused with the dots: z = invDiag .* r; rdotz = <r,z>_w,
 // zdotAp = <z,Ap>_w (flexible); then beta from rdotz1/rdotz2 (or -alpha zdotAp/rdotz2).
 void pcg_precon_dots(Reducer &R, int64_t n, const double *w, const double *invDiag,
                      const double *r, double *z, const double *Ap, bool jacobi, bool flexible,
-                     bool none_precon, int iter, double *scal, hipStream_t s);
+                     bool none_precon, double *scal, hipStream_t s);
 // p = z + beta p
 void pcg_update_p(int64_t n, const double *z, double *p, const double *scal, hipStream_t s);
 // alpha = rdotz1 / (<p,Ap>_w + 1e-300)

@@ -1,0 +1,246 @@
+"""Oracle for the p-multigrid preconditioned (flexible) PCG.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).  A numpy driver over the
+oracle's C kernels that restates the reference's multigrid:
+
+  V-cycle                   src/solvers/elliptic/MG/MGSolver.cpp:167-193
+  smooth / residual         MG/ellipticMultiGridLevel.cpp:37-41, 85-277
+  coarsen / prolongate      MG/ellipticMultiGridLevel.cpp:43-83
+  Chebyshev bounds          MG/ellipticMultiGridLevelSetup.cpp:109-180, 292-453
+  level meshes              src/mesh/meshSetup.cpp:293-376 (coordinates
+                            interpolated from the fine mesh, geometric factors
+                            recomputed, FP32 copies)
+  preconditioner casts      src/solvers/elliptic/ellipticPreconditioner.cpp:57-62
+  PCG / flexible PCG        src/solvers/elliptic/linearSolver/PCG.cpp:85-203
+
+Level arithmetic is FP32 (pfloat) or FP64; the coarsest level (N=1) is solved
+by Jacobi-PCG in FP64 to a relative tolerance, where the reference calls
+BoomerAMG.  lambda_max(S A) comes from a 10-step Arnoldi started from a
+splitmix64 vector (the reference seeds from std::random_device, which is not
+reproducible) and numpy's eigvals (LAPACK geev, as the reference's dgeev).
+"""
+import numpy as np
+
+from . import (BoxMesh, Elliptic, GWJ, Ogs, gll, interp, lib, mask_ids, masked_gids,
+               transfer_R, build_diag)
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix_uniform(seed, n):
+    """n draws of splitmix64 (state advanced by the golden-ratio increment)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def interp_coords(Nf, E, xf, Nc):
+    zf, _ = gll(Nf)
+    zc, _ = gll(Nc)
+    I = interp(zf, zc)
+    u = xf.reshape(E, Nf + 1, Nf + 1, Nf + 1)
+    return np.einsum("ck,bj,ai,ekji->ecba", I, I, I, u, optimize=True).ravel()
+
+
+class Level:
+    pass
+
+
+class MG:
+    """p-multigrid V-cycle on the levels `orders` of a box mesh."""
+
+    def __init__(self, mesh, orders, T=np.float32, degree=2, min_factor=0.1, max_factor=1.1,
+                 coarse_tol=1e-2, coarse_maxit=200, seed=0, smoother="cheby1"):
+        assert orders[0] == mesh.N
+        self.T = T
+        self.smoother = smoother
+        self.coarse_tol, self.coarse_maxit = coarse_tol, coarse_maxit
+        self.levels = []
+        x, y, z = mesh.x, mesh.y, mesh.z
+        Nprev = mesh.N
+        for li, N in enumerate(orders):
+            L = Level()
+            L.N, L.Np, L.E = N, (N + 1) ** 3, mesh.E
+            L.isCoarse = li == len(orders) - 1
+            if li > 0:
+                x = interp_coords(Nprev, mesh.E, x, N)
+                y = interp_coords(Nprev, mesh.E, y, N)
+                z = interp_coords(Nprev, mesh.E, z, N)
+            m = BoxMesh(mesh.nx, mesh.ny, mesh.nz, N)          # structured ids and BCs at order N
+            m.x, m.y, m.z = x, y, z
+            r, w = gll(N)
+            m.ggeo = np.zeros(m.E * 7 * m.Np)
+            lib().orc_geofacs(N, m.E, m.D.ravel().copy(), w, x, y, z, m.ggeo, None)
+            L.mesh = m
+            L.el = Elliptic(m)                                  # masked ogs, mask, invDegree
+            L.ggeoT = m.ggeo.astype(T)
+            L.DT = m.D.astype(T)
+            L.invDegreeT = L.el.invDegree.astype(T)
+            if li > 0:
+                L.R = transfer_R(Nprev, N).astype(T)
+            if not L.isCoarse:
+                d = build_diag(N, m.ggeo, m.D)
+                d = L.el.ogs.gs(d, "add")
+                L.invDiagT = (1.0 / d).astype(T)
+                L.degree = degree
+            self.levels.append(L)
+            Nprev = N
+        for L in self.levels:
+            if not L.isCoarse:
+                L.rho = self._max_eig(L, seed)
+                L.lmax, L.lmin = max_factor * L.rho, min_factor * L.rho
+        cl = self.levels[-1]
+        cl.invDiag64 = cl.el.inv_diag()
+
+    # ---- level operators
+    def op(self, L, q):
+        q = np.ascontiguousarray(q, dtype=self.T)
+        fn = lib().orc_ax_f64 if self.T == np.float64 else lib().orc_ax_f32
+        Aq = np.zeros_like(q)
+        one = np.ones(1, dtype=self.T)
+        zero = np.zeros(1, dtype=self.T)
+        fn(L.N, L.E, None, L.ggeoT, L.DT.ravel().copy(), one, zero, 1, 0, q, Aq)
+        Aq[L.el.maskIds] = 0
+        return L.el.ogs.gs(Aq, "add")
+
+    def _max_eig(self, L, seed):
+        k = 10
+        n = L.E * L.Np
+        vx = splitmix_uniform(seed + 0x5EED * (L.N + 1), n)
+        vx = L.el.ogs_full.gs(vx, "add")
+        vx[L.el.maskIds] = 0.0
+        w = L.el.invDegree
+        V = [vx / np.sqrt(np.sum(w * vx * vx))]
+        H = np.zeros((k, k))
+        for j in range(k):
+            av = self.op(L, V[j].astype(self.T))
+            v = (L.invDiagT * av).astype(np.float64)
+            for i in range(j + 1):
+                hij = np.sum(w * V[i] * v)
+                v = v - hij * V[i]
+                H[i, j] = hij
+            if j + 1 < k:
+                nj = np.sqrt(np.sum(w * v * v))
+                v = v / nj
+                H[j + 1, j] = nj
+            V.append(v)
+        return float(np.max(np.abs(np.linalg.eigvals(H))))
+
+    # ---- smoothing: smoothChebyshev (ellipticMultiGridLevel.cpp:144-210)
+    def smooth(self, L, rhs, x, x_is_zero):
+        T = self.T
+        deg = L.degree
+        theta = T(0.5) * (T(L.lmax) + T(L.lmin))
+        delta = T(0.5) * (T(L.lmax) - T(L.lmin))
+        invTheta = T(1) / theta
+        sigma = theta / delta
+        rho_n = T(1) / sigma
+        if x_is_zero:
+            x[:] = 0
+            res = rhs.copy()
+        else:
+            res = (rhs - self.op(L, x)).astype(T)
+        res = (L.invDiagT * res).astype(T)
+        d = (invTheta * res).astype(T)
+        for _ in range(1, deg):
+            Ad = (L.invDiagT * self.op(L, d)).astype(T)
+            rhoSave = rho_n
+            rho_n = T(1) / (T(2) * sigma - rho_n)
+            rC = T(2) * rho_n / delta
+            dC = rho_n * rhoSave
+            x += d
+            res = (res - Ad).astype(T)
+            d = (dC * d + rC * res).astype(T)
+        x += d
+        x[L.el.maskIds] = 0
+        return x
+
+    def coarse_solve(self, L, rhs):
+        el = L.el
+        r = rhs.astype(np.float64).copy()
+        x = np.zeros_like(r)
+        w = el.invDegree
+        r0 = np.sqrt(np.sum(w * r * r) / el.volume)
+        if not r0 > 0:
+            return x.astype(self.T), 0
+        import ctypes as C
+        rd = C.c_double(r0)
+        it = lib().orc_pcg(C.byref(el._c), 1, L.invDiag64.ctypes.data, 0, r, x,
+                           self.coarse_tol * r0, self.coarse_maxit, C.byref(rd), None)
+        return x.astype(self.T), it
+
+    def vcycle(self, k, rhs):
+        L = self.levels[k]
+        if L.isCoarse:
+            x, self.last_coarse_it = self.coarse_solve(L, rhs)
+            return x
+        C_ = self.levels[k + 1]
+        x = np.zeros(L.E * L.Np, dtype=self.T)
+        self.smooth(L, rhs, x, True)
+        res = (rhs - self.op(L, x)).astype(self.T)
+        res = (res * L.invDegreeT).astype(self.T)
+        rc = self._coarsen(L.N, C_.N, C_.R, res, L.E)
+        rc = C_.el.ogs.gs(rc, "add")
+        rc[C_.el.maskIds] = 0
+        xc = self.vcycle(k + 1, rc)
+        x = (x + self._prolong(L.N, C_.N, C_.R, xc, L.E)).astype(self.T)
+        self.smooth(L, rhs, x, False)
+        return x
+
+    def _coarsen(self, Nf, Nc, R, qf, E):
+        u = qf.reshape(E, Nf + 1, Nf + 1, Nf + 1)
+        return np.einsum("ck,bj,ai,ekji->ecba", R, R, R, u, optimize=True).astype(self.T).ravel()
+
+    def _prolong(self, Nf, Nc, R, qc, E):
+        u = qc.reshape(E, Nc + 1, Nc + 1, Nc + 1)
+        return np.einsum("ck,bj,ai,ekji->ecba", R.T, R.T, R.T, u, optimize=True).astype(self.T).ravel()
+
+    def apply(self, r):
+        """z = V-cycle(r) with FP64 in/out (ellipticPreconditioner.cpp:57-62)."""
+        return self.vcycle(0, r.astype(self.T)).astype(np.float64)
+
+
+def fcg_solve(el, rhs, precon, flexible=True, tol=1e-8, maxit=500, x0=None):
+    """ellipticSolve + pcg with a preconditioner callback (FP64 numpy)."""
+    m = el.mesh
+    x = np.zeros(m.Nlocal) if x0 is None else x0.copy()
+    from . import ax
+    Ax0 = ax(m.N, m.ggeo, m.D, x)
+    r = rhs - Ax0
+    r[el.maskIds] = 0
+    r = el.ogs.gs(r, "add")
+    w = el.invDegree
+    vol = el.volume
+    res0 = np.sqrt(np.sum(w * r * r) / vol)
+    tol_abs = tol * res0
+    p = np.zeros_like(r)
+    Ap = np.zeros_like(r)
+    rdotz1 = 0.0
+    alpha = 0.0
+    it = 0
+    hist = []
+    xs = np.zeros_like(r)
+    while True:
+        it += 1
+        z = precon(r)
+        rdotz2 = rdotz1
+        rdotz1 = np.sum(w * r * z)
+        beta = 0.0
+        if it > 1:
+            beta = rdotz1 / rdotz2
+            if flexible:
+                beta = -alpha * np.sum(w * z * Ap) / rdotz2
+        p = z + beta * p
+        Ap = el.operator(p)
+        alpha = rdotz1 / (np.sum(w * p * Ap) + 1e-300)
+        r = r - alpha * Ap
+        xs = xs + alpha * p
+        rdotr = np.sqrt(np.sum(w * r * r) / vol)
+        hist.append(rdotr)
+        if not (rdotr > tol_abs and it < maxit):
+            break
+    return xs + x, {"iterations": it, "res0": res0, "res": rdotr, "history": np.array(hist)}

@@ -89,3 +89,64 @@ def test_default_opts():
     o = nekrs_amd.default_opts()
     assert o.solver == 0 and o.preconditioner == 1 and o.tol == 1e-8 and o.relative == 1
     assert o.chebyMinFactor == 0.1 and o.chebyMaxFactor == 1.1
+
+
+def _partition(ids):
+    """Canonical form of the node partition induced by an id list."""
+    _, inv = np.unique(ids, return_inverse=True)
+    first = {}
+    return np.array([first.setdefault(g, i) for i, g in enumerate(inv)])
+
+
+@pytest.mark.parametrize("Nf,Nc", [(7, 3), (7, 1), (3, 1), (7, 5), (9, 5), (4, 2)])
+def test_coarse_global_ids_partition(orc, Nf, Nc):
+    """Derived order-Nc ids induce the same node partition as the structured
+    box numbering at order Nc (and vertices keep their fine ids)."""
+    import nekrs_amd
+    nx, ny, nz = 3, 2, 4
+    fine = orc.BoxMesh(nx, ny, nz, Nf, "kershaw", 0.3)
+    coarse = orc.BoxMesh(nx, ny, nz, Nc, "kershaw", 0.3)
+    ids = nekrs_amd.coarse_global_ids(Nf, fine.E, fine.gid, Nc)
+    assert np.array_equal(_partition(ids), _partition(coarse.gid))
+
+
+def test_coarse_global_ids_rotated_elements():
+    """Element-local frames that are rotated/reflected against their neighbours
+    (generic conforming meshes) still produce matching coarse ids."""
+    import nekrs_amd
+    rng = np.random.default_rng(4)
+    Nf, Nc = 4, 2
+    Nq = Nf + 1
+    # two elements sharing the face x=1 of element 0; element 1 is stored with a
+    # random permutation/reflection of its local axes
+    def gid_of(gx, gy, gz):
+        return 1 + gx + (2 * Nf + 1) * (gy + (Nf + 1) * gz)
+    ids = np.zeros(2 * Nq ** 3, dtype=np.int64)
+    perm = rng.permutation(3)
+    flip = rng.integers(0, 2, 3)
+    for e in range(2):
+        for k in range(Nq):
+            for j in range(Nq):
+                for i in range(Nq):
+                    loc = [i, j, k]
+                    if e == 1:
+                        loc = [loc[perm[a]] for a in range(3)]
+                        loc = [Nf - c if flip[a] else c for a, c in enumerate(loc)]
+                    gx, gy, gz = loc[0] + e * Nf, loc[1], loc[2]
+                    ids[e * Nq ** 3 + i + Nq * (j + Nq * k)] = gid_of(gx, gy, gz)
+    coarse = nekrs_amd.coarse_global_ids(Nf, 2, ids, Nc)
+    # shared coarse nodes = coarse nodes at the same physical position
+    Nqc = Nc + 1
+    pos = {}
+    for e in range(2):
+        for k in range(Nqc):
+            for j in range(Nqc):
+                for i in range(Nqc):
+                    loc = [i, j, k]
+                    if e == 1:
+                        loc = [loc[perm[a]] for a in range(3)]
+                        loc = [Nc - c if flip[a] else c for a, c in enumerate(loc)]
+                    key = (loc[0] + e * Nc, loc[1], loc[2])
+                    pos.setdefault(key, set()).add(coarse[e * Nqc ** 3 + i + Nqc * (j + Nqc * k)])
+    assert all(len(v) == 1 for v in pos.values())
+    assert len(set(coarse)) == len(pos)

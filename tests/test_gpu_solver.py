@@ -162,11 +162,11 @@ def test_c1_shape_properties(gpu, torch_mod):
     p = torch.rand(L, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
     q = torch.rand(L, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
     ogs.gather_scatter(p, "max"); ogs.gather_scatter(q, "max")     # continuous fields
-    Ap, Aq = torch.empty_like(p), torch.empty_like(q)
-    el.operator(p, Ap); el.operator(q, Aq)
     w = el.inv_degree
     mask = el.mask_ids.long()
-    p[mask] = 0; q[mask] = 0
+    p[mask] = 0; q[mask] = 0                                       # in the masked space
+    Ap, Aq = torch.empty_like(p), torch.empty_like(q)
+    el.operator(p, Ap); el.operator(q, Aq)
     a, b = torch.sum(w * q * Ap).item(), torch.sum(w * p * Aq).item()
     assert abs(a - b) <= 1e-11 * abs(a)
     Apq = torch.empty_like(p)

@@ -1,0 +1,115 @@
+"""Pinning of the multigrid oracle (oracle/mg.py) by what the mathematics fixes:
+the Chebyshev smoother's error propagator equals the scaled Chebyshev
+polynomial of S A, the Arnoldi bound brackets the dense spectrum, the V-cycle
+with an exact coarse solve is a symmetric operator in the assembled inner
+product, and MG-preconditioned flexible CG reaches the same solution as the
+independently pinned Jacobi PCG."""
+import numpy as np
+import pytest
+
+
+def _dense(op, gid, keep):
+    """Matrix of op on unique unmasked dofs (inputs continuous: 1 at every copy)."""
+    cols = []
+    for c in np.nonzero(keep)[0]:
+        e = (gid == gid[c]).astype(float)
+        cols.append(op(e)[keep])
+    return np.array(cols).T
+
+
+@pytest.fixture(scope="module")
+def small(orc):
+    import oracle.mg as omg
+    mo = orc.BoxMesh(2, 2, 2, 3, "kershaw", 0.3)
+    om = omg.MG(mo, [3, 1], T=np.float64, coarse_tol=1e-14, coarse_maxit=1000)
+    return mo, om
+
+
+def test_arnoldi_bound_brackets_spectrum(small):
+    mo, om = small
+    L = om.levels[0]
+    # unique unmasked nodes: one representative per gather row
+    keep = np.zeros(mo.Nlocal, bool)
+    seen = set()
+    masked = set(L.el.maskIds.tolist())
+    for i, g in enumerate(mo.gid):
+        if i not in masked and g not in seen:
+            seen.add(g)
+            keep[i] = True
+    SA = _dense(lambda v: L.invDiagT * om.op(L, v), mo.gid, keep)
+    ev = np.linalg.eigvals(SA).real
+    # Ritz values of the Arnoldi process (standard inner product on unique
+    # dofs, where S A is not normal) lie in the field of values of S A
+    fov_max = np.linalg.eigvalsh(0.5 * (SA + SA.T)).max()
+    assert L.rho <= fov_max * (1 + 1e-10)
+    assert L.rho >= 0.95 * ev.max()
+
+
+def test_chebyshev_error_propagator(small):
+    mo, om = small
+    L = om.levels[0]
+    keep = np.zeros(mo.Nlocal, bool)
+    seen = set()
+    masked = set(L.el.maskIds.tolist())
+    for i, g in enumerate(mo.gid):
+        if i not in masked and g not in seen:
+            seen.add(g)
+            keep[i] = True
+    A = _dense(lambda v: om.op(L, v), mo.gid, keep)
+    S = np.diag(L.invDiagT[keep])
+    # smoother with x = 0: x = q(SA) S b; error propagator E = I - q(SA) S A
+    col = {g: c for c, g in enumerate(mo.gid[keep])}
+    idx = np.array([col.get(g, -1) for g in mo.gid])
+    def smooth_unique(b_u):
+        b = np.where(idx >= 0, b_u[np.maximum(idx, 0)], 0.0)   # continuous representation
+        b[L.el.maskIds] = 0
+        x = np.zeros(mo.Nlocal)
+        om.smooth(L, b, x, True)
+        return x[keep]
+    n = keep.sum()
+    E = np.eye(n) - np.array([smooth_unique(c) for c in A.T]).T
+    theta, delta = 0.5 * (L.lmax + L.lmin), 0.5 * (L.lmax - L.lmin)
+    lam, V = np.linalg.eig(S @ A)
+    k = L.degree
+    T = np.polynomial.chebyshev.Chebyshev.basis(k)
+    expect = T((theta - lam.real) / delta) / T(theta / delta)
+    got = np.diag(np.linalg.solve(V, E @ V)).real
+    assert np.allclose(got, expect, atol=1e-8)
+
+
+def test_vcycle_symmetric_with_exact_coarse(small, orc):
+    mo, om = small
+    el = om.levels[0].el
+    rng = np.random.default_rng(0)
+    def assembled():
+        r = rng.uniform(-1, 1, mo.Nlocal)
+        r[el.maskIds] = 0
+        return el.ogs.gs(r, "add")
+    r1, r2 = assembled(), assembled()
+    z1, z2 = om.apply(r1), om.apply(r2)
+    w = el.invDegree
+    a, b = np.sum(w * z1 * r2), np.sum(w * r1 * z2)
+    assert abs(a - b) < 1e-10 * abs(a)
+    # and positive definite on these samples
+    assert np.sum(w * z1 * r1) > 0 and np.sum(w * z2 * r2) > 0
+
+
+def test_mg_fcg_matches_jacobi_solution(orc):
+    import oracle.mg as omg
+    mo = orc.BoxMesh(3, 3, 3, 7, "kershaw", 0.3)
+    eo = orc.Elliptic(mo)
+    rhs = np.random.default_rng(7).uniform(-1, 1, mo.Nlocal)
+    xj, ij = eo.solve(rhs, tol=1e-12, maxit=5000)
+    for levels in ([7, 3, 1], [7, 5, 3, 1]):
+        om = omg.MG(mo, levels)
+        xm, im = omg.fcg_solve(eo, rhs, om.apply, tol=1e-12, maxit=400)
+        assert im["iterations"] < ij["iterations"] / 4
+        assert np.max(np.abs(xj - xm)) <= 1e-9 * np.max(np.abs(xj))
+
+
+def test_splitmix_reference_values():
+    import oracle.mg as omg
+    # splitmix64 with state 0: first outputs 0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4
+    v = omg.splitmix_uniform(0, 2)
+    assert v[0] == (0xE220A8397B1DCDAF >> 11) / 2.0 ** 53
+    assert v[1] == (0x6E789E6AA1B965F4 >> 11) / 2.0 ** 53
