@@ -171,8 +171,10 @@ enum nrs_smoother {
   NRS_SMOOTHER_OPT4_CHEBYSHEV_JACOBI = 1,
   NRS_SMOOTHER_CHEBYSHEV_ASM = 2,     /* CHEBYSHEV+ASM (FDM/Schwarz)             */
   NRS_SMOOTHER_OPT4_CHEBYSHEV_ASM = 3,
-  NRS_SMOOTHER_ASM = 4,
-  NRS_SMOOTHER_RAS = 5
+  NRS_SMOOTHER_ASM = 4,               /* ASM (no post-smoothing)                 */
+  NRS_SMOOTHER_RAS = 5,               /* RAS (no post-smoothing)                 */
+  NRS_SMOOTHER_CHEBYSHEV_RAS = 6,     /* CHEBYSHEV+RAS                           */
+  NRS_SMOOTHER_OPT4_CHEBYSHEV_RAS = 7 /* FOURTHOPTCHEBYSHEV+RAS                  */
 };
 enum nrs_coarse { NRS_COARSE_PCG_F64 = 0, NRS_COARSE_SMOOTH = 1 };
 enum nrs_mg_precision { NRS_MG_F32 = 0, NRS_MG_F64 = 1 };
@@ -254,6 +256,29 @@ int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats
  * coarseIterations = PCG iterations of the last coarse solve. */
 int nrs_elliptic_mg_info(nrs_elliptic *el, int max, int *nLevels, int *orders, int64_t *Nlocal,
                          double *rho, int *degree, int *coarseIterations);
+
+/* out = S in on multigrid level `level` (0 = finest, not the coarsest): the
+ * level's smoother operator, 1/diag(A) (Jacobi) or the overlapping Schwarz
+ * solve with fast diagonalisation (ASM / RAS), including its gather-scatter
+ * and mask.  in/out are device E-vectors of that level in the multigrid
+ * precision (float for NRS_MG_F32, double for NRS_MG_F64); out != in.
+ * Replaces pMGLevel::smoother (MG/ellipticMultiGridLevel.cpp:104-111) and
+ * pMGLevel::smoothSchwarz (MG/ellipticMultiGridSchwarz.cpp:1056-1156). */
+int nrs_elliptic_mg_smoother(nrs_elliptic *el, int level, const void *in, void *out, void *stream);
+
+/* Host-only FDM setup pieces of the Schwarz smoother (no device work):
+ * the 1D operator pair of one element direction on N+3 points, with
+ * boundary flags lbc/rbc (0 interior, 1 Dirichlet, 2 Neumann) and the
+ * left/middle/right element lengths: S[nl*nl] (row = point, column =
+ * eigenvector, S^T B S = I, ghost/Dirichlet rows zeroed) and ascending
+ * lam[nl].  Replaces compute_1d_matrices (MG/ellipticMultiGridSchwarz.cpp:505-556,
+ * LAPACK dsygv). */
+int nrs_fdm_1d(int N, int lbc, int rbc, double ll, double lm, double lr, double *S, double *lam);
+/* Element lengths [9][E] (left x,y,z, middle x,y,z, right x,y,z) from the
+ * order-N node coordinates and ids.  Replaces compute_element_lengths
+ * (MG/ellipticMultiGridSchwarz.cpp:58-276). */
+int nrs_fdm_element_lengths(int N, int Nelements, const double *x, const double *y, const double *z,
+                            const long long *globalIds, double *lengths);
 
 /* Global ids of the order-Nc nodes of a conforming hexahedral mesh derived from
  * its order-Nf ids (host arrays [E*(Nf+1)^3] -> [E*(Nc+1)^3]); replaces Nek5000's

@@ -20,14 +20,17 @@ from . import _lib
 from ._lib import NrsError
 
 __all__ = ["NrsError", "init", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
-           "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh"]
+           "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh", "fdm_1d",
+           "fdm_element_lengths"]
 
 OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
 MAPS = {"none": 0, "sine": 1, "kershaw": 2}
 SOLVERS = {"pcg": 0, "fcg": 1, "pcg+flexible": 1}
 PRECONS = {"none": 0, "jacobi": 1, "multigrid": 2}
 SMOOTHERS = {"chebyshev+jacobi": 0, "opt4chebyshev+jacobi": 1, "chebyshev+asm": 2,
-             "opt4chebyshev+asm": 3, "asm": 4, "ras": 5}
+             "opt4chebyshev+asm": 3, "asm": 4, "ras": 5, "chebyshev+ras": 6,
+             "opt4chebyshev+ras": 7}
+FDM_PRECISIONS = {"f32": 0, "bf16": 1, "fp16": 2}
 
 
 def _torch():
@@ -188,6 +191,27 @@ class Ogs:
         return {"rows": a.value, "members": b.value, "singletons": c.value}
 
 
+def fdm_1d(N, lbc, rbc, ll, lm, lr):
+    """Host-only 1D FDM pair: (S[nl][nl] (point, eigenvector), lam[nl]), nl = N + 3."""
+    import numpy as np
+    nl = N + 3
+    S = np.zeros(nl * nl)
+    lam = np.zeros(nl)
+    _lib.call("nrs_fdm_1d", N, lbc, rbc, float(ll), float(lm), float(lr), S.ctypes.data,
+              lam.ctypes.data)
+    return S.reshape(nl, nl), lam
+
+
+def fdm_element_lengths(mesh):
+    """Host-only element lengths of a mesh: (left, middle, right), each [3][E]."""
+    import numpy as np
+    out = np.zeros(9 * mesh.E)
+    _lib.call("nrs_fdm_element_lengths", mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
+              mesh.z.ctypes.data, mesh.gid.ctypes.data, out.ctypes.data)
+    out = out.reshape(3, 3, mesh.E)
+    return out[0], out[1], out[2]
+
+
 def default_opts():
     o = _lib.EllipticOpts()
     _lib.load().nrs_elliptic_default_opts(C.byref(o))
@@ -282,6 +306,11 @@ class Elliptic:
         lv = [{"order": orders[i], "Nlocal": nl[i], "rho": rho[i], "degree": deg[i]}
               for i in range(min(n.value, 8))]
         return lv, cit.value
+
+    def mg_smoother(self, level, x, out, stream=None):
+        """out = S x on a non-coarse multigrid level (tensors of the MG precision)."""
+        _lib.call("nrs_elliptic_mg_smoother", self._h, int(level), _ptr(x), _ptr(out),
+                  _stream(stream))
 
     def profile(self, on=True):
         """Enable/disable per-launch device timing of the operator's kernels."""

@@ -74,6 +74,9 @@ SIGNATURES = {
     "nrs_elliptic_preconditioner": ([_vp, _vp, _vp, _vp], _i),
     "nrs_elliptic_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats), _vp], _i),
     "nrs_elliptic_mg_info": ([_vp, _i, C.POINTER(_i), _vp, _vp, _vp, _vp, C.POINTER(_i)], _i),
+    "nrs_elliptic_mg_smoother": ([_vp, _i, _vp, _vp, _vp], _i),
+    "nrs_fdm_1d": ([_i, _i, _i, _d, _d, _d, _vp, _vp], _i),
+    "nrs_fdm_element_lengths": ([_i, _i, _vp, _vp, _vp, _vp, _vp], _i),
     "nrs_coarse_global_ids": ([_i, _i, _vp, _i, _vp], _i),
     "nrs_elliptic_profile_enable": ([_vp, _i], _i),
     "nrs_elliptic_profile_read": ([_vp, C.POINTER(_d), C.POINTER(_i64)], _i),

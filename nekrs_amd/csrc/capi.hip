@@ -5,6 +5,7 @@
 #include <string>
 
 #include "elliptic.hpp"
+#include "fdm.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "mg.hpp"
@@ -357,6 +358,42 @@ int nrs_elliptic_mg_info(nrs_elliptic *el, int max, int *nLevels, int *orders, i
     if (degree) degree[i] = info[i].degree;
   }
   if (coarseIterations) *coarseIterations = el->mg->coarse_iterations();
+  NRS_API_END
+}
+
+int nrs_elliptic_mg_smoother(nrs_elliptic *el, int level, const void *in, void *out, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && in && out, "mg smoother: null argument");
+  NRS_CHECK(el->mg != nullptr, "mg smoother: the preconditioner is not multigrid");
+  el->mg->level_smoother(level, in, out, nrs::as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_fdm_1d(int N, int lbc, int rbc, double ll, double lm, double lr, double *S, double *lam)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(N >= 1 && N <= 15, "fdm 1d: order out of range");
+  NRS_CHECK(S && lam, "fdm 1d: null argument");
+  NRS_CHECK(lbc >= 0 && lbc <= 2 && rbc >= 0 && rbc <= 2, "fdm 1d: boundary flags are 0, 1 or 2");
+  NRS_CHECK(ll > 0 && lm > 0 && lr > 0, "fdm 1d: lengths must be positive");
+  std::vector<double> z, w;
+  nrs::gll_nodes(N, z, w);
+  const auto D = nrs::dmatrix(z);
+  NRS_CHECK(nrs::fdm_1d(N, D.data(), w.data(), lbc, rbc, ll, lm, lr, S, lam),
+            "fdm 1d: mass matrix not positive definite");
+  NRS_API_END
+}
+
+int nrs_fdm_element_lengths(int N, int Nelements, const double *x, const double *y, const double *z,
+                            const long long *globalIds, double *lengths)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(N >= 1 && Nelements >= 0, "element lengths: bad sizes");
+  NRS_CHECK(Nelements == 0 || (x && y && z && globalIds && lengths), "element lengths: null array");
+  const auto links = nrs::face_links(N, Nelements, globalIds);
+  const auto len = nrs::element_lengths(N, Nelements, x, y, z, links);
+  std::memcpy(lengths, len.data(), len.size() * sizeof(double));
   NRS_API_END
 }
 

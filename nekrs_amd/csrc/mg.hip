@@ -1,5 +1,6 @@
-// mg.hip -- p-multigrid preconditioner: level construction, Chebyshev-Jacobi
-// smoothing, intergrid transfers, FP64 coarse-grid PCG and the V-cycle.
+// mg.hip -- p-multigrid preconditioner: level construction, Chebyshev smoothing
+// around Jacobi or overlapping Schwarz (fdm.hip), intergrid transfers, FP64
+// coarse-grid PCG and the V-cycle.
 //
 // Reference behaviour followed (src/solvers/elliptic/MG/):
 //   V-cycle                  MGSolver.cpp:167-193
@@ -7,6 +8,8 @@
 //   coarsen / prolongate     ellipticMultiGridLevel.cpp:43-83
 //   Chebyshev bounds         ellipticMultiGridLevelSetup.cpp:109-180 (lambda_max by a
 //                            10-step Arnoldi on S A, :292-453)
+//   smoother choice          ellipticMultiGridLevel.cpp:85-111,
+//                            ellipticMultiGridLevelSetup.cpp:109-180
 //   level schedule           determineMGLevels.cpp:58-95
 // The coarse level is solved by Jacobi-PCG in FP64 on the GPU (the reference uses
 // BoomerAMG on an assembled FEM matrix on the host).
@@ -17,6 +20,7 @@
 #include <map>
 #include <type_traits>
 
+#include "fdm.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "mg.hpp"
@@ -264,6 +268,7 @@ struct Level {
   T *rhs = nullptr, *x = nullptr;
   double rho = 0, lmin = 0, lmax = 0;
   int degDown = 0, degUp = 0;
+  std::unique_ptr<Schwarz<T>> schwarz;   // ASM / RAS smoother (else Jacobi)
   ~Level()
   {
     if (ownOgs) { delete ogs; delete ogsFull; }
@@ -364,10 +369,12 @@ template <typename T>
 struct MGImpl final : MGSolver {
   nrs_elliptic *el = nullptr;
   std::vector<std::unique_ptr<Level<T>>> levels;
-  DevBuf<T> sRes, sAd, sD;        // shared smoother work vectors (finest size)
+  DevBuf<T> sRes, sAd, sD, sTmp;  // shared smoother work vectors (finest size)
   DevBuf<T> rP, zP;               // level-0 FP32 copies of r and z
   std::unique_ptr<CoarsePCG> coarse;
-  int smoother = NRS_SMOOTHER_CHEBYSHEV_JACOBI;
+  enum Cheby { NO_CHEBY, CHEBY1, CHEBY_OPT4 };
+  int cheby = CHEBY1;
+  int schwarzKind = 0;            // 0 Jacobi, 1 ASM, 2 RAS
   std::vector<T> betasDown, betasUp;
 
   // ---- level operators (pMGLevel::Ax / residual / smoother)
@@ -378,22 +385,26 @@ struct MGImpl final : MGSolver {
     ogs_gather_scatter<T>(l.ogs, Aq, 1, l.L, NRS_ADD, s);
   }
 
+  // pMGLevel::smoother (ellipticMultiGridLevel.cpp:104-111); out != in
   void smootherS(Level<T> &l, const T *in, T *out, hipStream_t s)
   {
-    k_wmul<T>(l.L, l.invDiagA.get(), in, out, s);
+    if (l.schwarz) l.schwarz->apply(in, out, s);
+    else k_wmul<T>(l.L, l.invDiagA.get(), in, out, s);
   }
 
-  // smoothChebyshev (ellipticMultiGridLevel.cpp:144-210)
+  // smoothChebyshev (ellipticMultiGridLevel.cpp:144-210).  The reference
+  // smooths res and Ad in place; here S writes to a spare vector whose role is
+  // swapped with the input's.  Scalars are formed in FP64 and rounded to T.
   void smooth_cheby1(Level<T> &l, const T *rhs, T *x, bool xIsZero, hipStream_t s)
   {
     const int deg = xIsZero ? l.degDown : l.degUp;
     if (deg == 0) return;
-    const T theta = T(0.5) * (T(l.lmax) + T(l.lmin));
-    const T delta = T(0.5) * (T(l.lmax) - T(l.lmin));
-    const T invTheta = T(1) / theta;
+    const T theta = T(0.5 * (l.lmax + l.lmin));
+    const T delta = T(0.5 * (l.lmax - l.lmin));
+    const T invTheta = T(1.0 / (double)theta);
     const T sigma = theta / delta;
-    T rho_n = T(1) / sigma;
-    T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get();
+    T rho_n = T(1.0 / (double)sigma);
+    T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get(), *spare = sTmp.get();
     if (xIsZero) {
       fill<T>(l.L, T(0), x, s);
       NRS_HIP(hipMemcpyAsync(res, rhs, l.L * sizeof(T), hipMemcpyDeviceToDevice, s));
@@ -401,14 +412,16 @@ struct MGImpl final : MGSolver {
       op(l, x, res, s);
       k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
     }
-    smootherS(l, res, res, s);
+    smootherS(l, res, spare, s);
+    std::swap(res, spare);
     k_axpby<T>(l.L, invTheta, res, T(0), d, s);
     for (int k = 1; k < deg; ++k) {
       op(l, d, Ad, s);
-      smootherS(l, Ad, Ad, s);
+      smootherS(l, Ad, spare, s);
+      std::swap(Ad, spare);
       const T rhoSave = rho_n;
-      rho_n = T(1) / (T(2) * sigma - rho_n);
-      const T rC = T(2) * rho_n / delta;
+      rho_n = T(1.0 / (2.0 * (double)sigma - (double)rho_n));
+      const T rC = T(2.0 * (double)rho_n / (double)delta);
       const T dC = rho_n * rhoSave;
       cheby1_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, dC, rC, Ad, d, res, x);
       NRS_LAUNCH_CHECK();
@@ -423,7 +436,7 @@ struct MGImpl final : MGSolver {
     const int deg = xIsZero ? l.degDown : l.degUp;
     if (deg == 0) return;
     const auto &betas = xIsZero ? betasDown : betasUp;
-    const T rho = T(l.lmax);
+    const double rho = l.lmax;
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get();
     if (xIsZero) {
       fill<T>(l.L, T(0), x, s);
@@ -433,24 +446,30 @@ struct MGImpl final : MGSolver {
       k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
     }
     smootherS(l, res, Ad, s);
-    const T coeff = T(4.0) / (T(3.0) * rho);
+    const T coeff = T(4.0 / (3.0 * rho));
     k_axpby<T>(l.L, coeff, Ad, T(0), d, s);
     for (int k = 1; k < deg; ++k) {
       op(l, d, Ad, s);
       cheby4_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, betas[k - 1], Ad, d, res, x);
       NRS_LAUNCH_CHECK();
       smootherS(l, res, Ad, s);
-      const T dC = (T(2) * k - T(1)) / (T(2) * k + T(3));
-      const T rC = (T(8) * k + T(4)) / ((T(2) * k + T(3)) * rho);
+      const T dC = T((2.0 * k - 1.0) / (2.0 * k + 3.0));
+      const T rC = T((8.0 * k + 4.0) / ((2.0 * k + 3.0) * rho));
       k_axpby<T>(l.L, rC, Ad, dC, d, s);
     }
     k_axpby<T>(l.L, betas.back(), d, T(1), x, s);
     launch_mask<T>(l.Nmasked, l.maskIds, x, s);
   }
 
+  // pMGLevel::smooth (ellipticMultiGridLevel.cpp:85-103): pure ASM / RAS
+  // smooths only on the way down (x = 0)
   void smooth(Level<T> &l, const T *rhs, T *x, bool xIsZero, hipStream_t s)
   {
-    if (smoother == NRS_SMOOTHER_OPT4_CHEBYSHEV_JACOBI) smooth_cheby4(l, rhs, x, xIsZero, s);
+    if (cheby == NO_CHEBY) {
+      if (xIsZero) smootherS(l, rhs, x, s);
+      return;
+    }
+    if (cheby == CHEBY_OPT4) smooth_cheby4(l, rhs, x, xIsZero, s);
     else smooth_cheby1(l, rhs, x, xIsZero, s);
   }
 
@@ -499,6 +518,12 @@ struct MGImpl final : MGSolver {
     return out;
   }
   int coarse_iterations() const override { return coarse ? coarse->lastIters : 0; }
+  void level_smoother(int level, const void *in, void *out, hipStream_t s) override
+  {
+    NRS_CHECK(level >= 0 && level + 1 < (int)levels.size(), "mg smoother: not a smoothed level");
+    NRS_CHECK(in != out, "mg smoother: input and output must differ");
+    smootherS(*levels[level], (const T *)in, (T *)out, s);
+  }
 
   // ---- setup helpers
 
@@ -611,14 +636,30 @@ struct MGImpl final : MGSolver {
   {
     el = e;
     const auto &o = e->opts;
-    smoother = o.smoother;
-    NRS_CHECK(smoother == NRS_SMOOTHER_CHEBYSHEV_JACOBI || smoother == NRS_SMOOTHER_OPT4_CHEBYSHEV_JACOBI,
-              "multigrid: only Chebyshev-Jacobi smoothers are built");
+    switch (o.smoother) {
+      case NRS_SMOOTHER_CHEBYSHEV_JACOBI: cheby = CHEBY1; schwarzKind = 0; break;
+      case NRS_SMOOTHER_OPT4_CHEBYSHEV_JACOBI: cheby = CHEBY_OPT4; schwarzKind = 0; break;
+      case NRS_SMOOTHER_CHEBYSHEV_ASM: cheby = CHEBY1; schwarzKind = 1; break;
+      case NRS_SMOOTHER_OPT4_CHEBYSHEV_ASM: cheby = CHEBY_OPT4; schwarzKind = 1; break;
+      case NRS_SMOOTHER_CHEBYSHEV_RAS: cheby = CHEBY1; schwarzKind = 2; break;
+      case NRS_SMOOTHER_OPT4_CHEBYSHEV_RAS: cheby = CHEBY_OPT4; schwarzKind = 2; break;
+      case NRS_SMOOTHER_ASM: cheby = NO_CHEBY; schwarzKind = 1; break;
+      case NRS_SMOOTHER_RAS: cheby = NO_CHEBY; schwarzKind = 2; break;
+      default: throw Error(NRS_ERR_INVALID, "multigrid: unknown smoother");
+    }
     NRS_CHECK(o.coarseSolver == NRS_COARSE_PCG_F64, "multigrid: only the FP64 PCG coarse solve is built");
-    // level schedule (determineMGLevels.cpp:58-95, Chebyshev-Jacobi table)
+    NRS_CHECK(o.fdmPrecision == NRS_FDM_F32 || o.fdmPrecision == NRS_FDM_BF16 ||
+                  o.fdmPrecision == NRS_FDM_F16,
+              "multigrid: unknown FDM precision");
+    // level schedule (determineMGLevels.cpp:58-95)
     std::vector<int> orders;
     if (o.nLevels > 0) {
       for (int i = 0; i < o.nLevels && i < 8; ++i) orders.push_back(o.levels[i]);
+    } else if (schwarzKind) {
+      static const std::map<int, std::vector<int>> table = {
+          {1, {1}}, {2, {2, 1}}, {3, {3, 1}}, {4, {4, 2, 1}}, {5, {5, 3, 1}},
+          {6, {6, 3, 1}}, {7, {7, 3, 1}}, {8, {8, 5, 1}}, {9, {9, 5, 1}}};
+      orders = table.at(e->N);
     } else {
       static const std::map<int, std::vector<int>> table = {
           {1, {1}}, {2, {2, 1}}, {3, {3, 1}}, {4, {4, 2, 1}}, {5, {5, 3, 1}},
@@ -630,13 +671,21 @@ struct MGImpl final : MGSolver {
       NRS_CHECK(orders[i] >= 1 && orders[i] < orders[i - 1], "multigrid: orders must decrease");
     NRS_CHECK(orders.size() >= 2, "multigrid: at least two levels are required");
     NRS_CHECK(o.chebyDegree >= 1 && o.chebyDegree <= 16, "multigrid: Chebyshev degree must be 1..16");
-    if (smoother == NRS_SMOOTHER_OPT4_CHEBYSHEV_JACOBI) {
+    if (cheby == CHEBY_OPT4) {
       betasDown = opt4_betas<T>(o.chebyDegree);
       betasUp = betasDown;
     }
 
     std::vector<double> xf(d->x, d->x + e->Nlocal), yf(d->y, d->y + e->Nlocal), zf(d->z, d->z + e->Nlocal);
     std::vector<long long> gf(d->globalIds, d->globalIds + e->Nlocal);
+    // Schwarz: face links of the mesh and element lengths of the finest level
+    // (pMGLevel::build passes the finest elliptic to compute_element_lengths)
+    std::vector<FaceLink> links;
+    std::vector<double> lengths;
+    if (schwarzKind) {
+      links = face_links(e->N, e->E, gf.data());
+      lengths = element_lengths(e->N, e->E, xf.data(), yf.data(), zf.data(), links);
+    }
     int Nprev = e->N;
     for (size_t li = 0; li < orders.size(); ++li) {
       auto lp = std::make_unique<Level<T>>();
@@ -689,23 +738,30 @@ struct MGImpl final : MGSolver {
         l.rhs = l.rhsOwn.get(); l.x = l.xOwn.get();
       }
       if (!l.isCoarse) {
-        // Jacobi smoother: 1/diag(A), assembled (ellipticUpdateJacobi.cpp:36-61)
-        l.invDiagA.alloc(l.L);
-        launch_build_diag<T>(l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, false,
-                             l.invDiagA.get(), s);
-        ogs_gather_scatter<T>(l.ogs, l.invDiagA.get(), 1, l.L, NRS_ADD, s);
-        reciprocal<T>(l.L, l.invDiagA.get(), s);
-        l.degDown = l.degUp = o.chebyDegree;
+        if (schwarzKind) {
+          // Schwarz smoother (pMGLevel::build, ellipticMultiGridSchwarz.cpp:837-1054)
+          l.schwarz = std::make_unique<Schwarz<T>>();
+          l.schwarz->setup(l.N, l.E, schwarzKind == 2, o.fdmPrecision, lengths, d->EToB, links,
+                           l.ogs, l.invDegree.get(), l.Nmasked, l.maskIds, s);
+        } else {
+          // Jacobi smoother: 1/diag(A), assembled (ellipticUpdateJacobi.cpp:36-61)
+          l.invDiagA.alloc(l.L);
+          launch_build_diag<T>(l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, false,
+                               l.invDiagA.get(), s);
+          ogs_gather_scatter<T>(l.ogs, l.invDiagA.get(), 1, l.L, NRS_ADD, s);
+          reciprocal<T>(l.L, l.invDiagA.get(), s);
+        }
+        l.degDown = l.degUp = (cheby == NO_CHEBY) ? 1 : o.chebyDegree;
       }
       Nprev = l.N;
       levels.push_back(std::move(lp));
     }
     const int64_t L0 = levels[0]->L;
-    sRes.alloc(L0); sAd.alloc(L0); sD.alloc(L0);
+    sRes.alloc(L0); sAd.alloc(L0); sD.alloc(L0); sTmp.alloc(L0);
     if (!std::is_same<T, double>::value) { rP.alloc(L0); zP.alloc(L0); }
     for (auto &lp : levels) {
       Level<T> &l = *lp;
-      if (l.isCoarse) continue;
+      if (l.isCoarse || cheby == NO_CHEBY) continue;
       l.rho = max_eig(l, o.seed, s);
       l.lmax = o.chebyMaxFactor * l.rho;
       l.lmin = o.chebyMinFactor * l.rho;

@@ -68,6 +68,26 @@ for pair in 8:4 8:2 4:2 8:6 6:4 10:4 10:6 6:2; do
   done
 done
 
+# preFDM.c / fusedFDM.c (ASM: p_restrict=0, RAS: p_restrict=1) / postFDM.c, pfloat = float
+#   ref_prefdm_nq<Nq>, ref_fusedfdm_{asm,ras}_nq<Nq>, ref_postfdm_nq<Nq>
+for nq in 2 4 8; do
+  nqe=$((nq + 2))
+  F="-Dpfloat=float -Ddfloat=double -Dp_Nq=$nq -Dp_Nq_e=$nqe"
+  o="$OUT/obj/prefdm_${nq}.o"
+  $CXX $CXXFLAGS $F "-DFUNC(a)=ref_prefdm_nq${nq}" -c "$K/preFDM.c" -o "$o"
+  objs+=("$o")
+  for r in 0 1; do
+    name=$([ $r = 1 ] && echo ras || echo asm)
+    o="$OUT/obj/fusedfdm_${name}_${nq}.o"
+    $CXX $CXXFLAGS $F -Dp_restrict=$r "-DFUNC(a)=ref_fusedfdm_${name}_nq${nq}" \
+      -c "$K/fusedFDM.c" -o "$o"
+    objs+=("$o")
+  done
+  o="$OUT/obj/postfdm_${nq}.o"
+  $CXX $CXXFLAGS $F "-DFUNC(a)=ref_postfdm_nq${nq}" -c "$K/postFDM.c" -o "$o"
+  objs+=("$o")
+done
+
 # gslib (serial: MPI undefined), 3rd_party/gslib/src
 GSFLAGS="-O2 -fPIC -DUNDERSCORE -DGLOBAL_LONG_LONG -DUSE_NAIVE_BLAS -I$GS"
 for f in gs sort sarray_transfer sarray_sort gs_local fail crystal comm tensor; do

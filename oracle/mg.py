@@ -13,6 +13,12 @@ oracle's C kernels that restates the reference's multigrid:
   preconditioner casts      src/solvers/elliptic/ellipticPreconditioner.cpp:57-62
   PCG / flexible PCG        src/solvers/elliptic/linearSolver/PCG.cpp:85-203
 
+Smoothers: Chebyshev of the 1st kind (ellipticMultiGridLevel.cpp:144-210) or
+the optimal 4th kind (:212-270, betas of MG/optimalCoeffs.cpp) around S =
+Jacobi (1/diag A) or the overlapping Schwarz / FDM smoother (oracle/fdm.py,
+ASM or RAS); pure ASM / RAS smoothing (:85-103: no post-smoothing when x is
+nonzero).
+
 Level arithmetic is FP32 (pfloat) or FP64; the coarsest level (N=1) is solved
 by Jacobi-PCG in FP64 to a relative tolerance, where the reference calls
 BoomerAMG.  lambda_max(S A) comes from a 10-step Arnoldi started from a
@@ -23,6 +29,15 @@ import numpy as np
 
 from . import (BoxMesh, Elliptic, GWJ, Ogs, gll, interp, lib, mask_ids, masked_gids,
                transfer_R, build_diag)
+from . import fdm as _fdm
+
+# optimal 4th-kind Chebyshev betas, degrees 1..4 (MG/optimalCoeffs.cpp:4-215;
+# Lottes, arXiv:2202.08830)
+OPT4_BETAS = {
+    1: [1.12500000000000],
+    2: [1.02387287570313, 1.26408905371085],
+    3: [1.00842544782028, 1.08867839208730, 1.33753125909618],
+    4: [1.00391310427285, 1.04035811188593, 1.14863498546254, 1.38268869241000]}
 
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 
@@ -54,10 +69,15 @@ class MG:
     """p-multigrid V-cycle on the levels `orders` of a box mesh."""
 
     def __init__(self, mesh, orders, T=np.float32, degree=2, min_factor=0.1, max_factor=1.1,
-                 coarse_tol=1e-2, coarse_maxit=200, seed=0, smoother="cheby1"):
+                 coarse_tol=1e-2, coarse_maxit=200, seed=0, cheby="cheby1", S="jacobi",
+                 fmt=None):
+        """cheby: "cheby1", "opt4" or None (pure Schwarz); S: "jacobi", "asm", "ras";
+        fmt: None, "bf16" or "fp16" for the FDM contractions."""
         assert orders[0] == mesh.N
+        assert cheby in ("cheby1", "opt4", None) and S in ("jacobi", "asm", "ras")
+        assert cheby is not None or S != "jacobi"
         self.T = T
-        self.smoother = smoother
+        self.cheby, self.S = cheby, S
         self.coarse_tol, self.coarse_maxit = coarse_tol, coarse_maxit
         self.levels = []
         x, y, z = mesh.x, mesh.y, mesh.z
@@ -72,9 +92,9 @@ class MG:
                 z = interp_coords(Nprev, mesh.E, z, N)
             m = BoxMesh(mesh.nx, mesh.ny, mesh.nz, N)          # structured ids and BCs at order N
             m.x, m.y, m.z = x, y, z
-            r, w = gll(N)
+            _, wq = gll(N)
             m.ggeo = np.zeros(m.E * 7 * m.Np)
-            lib().orc_geofacs(N, m.E, m.D.ravel().copy(), w, x, y, z, m.ggeo, None)
+            lib().orc_geofacs(N, m.E, m.D.ravel().copy(), wq, x, y, z, m.ggeo, None)
             L.mesh = m
             L.el = Elliptic(m)                                  # masked ogs, mask, invDegree
             L.ggeoT = m.ggeo.astype(T)
@@ -83,14 +103,20 @@ class MG:
             if li > 0:
                 L.R = transfer_R(Nprev, N).astype(T)
             if not L.isCoarse:
-                d = build_diag(N, m.ggeo, m.D)
-                d = L.el.ogs.gs(d, "add")
-                L.invDiagT = (1.0 / d).astype(T)
+                if S == "jacobi":
+                    d = build_diag(N, m.ggeo, m.D)
+                    d = L.el.ogs.gs(d, "add")
+                    L.invDiagT = (1.0 / d).astype(T)
+                else:
+                    if li == 0:
+                        fine = (N, x, y, z, wq, L.el.ogs_full)
+                    L.schwarz = _fdm.Schwarz(m, fine, L.el.maskIds, L.el.ogs, L.el.invDegree,
+                                             mode=S, T=T, fmt=fmt)
                 L.degree = degree
             self.levels.append(L)
             Nprev = N
         for L in self.levels:
-            if not L.isCoarse:
+            if not L.isCoarse and cheby is not None:
                 L.rho = self._max_eig(L, seed)
                 L.lmax, L.lmin = max_factor * L.rho, min_factor * L.rho
         cl = self.levels[-1]
@@ -118,7 +144,7 @@ class MG:
         H = np.zeros((k, k))
         for j in range(k):
             av = self.op(L, V[j].astype(self.T))
-            v = (L.invDiagT * av).astype(np.float64)
+            v = self.smoother_S(L, av).astype(np.float64)
             for i in range(j + 1):
                 hij = np.sum(w * V[i] * v)
                 v = v - hij * V[i]
@@ -130,32 +156,74 @@ class MG:
             V.append(v)
         return float(np.max(np.abs(np.linalg.eigvals(H))))
 
-    # ---- smoothing: smoothChebyshev (ellipticMultiGridLevel.cpp:144-210)
+    def smoother_S(self, L, v):
+        """pMGLevel::smoother (ellipticMultiGridLevel.cpp:104-111)."""
+        if self.S == "jacobi":
+            return (L.invDiagT * v).astype(self.T)
+        return L.schwarz.apply(v)
+
     def smooth(self, L, rhs, x, x_is_zero):
+        """pMGLevel::smooth (ellipticMultiGridLevel.cpp:85-103)."""
+        if self.cheby is None:
+            if x_is_zero:
+                x[:] = L.schwarz.apply(rhs)
+            return x
+        if self.cheby == "opt4":
+            return self.smooth_opt4(L, rhs, x, x_is_zero)
+        return self.smooth_cheby1(L, rhs, x, x_is_zero)
+
+    # smoothChebyshev (ellipticMultiGridLevel.cpp:144-210); scalars are formed
+    # in FP64 and stored as pfloat, as there
+    def smooth_cheby1(self, L, rhs, x, x_is_zero):
         T = self.T
         deg = L.degree
-        theta = T(0.5) * (T(L.lmax) + T(L.lmin))
-        delta = T(0.5) * (T(L.lmax) - T(L.lmin))
-        invTheta = T(1) / theta
-        sigma = theta / delta
-        rho_n = T(1) / sigma
+        theta = T(0.5 * (L.lmax + L.lmin))
+        delta = T(0.5 * (L.lmax - L.lmin))
+        invTheta = T(1.0 / float(theta))
+        sigma = T(theta / delta)
+        rho_n = T(1.0 / float(sigma))
         if x_is_zero:
             x[:] = 0
             res = rhs.copy()
         else:
             res = (rhs - self.op(L, x)).astype(T)
-        res = (L.invDiagT * res).astype(T)
+        res = self.smoother_S(L, res)
         d = (invTheta * res).astype(T)
         for _ in range(1, deg):
-            Ad = (L.invDiagT * self.op(L, d)).astype(T)
+            Ad = self.smoother_S(L, self.op(L, d))
             rhoSave = rho_n
-            rho_n = T(1) / (T(2) * sigma - rho_n)
-            rC = T(2) * rho_n / delta
-            dC = rho_n * rhoSave
+            rho_n = T(1.0 / (2.0 * float(sigma) - float(rho_n)))
+            rC = T(2.0 * float(rho_n) / float(delta))
+            dC = T(rho_n * rhoSave)
             x += d
             res = (res - Ad).astype(T)
             d = (dC * d + rC * res).astype(T)
         x += d
+        x[L.el.maskIds] = 0
+        return x
+
+    # smoothFourthKindChebyshev (ellipticMultiGridLevel.cpp:212-270), rho = lambda1
+    def smooth_opt4(self, L, rhs, x, x_is_zero):
+        T = self.T
+        deg = L.degree
+        betas = [T(b) for b in OPT4_BETAS[deg]]
+        rho = L.lmax
+        if x_is_zero:
+            x[:] = 0
+            res = rhs.copy()
+        else:
+            res = (rhs - self.op(L, x)).astype(T)
+        Ad = self.smoother_S(L, res)
+        d = (T(4.0 / (3.0 * rho)) * Ad).astype(T)
+        for k in range(1, deg):
+            Ad = self.op(L, d)
+            x += (betas[k - 1] * d).astype(T)
+            res = (res - Ad).astype(T)
+            Ad = self.smoother_S(L, res)
+            dC = T((2.0 * k - 1.0) / (2.0 * k + 3.0))
+            rC = T((8.0 * k + 4.0) / ((2.0 * k + 3.0) * rho))
+            d = (rC * Ad + dC * d).astype(T)
+        x += (betas[-1] * d).astype(T)
         x[L.el.maskIds] = 0
         return x
 

@@ -9,11 +9,13 @@ records inputs and outputs of
   * ellipticBlockUpdatePCG;
   * ellipticPreconCoarsenHex3D / ProlongateHex3D for (Nf,Nc) pairs;
   * gslib gs() (double/float/int; add, mul, min, max) on box-mesh ids and on
-    random id lists with repeats and zeros.
+    random id lists with repeats and zeros;
+  * preFDM / fusedFDM (ASM and RAS) / postFDM in FP32 for Nq = 2, 4, 8.
 Inputs are seeded (numpy PCG64); element counts are tiny so the fixtures stay
 small.  Output: tests/golden/*.npz (data only).
 
-usage: python tests/golden/gen_golden.py [REFERENCE_ROOT]
+usage: python tests/golden/gen_golden.py [REFERENCE_ROOT] [fixture ...]
+(fixtures: ax block_ax updatepcg transfer gs fdm; default all)
 """
 import ctypes as C
 import os
@@ -147,19 +149,60 @@ def gen_gs(R, rng):
     return out
 
 
+def gen_fdm(R, rng):
+    out = {}
+    f32 = lambda n: rng.uniform(-1, 1, n).astype(np.float32)   # noqa: E731
+    for nq in [2, 4, 8]:
+        nl = nq + 2
+        E = 2
+        key = "fdm_nq%d" % nq
+        u = f32(E * nq ** 3)
+        work1 = np.zeros(E * nl ** 3, dtype=np.float32)
+        getattr(R, "ref_prefdm_nq%d" % nq)(ci(E), dp(u), dp(work1))
+        out.update({key + "_pre_u": u, key + "_pre_work1": work1})
+        Sx, Sy, Sz = f32(E * nl * nl), f32(E * nl * nl), f32(E * nl * nl)
+        invL = rng.uniform(0.1, 1.0, E * nl ** 3).astype(np.float32)
+        w1 = f32(E * nl ** 3)
+        elist = np.arange(E, dtype=np.int32)
+        out.update({key + "_Sx": Sx, key + "_Sy": Sy, key + "_Sz": Sz, key + "_invL": invL,
+                    key + "_w1": w1})
+        Su = np.zeros(E * nl ** 3, dtype=np.float32)
+        uw = w1.copy()
+        getattr(R, "ref_fusedfdm_asm_nq%d" % nq)(ci(E), dp(elist), dp(Su), dp(Sx), dp(Sy), dp(Sz),
+                                                  dp(invL), dp(uw))
+        out.update({key + "_asm_Su": Su, key + "_asm_u": uw})
+        wts = rng.uniform(0.1, 1.0, E * nq ** 3).astype(np.float32)
+        Su = np.zeros(E * nq ** 3, dtype=np.float32)
+        uw = w1.copy()
+        getattr(R, "ref_fusedfdm_ras_nq%d" % nq)(ci(E), dp(elist), dp(Su), dp(Sx), dp(Sy), dp(Sz),
+                                                  dp(invL), dp(wts), dp(uw))
+        out.update({key + "_ras_wts": wts, key + "_ras_Su": Su})
+        a, b = f32(E * nl ** 3), f32(E * nl ** 3)
+        Su = np.zeros(E * nq ** 3, dtype=np.float32)
+        getattr(R, "ref_postfdm_nq%d" % nq)(ci(E), dp(a.copy()), dp(b.copy()), dp(Su), dp(wts))
+        out.update({key + "_post_work1": a, key + "_post_work2": b, key + "_post_Su": Su})
+    return out
+
+
+GENERATORS = {"ax": gen_ax, "block_ax": gen_block_ax, "updatepcg": gen_updatepcg,
+              "transfer": gen_transfer, "gs": gen_gs}
+
+
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    only = set(sys.argv[2:])
     oracle.build()
     if oracle.build_ref(ref_root) is None:
         print("reference tree not found; nothing generated")
         return 1
     R = oracle.ref_lib()
-    rng = np.random.default_rng(20261202)
-    np.savez_compressed(os.path.join(HERE, "ax.npz"), **gen_ax(R, rng))
-    np.savez_compressed(os.path.join(HERE, "block_ax.npz"), **gen_block_ax(R, rng))
-    np.savez_compressed(os.path.join(HERE, "updatepcg.npz"), **gen_updatepcg(R, rng))
-    np.savez_compressed(os.path.join(HERE, "transfer.npz"), **gen_transfer(R, rng))
-    np.savez_compressed(os.path.join(HERE, "gs.npz"), **gen_gs(R, rng))
+    rng = np.random.default_rng(20261202)      # one stream through the original five, in order
+    for name, fn in GENERATORS.items():
+        data = fn(R, rng)
+        if not only or name in only:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **data)
+    if not only or "fdm" in only:
+        np.savez_compressed(os.path.join(HERE, "fdm.npz"), **gen_fdm(R, np.random.default_rng(20261203)))
     print("golden fixtures written to", HERE)
     return 0
 
