@@ -16,9 +16,11 @@ LIB = os.path.join(HERE, "libnekrs_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 VALU ops.
+# -amdgpu-mfma-vgpr-form: MFMA results in VGPRs (no accumulator-register copies).
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Wno-unused-function", "-Wno-unused-variable",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 

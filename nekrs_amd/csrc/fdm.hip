@@ -26,7 +26,9 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <thread>
+#include <type_traits>
 
 #include "fdm.hpp"
 #include "kernels.hpp"
@@ -100,6 +102,39 @@ void jacobi_eig(int n, double *a, double *d, double *v)
       }
   }
   for (int i = 0; i < n; ++i) d[i] = a[i * n + i];
+}
+
+// FP32 -> BF16 / FP16 bit patterns, round to nearest even (the device rounding)
+uint16_t host_bf16(float x)
+{
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+uint16_t host_f16(float x)
+{
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const uint32_t a = u & 0x7FFFFFFFu;
+  if (a >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (a > 0x7F800000u ? 0x200u : 0u));
+  if (a >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);          // rounds to >= 65520: inf
+  if (a < 0x38800000u) {                                            // below 2^-14: subnormal
+    const int ex = (int)(a >> 23);
+    const int shift = 126 - ex;                                     // value = m * 2^-shift units of 2^-24
+    if (ex == 0 || shift > 24) return (uint16_t)sign;
+    const uint32_t m = (a & 0x7FFFFFu) | 0x800000u;
+    uint32_t q = m >> shift;
+    const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t q = a - 0x38000000u;                                     // rebias 127 -> 15
+  const uint32_t rem = q & 0x1FFFu;
+  q >>= 13;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;
+  return (uint16_t)(sign | q);
 }
 
 template <typename F> void parallel_for(int64_t n, F &&fn)
@@ -329,12 +364,12 @@ template <int NQ> struct FdmCfg {
 // One element per TPE threads (whole waves, so the element's operators are
 // read with wave-uniform loads).  Thread (a, b) owns one pencil of the box in
 // each of the six contraction phases; the box lives in two LDS buffers.
-//   RAS: out = restricted result * w (w = invDegree)
-//   ASM: out = restricted result, gout = overlap (ghost face) results
+//   RAS and ASM: out = restricted result; ASM also gout = overlap (ghost face)
+//   results.  The RAS weight (invDegree) is applied inside the gather-scatter.
 template <typename T, int NQ, bool ASM>
 __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
     int E, const T *__restrict__ u, const T *__restrict__ S, const double *__restrict__ lam,
-    const int2 *__restrict__ links, const T *__restrict__ w, T *__restrict__ out, T *__restrict__ gout)
+    const int4 *__restrict__ gmap, T *__restrict__ out, T *__restrict__ gout)
 {
   using C = FdmCfg<NQ>;
   constexpr int NL = C::NL, NLP = C::NLP, NP = C::NP, NQ2 = NQ * NQ, TPE = C::TPE;
@@ -378,15 +413,9 @@ __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
         default: g = at(vv + 1, uu + 1, NL - 1); o = at(vv + 1, uu + 1, NL - 3); break;
       }
       const T own = X[o];
-      T val = T(0);
-      const int2 lk = links[(int64_t)e * 6 + f];
-      if (lk.x >= 0) {
-        int up, vp;
-        link_map(lk.y, NQ - 1, uu, vv, up, vp);
-        const T nb = u[(int64_t)lk.x * NP + face_node(lk.y & 7, up, vp, 1, NQ - 1)];
-        val = (own + nb) - own;
-      }
-      X[g] = val;
+      const int4 gm = gmap[(int64_t)e * 6 + f];
+      const T nb = (gm.x >= 0) ? u[gm.x + uu * gm.y + vv * gm.z] : T(0);
+      X[g] = (own + nb) - own;     // 0 without a neighbour, as the unshared extended node
     }
   }
   __syncthreads();
@@ -478,7 +507,7 @@ __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
 #pragma unroll
     for (int k = 1; k <= NL - 2; ++k) {
       const int64_t idx = eb + ((k - 1) * NQ + (a - 1)) * NQ + (b - 1);
-      out[idx] = ASM ? o[k] : o[k] * w[idx];
+      out[idx] = o[k];
     }
   }
   if (!ASM) return;
@@ -502,43 +531,53 @@ __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
 // ASM overlap exchange (extended gather-scatter + postFDM.c:3-155): each node
 // on the second layer from a face with a neighbour gets (own + neighbour) -
 // own of the overlap results there, in the order k faces, j faces, i faces;
-// then the Schwarz weight.
+// then the Schwarz weight.  One thread per (j, k) row of i values.
 template <typename T, int NQ>
-__global__ void fdm_post_kernel(int64_t n, const T *__restrict__ box, const T *__restrict__ gout,
-                                const int2 *__restrict__ links, const T *__restrict__ wts,
-                                T *__restrict__ Su)
+__global__ __launch_bounds__(256) void fdm_post_kernel(int E, const T *__restrict__ box,
+                                                       const T *__restrict__ gout,
+                                                       const int4 *__restrict__ smap,
+                                                       const T *__restrict__ wts, T *__restrict__ Su)
 {
   constexpr int NP = NQ * NQ * NQ, NQ2 = NQ * NQ;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = idx / NP;
-    const int r = (int)(idx - e * NP);
-    const int i = r % NQ, j = (r / NQ) % NQ, k = r / NQ2;
-    T v = box[idx];
-    auto add = [&](int f, int uu, int vv) {
-      const int2 lk = links[e * 6 + f];
-      if (lk.x < 0) return;
-      int up, vp;
-      link_map(lk.y, NQ - 1, uu, vv, up, vp);
-      const T own = gout[(e * 6 + f) * NQ2 + vv * NQ + uu];
-      const T nb = gout[((int64_t)lk.x * 6 + (lk.y & 7)) * NQ2 + vp * NQ + up];
-      v += (own + nb) - own;
-    };
-    if (k == 1) add(0, i, j);
-    if (k == NQ - 2) add(5, i, j);
-    if (j == 1) add(1, i, k);
-    if (j == NQ - 2) add(3, i, k);
-    if (i == 1) add(4, j, k);
-    if (i == NQ - 2) add(2, j, k);
-    Su[idx] = v * wts[idx];
-  }
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= (int64_t)E * NQ2) return;
+  const int64_t e = row / NQ2;
+  const int r = (int)(row - e * NQ2), j = r % NQ, k = r / NQ;
+  const int64_t base = e * NP + (int64_t)r * NQ;
+  const T *ge = gout + e * 6 * NQ2;
+  T v[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) v[i] = box[base + i];
+  // overlap value at face point (uu, vv) of face f: (own + neighbour) - own
+  auto ov = [&](int f, int uu, int vv) -> T {
+    const int4 sm = smap[e * 6 + f];
+    if (sm.x < 0) return T(0);
+    const T own = ge[f * NQ2 + vv * NQ + uu];
+    return (own + gout[sm.x + uu * sm.y + vv * sm.z]) - own;
+  };
+  if (k == 1)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) v[i] += ov(0, i, j);
+  if (k == NQ - 2)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) v[i] += ov(5, i, j);
+  if (j == 1)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) v[i] += ov(1, i, k);
+  if (j == NQ - 2)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) v[i] += ov(3, i, k);
+  v[1] += ov(4, j, k);
+  v[NQ - 2] += ov(2, j, k);
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) Su[base + i] = v[i] * wts[base + i];
 }
 
 // overlap counts behind the ASM weights (generate_weights, :798-835 before
 // its gather-scatter): 1 + number of faces with a neighbour whose second
 // layer holds the node
 template <typename T>
-__global__ void asm_count_kernel(int N, int64_t n, const int2 *__restrict__ links, T *__restrict__ c)
+__global__ void asm_count_kernel(int N, int64_t n, const int4 *__restrict__ gmap, T *__restrict__ c)
 {
   const int Nq = N + 1, Np = Nq * Nq * Nq;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
@@ -547,7 +586,7 @@ __global__ void asm_count_kernel(int N, int64_t n, const int2 *__restrict__ link
     const int r = (int)(idx - e * Np);
     const int i = r % Nq, j = (r / Nq) % Nq, k = r / (Nq * Nq);
     int cnt = 1;
-    const int2 *lk = links + e * 6;
+    const int4 *lk = gmap + e * 6;
     cnt += (k == 1 && lk[0].x >= 0) + (k == N - 1 && lk[5].x >= 0);
     cnt += (j == 1 && lk[1].x >= 0) + (j == N - 1 && lk[3].x >= 0);
     cnt += (i == 1 && lk[4].x >= 0) + (i == N - 1 && lk[2].x >= 0);
@@ -555,24 +594,407 @@ __global__ void asm_count_kernel(int N, int64_t n, const int2 *__restrict__ link
   }
 }
 
-template <typename T, int NQ>
-void launch_fdm_nq(bool ras, int E, const T *u, const T *S, const double *lam, const int2 *links,
-                   const T *w, T *out, T *gout, hipStream_t s)
+// ---- BF16 / FP16 variant: the six contractions on the matrix cores
+//
+// One element per wavefront.  The extended box lives in LDS as 16-bit values in
+// a [plane][row][16] layout; every contraction is a (16 x 16) x (16 x 16) MFMA
+// per plane (v_mfma_f32_16x16x16_{bf16,f16}, FP32 accumulation) whose result is
+// written back transposed so that the next contraction index is the contiguous
+// one: X[k][j][i] -> T1[m][k][j] -> T2[n][m][k] -> T3[p][n][m] (times invL) ->
+// T4[i][p][n] -> T5[j][i][p] -> out[k][j][i].  Each axis is stored in the
+// order (interior 0..NQ-1, left ghost, right ghost, zero padding), so interior
+// rows are contiguous and aligned; the 1D operators are stored with their rows
+// permuted the same way.  FP16 data are kept in range by a power-of-two scale
+// per element and contraction (block floating point), which leaves the
+// rounding of an 11-bit significand.
+
+typedef _Float16 fdm_h4 __attribute__((ext_vector_type(4)));
+typedef short fdm_s4 __attribute__((ext_vector_type(4)));
+typedef float fdm_f4 __attribute__((ext_vector_type(4)));
+
+typedef __bf16 fdm_bf2 __attribute__((ext_vector_type(2)));
+typedef _Float16 fdm_h2 __attribute__((ext_vector_type(2)));
+typedef float fdm_f2 __attribute__((ext_vector_type(2)));
+// two FP32 -> two 16-bit values in one word (v_cvt_pk_bf16_f32 / v_cvt_pk_f16_f32,
+// round to nearest even)
+template <int FMT> __device__ __forceinline__ uint32_t pack2(float a, float b)
 {
-  using C = FdmCfg<NQ>;
-  const unsigned grid = (unsigned)((E + C::EPB - 1) / C::EPB);
-  if (ras)
-    fdm_kernel<T, NQ, false><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, links, w, out, gout);
+  const fdm_f2 v = {a, b};
+  if constexpr (FMT == NRS_FDM_BF16) return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, fdm_bf2));
+  else return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, fdm_h2));
+}
+template <int FMT> __device__ __forceinline__ uint2 pack4(fdm_f4 v)
+{
+  return make_uint2(pack2<FMT>(v[0], v[1]), pack2<FMT>(v[2], v[3]));
+}
+template <int FMT> __device__ __forceinline__ fdm_f4 mfma16(uint2 a, uint2 b)
+{
+  const fdm_f4 z = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (FMT == NRS_FDM_BF16)
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(fdm_s4, a),
+                                                     __builtin_bit_cast(fdm_s4, b), z, 0, 0, 0);
   else
-    fdm_kernel<T, NQ, true><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, links, w, out, gout);
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(fdm_h4, a),
+                                                 __builtin_bit_cast(fdm_h4, b), z, 0, 0, 0);
+}
+// LDS traffic of one wavefront is ordered; the fence keeps the compiler from
+// moving LDS accesses across a contraction boundary.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <int NQ> struct MfmaCfg {
+  static constexpr int NL = NQ + 2;
+  // plane stride (halves): 16 x 16 + 8, more if the staged results of an
+  // element (ASM: NP + 6 NQ^2 floats) would not fit one box buffer
+  static constexpr int PSMIN = (2 * (NQ * NQ * NQ + 6 * NQ * NQ) + NL - 1) / NL;
+  static constexpr int PS = (PSMIN <= 264) ? 264 : (PSMIN + 7) / 8 * 8;
+  static constexpr int BUF = NL * PS;
+  static constexpr int SST = (3 * NL * NL + 7) / 8 * 8;   // operator halves per element, 16-byte multiple
+  static constexpr int WAVES = 4;
+  static_assert(NL <= 16, "one 16-wide MFMA tile per axis");
+};
+
+// Each wavefront walks its elements in a grid-stride loop.  The global loads
+// of the next element (interior rows, the own and neighbour values behind each
+// ghost, operators, eigenvalues, ghost maps) are issued before the contractions
+// of the current one, and the results of an element are staged in LDS and
+// written out (as whole rows) only after the next element's loads have
+// arrived, so no wait on outstanding stores sits in front of the prefetch.
+template <int NQ, int FMT, bool ASM>
+__global__ __launch_bounds__(256) void fdm_mfma_kernel(
+    int E, const float *__restrict__ u, const uint16_t *__restrict__ S16,
+    const float *__restrict__ lam, const int4 *__restrict__ gmap, float *__restrict__ out,
+    float *__restrict__ gout)
+{
+  using C = MfmaCfg<NQ>;
+  constexpr int NL = C::NL, PS = C::PS, BUF = C::BUF, NP = NQ * NQ * NQ, NQ2 = NQ * NQ;
+  constexpr int SST = C::SST;                  // halves of operators per element (16-byte rows)
+  constexpr int SQ4 = SST / 8;                 // uint4 per element's operators
+  constexpr bool SCALE = (FMT == NRS_FDM_F16);
+  constexpr int RI = (NQ2 + 63) / 64;          // interior rows per lane
+  constexpr int RG = (NQ2 + 63) / 64;          // positions per face per lane
+  constexpr int RS = (SQ4 + 63) / 64;          // operator uint4 per lane
+  constexpr int OUTF = ASM ? NP + 6 * NQ2 : NP;   // staged floats per element
+  static_assert(2 * OUTF <= BUF, "staged results fit one box buffer");
+  __shared__ __align__(16) uint16_t lds[C::WAVES][2 * BUF + SST];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  uint16_t *X = lds[wv], *Y = lds[wv] + BUF, *SB = lds[wv] + 2 * BUF;
+  float *stage = reinterpret_cast<float *>(Y);   // results of the previous element
+  {
+    uint4 *p = reinterpret_cast<uint4 *>(lds[wv]);
+    for (int n = lane; n < (2 * BUF + SST) * 2 / 16; n += 64) p[n] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  wave_lds_sync();
+  const int stride = gridDim.x * C::WAVES;
+  int e = __builtin_amdgcn_readfirstlane(blockIdx.x * C::WAVES + wv);
+  if (e >= E) return;
+
+  // ---- prefetch state of the next element
+  float pvi[RI][NQ], pown[6][RG], pnb[6][RG], plx[4], plz, ply[NL];
+  uint4 ps[RS];
+  int4 pgm[6];
+  auto load_a = [&](int en) {
+    const float *ue = u + (int64_t)en * NP;
+#pragma unroll
+    for (int t = 0; t < RI; ++t) {
+      const int n = lane + 64 * t;
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) pvi[t][i] = (n < NQ2) ? ue[n * NQ + i] : 0.f;
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+#pragma unroll
+      for (int t = 0; t < RG; ++t) {
+        const int n = lane + 64 * t;
+        pown[f][t] = (n < NQ2) ? ue[face_node(f, n % NQ, n / NQ, 1, NQ - 1)] : 0.f;
+      }
+    const uint4 *Se = reinterpret_cast<const uint4 *>(S16 + (int64_t)en * SST);
+#pragma unroll
+    for (int t = 0; t < RS; ++t) {
+      const int n = lane + 64 * t;
+      ps[t] = (n < SQ4) ? Se[n] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    const float *le = lam + (int64_t)en * 3 * NL;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) plx[q] = (4 * g + q < NL) ? le[4 * g + q] : 0.f;
+    plz = (c < NL) ? le[2 * NL + c] : 0.f;
+#pragma unroll
+    for (int a = 0; a < NL; ++a) ply[a] = le[NL + a];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) pgm[f] = gmap[(int64_t)en * 6 + f];
+  };
+  auto load_b = [&]() {   // neighbour layers through the affine ghost maps
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int4 gm = pgm[f];
+#pragma unroll
+      for (int t = 0; t < RG; ++t) {
+        const int n = lane + 64 * t;
+        pnb[f][t] = (gm.x >= 0 && n < NQ2) ? u[gm.x + (n % NQ) * gm.y + (n / NQ) * gm.z] : 0.f;
+      }
+    }
+  };
+  auto flush = [&](int ep) {   // staged results of element ep -> global rows
+    for (int n = lane; n < NQ2; n += 64) {
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) out[(int64_t)ep * NP + n * NQ + i] = stage[n * NQ + i];
+    }
+    if constexpr (ASM) {
+      for (int n = lane; n < 6 * NQ2; n += 64) gout[(int64_t)ep * 6 * NQ2 + n] = stage[NP + n];
+    }
+    // the padding rows NL..15 of every plane are read as zero by the contractions
+    for (int n = lane; n < NL * (16 - NL) * 2; n += 64) {
+      const int pl = n / ((16 - NL) * 2), rr = NL + (n / 2) % (16 - NL), hh = n & 1;
+      *reinterpret_cast<uint4 *>(Y + pl * PS + rr * 16 + hh * 8) = make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
+  load_a(e);
+  load_b();
+  int prev = -1;   // element whose results are staged
+
+  for (;;) {
+    float vi[RI][NQ], vg[6][RG], lx[4], lz, ly[NL];
+    uint4 sr[RS];
+#pragma unroll
+    for (int t = 0; t < RI; ++t)
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) vi[t][i] = pvi[t][i];
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+#pragma unroll
+      for (int t = 0; t < RG; ++t) vg[f][t] = (pown[f][t] + pnb[f][t]) - pown[f][t];   // fusedFDM.c:47-104
+#pragma unroll
+    for (int t = 0; t < RS; ++t) sr[t] = ps[t];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lx[q] = plx[q];
+    lz = plz;
+#pragma unroll
+    for (int a = 0; a < NL; ++a) ly[a] = ply[a];
+    const int en = e + stride;
+    const bool more = en < E;
+
+    // -- 1. the extended box: interior rows, ghost faces, zero edges; operators
+    float sc = 1.f;   // stored = true * sc
+    if constexpr (SCALE) {
+      float m = 0.f;
+#pragma unroll
+      for (int t = 0; t < RI; ++t)
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) m = fmaxf(m, fabsf(vi[t][i]));
+#pragma unroll
+      for (int f = 0; f < 6; ++f)
+#pragma unroll
+        for (int t = 0; t < RG; ++t) m = fmaxf(m, fabsf(vg[f][t]));
+      m = wave_max(m);
+      if (m > 0.f) {
+        int ex;
+        frexpf(m, &ex);
+        sc = ldexpf(1.f, 15 - ex);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < RI; ++t) {
+      const int n = lane + 64 * t;
+      if (n < NQ2) {
+        uint16_t *row = X + (n / NQ) * PS + (n % NQ) * 16;
+#pragma unroll
+        for (int i = 0; i + 1 < NQ; i += 2)
+          *reinterpret_cast<uint32_t *>(row + i) = pack2<FMT>(vi[t][i] * sc, vi[t][i + 1] * sc);
+        if (NQ & 1) row[NQ - 1] = (uint16_t)(pack2<FMT>(vi[t][NQ - 1] * sc, 0.f) & 0xFFFFu);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+#pragma unroll
+      for (int t = 0; t < RG; ++t) {
+        const int n = lane + 64 * t;
+        if (n < NQ2) {
+          const int uu = n % NQ, vv = n / NQ;
+          int pos;
+          switch (f) {   // (plane, row, col) of the ghost in storage order
+            case 0: pos = NQ * PS + vv * 16 + uu; break;
+            case 5: pos = (NQ + 1) * PS + vv * 16 + uu; break;
+            case 1: pos = vv * PS + NQ * 16 + uu; break;
+            case 3: pos = vv * PS + (NQ + 1) * 16 + uu; break;
+            case 4: pos = vv * PS + uu * 16 + NQ; break;
+            default: pos = vv * PS + uu * 16 + NQ + 1; break;
+          }
+          X[pos] = (uint16_t)(pack2<FMT>(vg[f][t] * sc, 0.f) & 0xFFFFu);
+        }
+      }
+    for (int n = lane; n < 12 * NL; n += 64) {   // box edges (two ghost indices) are zero
+      const int line = n / NL, sidx = n % NL;
+      const int c0 = NQ + (line & 1), c1 = NQ + ((line >> 1) & 1), dir = line >> 2;
+      const int pos = (dir == 0) ? c1 * PS + c0 * 16 + sidx
+                                 : (dir == 1) ? c1 * PS + sidx * 16 + c0 : sidx * PS + c1 * 16 + c0;
+      X[pos] = 0;
+    }
+#pragma unroll
+    for (int t = 0; t < RS; ++t) {
+      const int n = lane + 64 * t;
+      if (n < SQ4) reinterpret_cast<uint4 *>(SB)[n] = sr[t];
+    }
+    // the previous element's results leave before the next element's loads are
+    // issued, so waiting for those loads never waits on these stores
+    if (prev >= 0) flush(prev);
+    wave_lds_sync();
+    if (more) load_a(en);
+    // forward B[s][m] = S[s][m], backward B[m][s] = S[s][m] (rows in storage order)
+    uint2 bf[3], bb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      uint32_t f4[4], b4[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = 4 * g + q;
+        const bool ok = (k < NL && c < NL);
+        f4[q] = ok ? SB[(d * NL + k) * NL + c] : 0u;
+        b4[q] = ok ? SB[(d * NL + c) * NL + k] : 0u;
+      }
+      bf[d] = make_uint2(f4[0] | (f4[1] << 16), f4[2] | (f4[3] << 16));
+      bb[d] = make_uint2(b4[0] | (b4[1] << 16), b4[2] | (b4[3] << 16));
+    }
+
+    // -- 2. six contractions
+    fdm_f4 D[NL];
+    auto contract = [&](const uint16_t *in, uint2 bfrag) {
+#pragma unroll
+      for (int a = 0; a < NL; ++a)
+        D[a] = mfma16<FMT>(*reinterpret_cast<const uint2 *>(in + a * PS + c * 16 + 4 * g), bfrag);
+    };
+    auto rescale = [&]() {   // FP16: bring the phase result back to [2^14, 2^15)
+      if constexpr (SCALE) {
+        float m = 0.f;
+#pragma unroll
+        for (int a = 0; a < NL; ++a)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) m = fmaxf(m, fabsf(D[a][q]));
+        m = wave_max(m);
+        if (m > 0.f) {
+          int ex;
+          frexpf(m, &ex);
+          const float sf = ldexpf(1.f, 15 - ex);
+#pragma unroll
+          for (int a = 0; a < NL; ++a) D[a] *= sf;
+          sc *= sf;
+        }
+      }
+    };
+    auto store = [&](uint16_t *outb) {
+      if (c < NL) {
+#pragma unroll
+        for (int a = 0; a < NL; ++a)
+          *reinterpret_cast<uint2 *>(outb + c * PS + a * 16 + 4 * g) = pack4<FMT>(D[a]);
+      }
+      wave_lds_sync();
+    };
+    contract(X, bf[0]); rescale(); store(Y);          // T1[m][k][j]
+    contract(Y, bf[1]); rescale(); store(X);          // T2[n][m][k]
+    contract(X, bf[2]);                               // T3[p][n][m] = invL * (...)
+#pragma unroll
+    for (int a = 0; a < NL; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float dg = (lx[q] + ly[a]) + lz;
+        D[a][q] *= (dg > 1e-5f) ? __builtin_amdgcn_rcpf(dg) : 0.f;   // 1 ulp: below the 16-bit rounding
+      }
+    rescale(); store(Y);
+    if (more) load_b();
+    contract(Y, bb[0]); rescale(); store(X);          // T4[i][p][n]
+    contract(X, bb[1]); rescale(); store(Y);          // T5[j][i][p]
+    contract(Y, bb[2]);                               // out[k][j][i]: k = c, j = a, i = 4g+q
+    wave_lds_sync();                                  // T5 fully read: Y becomes the staging area
+    const float inv = 1.f / sc;
+    if (c < NQ) {
+#pragma unroll
+      for (int a = 0; a < NQ; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = 4 * g + q;
+          if (i < NQ) stage[(c * NQ + a) * NQ + i] = D[a][q] * inv;
+        }
+    }
+    if constexpr (ASM) {
+      float *ge = stage + NP;
+      if (c < NQ) {
+#pragma unroll
+        for (int a = 0; a < NQ; ++a)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int i = 4 * g + q;
+            if (i == NQ) ge[4 * NQ2 + c * NQ + a] = D[a][q] * inv;          // x faces: (u,v) = (j,k)
+            if (i == NQ + 1) ge[2 * NQ2 + c * NQ + a] = D[a][q] * inv;
+          }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = 4 * g + q;
+          if (i < NQ) {                                                       // y faces: (u,v) = (i,k)
+            ge[1 * NQ2 + c * NQ + i] = D[NQ][q] * inv;
+            ge[3 * NQ2 + c * NQ + i] = D[NQ + 1][q] * inv;
+          }
+        }
+      } else if (c == NQ || c == NQ + 1) {                                    // z faces: (u,v) = (i,j)
+        float *gf = ge + (c == NQ ? 0 : 5) * NQ2;
+#pragma unroll
+        for (int a = 0; a < NQ; ++a)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int i = 4 * g + q;
+            if (i < NQ) gf[a * NQ + i] = D[a][q] * inv;
+          }
+      }
+    }
+    wave_lds_sync();
+    prev = e;
+    if (!more) break;
+    e = en;
+  }
+  flush(prev);
+}
+
+template <int NQ, int FMT>
+void launch_fdm_mfma_nq(bool ras, int E, const float *u, const uint16_t *S16, const float *lam,
+                        const int4 *gmap, float *out, float *gout, hipStream_t s)
+{
+  using C = MfmaCfg<NQ>;
+  // about four elements per wavefront, so the prefetch has something to overlap
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)E + 15) / 16, 8192));
+  if (ras)
+    fdm_mfma_kernel<NQ, FMT, false><<<grid, 64 * C::WAVES, 0, s>>>(E, u, S16, lam, gmap, out, gout);
+  else
+    fdm_mfma_kernel<NQ, FMT, true><<<grid, 64 * C::WAVES, 0, s>>>(E, u, S16, lam, gmap, out, gout);
   NRS_LAUNCH_CHECK();
 }
 
 template <typename T, int NQ>
-void launch_post_nq(int64_t n, const T *box, const T *gout, const int2 *links, const T *wts, T *Su,
+void launch_fdm_nq(bool ras, int E, const T *u, const T *S, const double *lam, const int4 *gmap,
+                   T *out, T *gout, hipStream_t s)
+{
+  using C = FdmCfg<NQ>;
+  const unsigned grid = (unsigned)((E + C::EPB - 1) / C::EPB);
+  if (ras)
+    fdm_kernel<T, NQ, false><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, gmap, out, gout);
+  else
+    fdm_kernel<T, NQ, true><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, gmap, out, gout);
+  NRS_LAUNCH_CHECK();
+}
+
+template <typename T, int NQ>
+void launch_post_nq(int E, const T *box, const T *gout, const int4 *smap, const T *wts, T *Su,
                     hipStream_t s)
 {
-  fdm_post_kernel<T, NQ><<<grid_for(n, 256, 16384), 256, 0, s>>>(n, box, gout, links, wts, Su);
+  const int64_t rows = (int64_t)E * NQ * NQ;
+  fdm_post_kernel<T, NQ><<<(unsigned)((rows + 255) / 256), 256, 0, s>>>(E, box, gout, smap, wts, Su);
   NRS_LAUNCH_CHECK();
 }
 
@@ -585,7 +1007,8 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
 {
   N = N_; E = E_; ras = ras_; fdmPrecision = prec;
   NRS_CHECK(N >= 1 && N <= 9, "schwarz: orders 1..9 are built");
-  NRS_CHECK(prec == NRS_FDM_F32, "schwarz: only FP32/FP64 FDM contractions are built");
+  NRS_CHECK((prec == NRS_FDM_F32 || std::is_same<T, float>::value),
+            "schwarz: BF16/FP16 contractions need the FP32 multigrid");
   Nq = N + 1; Np = Nq * Nq * Nq; nl = N + 3;
   ogs = ogs_; invDegree = invDegree_; Nmasked = Nmasked_; maskIds = maskIds_;
   std::vector<double> r, wq;
@@ -612,13 +1035,59 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
             "schwarz: 1D mass matrix not positive definite");
   S.upload(Sh);
   lam.upload(lh);
-  std::vector<int2> l2((size_t)E * 6);
-  for (size_t q = 0; q < l2.size(); ++q) l2[q] = make_int2(lk[q].nbr, lk[q].code);
-  links.upload(l2);
+  if (prec != NRS_FDM_F32) {
+    // 16-bit operators with rows in the MFMA kernel's storage order (interior,
+    // left ghost, right ghost), rounded from the FP32 values; FP32 eigenvalues
+    const int sst = (3 * nl * nl + 7) / 8 * 8;      // 16-byte aligned per element
+    std::vector<uint16_t> s16((size_t)E * sst, 0);
+    std::vector<float> l32(lh.size());
+    for (size_t q = 0; q < lh.size(); ++q) l32[q] = (float)lh[q];
+    for (int64_t e = 0; e < E; ++e)
+      for (int d = 0; d < 3; ++d)
+        for (int sp = 0; sp < nl; ++sp) {
+          const int phys = (sp < Nq) ? sp + 1 : (sp == Nq ? 0 : nl - 1);
+          for (int m = 0; m < nl; ++m) {
+            const float v = (float)Sh[((e * 3 + d) * nl + phys) * nl + m];
+            s16[e * sst + (d * nl + sp) * nl + m] = (prec == NRS_FDM_BF16) ? host_bf16(v) : host_f16(v);
+          }
+        }
+    S16.upload(s16);
+    lam32.upload(l32);
+  }
+  // affine maps of the face links at this order: the neighbour's second-layer
+  // node behind ghost point (u, v) of face f is gm.x + u gm.y + v gm.z, its
+  // overlap-result slot sm.x + u sm.y + v sm.z (x < 0: no neighbour)
+  NRS_CHECK((int64_t)E * Np < INT32_MAX && (int64_t)E * 6 * Nq * Nq < INT32_MAX,
+            "schwarz: E-vector too long for 32-bit indices");
+  std::vector<int4> gm((size_t)E * 6), sm((size_t)E * 6);
+  for (int64_t q = 0; q < (int64_t)E * 6; ++q) {
+    const FaceLink &l = lk[q];
+    if (l.nbr < 0) {
+      gm[q] = make_int4(-1, 0, 0, 0);
+      sm[q] = make_int4(-1, 0, 0, 0);
+      continue;
+    }
+    const int fb = l.code & 7;
+    auto node = [&](int uu, int vv) {
+      int up, vp;
+      link_map(l.code, N, uu, vv, up, vp);
+      return (int)((int64_t)l.nbr * Np + face_node(fb, up, vp, 1, N));
+    };
+    auto slot = [&](int uu, int vv) {
+      int up, vp;
+      link_map(l.code, N, uu, vv, up, vp);
+      return (int)(((int64_t)l.nbr * 6 + fb) * Nq * Nq + vp * Nq + up);
+    };
+    const int n0 = node(0, 0), s0 = slot(0, 0);
+    gm[q] = make_int4(n0, node(1, 0) - n0, node(0, 1) - n0, 0);
+    sm[q] = make_int4(s0, slot(1, 0) - s0, slot(0, 1) - s0, 0);
+  }
+  gmap.upload(gm);
+  if (!ras) smap.upload(sm);
   const int64_t L = (int64_t)E * Np;
   if (!ras) {
     wts.alloc(L);
-    asm_count_kernel<T><<<grid_for(L, 256, 16384), 256, 0, s>>>(N, L, links.get(), wts.get());
+    asm_count_kernel<T><<<grid_for(L, 256, 16384), 256, 0, s>>>(N, L, gmap.get(), wts.get());
     NRS_LAUNCH_CHECK();
     ogs_gather_scatter<T>(ogs, wts.get(), 1, L, NRS_ADD, s);
     reciprocal<T>(L, wts.get(), s);
@@ -636,10 +1105,31 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
   T *out = ras ? Su : boxOut.get();
   T *gout = ras ? nullptr : ghostOut.get();
   const T *w = ras ? invDegree : nullptr;
+  if constexpr (std::is_same<T, float>::value) {
+    if (fdmPrecision != NRS_FDM_F32) {
+#define NRS_FDM_MFMA_CASE(NQ_)                                                                 \
+  case NQ_:                                                                                    \
+    if (fdmPrecision == NRS_FDM_BF16)                                                          \
+      launch_fdm_mfma_nq<NQ_, NRS_FDM_BF16>(ras, E, u, S16.get(), lam32.get(), gmap.get(), out, gout, s); \
+    else                                                                                       \
+      launch_fdm_mfma_nq<NQ_, NRS_FDM_F16>(ras, E, u, S16.get(), lam32.get(), gmap.get(), out, gout, s); \
+    if (!ras) launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); \
+    break;
+      switch (Nq) {
+        NRS_FDM_MFMA_CASE(2) NRS_FDM_MFMA_CASE(3) NRS_FDM_MFMA_CASE(4) NRS_FDM_MFMA_CASE(5)
+        NRS_FDM_MFMA_CASE(6) NRS_FDM_MFMA_CASE(7) NRS_FDM_MFMA_CASE(8) NRS_FDM_MFMA_CASE(9)
+        NRS_FDM_MFMA_CASE(10)
+        default: throw Error(NRS_ERR_INVALID, "schwarz: unsupported order");
+      }
+#undef NRS_FDM_MFMA_CASE
+      finish(Su, s);
+      return;
+    }
+  }
 #define NRS_FDM_CASE(NQ_)                                                                      \
   case NQ_:                                                                                    \
-    launch_fdm_nq<T, NQ_>(ras, E, u, S.get(), lam.get(), links.get(), w, out, gout, s);       \
-    if (!ras) launch_post_nq<T, NQ_>(L, boxOut.get(), ghostOut.get(), links.get(), wts.get(), Su, s); \
+    launch_fdm_nq<T, NQ_>(ras, E, u, S.get(), lam.get(), gmap.get(), out, gout, s);           \
+    if (!ras) launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); \
     break;
   switch (Nq) {
     NRS_FDM_CASE(2) NRS_FDM_CASE(3) NRS_FDM_CASE(4) NRS_FDM_CASE(5) NRS_FDM_CASE(6)
@@ -647,7 +1137,19 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
     default: throw Error(NRS_ERR_INVALID, "schwarz: unsupported order");
   }
 #undef NRS_FDM_CASE
-  ogs_gather_scatter<T>(ogs, Su, 1, L, NRS_ADD, s);
+  finish(Su, s);
+}
+
+// RAS: Su = gs(invDegree .* restricted) (fusedFDM.c p_restrict, then oogs); the
+// weighted gather-scatter multiplies each member by its weight before the row
+// sum, as the reference's separate multiply does.  ASM: Su = gs(postFDM(...)).
+// Then the mask (smoothSchwarz, :1146).
+template <typename T>
+void Schwarz<T>::finish(T *Su, hipStream_t s)
+{
+  const int64_t L = (int64_t)E * Np;
+  if (ras) ogs_gather_scatter_weighted<T>(ogs, Su, invDegree, s);
+  else ogs_gather_scatter<T>(ogs, Su, 1, L, NRS_ADD, s);
   launch_mask<T>(Nmasked, maskIds, Su, s);
 }
 

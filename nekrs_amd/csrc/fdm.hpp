@@ -40,7 +40,10 @@ struct Schwarz {
   int fdmPrecision = NRS_FDM_F32;
   DevBuf<T> S;              // [E][3][nl][nl]
   DevBuf<double> lam;       // [E][3][nl]
-  DevBuf<int2> links;       // [E][6] (nbr, code)
+  DevBuf<uint16_t> S16;     // BF16/FP16 operators, rows in MFMA storage order
+  DevBuf<float> lam32;
+  DevBuf<int4> gmap;        // [E][6] neighbour second layer: base, du, dv (base < 0: none)
+  DevBuf<int4> smap;        // [E][6] neighbour overlap-result slot (ASM)
   DevBuf<T> wts;            // ASM weights [E*Np] (RAS uses invDegree)
   DevBuf<T> boxOut;         // ASM: restricted FDM result [E*Np]
   DevBuf<T> ghostOut;       // ASM: overlap results [E][6][Nq][Nq]
@@ -57,6 +60,7 @@ struct Schwarz {
   // gather-scatter, mask.  Su must not alias u (the RAS kernel reads the
   // neighbours' u while writing its own Su).
   void apply(const T *u, T *Su, hipStream_t s);
+  void finish(T *Su, hipStream_t s);
 };
 
 }  // namespace nrs

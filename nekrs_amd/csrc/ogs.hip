@@ -55,7 +55,7 @@ template <typename T, int OP> struct Red {
 
 constexpr int kGsBlock = 256;
 
-template <typename T, int OP, int W>
+template <typename T, int OP, int W, bool WEIGHTED = false>
 __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__restrict__ v,
                                              int nvec, int64_t stride)
 {
@@ -75,6 +75,10 @@ __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__r
     T val[W];
 #pragma unroll
     for (int c = 0; c < W; ++c) val[c] = vk[m[c]];
+    if constexpr (WEIGHTED) {   // invDegree of a member of a W-row is 1/W, exact for W = 2, 4, 8
+#pragma unroll
+      for (int c = 0; c < W; ++c) val[c] = val[c] * (T(1) / T(W));
+    }
     T g = Red<T, OP>::id();
 #pragma unroll
     for (int c = 0; c < W; ++c) g = Red<T, OP>::apply(g, val[c]);
@@ -114,7 +118,51 @@ gs_kernel(int n2, int n4, int n8, int nx, int b2, int b4, int b8, const int *__r
   }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(kGsBlock)
+gs_weighted_kernel(int n2, int n4, int n8, int nx, int b2, int b4, int b8, const int *__restrict__ pairs,
+                   const int *__restrict__ quads, const int *__restrict__ octs,
+                   const int *__restrict__ xoff, const int *__restrict__ xids,
+                   const T *__restrict__ w, T *__restrict__ v)
+{
+  const int b = blockIdx.x;
+  if (b < b2) {
+    const int r = b * kGsBlock + threadIdx.x;
+    if (r < n2) gs_fixed_row<T, NRS_ADD, 2, true>(pairs + 2 * (int64_t)r, v, 1, 0);
+  } else if (b < b4) {
+    const int r = (b - b2) * kGsBlock + threadIdx.x;
+    if (r < n4) gs_fixed_row<T, NRS_ADD, 4, true>(quads + 4 * (int64_t)r, v, 1, 0);
+  } else if (b < b8) {
+    const int r = (b - b4) * kGsBlock + threadIdx.x;
+    if (r < n8) gs_fixed_row<T, NRS_ADD, 8, true>(octs + 8 * (int64_t)r, v, 1, 0);
+  } else {
+    const int r = (b - b8) * kGsBlock + threadIdx.x;
+    if (r >= nx) return;
+    const int s0 = xoff[r], e0 = xoff[r + 1];
+    T g = T(0);
+    for (int m = s0; m < e0; ++m) g = g + v[xids[m]] * w[xids[m]];
+    for (int m = s0; m < e0; ++m) v[xids[m]] = g;
+  }
+}
+
 }  // namespace
+
+template <typename T>
+void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s)
+{
+  NRS_CHECK(o != nullptr, "ogs: null handle");
+  if (o->Nrows == 0) return;
+  const int b2 = (int)((o->n2 + kGsBlock - 1) / kGsBlock);
+  const int b4 = b2 + (int)((o->n4 + kGsBlock - 1) / kGsBlock);
+  const int b8 = b4 + (int)((o->n8 + kGsBlock - 1) / kGsBlock);
+  const int bt = b8 + (int)((o->nx + kGsBlock - 1) / kGsBlock);
+  gs_weighted_kernel<T><<<bt, kGsBlock, 0, s>>>((int)o->n2, (int)o->n4, (int)o->n8, (int)o->nx, b2, b4,
+                                               b8, o->pairs.get(), o->quads.get(), o->octs.get(),
+                                               o->xoff.get(), o->xids.get(), w, v);
+  NRS_LAUNCH_CHECK();
+}
+template void ogs_gather_scatter_weighted<double>(const nrs_ogs *, double *, const double *, hipStream_t);
+template void ogs_gather_scatter_weighted<float>(const nrs_ogs *, float *, const float *, hipStream_t);
 
 template <typename T>
 void ogs_gather_scatter(const nrs_ogs *o, T *v, int nvec, int64_t stride, int op, hipStream_t s)

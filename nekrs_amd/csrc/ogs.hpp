@@ -25,4 +25,9 @@ namespace nrs {
 template <typename T>
 void ogs_gather_scatter(const nrs_ogs *o, T *v, int nvec, int64_t stride, int op, hipStream_t s);
 nrs_ogs *ogs_setup(int64_t N, const long long *ids);
+// v <- gs_add(w .* v) for one vector, w = the handle's invDegree in T: each
+// member is scaled before its row sum; nodes outside any row are left as they
+// are (their weight is 1).
+template <typename T>
+void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s);
 }  // namespace nrs

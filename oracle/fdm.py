@@ -40,13 +40,22 @@ BIG = 1000000000
 
 
 # ------------------------------------------------------------ rounding
-def round_fmt(x, fmt):
-    """Round float32 values to bf16 / fp16 (nearest-even) and back to float32."""
+def round_fmt(x, fmt, operator=False):
+    """Round float32 values to the 16-bit format (nearest even) and back.
+
+    bf16: 8-bit significand.  fp16: the operators (operator=True) are rounded
+    to IEEE half; data are rounded to an 11-bit significand with an unbounded
+    exponent, which is what half precision gives for data scaled by a power of
+    two per element and contraction (block floating point) as the GPU path does."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     if fmt is None or fmt == "f32":
         return x
     if fmt == "fp16":
-        return x.astype(np.float16).astype(np.float32)
+        if operator:
+            return x.astype(np.float16).astype(np.float32)
+        u = x.view(np.uint32).astype(np.uint64)
+        u = (u + 0xFFF + ((u >> 13) & 1)) & 0xFFFFE000
+        return u.astype(np.uint32).view(np.float32)
     if fmt == "bf16":
         u = x.view(np.uint32).astype(np.uint64)
         u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
@@ -261,7 +270,8 @@ def fdm_box(w, S, invL, T=np.float32, fmt=None):
         t = np.einsum("eim,epnm->epni", Sx, t)
         t = np.einsum("ejn,epni->epji", Sy, t)
         return np.einsum("ekp,epji->ekji", Sz, t).astype(T)
-    Sr = [round_fmt(np.asarray(S[d], dtype=np.float32), fmt).astype(np.float64) for d in range(3)]
+    Sr = [round_fmt(np.asarray(S[d], dtype=np.float32), fmt, operator=True).astype(np.float64)
+          for d in range(3)]
     r = lambda a: round_fmt(a.astype(np.float32), fmt).astype(np.float64)   # noqa: E731
     f32 = lambda a: a.astype(np.float32)                                      # noqa: E731
     t = f32(np.einsum("elm,ekjl->ekjm", Sr[0], r(w)))

@@ -154,3 +154,65 @@ def test_schwarz_default_schedule(gpu, torch_mod):
                       cheby_degree=3)
     lv, _ = el.mg_info()
     assert [l["order"] for l in lv] == [7, 3, 1]
+
+
+FMTS = {"bf16": 1, "fp16": 2}
+
+
+def rel2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "fp16"])
+@pytest.mark.parametrize("S", ["asm", "ras"])
+def test_mfma_smoother_vs_oracle(gpu, torch_mod, orc, fmt, S):
+    """BF16/FP16 contractions on the matrix cores against the oracle's
+    emulation of the same roundings (FP32 accumulation), and against FP32."""
+    import oracle.mg as omg
+    torch = torch_mod
+    mg = gpu.BoxMesh(4, 3, 3, 7, "kershaw", 0.3)
+    eg = gpu.Elliptic(mg, precon="multigrid", solver="fcg", levels=[7, 3, 1],
+                      smoother=SM[("opt4", S)], cheby_degree=3, fdmPrecision=FMTS[fmt])
+    mo = orc.BoxMesh(4, 3, 3, 7, "kershaw", 0.3)
+    om = omg.MG(mo, [7, 3, 1], T=np.float32, degree=3, cheby="opt4", S=S, fmt=fmt)
+    o32 = omg.MG(mo, [7, 3, 1], T=np.float32, degree=3, cheby="opt4", S=S)
+    for lvl in (0, 1):
+        L = om.levels[lvl]
+        x = np.random.default_rng(10 + lvl).uniform(-1, 1, L.E * L.Np)
+        x[L.el.maskIds] = 0
+        x = L.el.ogs.gs(x, "add").astype(np.float32)
+        ref = L.schwarz.apply(x)
+        ref32 = o32.levels[lvl].schwarz.apply(x)
+        out = torch.zeros(L.E * L.Np, dtype=torch.float32, device="cuda")
+        eg.mg_smoother(lvl, dev(torch, x), out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        e_emul, e_f32 = rel2(got, ref), rel2(got, ref32)
+        print(fmt, S, lvl, "vs emulation %.2e, vs FP32 %.2e, emulation vs FP32 %.2e" %
+              (e_emul, e_f32, rel2(ref, ref32)))
+        assert e_emul < (2e-3 if fmt == "bf16" else 3e-4)
+        assert e_emul < 0.5 * rel2(ref, ref32)      # closer to the emulation than to FP32
+        assert np.all(got[L.el.maskIds] == 0)
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "fp16"])
+def test_mg_mfma_fcg_vs_oracle(gpu, torch_mod, orc, fmt):
+    import oracle.mg as omg
+    torch = torch_mod
+    mg = gpu.BoxMesh(4, 4, 4, 7, "kershaw", 0.3)
+    eg = gpu.Elliptic(mg, precon="multigrid", solver="fcg", levels=[7, 3, 1], tol=1e-10,
+                      max_iter=200, smoother="opt4chebyshev+ras", cheby_degree=3,
+                      fdmPrecision=FMTS[fmt])
+    mo = orc.BoxMesh(4, 4, 4, 7, "kershaw", 0.3)
+    eo = orc.Elliptic(mo)
+    om = omg.MG(mo, [7, 3, 1], T=np.float32, degree=3, cheby="opt4", S="ras", fmt=fmt)
+    rhs = np.random.default_rng(7).uniform(-1, 1, eg.Nlocal)
+    xo, info = omg.fcg_solve(eo, rhs, om.apply, flexible=True, tol=1e-10, maxit=200)
+    x = torch.zeros(eg.Nlocal, dtype=torch.float64, device="cuda")
+    st = eg.solve(dev(torch, rhs), x)
+    print(fmt, "gpu it", st["iterations"], "oracle it", info["iterations"])
+    assert st["converged"]
+    assert abs(st["iterations"] - info["iterations"]) <= 1
+    assert rel(x.cpu().numpy(), xo) <= 1e-8
