@@ -200,6 +200,7 @@ typedef struct {
   int fdmPrecision;              /* nrs_fdm_precision                           */
   unsigned long long seed;       /* Arnoldi start vector seed                   */
   int useGraphs;                 /* replay captured HIP graphs for the coarse PCG */
+  int autotune;                  /* KERNEL AUTOTUNING: time launch variants at setup */
 } nrs_elliptic_opts;
 
 typedef struct {
@@ -284,6 +285,42 @@ int nrs_fdm_element_lengths(int N, int Nelements, const double *x, const double 
  * its order-Nf ids (host arrays [E*(Nf+1)^3] -> [E*(Nc+1)^3]); replaces Nek5000's
  * numbering of the multigrid meshes (createMeshMG -> meshGlobalIds). */
 int nrs_coarse_global_ids(int Nf, int Nelements, const long long *idsFine, int Nc, long long *idsCoarse);
+
+/* ------------------------------------------------------------------------- */
+/* Batched independent solves                                                 */
+/* ------------------------------------------------------------------------- */
+/* No counterpart in the reference (one elliptic_t per field and rank there).
+ * n independent problems (any order, size, lambda, options), each set up as by
+ * nrs_elliptic_setup; multigrid problems must use NRS_COARSE_SMOOTH (the solve
+ * is device-driven).  Setup captures itersPerLaunch iterations of all problems
+ * into one HIP graph (problems as parallel branches). */
+typedef struct nrs_batch nrs_batch;
+int nrs_batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic_opts *opts,
+                    int itersPerLaunch, nrs_batch **out);
+/* Solve all problems: rhs[p], x[p] device arrays as in nrs_elliptic_solve
+ * (x: initial guess in, solution out; rhs: final residual out); stats[n].
+ * The graph is replayed until every problem has converged or reached maxIter. */
+int nrs_batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_stats *stats,
+                    void *stream);
+/* Problem count, nodes of the captured graph, graph launches of the last solve. */
+int nrs_batch_info(nrs_batch *b, int *n, int64_t *graphNodes, int *lastLaunches);
+int nrs_batch_free(nrs_batch *b);
+
+/* ------------------------------------------------------------------------- */
+/* Kernel launch variants (KERNEL AUTOTUNING; the reference times p_knl       */
+/* variants at setup: src/bench/axHelm/benchmarkAx.cpp:268-383,               */
+/* src/bench/fdm/benchmarkFDM.cpp:180-210)                                    */
+/* ------------------------------------------------------------------------- */
+/* Variants: NRS_TUNE_AX = elements-per-block multiplier {1, 2, 4} of the
+ * operator kernel, keyed by dtype (nrs_dtype); NRS_TUNE_FDM = elements per
+ * wavefront {1, 2, 4, 8} of the matrix-core Schwarz kernel, keyed by the
+ * nrs_fdm_precision.  Keys are (kernel, N, type, floor(log2 Nelements)); a
+ * setup with opts.autotune records the fastest variant whose result matches
+ * the default's.  variant 0 = none recorded (the default is used). */
+enum nrs_tune_kernel { NRS_TUNE_AX = 0, NRS_TUNE_FDM = 1 };
+int nrs_tuned_variant(int kernel, int N, int type, int64_t Nelements, int *variant);
+/* Record a variant by hand (0 clears it); later launches with this key use it. */
+int nrs_set_variant(int kernel, int N, int type, int64_t Nelements, int variant);
 
 /* ------------------------------------------------------------------------- */
 /* Device timers (replaces platform->timer device tags, src/core/timer.cpp)   */

@@ -21,7 +21,7 @@ from ._lib import NrsError
 
 __all__ = ["NrsError", "init", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
            "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh", "fdm_1d",
-           "fdm_element_lengths"]
+           "fdm_element_lengths", "Batch", "make_desc", "make_opts", "tuned_variant", "set_variant"]
 
 OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
 MAPS = {"none": 0, "sine": 1, "kershaw": 2}
@@ -31,6 +31,7 @@ SMOOTHERS = {"chebyshev+jacobi": 0, "opt4chebyshev+jacobi": 1, "chebyshev+asm": 
              "opt4chebyshev+asm": 3, "asm": 4, "ras": 5, "chebyshev+ras": 6,
              "opt4chebyshev+ras": 7}
 FDM_PRECISIONS = {"f32": 0, "bf16": 1, "fp16": 2}
+DTYPES = {"f64": _lib.F64, "f32": _lib.F32}
 
 
 def _torch():
@@ -218,36 +219,113 @@ def default_opts():
     return o
 
 
+COARSE = {"pcg": 0, "smooth": 1}
+
+
+def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True):
+    """nrs_elliptic_desc of a mesh (host arrays stay owned by the mesh)."""
+    return _lib.EllipticDesc(mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
+                             mesh.z.ctypes.data, mesh.gid.ctypes.data, mesh.EToB.ctypes.data,
+                             int(poisson), float(lambda0), float(lambda1))
+
+
+def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=500, levels=None,
+              smoother="chebyshev+jacobi", cheby_degree=2, cheby_min=0.1, cheby_max=1.1,
+              coarse_tol=1e-2, coarse_max_iter=200, seed=0, coarse="pcg", **extra):
+    """nrs_elliptic_opts from keyword options (extra: raw field names)."""
+    o = default_opts()
+    o.solver = SOLVERS[solver]
+    o.preconditioner = PRECONS[precon]
+    o.tol = tol
+    o.relative = int(relative)
+    o.maxIter = max_iter
+    if levels:
+        o.nLevels = len(levels)
+        for i, l in enumerate(levels):
+            o.levels[i] = l
+    o.smoother = SMOOTHERS[smoother]
+    o.chebyDegree = cheby_degree
+    o.chebyMinFactor = cheby_min
+    o.chebyMaxFactor = cheby_max
+    o.coarseTol = coarse_tol
+    o.coarseMaxIter = coarse_max_iter
+    o.coarseSolver = COARSE[coarse]
+    o.seed = seed
+    for k, v in extra.items():
+        setattr(o, k, v)
+    return o
+
+
+TUNE_KERNELS = {"ax": 0, "fdm": 1}
+
+
+def tuned_variant(kernel, N, dtype, nelements):
+    """Recorded launch variant for (kernel, N, type, E), 0 if none (nrs_tuned_variant).
+    type: a DTYPES name for "ax", an FDM_PRECISIONS name for "fdm"."""
+    t = DTYPES[dtype] if kernel == "ax" else FDM_PRECISIONS[dtype]
+    v = C.c_int()
+    _lib.call("nrs_tuned_variant", TUNE_KERNELS[kernel], int(N), t, int(nelements), C.byref(v))
+    return v.value
+
+
+def set_variant(kernel, N, dtype, nelements, variant):
+    """Force a launch variant (0 clears) for later launches with this key."""
+    t = DTYPES[dtype] if kernel == "ax" else FDM_PRECISIONS[dtype]
+    _lib.call("nrs_set_variant", TUNE_KERNELS[kernel], int(N), t, int(nelements), int(variant))
+
+
+class Batch:
+    """Independent solves advanced together by one captured graph (nrs_batch_*).
+
+    problems: list of (mesh, dict(lambda0=, lambda1=, poisson=, **solver options))."""
+
+    def __init__(self, problems, iters_per_launch=1):
+        n = len(problems)
+        self.meshes = [m for m, _ in problems]
+        descs = (_lib.EllipticDesc * n)()
+        opts = (_lib.EllipticOpts * n)()
+        for i, (m, kw) in enumerate(problems):
+            kw = dict(kw)
+            lam = {k: kw.pop(k) for k in ("lambda0", "lambda1", "poisson") if k in kw}
+            descs[i] = make_desc(m, **lam)
+            opts[i] = make_opts(**kw)
+        h = C.c_void_p()
+        _lib.call("nrs_batch_setup", n, descs, opts, int(iters_per_launch), C.byref(h))
+        self._h = h
+        self.n = n
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().nrs_batch_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def info(self):
+        n, nodes, launches = C.c_int(), C.c_int64(), C.c_int()
+        _lib.call("nrs_batch_info", self._h, C.byref(n), C.byref(nodes), C.byref(launches))
+        return {"problems": n.value, "graph_nodes": nodes.value, "last_launches": launches.value}
+
+    def solve(self, rhs, x, stream=None):
+        """In place per problem (as Elliptic.solve); returns a list of stats dicts."""
+        n = self.n
+        rp = (C.c_void_p * n)(*[_ptr(t) for t in rhs])
+        xp = (C.c_void_p * n)(*[_ptr(t) for t in x])
+        st = (_lib.SolveStats * n)()
+        _lib.call("nrs_batch_solve", self._h, rp, xp, st, _stream(stream))
+        return [{"iterations": s.iterations, "converged": bool(s.converged), "res0": s.res0,
+                 "res": s.res, "seconds": s.seconds} for s in st]
+
+
 class Elliptic:
     """elliptic_t for one scalar field (ellipticSolveSetup / ellipticSolve)."""
 
-    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, solver="pcg",
-                 precon="jacobi", tol=1e-8, relative=True, max_iter=500, levels=None,
-                 smoother="chebyshev+jacobi", cheby_degree=2, cheby_min=0.1, cheby_max=1.1,
-                 coarse_tol=1e-2, coarse_max_iter=200, seed=0, **extra):
+    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, **kw):
         self.mesh = mesh
-        d = _lib.EllipticDesc(mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
-                              mesh.z.ctypes.data, mesh.gid.ctypes.data, mesh.EToB.ctypes.data,
-                              int(poisson), float(lambda0), float(lambda1))
-        o = default_opts()
-        o.solver = SOLVERS[solver]
-        o.preconditioner = PRECONS[precon]
-        o.tol = tol
-        o.relative = int(relative)
-        o.maxIter = max_iter
-        if levels:
-            o.nLevels = len(levels)
-            for i, l in enumerate(levels):
-                o.levels[i] = l
-        o.smoother = SMOOTHERS[smoother]
-        o.chebyDegree = cheby_degree
-        o.chebyMinFactor = cheby_min
-        o.chebyMaxFactor = cheby_max
-        o.coarseTol = coarse_tol
-        o.coarseMaxIter = coarse_max_iter
-        o.seed = seed
-        for k, v in extra.items():
-            setattr(o, k, v)
+        d = make_desc(mesh, lambda0, lambda1, poisson)
+        o = make_opts(**kw)
         self.opts = o
         h = C.c_void_p()
         _lib.call("nrs_elliptic_setup", C.byref(d), C.byref(o), C.byref(h))

@@ -36,7 +36,7 @@ class EllipticOpts(C.Structure):
                 ("chebyMinFactor", C.c_double), ("chebyMaxFactor", C.c_double),
                 ("coarseSolver", C.c_int), ("coarseTol", C.c_double), ("coarseMaxIter", C.c_int),
                 ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong),
-                ("useGraphs", C.c_int)]
+                ("useGraphs", C.c_int), ("autotune", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -78,6 +78,12 @@ SIGNATURES = {
     "nrs_fdm_1d": ([_i, _i, _i, _d, _d, _d, _vp, _vp], _i),
     "nrs_fdm_element_lengths": ([_i, _i, _vp, _vp, _vp, _vp, _vp], _i),
     "nrs_coarse_global_ids": ([_i, _i, _vp, _i, _vp], _i),
+    "nrs_batch_setup": ([_i, _vp, _vp, _i, C.POINTER(_vp)], _i),
+    "nrs_batch_solve": ([_vp, _vp, _vp, _vp, _vp], _i),
+    "nrs_batch_info": ([_vp, C.POINTER(_i), C.POINTER(_i64), C.POINTER(_i)], _i),
+    "nrs_batch_free": ([_vp], _i),
+    "nrs_tuned_variant": ([_i, _i, _i, _i64, C.POINTER(_i)], _i),
+    "nrs_set_variant": ([_i, _i, _i, _i64, _i], _i),
     "nrs_elliptic_profile_enable": ([_vp, _i], _i),
     "nrs_elliptic_profile_read": ([_vp, C.POINTER(_d), C.POINTER(_i64)], _i),
 }

@@ -12,6 +12,7 @@
 // Small orders pack several elements per block so that a block is >= 64 lanes.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "tune.hpp"
 
 namespace nrs {
 
@@ -30,9 +31,9 @@ template <typename T> __device__ __forceinline__ T stream_load(const T *p)
   return __builtin_nontemporal_load(p);
 }
 
-template <int NQ> struct AxShape {
-  // elements per block: fill at least one wave64
-  static constexpr int NE = (NQ * NQ >= 64) ? 1 : (64 + NQ * NQ - 1) / (NQ * NQ);
+template <int NQ, int MULT = 1> struct AxShape {
+  // elements per block: fill at least one wave64, times the tuned multiplier
+  static constexpr int NE = ((NQ * NQ >= 64) ? 1 : (64 + NQ * NQ - 1) / (NQ * NQ)) * MULT;
   static constexpr int THREADS = NQ * NQ * NE;
 };
 
@@ -43,13 +44,13 @@ template <int NQ> struct AxShape {
 //   [Gqr,Gqs,Gqt] = lambda0 * G * [qr,qs,qt]
 //   Aq = sum_m D[m][i] Gqr(m,j,k) + D[m][j] Gqs(i,m,k) + D[m][k] Gqt(i,j,m) (+ GwJ lambda1 q)
 // -------------------------------------------------------------------------------------
-template <typename T, int NQ, bool POISSON, bool LFIELD>
-__global__ void __launch_bounds__(AxShape<NQ>::THREADS, AX_WAVES_PER_EU)
+template <typename T, int NQ, int MULT, bool POISSON, bool LFIELD>
+__global__ void __launch_bounds__((AxShape<NQ, MULT>::THREADS), AX_WAVES_PER_EU)
 ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
           const T *__restrict__ Dg, const T *__restrict__ lam0, const T *__restrict__ lam1,
           const T *__restrict__ q, T *__restrict__ Aq, const uint32_t *__restrict__ maskBits)
 {
-  constexpr int NE = AxShape<NQ>::NE;
+  constexpr int NE = AxShape<NQ, MULT>::NE;
   constexpr int NP = NQ * NQ * NQ;
   constexpr int NW = (NP + 31) / 32;
   __shared__ T s_D[NQ][NQ];
@@ -64,7 +65,7 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
   const int eidx = blockIdx.x * NE + le;
   const bool active = eidx < E;
   int64_t e = active ? (elist ? elist[eidx] : eidx) : 0;
-  if (NE == 1) e = __builtin_amdgcn_readfirstlane((int)e);   // uniform element -> scalar base
+  if (NE == 1 || NQ * NQ == 64) e = __builtin_amdgcn_readfirstlane((int)e);   // wave-uniform element
 
   if (t < NQ * NQ) s_D[t / NQ][t % NQ] = Dg[t];
   __syncthreads();
@@ -166,43 +167,67 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
   }
 }
 
-template <typename T, int NQ>
-static void launch_ax_nq(int E, const int *elist, const T *ggeo, const T *D, const T *l0,
-                         const T *l1, bool poisson, bool lfield, const T *q, T *Aq,
-                         const uint32_t *maskBits, hipStream_t s)
+template <typename T, int NQ, int MULT>
+static void launch_ax_nqm(int E, const int *elist, const T *ggeo, const T *D, const T *l0,
+                          const T *l1, bool poisson, bool lfield, const T *q, T *Aq,
+                          const uint32_t *maskBits, hipStream_t s)
 {
-  using S = AxShape<NQ>;
+  using S = AxShape<NQ, MULT>;
   const unsigned grid = (unsigned)((E + S::NE - 1) / S::NE);
   if (grid == 0) return;
   if (poisson && !lfield)
-    ax_kernel<T, NQ, true, false><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+    ax_kernel<T, NQ, MULT, true, false><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
   else if (poisson && lfield)
-    ax_kernel<T, NQ, true, true><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+    ax_kernel<T, NQ, MULT, true, true><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
   else if (!poisson && !lfield)
-    ax_kernel<T, NQ, false, false><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+    ax_kernel<T, NQ, MULT, false, false><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
   else
-    ax_kernel<T, NQ, false, true><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
+    ax_kernel<T, NQ, MULT, false, true><<<grid, S::THREADS, 0, s>>>(E, elist, ggeo, D, l0, l1, q, Aq, maskBits);
   NRS_LAUNCH_CHECK();
 }
 
+template <typename T, int NQ>
+static void launch_ax_nq(int mult, int E, const int *elist, const T *ggeo, const T *D, const T *l0,
+                         const T *l1, bool poisson, bool lfield, const T *q, T *Aq,
+                         const uint32_t *maskBits, hipStream_t s)
+{
+  switch (mult) {
+    case 2: launch_ax_nqm<T, NQ, 2>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    case 4: launch_ax_nqm<T, NQ, 4>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    default: launch_ax_nqm<T, NQ, 1>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+  }
+}
+
+template <typename T>
+void launch_ax_variant(int N, int mult, int E, const int *elist, const T *ggeo, const T *D,
+                       const T *l0, const T *l1, bool poisson, bool lfield, const T *q, T *Aq,
+                       const uint32_t *maskBits, hipStream_t s)
+{
+  switch (N + 1) {
+#define NRS_AX_CASE(NQ_)                                                                          \
+  case NQ_: launch_ax_nq<T, NQ_>(mult, E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
+    NRS_AX_CASE(2) NRS_AX_CASE(3) NRS_AX_CASE(4) NRS_AX_CASE(5) NRS_AX_CASE(6) NRS_AX_CASE(7)
+    NRS_AX_CASE(8) NRS_AX_CASE(9) NRS_AX_CASE(10)
+#undef NRS_AX_CASE
+    default: throw Error(NRS_ERR_UNSUPPORTED, "Ax: order N must be 1..9");
+  }
+}
+
+// elements-per-block multiplier: the autotuner's choice for (N, type), else 1
 template <typename T>
 void launch_ax(int N, int E, const int *elist, const T *ggeo, const T *D, const T *l0, const T *l1,
                bool poisson, bool lfield, const T *q, T *Aq, const uint32_t *maskBits,
                hipStream_t s)
 {
-  switch (N + 1) {
-    case 2: launch_ax_nq<T, 2>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 3: launch_ax_nq<T, 3>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 4: launch_ax_nq<T, 4>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 5: launch_ax_nq<T, 5>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 6: launch_ax_nq<T, 6>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 7: launch_ax_nq<T, 7>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 8: launch_ax_nq<T, 8>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 9: launch_ax_nq<T, 9>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    case 10: launch_ax_nq<T, 10>(E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s); break;
-    default: throw Error(NRS_ERR_UNSUPPORTED, "Ax: order N must be 1..9");
-  }
+  const int mult = tuned(TUNE_AX, N, dtype_of<T>::value, E, 1);
+  launch_ax_variant<T>(N, mult, E, elist, ggeo, D, l0, l1, poisson, lfield, q, Aq, maskBits, s);
 }
+template void launch_ax_variant<double>(int, int, int, const int *, const double *, const double *,
+                                        const double *, const double *, bool, bool, const double *,
+                                        double *, const uint32_t *, hipStream_t);
+template void launch_ax_variant<float>(int, int, int, const int *, const float *, const float *,
+                                       const float *, const float *, bool, bool, const float *,
+                                       float *, const uint32_t *, hipStream_t);
 
 template void launch_ax<double>(int, int, const int *, const double *, const double *, const double *,
                                 const double *, bool, bool, const double *, double *,

@@ -1,0 +1,144 @@
+// batch.hip -- many independent elliptic solves of uneven size and order,
+// advanced together by one captured HIP graph.
+//
+// Each problem keeps its own handle (operator, gather-scatter, multigrid with a
+// smoothed coarse level, device scalars).  Setup captures, once, `k`
+// iterations of every problem into one graph: the problems are independent
+// branches (fork/join through events), so the graph executor overlaps the
+// small problems' kernels; a last node gathers the problems' "active" flags
+// into one array.  A solve runs each problem's start-up work, then replays the
+// graph until no problem is active (one 8-byte-per-problem readback per replay),
+// then adds the initial guesses back.  Iterations of a converged problem leave
+// its state unchanged (its update kernels test the device flag), so a problem's
+// result equals its stand-alone device-driven solve.
+//
+// The reference runs one elliptic_t per field and rank (src/solvers/elliptic/
+// ellipticSolve.cpp:32-190); batching independent problems has no counterpart there.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "batch.hpp"
+#include "elliptic.hpp"
+#include "kernels.hpp"
+#include "linalg.hpp"
+#include "mg.hpp"
+
+namespace nrs {
+
+namespace {
+
+__global__ void gather_flags_kernel(int n, double *const *scal, double *flags)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) flags[p] = scal[p][S_ACTIVE];
+}
+
+}  // namespace
+
+nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic_opts *opts,
+                       int itersPerLaunch)
+{
+  NRS_CHECK(n >= 1, "batch: at least one problem");
+  NRS_CHECK(itersPerLaunch >= 1 && itersPerLaunch <= 64, "batch: iterations per launch 1..64");
+  auto b = std::make_unique<nrs_batch>();
+  b->itersPerLaunch = itersPerLaunch;
+  for (int p = 0; p < n; ++p) {
+    NRS_CHECK(!(opts[p].preconditioner == NRS_PRECON_MULTIGRID && opts[p].coarseSolver != NRS_COARSE_SMOOTH),
+              "batch: multigrid problems need the smoothed coarse level (no host round trip)");
+    b->probs.push_back(elliptic_setup(&descs[p], &opts[p]));
+    const int64_t L = b->probs.back()->Nlocal;
+    b->r.emplace_back(L);
+    b->x.emplace_back(L);
+  }
+  std::vector<double *> sp;
+  for (auto *e : b->probs) sp.push_back(e->scal.get());
+  b->scalPtrs.upload(sp);
+  b->flags.alloc(n);
+  NRS_HIP(hipHostMalloc((void **)&b->flagsHost, n * sizeof(double), hipHostMallocDefault));
+  b->streams.resize(n);
+  b->joins.resize(n);
+  for (int p = 0; p < n; ++p) {
+    NRS_HIP(hipStreamCreateWithFlags(&b->streams[p], hipStreamNonBlocking));
+    NRS_HIP(hipEventCreateWithFlags(&b->joins[p], hipEventDisableTiming));
+  }
+  NRS_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  NRS_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+  // one graph: itersPerLaunch iterations of every problem, problems in parallel
+  NRS_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+  NRS_HIP(hipEventRecord(b->fork, b->cap));
+  for (int p = 0; p < n; ++p) {
+    hipStream_t st = b->streams[p];
+    NRS_HIP(hipStreamWaitEvent(st, b->fork, 0));
+    for (int it = 0; it < itersPerLaunch; ++it)
+      elliptic_iterate(b->probs[p], b->r[p].get(), b->x[p].get(), st);
+    NRS_HIP(hipEventRecord(b->joins[p], st));
+    NRS_HIP(hipStreamWaitEvent(b->cap, b->joins[p], 0));
+  }
+  gather_flags_kernel<<<(n + 63) / 64, 64, 0, b->cap>>>(n, b->scalPtrs.get(), b->flags.get());
+  NRS_LAUNCH_CHECK();
+  NRS_HIP(hipStreamEndCapture(b->cap, &b->graph));
+  NRS_HIP(hipGraphGetNodes(b->graph, nullptr, &b->graphNodes));
+  NRS_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return b.release();
+}
+
+int batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_stats *st, hipStream_t s)
+{
+  const int n = (int)b->probs.size();
+  auto t0 = std::chrono::steady_clock::now();
+  int maxit = 0;
+  for (int p = 0; p < n; ++p) {
+    nrs_elliptic *e = b->probs[p];
+    const int64_t L = e->Nlocal;
+    NRS_HIP(hipMemcpyAsync(b->r[p].get(), rhs[p], L * sizeof(double), hipMemcpyDeviceToDevice, s));
+    NRS_HIP(hipMemcpyAsync(b->x[p].get(), x[p], L * sizeof(double), hipMemcpyDeviceToDevice, s));
+    elliptic_solve_begin(e, b->r[p].get(), b->x[p].get(), s);
+    maxit = std::max(maxit, e->opts.maxIter);
+  }
+  const int maxLaunches = (maxit + b->itersPerLaunch - 1) / b->itersPerLaunch;
+  int launches = 0;
+  for (; launches < maxLaunches; ++launches) {
+    NRS_HIP(hipGraphLaunch(b->exec, s));
+    NRS_HIP(hipMemcpyAsync(b->flagsHost, b->flags.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    bool any = false;
+    for (int p = 0; p < n; ++p) any |= (b->flagsHost[p] != 0.0);
+    if (!any) {
+      ++launches;
+      break;
+    }
+  }
+  int status = NRS_OK;
+  for (int p = 0; p < n; ++p) {
+    nrs_elliptic *e = b->probs[p];
+    elliptic_solve_end(e, b->x[p].get(), s);
+    NRS_HIP(hipMemcpyAsync(x[p], b->x[p].get(), e->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
+    NRS_HIP(hipMemcpyAsync(rhs[p], b->r[p].get(), e->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
+    NRS_HIP(hipMemcpyAsync(e->scalHost, e->scal.get(), S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  NRS_HIP(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::string nanProblems;
+  for (int p = 0; p < n; ++p) {
+    const double *h = b->probs[p]->scalHost;
+    if (std::isnan(h[S_RDOTR]) || std::isnan(h[S_TMP3])) {
+      status = NRS_ERR_NAN;
+      nanProblems += (nanProblems.empty() ? "" : ", ") + std::to_string(p);
+    }
+    if (st) {
+      st[p].iterations = (int)h[S_ITER];
+      st[p].res0 = h[S_TMP3];
+      st[p].res = h[S_RDOTR];
+      st[p].converged = (h[S_RDOTR] <= h[S_TOL]) ? 1 : 0;
+      st[p].seconds = secs;
+      st[p].precon_seconds = 0.0;
+    }
+  }
+  b->lastLaunches = launches;
+  if (status != NRS_OK) set_last_error("batch solve: NaN residual norm in problem(s) " + nanProblems);
+  return status;
+}
+
+}  // namespace nrs

@@ -1,0 +1,34 @@
+// batch.hpp -- independent solves advanced by one captured graph (internal layout of nrs_batch).
+#pragma once
+#include <vector>
+
+#include "elliptic.hpp"
+
+struct nrs_batch {
+  std::vector<nrs_elliptic *> probs;
+  std::vector<nrs::DevBuf<double>> r, x;     // internal rhs / solution per problem (graph-fixed)
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t graphNodes = 0;
+  int itersPerLaunch = 1;
+  int lastLaunches = 0;
+  nrs::DevBuf<double *> scalPtrs;
+  nrs::DevBuf<double> flags;
+  double *flagsHost = nullptr;
+  ~nrs_batch()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : joins) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto st : streams) (void)hipStreamDestroy(st);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (flagsHost) (void)hipHostFree(flagsHost);
+    for (auto *p : probs) delete p;
+  }
+};
+

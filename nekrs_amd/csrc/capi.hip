@@ -4,7 +4,9 @@
 #include <cstring>
 #include <string>
 
+#include "batch.hpp"
 #include "elliptic.hpp"
+#include "tune.hpp"
 #include "fdm.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
@@ -265,6 +267,7 @@ void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
   o->fdmPrecision = NRS_FDM_F32;
   o->seed = 0;
   o->useGraphs = 1;
+  o->autotune = 1;
 }
 
 int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts, nrs_elliptic **out)
@@ -358,6 +361,63 @@ int nrs_elliptic_mg_info(nrs_elliptic *el, int max, int *nLevels, int *orders, i
     if (degree) degree[i] = info[i].degree;
   }
   if (coarseIterations) *coarseIterations = el->mg->coarse_iterations();
+  NRS_API_END
+}
+
+int nrs_batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic_opts *opts,
+                    int itersPerLaunch, nrs_batch **out)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(descs && opts && out, "batch setup: null argument");
+  *out = nullptr;
+  *out = nrs::batch_setup(n, descs, opts, itersPerLaunch);
+  NRS_API_END
+}
+
+int nrs_batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_stats *stats,
+                    void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(b && rhs && x, "batch solve: null argument");
+  return nrs::batch_solve(b, rhs, x, stats, nrs::as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_batch_info(nrs_batch *b, int *n, int64_t *graphNodes, int *lastLaunches)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(b, "batch info: null handle");
+  if (n) *n = (int)b->probs.size();
+  if (graphNodes) *graphNodes = (int64_t)b->graphNodes;
+  if (lastLaunches) *lastLaunches = b->lastLaunches;
+  NRS_API_END
+}
+
+int nrs_batch_free(nrs_batch *b)
+{
+  NRS_API_BEGIN
+  delete b;
+  NRS_API_END
+}
+
+int nrs_tuned_variant(int kernel, int N, int type, int64_t Nelements, int *variant)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(variant != nullptr, "tuned variant: null output");
+  NRS_CHECK(kernel == NRS_TUNE_AX || kernel == NRS_TUNE_FDM, "tuned variant: unknown kernel");
+  NRS_CHECK(N >= 1 && N <= 9 && type >= 0 && type < nrs::TUNE_TYPES && Nelements >= 1,
+            "tuned variant: bad key");
+  *variant = nrs::tuned(kernel, N, type, Nelements, 0);
+  NRS_API_END
+}
+
+int nrs_set_variant(int kernel, int N, int type, int64_t Nelements, int variant)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(kernel == NRS_TUNE_AX || kernel == NRS_TUNE_FDM, "set variant: unknown kernel");
+  NRS_CHECK(N >= 1 && N <= 9 && type >= 0 && type < nrs::TUNE_TYPES && Nelements >= 1,
+            "set variant: bad key");
+  nrs::set_tuned(kernel, N, type, Nelements, variant);
   NRS_API_END
 }
 

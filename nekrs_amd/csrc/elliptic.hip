@@ -4,6 +4,7 @@
 #include <cmath>
 
 #include "elliptic.hpp"
+#include "tune.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "mg.hpp"
@@ -139,6 +140,11 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     el->ogs = ogs_setup(el->Nlocal, mg.data());
   }
 
+  // operator launch variants (KERNEL AUTOTUNING, registerEllipticKernels.cpp:170-183)
+  if (o->autotune)
+    autotune_ax<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(), el->lam1.get(),
+                        el->poisson, s);
+
   // workspace
   el->p.alloc(el->Nlocal); el->zv.alloc(el->Nlocal); el->Ap.alloc(el->Nlocal); el->x0.alloc(el->Nlocal);
   NRS_HIP(hipEventCreate(&el->ev0));
@@ -266,6 +272,52 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
     st->precon_seconds = precon_s;
   }
   return status;
+}
+
+// ---- the same solve as enqueued work only (no host synchronisation), for
+// callers that capture iterations into a graph.  begin: r = gs(mask(rhs - A x)),
+// x0 = x, x = 0, res0 and tolerance on the device; iterate: one (flexible) PCG
+// iteration, a no-op once the solve has converged or reached maxIter (the
+// update kernels test the device flag; the preconditioner and operator still
+// run); end: x += x0.  The preconditioner must not synchronise (multigrid with a
+// smoothed coarse level, Jacobi, or none).
+void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t s)
+{
+  const int64_t L = el->Nlocal;
+  NRS_CHECK(!(el->mg && !el->mg->device_only()),
+            "device-driven solve: the multigrid coarse level must be smoothed, not solved");
+  launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                    el->lam1.get(), el->poisson, false, x, el->Ap.get(), nullptr, s);
+  axpby<double>(L, -1.0, el->Ap.get(), 1.0, rhs, s);
+  launch_mask<double>(el->Nmasked, el->maskIds.get(), rhs, s);
+  ogs_gather_scatter<double>(el->ogs, rhs, 1, L, NRS_ADD, s);
+  axpby<double>(L, 1.0, x, 0.0, el->x0.get(), s);
+  fill<double>(L, 0.0, x, s);
+  wnorm(*el->red, L, el->ogs->invDegree.get(), rhs, 1.0 / el->volume, el->scal.get(), S_RDOTR, s);
+  pcg_init_device(el->scal.get(), el->opts.tol, el->opts.relative != 0, el->opts.maxIter, s);
+  fill<double>(L, 0.0, el->p.get(), s);
+}
+
+void elliptic_iterate(nrs_elliptic *el, double *r, double *x, hipStream_t s)
+{
+  const int64_t L = el->Nlocal;
+  const double *w = el->ogs->invDegree.get();
+  const auto &o = el->opts;
+  const bool none = o.preconditioner == NRS_PRECON_NONE;
+  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI;
+  double *z = none ? r : el->zv.get();
+  if (!none && !jacobi) elliptic_preconditioner(el, r, z, s);
+  pcg_precon_dots(*el->red, L, w, el->invDiagA.get(), r, z, el->Ap.get(), jacobi,
+                  o.solver == NRS_SOLVER_FCG, none, el->scal.get(), s);
+  pcg_update_p(L, z, el->p.get(), el->scal.get(), s);
+  elliptic_operator(el, el->p.get(), el->Ap.get(), true, s);
+  pcg_alpha(*el->red, L, w, el->p.get(), el->Ap.get(), el->scal.get(), s);
+  pcg_update_rx(*el->red, L, w, el->Ap.get(), el->p.get(), r, x, 1.0 / el->volume, el->scal.get(), s);
+}
+
+void elliptic_solve_end(nrs_elliptic *el, double *x, hipStream_t s)
+{
+  axpby<double>(el->Nlocal, 1.0, el->x0.get(), 1.0, x, s);
 }
 
 }  // namespace nrs

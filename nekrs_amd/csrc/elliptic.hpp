@@ -56,6 +56,14 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
 void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s);
 void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s);
 int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s);
+// Device-driven pieces of the same solve (no host synchronisation).
+void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t s);
+void elliptic_iterate(nrs_elliptic *el, double *r, double *x, hipStream_t s);
+void elliptic_solve_end(nrs_elliptic *el, double *x, hipStream_t s);
+
+nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic_opts *opts,
+                       int itersPerLaunch);
+int batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_stats *st, hipStream_t s);
 
 // Shared helpers for the multigrid levels.
 std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds);

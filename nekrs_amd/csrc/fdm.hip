@@ -31,6 +31,7 @@
 #include <type_traits>
 
 #include "fdm.hpp"
+#include "tune.hpp"
 #include "kernels.hpp"
 #include "linalg.hpp"
 #include "mesh.hpp"
@@ -962,13 +963,15 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
   flush(prev);
 }
 
+// epw: elements per wavefront (the tuned variant; about four by default, so
+// the prefetch has something to overlap)
 template <int NQ, int FMT>
 void launch_fdm_mfma_nq(bool ras, int E, const float *u, const uint16_t *S16, const float *lam,
-                        const int4 *gmap, float *out, float *gout, hipStream_t s)
+                        const int4 *gmap, float *out, float *gout, int epw, hipStream_t s)
 {
   using C = MfmaCfg<NQ>;
-  // about four elements per wavefront, so the prefetch has something to overlap
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)E + 15) / 16, 8192));
+  const int64_t perBlock = (int64_t)C::WAVES * std::max(1, epw);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((E + perBlock - 1) / perBlock, 65536));
   if (ras)
     fdm_mfma_kernel<NQ, FMT, false><<<grid, 64 * C::WAVES, 0, s>>>(E, u, S16, lam, gmap, out, gout);
   else
@@ -1003,7 +1006,8 @@ void launch_post_nq(int E, const T *box, const T *gout, const int4 *smap, const 
 template <typename T>
 void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<double> &lengths,
                        const int *EToB, const std::vector<FaceLink> &lk, const nrs_ogs *ogs_,
-                       const T *invDegree_, int64_t Nmasked_, const int *maskIds_, hipStream_t s)
+                       const T *invDegree_, int64_t Nmasked_, const int *maskIds_, bool poisson,
+                       double lambda0, double lambda1, hipStream_t s)
 {
   N = N_; E = E_; ras = ras_; fdmPrecision = prec;
   NRS_CHECK(N >= 1 && N <= 9, "schwarz: orders 1..9 are built");
@@ -1033,6 +1037,8 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
   });
   NRS_CHECK(std::find(bad.begin(), bad.end(), 1) == bad.end(),
             "schwarz: 1D mass matrix not positive definite");
+  if (!poisson)   // lambda0 (lx + ly + lz) + lambda1, one third of the shift per direction
+    for (auto &v : lh) v = lambda0 * v + lambda1 / 3.0;
   S.upload(Sh);
   lam.upload(lh);
   if (prec != NRS_FDM_F32) {
@@ -1097,31 +1103,48 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
   NRS_HIP(hipStreamSynchronize(s));
 }
 
+// the matrix-core box solves of one Schwarz level (FP32 vectors)
+static void fdm_mfma_boxes(int Nq, int fmt, bool ras, int E, const float *u, const uint16_t *S16,
+                           const float *lam, const int4 *gmap, float *out, float *gout, int epw,
+                           hipStream_t s)
+{
+#define NRS_FDM_MFMA_CASE(NQ_)                                                                 \
+  case NQ_:                                                                                    \
+    if (fmt == NRS_FDM_BF16)                                                                   \
+      launch_fdm_mfma_nq<NQ_, NRS_FDM_BF16>(ras, E, u, S16, lam, gmap, out, gout, epw, s);     \
+    else                                                                                       \
+      launch_fdm_mfma_nq<NQ_, NRS_FDM_F16>(ras, E, u, S16, lam, gmap, out, gout, epw, s);      \
+    break;
+  switch (Nq) {
+    NRS_FDM_MFMA_CASE(2) NRS_FDM_MFMA_CASE(3) NRS_FDM_MFMA_CASE(4) NRS_FDM_MFMA_CASE(5)
+    NRS_FDM_MFMA_CASE(6) NRS_FDM_MFMA_CASE(7) NRS_FDM_MFMA_CASE(8) NRS_FDM_MFMA_CASE(9)
+    NRS_FDM_MFMA_CASE(10)
+    default: throw Error(NRS_ERR_INVALID, "schwarz: unsupported order");
+  }
+#undef NRS_FDM_MFMA_CASE
+}
+
 template <typename T>
 void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
 {
   NRS_CHECK((const void *)u != (const void *)Su, "schwarz: input and output must differ");
-  const int64_t L = (int64_t)E * Np;
   T *out = ras ? Su : boxOut.get();
   T *gout = ras ? nullptr : ghostOut.get();
-  const T *w = ras ? invDegree : nullptr;
   if constexpr (std::is_same<T, float>::value) {
     if (fdmPrecision != NRS_FDM_F32) {
-#define NRS_FDM_MFMA_CASE(NQ_)                                                                 \
-  case NQ_:                                                                                    \
-    if (fdmPrecision == NRS_FDM_BF16)                                                          \
-      launch_fdm_mfma_nq<NQ_, NRS_FDM_BF16>(ras, E, u, S16.get(), lam32.get(), gmap.get(), out, gout, s); \
-    else                                                                                       \
-      launch_fdm_mfma_nq<NQ_, NRS_FDM_F16>(ras, E, u, S16.get(), lam32.get(), gmap.get(), out, gout, s); \
-    if (!ras) launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); \
-    break;
-      switch (Nq) {
-        NRS_FDM_MFMA_CASE(2) NRS_FDM_MFMA_CASE(3) NRS_FDM_MFMA_CASE(4) NRS_FDM_MFMA_CASE(5)
-        NRS_FDM_MFMA_CASE(6) NRS_FDM_MFMA_CASE(7) NRS_FDM_MFMA_CASE(8) NRS_FDM_MFMA_CASE(9)
-        NRS_FDM_MFMA_CASE(10)
-        default: throw Error(NRS_ERR_INVALID, "schwarz: unsupported order");
+      const int epw = tuned(TUNE_FDM, N, fdmPrecision, E, 4);
+      fdm_mfma_boxes(Nq, fdmPrecision, ras, E, u, S16.get(), lam32.get(), gmap.get(), out, gout, epw, s);
+      if (!ras) {
+#define NRS_FDM_POST_CASE(NQ_)                                                                 \
+  case NQ_: launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); break;
+        switch (Nq) {
+          NRS_FDM_POST_CASE(2) NRS_FDM_POST_CASE(3) NRS_FDM_POST_CASE(4) NRS_FDM_POST_CASE(5)
+          NRS_FDM_POST_CASE(6) NRS_FDM_POST_CASE(7) NRS_FDM_POST_CASE(8) NRS_FDM_POST_CASE(9)
+          NRS_FDM_POST_CASE(10)
+          default: throw Error(NRS_ERR_INVALID, "schwarz: unsupported order");
+        }
+#undef NRS_FDM_POST_CASE
       }
-#undef NRS_FDM_MFMA_CASE
       finish(Su, s);
       return;
     }
@@ -1151,6 +1174,41 @@ void Schwarz<T>::finish(T *Su, hipStream_t s)
   if (ras) ogs_gather_scatter_weighted<T>(ogs, Su, invDegree, s);
   else ogs_gather_scatter<T>(ogs, Su, 1, L, NRS_ADD, s);
   launch_mask<T>(Nmasked, maskIds, Su, s);
+}
+
+// Time the matrix-core box solve with each elements-per-wavefront variant on
+// a fixed pattern, keep the variants whose result matches the default's within
+// 100 eps relative (src/bench/fdm/benchmarkFDM.cpp:180-210), record the fastest.
+template <typename T>
+int Schwarz<T>::autotune(hipStream_t s)
+{
+  if (!std::is_same<T, float>::value || fdmPrecision == NRS_FDM_F32) return 0;
+  const int64_t L = (int64_t)E * Np;
+  const int64_t G = ras ? 0 : (int64_t)E * 6 * Nq * Nq;
+  DevBuf<float> u(L), ref(L + G), out(L + G);
+  fill_pattern(L, u.get(), s);
+  int vals[8];
+  const int nv = tune_variants(TUNE_FDM, vals, 8);
+  auto run = [&](int epw, float *o) {
+    fdm_mfma_boxes(Nq, fdmPrecision, ras, E, u.get(), S16.get(), lam32.get(), gmap.get(), o,
+                   ras ? nullptr : o + L, epw, s);
+  };
+  run(4, ref.get());
+  std::vector<float> hr(L + G), ho(L + G);
+  NRS_HIP(hipMemcpyAsync(hr.data(), ref.get(), (L + G) * sizeof(float), hipMemcpyDeviceToHost, s));
+  NRS_HIP(hipStreamSynchronize(s));
+  int best = 4;
+  double bestMs = 1e300;
+  for (int v = 0; v < nv; ++v) {
+    run(vals[v], out.get());
+    NRS_HIP(hipMemcpyAsync(ho.data(), out.get(), (L + G) * sizeof(float), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    if (max_rel_diff(ho.data(), hr.data(), L + G) > 100.0 * 1.1920928955078125e-07) continue;
+    const double ms = time_launches([&] { run(vals[v], out.get()); }, s);
+    if (ms < bestMs) { bestMs = ms; best = vals[v]; }
+  }
+  set_tuned(TUNE_FDM, N, fdmPrecision, E, best);
+  return best;
 }
 
 template struct Schwarz<float>;

@@ -53,14 +53,21 @@ struct Schwarz {
   const int *maskIds = nullptr;
 
   // lengths: [9][E] of the finest level; EToB [E][6]; links of the mesh.
+  // helmholtz: operator lambda0 * (stiffness) + lambda1 * (mass) with constant
+  // coefficients; the box solve then inverts lambda0 (Lx + Ly + Lz) + lambda1
+  // in the eigenbasis (S^T B S = I per direction).
   void setup(int N, int E, bool ras, int fdmPrecision, const std::vector<double> &lengths,
              const int *EToB, const std::vector<FaceLink> &links, const nrs_ogs *ogs,
-             const T *invDegree, int64_t Nmasked, const int *maskIds, hipStream_t s);
+             const T *invDegree, int64_t Nmasked, const int *maskIds, bool poisson,
+             double lambda0, double lambda1, hipStream_t s);
   // Su = smoothSchwarz(u): FDM solves on the extended boxes, overlap exchange,
   // gather-scatter, mask.  Su must not alias u (the RAS kernel reads the
   // neighbours' u while writing its own Su).
   void apply(const T *u, T *Su, hipStream_t s);
   void finish(T *Su, hipStream_t s);
+  // choose the matrix-core kernel's elements per wavefront for this level
+  // (FP32 vectors with BF16/FP16 operators; otherwise nothing to tune); returns it
+  int autotune(hipStream_t s);
 };
 
 }  // namespace nrs

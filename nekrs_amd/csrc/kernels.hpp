@@ -11,6 +11,11 @@ template <typename T>
 void launch_ax(int N, int E, const int *elist, const T *ggeo, const T *D, const T *l0, const T *l1,
                bool poisson, bool lfield, const T *q, T *Aq, const uint32_t *maskBits,
                hipStream_t s);
+// the same with an explicit elements-per-block multiplier (1, 2 or 4)
+template <typename T>
+void launch_ax_variant(int N, int mult, int E, const int *elist, const T *ggeo, const T *D,
+                       const T *l0, const T *l1, bool poisson, bool lfield, const T *q, T *Aq,
+                       const uint32_t *maskBits, hipStream_t s);
 void launch_block_ax(int N, int E, int64_t offset, int64_t loffset, const double *ggeo,
                      const double *D, const double *l0, const double *l1, const double *q,
                      double *Aq, hipStream_t s);

@@ -225,6 +225,23 @@ __global__ void pcg_init_kernel(double *scal, double rdotr0, double tol, int max
   }
 }
 
+// The same initialisation from the device-resident initial norm in S_RDOTR
+// (no host round trip): S_TMP3 keeps res0, the tolerance is relative to it
+// when asked; a NaN norm leaves the iteration inactive.
+__global__ void pcg_init_dev_kernel(double *scal, double tol, int relative, int maxit)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double r0 = scal[S_RDOTR];
+    scal[S_TMP3] = r0;
+    scal[S_RDOTZ1] = 0.0;
+    scal[S_ALPHA] = 0.0;
+    scal[S_TOL] = relative ? tol * r0 : tol;
+    scal[S_ITER] = 0.0;
+    scal[S_MAXIT] = (double)maxit;
+    scal[S_ACTIVE] = (r0 == r0) ? 1.0 : 0.0;
+  }
+}
+
 // ------------------------------------------------------------------ element-wise kernels
 
 template <typename T> __global__ void fill_kernel(int64_t n, T v, T *x)
@@ -346,6 +363,12 @@ template void convert<float, double>(int64_t, const float *, double *, hipStream
 void pcg_init(double *scal, double rdotr0, double tol, int maxit, hipStream_t s)
 {
   pcg_init_kernel<<<1, 64, 0, s>>>(scal, rdotr0, tol, maxit);
+  NRS_LAUNCH_CHECK();
+}
+
+void pcg_init_device(double *scal, double tol, bool relative, int maxit, hipStream_t s)
+{
+  pcg_init_dev_kernel<<<1, 64, 0, s>>>(scal, tol, relative ? 1 : 0, maxit);
   NRS_LAUNCH_CHECK();
 }
 

@@ -51,6 +51,8 @@ template <typename A, typename B> void convert(int64_t n, const A *x, B *y, hipS
 // reached S_TOL or S_ITER reached S_MAXIT, so any number of iterations can be
 // enqueued ahead of the host's convergence check without changing the result.
 void pcg_init(double *scal, double rdotr0, double tol, int maxit, hipStream_t s);
+// pcg_init from the initial norm already in scal[S_RDOTR] (kept in S_TMP3)
+void pcg_init_device(double *scal, double tol, bool relative, int maxit, hipStream_t s);
 // Jacobi preconditioner fused with the dots: z = invDiag .* r; rdotz = <r,z>_w,
 // zdotAp = <z,Ap>_w (flexible); then beta from rdotz1/rdotz2 (or -alpha zdotAp/rdotz2).
 void pcg_precon_dots(Reducer &R, int64_t n, const double *w, const double *invDiag,

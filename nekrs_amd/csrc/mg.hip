@@ -24,6 +24,7 @@
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "mg.hpp"
+#include "tune.hpp"
 
 namespace nrs {
 
@@ -375,7 +376,8 @@ struct MGImpl final : MGSolver {
   enum Cheby { NO_CHEBY, CHEBY1, CHEBY_OPT4 };
   int cheby = CHEBY1;
   int schwarzKind = 0;            // 0 Jacobi, 1 ASM, 2 RAS
-  std::vector<T> betasDown, betasUp;
+  std::vector<T> betasDown, betasUp, betasCoarse;
+  bool coarseSmoothing = false;      // coarse level smoothed (degree 3), no coarse solve
 
   // ---- level operators (pMGLevel::Ax / residual / smoother)
   void op(Level<T> &l, const T *q, T *Aq, hipStream_t s)
@@ -407,7 +409,7 @@ struct MGImpl final : MGSolver {
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get(), *spare = sTmp.get();
     if (xIsZero) {
       fill<T>(l.L, T(0), x, s);
-      NRS_HIP(hipMemcpyAsync(res, rhs, l.L * sizeof(T), hipMemcpyDeviceToDevice, s));
+      k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
     } else {
       op(l, x, res, s);
       k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
@@ -435,12 +437,12 @@ struct MGImpl final : MGSolver {
   {
     const int deg = xIsZero ? l.degDown : l.degUp;
     if (deg == 0) return;
-    const auto &betas = xIsZero ? betasDown : betasUp;
+    const auto &betas = l.isCoarse ? betasCoarse : (xIsZero ? betasDown : betasUp);
     const double rho = l.lmax;
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get();
     if (xIsZero) {
       fill<T>(l.L, T(0), x, s);
-      NRS_HIP(hipMemcpyAsync(res, rhs, l.L * sizeof(T), hipMemcpyDeviceToDevice, s));
+      k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
     } else {
       op(l, x, res, s);
       k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
@@ -477,7 +479,8 @@ struct MGImpl final : MGSolver {
   {
     Level<T> &l = *levels[k];
     if (l.isCoarse) {
-      coarse->solve<T>(l.rhs, l.x, s);
+      if (coarse) coarse->solve<T>(l.rhs, l.x, s);
+      else smooth(l, l.rhs, l.x, true, s);   // MULTIGRID COARSE SOLVE = FALSE (ellipticMultiGridSetup.cpp:315-320)
       return;
     }
     Level<T> &c = *levels[k + 1];
@@ -518,9 +521,11 @@ struct MGImpl final : MGSolver {
     return out;
   }
   int coarse_iterations() const override { return coarse ? coarse->lastIters : 0; }
+  bool device_only() const override { return coarse == nullptr; }
   void level_smoother(int level, const void *in, void *out, hipStream_t s) override
   {
-    NRS_CHECK(level >= 0 && level + 1 < (int)levels.size(), "mg smoother: not a smoothed level");
+    NRS_CHECK(level >= 0 && level < (int)levels.size() && (level + 1 < (int)levels.size() || !coarse),
+              "mg smoother: not a smoothed level");
     NRS_CHECK(in != out, "mg smoother: input and output must differ");
     smootherS(*levels[level], (const T *)in, (T *)out, s);
   }
@@ -553,23 +558,39 @@ struct MGImpl final : MGSolver {
       return h[0];
     };
     const double nv = std::sqrt(dot(vx.get(), vx.get()));
+    // a level without unknowns (every node masked): nothing to smooth, any
+    // positive bound serves (the reference divides by the zero norm)
+    if (!(nv > 0.0)) return 1.0;
     axpby<double>(L, 1.0 / nv, vx.get(), 0.0, V[0].get(), s);
     std::vector<double> H((size_t)k * k, 0.0);
+    int m = k;   // dimension of the Krylov space reached
     for (int j = 0; j < k; ++j) {
       k_cast<double, T>(L, V[j].get(), vT.get(), s);
       op(l, vT.get(), avT.get(), s);
       smootherS(l, avT.get(), vT.get(), s);
       k_cast<T, double>(L, vT.get(), V[j + 1].get(), s);
+      double hn = 0.0;
       for (int i = 0; i <= j; ++i) {
         const double hij = dot(V[i].get(), V[j + 1].get());
         axpby<double>(L, -hij, V[i].get(), 1.0, V[j + 1].get(), s);
         H[i + j * k] = hij;
+        hn += hij * hij;
       }
       if (j + 1 < k) {
         const double nj = std::sqrt(dot(V[j + 1].get(), V[j + 1].get()));
+        // breakdown: the space is invariant (a level with fewer unknowns than
+        // steps), its leading block holds exact eigenvalues; the reference
+        // normalises the rounding residue instead (NaN or a spurious rho)
+        if (nj <= 1e-8 * std::sqrt(hn + nj * nj)) { m = j + 1; break; }
         axpby<double>(L, 1.0 / nj, V[j + 1].get(), 0.0, V[j + 1].get(), s);
         H[j + 1 + j * k] = nj;
       }
+    }
+    if (m < k) {
+      std::vector<double> Hm((size_t)m * m);
+      for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) Hm[i + j * m] = H[i + j * k];
+      return spectral_radius(Hm, m);
     }
     return spectral_radius(H, k);
   }
@@ -604,7 +625,7 @@ struct MGImpl final : MGSolver {
     for (int id : mids) mg[id] = 0;
     l.ogs = ogs_setup(l.L, mg.data());
     l.ownOgs = true;
-    if (l.isCoarse) {   // FP64 operator for the coarse PCG
+    if (l.isCoarse && !coarseSmoothing) {   // FP64 operator for the coarse PCG
       coarse = std::make_unique<CoarsePCG>();
       CoarsePCG &c = *coarse;
       c.N = l.N; c.E = l.E; c.L = l.L;
@@ -647,7 +668,9 @@ struct MGImpl final : MGSolver {
       case NRS_SMOOTHER_RAS: cheby = NO_CHEBY; schwarzKind = 2; break;
       default: throw Error(NRS_ERR_INVALID, "multigrid: unknown smoother");
     }
-    NRS_CHECK(o.coarseSolver == NRS_COARSE_PCG_F64, "multigrid: only the FP64 PCG coarse solve is built");
+    NRS_CHECK(o.coarseSolver == NRS_COARSE_PCG_F64 || o.coarseSolver == NRS_COARSE_SMOOTH,
+              "multigrid: unknown coarse solver");
+    coarseSmoothing = o.coarseSolver == NRS_COARSE_SMOOTH;
     NRS_CHECK(o.fdmPrecision == NRS_FDM_F32 || o.fdmPrecision == NRS_FDM_BF16 ||
                   o.fdmPrecision == NRS_FDM_F16,
               "multigrid: unknown FDM precision");
@@ -674,6 +697,7 @@ struct MGImpl final : MGSolver {
     if (cheby == CHEBY_OPT4) {
       betasDown = opt4_betas<T>(o.chebyDegree);
       betasUp = betasDown;
+      betasCoarse = opt4_betas<T>(3);
     }
 
     std::vector<double> xf(d->x, d->x + e->Nlocal), yf(d->y, d->y + e->Nlocal), zf(d->z, d->z + e->Nlocal);
@@ -737,12 +761,13 @@ struct MGImpl final : MGSolver {
         l.rhsOwn.alloc(l.L); l.xOwn.alloc(l.L);
         l.rhs = l.rhsOwn.get(); l.x = l.xOwn.get();
       }
-      if (!l.isCoarse) {
+      if (!l.isCoarse || coarseSmoothing) {
         if (schwarzKind) {
           // Schwarz smoother (pMGLevel::build, ellipticMultiGridSchwarz.cpp:837-1054)
           l.schwarz = std::make_unique<Schwarz<T>>();
           l.schwarz->setup(l.N, l.E, schwarzKind == 2, o.fdmPrecision, lengths, d->EToB, links,
-                           l.ogs, l.invDegree.get(), l.Nmasked, l.maskIds, s);
+                           l.ogs, l.invDegree.get(), l.Nmasked, l.maskIds, el->poisson,
+                           el->lambda0, el->lambda1, s);
         } else {
           // Jacobi smoother: 1/diag(A), assembled (ellipticUpdateJacobi.cpp:36-61)
           l.invDiagA.alloc(l.L);
@@ -751,8 +776,12 @@ struct MGImpl final : MGSolver {
           ogs_gather_scatter<T>(l.ogs, l.invDiagA.get(), 1, l.L, NRS_ADD, s);
           reciprocal<T>(l.L, l.invDiagA.get(), s);
         }
-        l.degDown = l.degUp = (cheby == NO_CHEBY) ? 1 : o.chebyDegree;
+        // the coarsest level, when smoothed, uses degree 3 (ellipticMultiGridLevelSetup.cpp:143-150)
+        l.degDown = l.degUp = (cheby == NO_CHEBY) ? 1 : (l.isCoarse ? 3 : o.chebyDegree);
+        if (o.autotune && l.schwarz) l.schwarz->autotune(s);
       }
+      // level operator launch variants (registerEllipticPreconditionerKernels.cpp:52-66)
+      if (o.autotune) autotune_ax<T>(l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, s);
       Nprev = l.N;
       levels.push_back(std::move(lp));
     }
@@ -761,7 +790,7 @@ struct MGImpl final : MGSolver {
     if (!std::is_same<T, double>::value) { rP.alloc(L0); zP.alloc(L0); }
     for (auto &lp : levels) {
       Level<T> &l = *lp;
-      if (l.isCoarse || cheby == NO_CHEBY) continue;
+      if ((l.isCoarse && !coarseSmoothing) || cheby == NO_CHEBY) continue;
       l.rho = max_eig(l, o.seed, s);
       l.lmax = o.chebyMaxFactor * l.rho;
       l.lmin = o.chebyMinFactor * l.rho;

@@ -22,8 +22,10 @@ struct MGSolver {
   virtual void apply(const double *r, double *z, hipStream_t s) = 0;
   virtual std::vector<MGLevelInfo> info() const = 0;
   virtual int coarse_iterations() const = 0;   // iterations of the last coarse solve
-  // out = S in on a non-coarse level, in the level arithmetic type
+  // out = S in on a smoothed level, in the level arithmetic type
   virtual void level_smoother(int level, const void *in, void *out, hipStream_t s) = 0;
+  // true when apply() enqueues work only (coarse level smoothed, no host round trip)
+  virtual bool device_only() const = 0;
 };
 
 std::unique_ptr<MGSolver> mg_create(nrs_elliptic *el, const nrs_elliptic_desc *d, hipStream_t s);

@@ -321,7 +321,7 @@ class Schwarz:
     maskIds, ogs (masked), invDegree of the level."""
 
     def __init__(self, level_mesh, fine, maskIds, ogs, invDegree, mode="asm", T=np.float32,
-                 fmt=None, mesh_ext=None):
+                 fmt=None, mesh_ext=None, poisson=True, lambda0=1.0, lambda1=0.0):
         m = level_mesh
         self.mode, self.T, self.fmt = mode, T, fmt
         self.N, self.E = m.N, m.E
@@ -340,6 +340,11 @@ class Schwarz:
             for d in range(3):
                 S[d, e], lam[d, e] = matrices_1d(D, w, bc[2 * d], bc[2 * d + 1],
                                                  self.left[d, e], self.mid[d, e], self.right[d, e])
+        if not poisson:
+            # Helmholtz (an extension: the reference's Schwarz smoother is
+            # Poisson-only, ellipticSetup.cpp:59-64): invert lambda0 (Lx + Ly + Lz)
+            # + lambda1, the mass matrix being the identity in the eigenbasis
+            lam = lambda0 * lam + lambda1 / 3.0
         self.S64, self.lam = S, lam
         diag = lam[2][:, :, None, None] + lam[1][:, None, :, None] + lam[0][:, None, None, :]
         with np.errstate(divide="ignore"):

@@ -70,14 +70,20 @@ class MG:
 
     def __init__(self, mesh, orders, T=np.float32, degree=2, min_factor=0.1, max_factor=1.1,
                  coarse_tol=1e-2, coarse_maxit=200, seed=0, cheby="cheby1", S="jacobi",
-                 fmt=None):
+                 fmt=None, lambda0=1.0, lambda1=0.0, poisson=True, coarse="pcg"):
         """cheby: "cheby1", "opt4" or None (pure Schwarz); S: "jacobi", "asm", "ras";
-        fmt: None, "bf16" or "fp16" for the FDM contractions."""
+        fmt: None, "bf16" or "fp16" for the FDM contractions; lambda0/lambda1/poisson:
+        the (constant-coefficient) operator; coarse: "pcg" (FP64 Jacobi-PCG) or
+        "smooth" (MULTIGRID COARSE SOLVE = FALSE: the coarsest level is smoothed,
+        degree 3, ellipticMultiGridSetup.cpp:315-320)."""
         assert orders[0] == mesh.N
         assert cheby in ("cheby1", "opt4", None) and S in ("jacobi", "asm", "ras")
         assert cheby is not None or S != "jacobi"
+        assert coarse in ("pcg", "smooth")
         self.T = T
         self.cheby, self.S = cheby, S
+        self.lambda0, self.lambda1, self.poisson = lambda0, lambda1, poisson
+        self.coarse = coarse
         self.coarse_tol, self.coarse_maxit = coarse_tol, coarse_maxit
         self.levels = []
         x, y, z = mesh.x, mesh.y, mesh.z
@@ -96,27 +102,28 @@ class MG:
             m.ggeo = np.zeros(m.E * 7 * m.Np)
             lib().orc_geofacs(N, m.E, m.D.ravel().copy(), wq, x, y, z, m.ggeo, None)
             L.mesh = m
-            L.el = Elliptic(m)                                  # masked ogs, mask, invDegree
+            L.el = Elliptic(m, lambda0, lambda1, poisson)       # masked ogs, mask, invDegree
             L.ggeoT = m.ggeo.astype(T)
             L.DT = m.D.astype(T)
             L.invDegreeT = L.el.invDegree.astype(T)
             if li > 0:
                 L.R = transfer_R(Nprev, N).astype(T)
-            if not L.isCoarse:
+            if li == 0 and S != "jacobi":
+                fine = (N, x, y, z, wq, L.el.ogs_full)
+            if not L.isCoarse or coarse == "smooth":
                 if S == "jacobi":
-                    d = build_diag(N, m.ggeo, m.D)
+                    d = build_diag(N, m.ggeo, m.D, lambda0, lambda1, poisson)
                     d = L.el.ogs.gs(d, "add")
                     L.invDiagT = (1.0 / d).astype(T)
                 else:
-                    if li == 0:
-                        fine = (N, x, y, z, wq, L.el.ogs_full)
                     L.schwarz = _fdm.Schwarz(m, fine, L.el.maskIds, L.el.ogs, L.el.invDegree,
-                                             mode=S, T=T, fmt=fmt)
-                L.degree = degree
+                                             mode=S, T=T, fmt=fmt, poisson=poisson,
+                                             lambda0=lambda0, lambda1=lambda1)
+                L.degree = 3 if L.isCoarse else degree
             self.levels.append(L)
             Nprev = N
         for L in self.levels:
-            if not L.isCoarse and cheby is not None:
+            if (not L.isCoarse or coarse == "smooth") and cheby is not None:
                 L.rho = self._max_eig(L, seed)
                 L.lmax, L.lmin = max_factor * L.rho, min_factor * L.rho
         cl = self.levels[-1]
@@ -127,9 +134,9 @@ class MG:
         q = np.ascontiguousarray(q, dtype=self.T)
         fn = lib().orc_ax_f64 if self.T == np.float64 else lib().orc_ax_f32
         Aq = np.zeros_like(q)
-        one = np.ones(1, dtype=self.T)
-        zero = np.zeros(1, dtype=self.T)
-        fn(L.N, L.E, None, L.ggeoT, L.DT.ravel().copy(), one, zero, 1, 0, q, Aq)
+        l0 = np.full(1, self.lambda0, dtype=self.T)
+        l1 = np.full(1, self.lambda1, dtype=self.T)
+        fn(L.N, L.E, None, L.ggeoT, L.DT.ravel().copy(), l0, l1, int(self.poisson), 0, q, Aq)
         Aq[L.el.maskIds] = 0
         return L.el.ogs.gs(Aq, "add")
 
@@ -140,21 +147,32 @@ class MG:
         vx = L.el.ogs_full.gs(vx, "add")
         vx[L.el.maskIds] = 0.0
         w = L.el.invDegree
-        V = [vx / np.sqrt(np.sum(w * vx * vx))]
+        nv = np.sqrt(np.sum(w * vx * vx))
+        if not nv > 0.0:
+            return 1.0              # no unknowns on this level (the reference divides by 0)
+        V = [vx / nv]
         H = np.zeros((k, k))
+        m = k
         for j in range(k):
             av = self.op(L, V[j].astype(self.T))
             v = self.smoother_S(L, av).astype(np.float64)
+            hn = 0.0
             for i in range(j + 1):
                 hij = np.sum(w * V[i] * v)
                 v = v - hij * V[i]
                 H[i, j] = hij
+                hn += hij * hij
             if j + 1 < k:
                 nj = np.sqrt(np.sum(w * v * v))
+                # breakdown: invariant Krylov space, its leading block is exact
+                # (the reference normalises the rounding residue instead)
+                if nj <= 1e-8 * np.sqrt(hn + nj * nj):
+                    m = j + 1
+                    break
                 v = v / nj
                 H[j + 1, j] = nj
             V.append(v)
-        return float(np.max(np.abs(np.linalg.eigvals(H))))
+        return float(np.max(np.abs(np.linalg.eigvals(H[:m, :m]))))
 
     def smoother_S(self, L, v):
         """pMGLevel::smoother (ellipticMultiGridLevel.cpp:104-111)."""
@@ -244,6 +262,9 @@ class MG:
     def vcycle(self, k, rhs):
         L = self.levels[k]
         if L.isCoarse:
+            if self.coarse == "smooth":
+                x = np.zeros(L.E * L.Np, dtype=self.T)
+                return self.smooth(L, rhs, x, True)
             x, self.last_coarse_it = self.coarse_solve(L, rhs)
             return x
         C_ = self.levels[k + 1]
@@ -277,7 +298,7 @@ def fcg_solve(el, rhs, precon, flexible=True, tol=1e-8, maxit=500, x0=None):
     m = el.mesh
     x = np.zeros(m.Nlocal) if x0 is None else x0.copy()
     from . import ax
-    Ax0 = ax(m.N, m.ggeo, m.D, x)
+    Ax0 = ax(m.N, m.ggeo, m.D, x, el.lambda0, el.lambda1, el.poisson)
     r = rhs - Ax0
     r[el.maskIds] = 0
     r = el.ogs.gs(r, "add")

@@ -1,0 +1,114 @@
+"""Batched independent solves (one captured graph advancing every problem):
+each problem's result must equal its stand-alone solve with the same handle
+options bit for bit (the graph only reorders independent work), and the
+Helmholtz p-MG problems with a smoothed coarse level must agree with the
+oracle (iterations +-1, solution 1e-8 relative).  The Helmholtz Schwarz
+smoother and the smoothed coarse level extend the reference (which rejects
+non-Poisson Schwarz smoothing), so the oracle restatement plus the solved
+residual pin them."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+MG = dict(solver="fcg", precon="multigrid", smoother="opt4chebyshev+ras", cheby_degree=3,
+          coarse="smooth", tol=1e-9, max_iter=300)
+
+
+def problems(gpu):
+    rng = np.random.default_rng(0)
+    out = []
+    for i, (shape, N) in enumerate([((2, 2, 2), 3), ((3, 2, 2), 5), ((2, 3, 2), 7), ((3, 3, 2), 3),
+                                    ((2, 2, 3), 5), ((2, 2, 2), 7)]):
+        eps = float(rng.uniform(0.0, 0.3))
+        m = gpu.BoxMesh(*shape, N, "kershaw", max(eps, 1e-3))
+        lam = float(rng.uniform(0.0, 10.0))
+        kw = dict(MG, lambda0=1.0, lambda1=lam, poisson=False, fdmPrecision=1 if i % 2 else 0)
+        out.append((m, kw))
+    # plain Jacobi-PCG Poisson problems ride along
+    out.append((gpu.BoxMesh(3, 3, 3, 4, "sine"), dict(solver="pcg", precon="jacobi", tol=1e-9,
+                                                      max_iter=500)))
+    return out
+
+
+def test_batch_equals_standalone(gpu, torch_mod):
+    torch = torch_mod
+    probs = problems(gpu)
+    batch = gpu.Batch(probs, iters_per_launch=2)
+    info = batch.info()
+    assert info["problems"] == len(probs) and info["graph_nodes"] > len(probs)
+    rng = np.random.default_rng(5)
+    rhs = [rng.uniform(-1, 1, m.Nlocal) for m, _ in probs]
+    rb = [dev(torch, r) for r in rhs]
+    xb = [torch.zeros(m.Nlocal, dtype=torch.float64, device="cuda") for m, _ in probs]
+    st = batch.solve(rb, xb)
+    for p, (m, kw) in enumerate(probs):
+        kw = dict(kw)
+        lam = {k: kw.pop(k) for k in ("lambda0", "lambda1", "poisson") if k in kw}
+        el = gpu.Elliptic(m, **lam, **kw)
+        x = torch.zeros(m.Nlocal, dtype=torch.float64, device="cuda")
+        s1 = el.solve(dev(torch, rhs[p]), x)
+        assert st[p]["converged"] and s1["converged"]
+        assert st[p]["iterations"] == s1["iterations"], (p, st[p]["iterations"], s1["iterations"])
+        assert torch.equal(xb[p], x), p
+    # a second solve from the same state reproduces the first
+    rb2 = [dev(torch, r) for r in rhs]
+    xb2 = [torch.zeros(m.Nlocal, dtype=torch.float64, device="cuda") for m, _ in probs]
+    st2 = batch.solve(rb2, xb2)
+    for p in range(len(probs)):
+        assert st2[p]["iterations"] == st[p]["iterations"]
+        assert torch.equal(xb2[p], xb[p])
+
+
+@pytest.mark.parametrize("N,lam,S", [(5, 3.0, "ras"), (7, 8.0, "ras"), (5, 0.5, "jacobi")])
+def test_helmholtz_mg_smoothed_coarse_vs_oracle(gpu, torch_mod, orc, N, lam, S):
+    import oracle.mg as omg
+    torch = torch_mod
+    shape = (4, 3, 3)
+    mg = gpu.BoxMesh(*shape, N, "kershaw", 0.25)
+    smoother = "opt4chebyshev+ras" if S == "ras" else "opt4chebyshev+jacobi"
+    kw = dict(MG, smoother=smoother, tol=1e-10, fdmPrecision=1 if S == "ras" else 0)
+    eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=lam, poisson=False, levels=[N, 3, 1], **kw)
+    mo = orc.BoxMesh(*shape, N, "kershaw", 0.25)
+    eo = orc.Elliptic(mo, 1.0, lam, False)
+    om = omg.MG(mo, [N, 3, 1], T=np.float32, degree=3, cheby="opt4", S=S, lambda0=1.0,
+                lambda1=lam, poisson=False, coarse="smooth", fmt="bf16" if S == "ras" else None)
+    rhs = np.random.default_rng(N).uniform(-1, 1, mo.Nlocal)
+    xo, info = omg.fcg_solve(eo, rhs, om.apply, flexible=True, tol=1e-10, maxit=300)
+    batch = gpu.Batch([(mg, dict(kw, lambda0=1.0, lambda1=lam, poisson=False, levels=[N, 3, 1]))])
+    x = [torch.zeros(mo.Nlocal, dtype=torch.float64, device="cuda")]
+    st = batch.solve([dev(torch, rhs)], x)[0]
+    print(N, lam, S, "gpu it", st["iterations"], "oracle it", info["iterations"])
+    assert st["converged"]
+    assert abs(st["iterations"] - info["iterations"]) <= 1
+    assert rel(x[0].cpu().numpy(), xo) <= 1e-8
+    # the converged solution satisfies the Helmholtz system
+    b = eo.ogs.gs(np.where(np.isin(np.arange(mo.Nlocal), eo.maskIds), 0.0, rhs), "add")
+    assert rel(eo.operator(x[0].cpu().numpy()), b) < 1e-8
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 2, 2), (1, 1, 1)])
+def test_tiny_coarse_level_rho_vs_oracle(gpu, orc, shape):
+    """Coarse levels with 1, 2 and 0 unknowns: the Arnoldi estimate stops at
+    the invariant space (rho exact; 1 on a level without unknowns)."""
+    import oracle.mg as omg
+    mg = gpu.BoxMesh(*shape, 3, "kershaw", 0.2)
+    eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=2.5, poisson=False, solver="fcg", precon="multigrid",
+                      smoother="chebyshev+jacobi", coarse="smooth", levels=[3, 1])
+    lv, _ = eg.mg_info()
+    mo = orc.BoxMesh(*shape, 3, "kershaw", 0.2)
+    om = omg.MG(mo, [3, 1], T=np.float32, coarse="smooth", lambda0=1.0, lambda1=2.5, poisson=False)
+    for L, g in zip(om.levels, lv):
+        assert np.isfinite(g["rho"])
+        assert abs(g["rho"] - L.rho) <= 1e-5 * L.rho, (g["rho"], L.rho)

@@ -113,3 +113,36 @@ def test_splitmix_reference_values():
     v = omg.splitmix_uniform(0, 2)
     assert v[0] == (0xE220A8397B1DCDAF >> 11) / 2.0 ** 53
     assert v[1] == (0x6E789E6AA1B965F4 >> 11) / 2.0 ** 53
+
+
+@pytest.mark.parametrize("shape,unknowns", [((2, 2, 2), 1), ((3, 2, 2), 2), ((1, 1, 1), 0)])
+def test_arnoldi_breakdown_on_tiny_levels(orc, shape, unknowns):
+    """A smoothed coarse level with fewer unknowns than Arnoldi steps: the
+    Krylov space becomes invariant, and the leading block's eigenvalues are the
+    exact spectrum of S A (1 for Jacobi on one unknown); a level with no
+    unknowns gets rho = 1 and smooths to zero."""
+    import oracle.mg as omg
+    mo = orc.BoxMesh(*shape, 3, "kershaw", 0.2)
+    om = omg.MG(mo, [3, 1], T=np.float64, coarse="smooth", lambda0=1.0, lambda1=2.5,
+                poisson=False)
+    L = om.levels[-1]
+    keep = np.zeros(L.E * L.Np, bool)
+    seen = set()
+    masked = set(L.el.maskIds.tolist())
+    for i, g in enumerate(L.mesh.gid):
+        if i not in masked and g not in seen:
+            seen.add(g)
+            keep[i] = True
+    assert keep.sum() == unknowns
+    if unknowns == 0:
+        assert L.rho == 1.0
+    else:
+        SA = _dense(lambda v: L.invDiagT * om.op(L, v), L.mesh.gid, keep)
+        exact = np.abs(np.linalg.eigvals(SA)).max()
+        assert abs(L.rho - exact) <= 1e-10 * exact
+        if unknowns == 1:
+            assert abs(L.rho - 1.0) <= 1e-12
+    rhs = np.random.default_rng(1).uniform(-1, 1, mo.Nlocal)
+    eo = orc.Elliptic(mo, 1.0, 2.5, False)
+    z = om.apply(eo.ogs.gs(np.where(np.isin(np.arange(mo.Nlocal), eo.maskIds), 0.0, rhs), "add"))
+    assert np.all(np.isfinite(z))
