@@ -135,6 +135,105 @@ def solve_cases(n, N, seed=0):
     return out
 
 
+BATCH_KW = dict(solver="fcg", precon="multigrid", smoother="opt4chebyshev+ras", cheby_degree=3,
+                mgPrecision=0, fdmPrecision=1, coarse="smooth", tol=1e-8, max_iter=500)
+
+
+def box_dims(E):
+    """A near-cubic nx*ny*nz box with about E elements (each side >= 2)."""
+    nx = max(2, int(round(E ** (1.0 / 3.0))))
+    ny = max(2, int(round((E / nx) ** 0.5)))
+    nz = max(2, int(round(E / (nx * ny))))
+    return nx, ny, nz
+
+
+def run_batch(probs, iters_per_launch, seed):
+    """Set up one graph batch, solve it (untimed, then timed), then solve the
+    same handles one after another (host-driven); returns the measurements."""
+    import numpy as np
+    import torch
+    import nekrs_amd
+    t0 = time.perf_counter()
+    b = nekrs_amd.Batch(probs, iters_per_launch=iters_per_launch)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    rhs = [torch.from_numpy(rng.uniform(-1, 1, m.Nlocal)).cuda() for m, _ in probs]
+    for rep in range(2):
+        r = [t.clone() for t in rhs]
+        x = [torch.zeros_like(t) for t in rhs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = b.solve(r, x)
+        torch.cuda.synchronize()
+        t_batch = time.perf_counter() - t0
+    xb = [t.cpu().numpy() for t in x]
+    info = b.info()
+    # the same handles, one problem at a time (host loop per iteration)
+    t_seq, seq_it, diff = 0.0, [], 0.0
+    for p in range(len(probs)):
+        el = b.problem(p)
+        r = rhs[p].clone()
+        xs = torch.zeros_like(r)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s1 = el.solve(r, xs)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+        seq_it.append(s1["iterations"])
+        diff = max(diff, float(np.max(np.abs(xs.cpu().numpy() - xb[p]))))
+    its = [s["iterations"] for s in st]
+    dofs = [m.E * m.N ** 3 for m, _ in probs]
+    work = float(sum(d * i for d, i in zip(dofs, its)))
+    del b
+    return {"problems": len(probs), "setup_seconds": t_setup, "seconds_batched": t_batch,
+            "seconds_sequential": t_seq, "speedup_batched_vs_sequential": t_seq / t_batch,
+            "dof_iterations_per_s_batched": work / t_batch,
+            "iterations": its, "iterations_sequential": seq_it,
+            "all_converged": all(s["converged"] for s in st),
+            "max_abs_diff_batched_vs_sequential": diff,
+            "graph_nodes": info["graph_nodes"], "graph_launches": info["last_launches"],
+            "iters_per_launch": iters_per_launch, "dofs": dofs}
+
+
+def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
+    """The uneven Helmholtz batch and the 3-component velocity Helmholtz solve."""
+    import numpy as np
+    import nekrs_amd
+    rng = np.random.default_rng(seed)
+    probs, desc = [], []
+    for p in range(count):
+        N = int(rng.choice([3, 5, 7, 9]))
+        Et = int(rng.integers(512, 8193))
+        lam = float(rng.uniform(0.0, 10.0))
+        eps = float(rng.uniform(0.05, 0.3))
+        nx, ny, nz = box_dims(Et)
+        probs.append((nekrs_amd.BoxMesh(nx, ny, nz, N, "kershaw", eps),
+                      dict(BATCH_KW, lambda0=1.0, lambda1=lam, poisson=False)))
+        desc.append({"N": N, "E": nx * ny * nz, "box": [nx, ny, nz], "lambda1": lam, "eps": eps})
+    log("batch: %d problems, %d elements in all" % (count, sum(d["E"] for d in desc)))
+    out = {"uneven": run_batch(probs, iters_per_launch, seed + 1)}
+    out["uneven"]["cases"] = desc
+    u = out["uneven"]
+    log("batch: %.3f s batched vs %.3f s one by one (x%.2f), %.2f G dof-iterations/s, its %d..%d" %
+        (u["seconds_batched"], u["seconds_sequential"], u["speedup_batched_vs_sequential"],
+         u["dof_iterations_per_s_batched"] / 1e9, min(u["iterations"]), max(u["iterations"])))
+    del probs
+    if block_n:
+        m = nekrs_amd.BoxMesh(block_n, block_n, block_n, 7, "kershaw", 0.3)
+        vel = [(m, dict(BATCH_KW, lambda0=1.0, lambda1=10.0, poisson=False)) for _ in range(3)]
+        out["velocity3"] = run_batch(vel, iters_per_launch, seed + 2)
+        v = out["velocity3"]
+        log("velocity x3 at %d^3 N=7: %.3f s batched vs %.3f s one by one, its %s" %
+            (block_n, v["seconds_batched"], v["seconds_sequential"], v["iterations"]))
+    out["workload"] = ("%d Helmholtz problems (seed %d): N in {3,5,7,9}, E ~ U[512,8192] as "
+                       "near-cubic boxes, lambda1 ~ U[0,10], Kershaw eps ~ U[0.05,0.3]; plus 3 "
+                       "velocity components (lambda1=10) at %d^3 N=7 eps 0.3; FP32 p-MG, opt. "
+                       "4th-kind Chebyshev + RAS with BF16 FDM, smoothed coarse level, FCG to "
+                       "1e-8 relative" % (count, seed, block_n))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +245,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solve-n", type=int, default=48, help="elements per direction of the solves")
     ap.add_argument("--no-solves", action="store_true")
+    ap.add_argument("--batch-problems", type=int, default=64)
+    ap.add_argument("--batch-iters-per-launch", type=int, default=4)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the multi-GPU halo exchange of this path is "
@@ -201,6 +303,11 @@ def main():
         log("p-MG solves at %d^3 ..." % args.solve_n)
         solves = solve_cases(args.solve_n, N)
 
+    batch = None
+    if not args.no_batch:
+        log("batched solves ...")
+        batch = batch_cases(args.batch_problems, 0, args.batch_iters_per_launch, n)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -233,6 +340,7 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles()},
         "cpu_baseline": cpu,
         "solves": solves,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

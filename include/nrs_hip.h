@@ -176,7 +176,12 @@ enum nrs_smoother {
   NRS_SMOOTHER_CHEBYSHEV_RAS = 6,     /* CHEBYSHEV+RAS                           */
   NRS_SMOOTHER_OPT4_CHEBYSHEV_RAS = 7 /* FOURTHOPTCHEBYSHEV+RAS                  */
 };
-enum nrs_coarse { NRS_COARSE_PCG_F64 = 0, NRS_COARSE_SMOOTH = 1 };
+/* Coarsest level: FP64 Jacobi-PCG solve; smoothed with the level smoother
+ * (MULTIGRID COARSE SOLVE = FALSE, ellipticMultiGridSetup.cpp:315-320); or
+ * smoothed with Chebyshev-Jacobi even when the finer levels use Schwarz (no
+ * counterpart in the reference: its Schwarz boxes on an order-1 level with
+ * free box ends make a poor smoother there). */
+enum nrs_coarse { NRS_COARSE_PCG_F64 = 0, NRS_COARSE_SMOOTH = 1, NRS_COARSE_SMOOTH_JACOBI = 2 };
 enum nrs_mg_precision { NRS_MG_F32 = 0, NRS_MG_F64 = 1 };
 enum nrs_fdm_precision { NRS_FDM_F32 = 0, NRS_FDM_BF16 = 1, NRS_FDM_F16 = 2 };
 
@@ -305,6 +310,9 @@ int nrs_batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solv
 /* Problem count, nodes of the captured graph, graph launches of the last solve. */
 int nrs_batch_info(nrs_batch *b, int *n, int64_t *graphNodes, int *lastLaunches);
 int nrs_batch_free(nrs_batch *b);
+/* The handle of problem p (owned by the batch: do not free it).  A stand-alone
+ * nrs_elliptic_solve on it runs the same iteration one problem at a time. */
+int nrs_batch_problem(nrs_batch *b, int p, nrs_elliptic **el);
 
 /* ------------------------------------------------------------------------- */
 /* Kernel launch variants (KERNEL AUTOTUNING; the reference times p_knl       */

@@ -219,7 +219,7 @@ def default_opts():
     return o
 
 
-COARSE = {"pcg": 0, "smooth": 1}
+COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2}
 
 
 def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True):
@@ -308,6 +308,12 @@ class Batch:
         _lib.call("nrs_batch_info", self._h, C.byref(n), C.byref(nodes), C.byref(launches))
         return {"problems": n.value, "graph_nodes": nodes.value, "last_launches": launches.value}
 
+    def problem(self, p):
+        """Problem p's solver handle (borrowed: valid while the batch lives)."""
+        h = C.c_void_p()
+        _lib.call("nrs_batch_problem", self._h, int(p), C.byref(h))
+        return Elliptic._borrow(h, self.meshes[p], self)
+
     def solve(self, rhs, x, stream=None):
         """In place per problem (as Elliptic.solve); returns a list of stats dicts."""
         n = self.n
@@ -334,8 +340,20 @@ class Elliptic:
         _lib.call("nrs_elliptic_info", h, C.byref(n), C.byref(m), C.byref(vol))
         self.Nlocal, self.Nmasked, self.volume = n.value, m.value, vol.value
 
+    @classmethod
+    def _borrow(cls, h, mesh, owner):
+        """Wrap a handle owned elsewhere (kept alive through `owner`, not freed here)."""
+        self = cls.__new__(cls)
+        self.mesh, self._h, self._owner = mesh, h, owner
+        n, m, vol = C.c_int64(), C.c_int64(), C.c_double()
+        _lib.call("nrs_elliptic_info", h, C.byref(n), C.byref(m), C.byref(vol))
+        self.Nlocal, self.Nmasked, self.volume = n.value, m.value, vol.value
+        return self
+
     def __del__(self):
         h = getattr(self, "_h", None)
+        if getattr(self, "_owner", None) is not None:
+            return
         if h is not None and h.value:
             try:
                 _lib.load().nrs_elliptic_free(h)

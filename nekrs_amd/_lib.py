@@ -82,6 +82,7 @@ SIGNATURES = {
     "nrs_batch_solve": ([_vp, _vp, _vp, _vp, _vp], _i),
     "nrs_batch_info": ([_vp, C.POINTER(_i), C.POINTER(_i64), C.POINTER(_i)], _i),
     "nrs_batch_free": ([_vp], _i),
+    "nrs_batch_problem": ([_vp, _i, C.POINTER(_vp)], _i),
     "nrs_tuned_variant": ([_i, _i, _i, _i64, C.POINTER(_i)], _i),
     "nrs_set_variant": ([_i, _i, _i, _i64, _i], _i),
     "nrs_elliptic_profile_enable": ([_vp, _i], _i),
@@ -100,6 +101,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError("libnekrs_hip.so is not built: run `python nekrs_amd/build.py` "
                           "(or __graft_entry__.build()) first; there is no CPU fallback")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7, and
+    # whichever copy is loaded first serves both.  Loading torch first makes
+    # that torch's, so its streams and allocations are the library's too
+    # (loading the system copy first leaves torch without devices).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (args, res) in SIGNATURES.items():
         fn = getattr(lib, name)

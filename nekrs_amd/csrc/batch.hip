@@ -8,7 +8,10 @@
 // small problems' kernels; a last node gathers the problems' "active" flags
 // into one array.  A solve runs each problem's start-up work, then replays the
 // graph until no problem is active (one 8-byte-per-problem readback per replay),
-// then adds the initial guesses back.  Iterations of a converged problem leave
+// then adds the initial guesses back.  The kernel nodes of a problem that has
+// converged are switched off (hipGraphNodeSetEnabled) for the remaining
+// replays of that solve, so a batch of uneven problems costs what each
+// problem's own iterations cost, not the slowest problem's count times all.  Iterations of a converged problem leave
 // its state unchanged (its update kernels test the device flag), so a problem's
 // result equals its stand-alone device-driven solve.
 //
@@ -18,6 +21,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_set>
 
 #include "batch.hpp"
 #include "elliptic.hpp"
@@ -45,8 +49,8 @@ nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic
   auto b = std::make_unique<nrs_batch>();
   b->itersPerLaunch = itersPerLaunch;
   for (int p = 0; p < n; ++p) {
-    NRS_CHECK(!(opts[p].preconditioner == NRS_PRECON_MULTIGRID && opts[p].coarseSolver != NRS_COARSE_SMOOTH),
-              "batch: multigrid problems need the smoothed coarse level (no host round trip)");
+    NRS_CHECK(!(opts[p].preconditioner == NRS_PRECON_MULTIGRID && opts[p].coarseSolver == NRS_COARSE_PCG_F64),
+              "batch: multigrid problems need a smoothed coarse level (no host round trip)");
     b->probs.push_back(elliptic_setup(&descs[p], &opts[p]));
     const int64_t L = b->probs.back()->Nlocal;
     b->r.emplace_back(L);
@@ -68,11 +72,38 @@ nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic
   // one graph: itersPerLaunch iterations of every problem, problems in parallel
   NRS_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
   NRS_HIP(hipEventRecord(b->fork, b->cap));
+  // the graph under construction, to attribute its nodes to problems
+  hipGraph_t building = nullptr;
+  {
+    hipStreamCaptureStatus cs;
+    b->canDisable = hipStreamGetCaptureInfo_v2(b->cap, &cs, nullptr, &building, nullptr, nullptr) ==
+                        hipSuccess && building != nullptr;
+    (void)hipGetLastError();
+  }
+  std::vector<hipGraphNode_t> all;
+  std::unordered_set<hipGraphNode_t> seen;
+  auto new_nodes = [&](std::vector<hipGraphNode_t> &mine) {
+    if (!b->canDisable) return;
+    size_t cnt = 0;
+    if (hipGraphGetNodes(building, nullptr, &cnt) != hipSuccess) { b->canDisable = false; (void)hipGetLastError(); return; }
+    all.resize(cnt);
+    if (hipGraphGetNodes(building, all.data(), &cnt) != hipSuccess) { b->canDisable = false; (void)hipGetLastError(); return; }
+    for (size_t i = 0; i < cnt; ++i) {
+      if (!seen.insert(all[i]).second) continue;
+      hipGraphNodeType t;
+      if (hipGraphNodeGetType(all[i], &t) != hipSuccess) { b->canDisable = false; (void)hipGetLastError(); return; }
+      if (t == hipGraphNodeTypeKernel || t == hipGraphNodeTypeMemcpy || t == hipGraphNodeTypeMemset)
+        mine.push_back(all[i]);
+    }
+  };
+  b->nodes.resize(n);
+  b->disabled.assign(n, 0);
   for (int p = 0; p < n; ++p) {
     hipStream_t st = b->streams[p];
     NRS_HIP(hipStreamWaitEvent(st, b->fork, 0));
     for (int it = 0; it < itersPerLaunch; ++it)
       elliptic_iterate(b->probs[p], b->r[p].get(), b->x[p].get(), st);
+    new_nodes(b->nodes[p]);   // the nodes not seen before are p's
     NRS_HIP(hipEventRecord(b->joins[p], st));
     NRS_HIP(hipStreamWaitEvent(b->cap, b->joins[p], 0));
   }
@@ -98,13 +129,25 @@ int batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_st
     maxit = std::max(maxit, e->opts.maxIter);
   }
   const int maxLaunches = (maxit + b->itersPerLaunch - 1) / b->itersPerLaunch;
+  // every branch runs again (a converged problem's iterations are no-ops on
+  // its state; its nodes are switched off only to save the device time)
+  auto set_enabled = [&](int p, bool on) {
+    for (hipGraphNode_t nd : b->nodes[p]) NRS_HIP(hipGraphNodeSetEnabled(b->exec, nd, on ? 1u : 0u));
+    b->disabled[p] = on ? 0 : 1;
+  };
+  for (int p = 0; p < n; ++p)
+    if (b->disabled[p]) set_enabled(p, true);
   int launches = 0;
   for (; launches < maxLaunches; ++launches) {
     NRS_HIP(hipGraphLaunch(b->exec, s));
     NRS_HIP(hipMemcpyAsync(b->flagsHost, b->flags.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
     NRS_HIP(hipStreamSynchronize(s));
     bool any = false;
-    for (int p = 0; p < n; ++p) any |= (b->flagsHost[p] != 0.0);
+    for (int p = 0; p < n; ++p) {
+      const bool active = b->flagsHost[p] != 0.0;
+      any |= active;
+      if (!active && !b->disabled[p] && b->canDisable) set_enabled(p, false);
+    }
     if (!any) {
       ++launches;
       break;

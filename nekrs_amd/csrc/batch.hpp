@@ -19,6 +19,10 @@ struct nrs_batch {
   nrs::DevBuf<double *> scalPtrs;
   nrs::DevBuf<double> flags;
   double *flagsHost = nullptr;
+  // kernel/copy nodes of each problem's branch, switched off once it converges
+  std::vector<std::vector<hipGraphNode_t>> nodes;
+  std::vector<char> disabled;
+  bool canDisable = false;
   ~nrs_batch()
   {
     if (exec) (void)hipGraphExecDestroy(exec);

@@ -400,6 +400,15 @@ int nrs_batch_free(nrs_batch *b)
   NRS_API_END
 }
 
+int nrs_batch_problem(nrs_batch *b, int p, nrs_elliptic **el)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(b && el, "batch problem: null argument");
+  NRS_CHECK(p >= 0 && p < (int)b->probs.size(), "batch problem: index out of range");
+  *el = b->probs[p];
+  NRS_API_END
+}
+
 int nrs_tuned_variant(int kernel, int N, int type, int64_t Nelements, int *variant)
 {
   NRS_API_BEGIN

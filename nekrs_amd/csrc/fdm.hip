@@ -370,7 +370,7 @@ template <int NQ> struct FdmCfg {
 template <typename T, int NQ, bool ASM>
 __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
     int E, const T *__restrict__ u, const T *__restrict__ S, const double *__restrict__ lam,
-    const int4 *__restrict__ gmap, T *__restrict__ out, T *__restrict__ gout)
+    double2 helm, const int4 *__restrict__ gmap, T *__restrict__ out, T *__restrict__ gout)
 {
   using C = FdmCfg<NQ>;
   constexpr int NL = C::NL, NLP = C::NLP, NP = C::NP, NQ2 = NQ * NQ, TPE = C::TPE;
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
 #pragma unroll
       for (int l = 0; l < NL; ++l) s += Sz[l * NL + p] * r[l];
       const double dg = lxy + le[2 * NL + p];
-      const T inv = (dg > 1e-5) ? T(1.0 / dg) : T(0);
+      const T inv = (dg > 1e-5) ? T(1.0 / (helm.x * dg + helm.y)) : T(0);
       Y[at(p, a, b)] = s * inv;
     }
   }
@@ -674,8 +674,8 @@ template <int NQ> struct MfmaCfg {
 template <int NQ, int FMT, bool ASM>
 __global__ __launch_bounds__(256) void fdm_mfma_kernel(
     int E, const float *__restrict__ u, const uint16_t *__restrict__ S16,
-    const float *__restrict__ lam, const int4 *__restrict__ gmap, float *__restrict__ out,
-    float *__restrict__ gout)
+    const float *__restrict__ lam, float2 helm, const int4 *__restrict__ gmap,
+    float *__restrict__ out, float *__restrict__ gout)
 {
   using C = MfmaCfg<NQ>;
   constexpr int NL = C::NL, PS = C::PS, BUF = C::BUF, NP = NQ * NQ * NQ, NQ2 = NQ * NQ;
@@ -907,7 +907,7 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float dg = (lx[q] + ly[a]) + lz;
-        D[a][q] *= (dg > 1e-5f) ? __builtin_amdgcn_rcpf(dg) : 0.f;   // 1 ulp: below the 16-bit rounding
+        D[a][q] *= (dg > 1e-5f) ? __builtin_amdgcn_rcpf(helm.x * dg + helm.y) : 0.f;   // 1 ulp: below the 16-bit rounding
       }
     rescale(); store(Y);
     if (more) load_b();
@@ -967,28 +967,28 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
 // the prefetch has something to overlap)
 template <int NQ, int FMT>
 void launch_fdm_mfma_nq(bool ras, int E, const float *u, const uint16_t *S16, const float *lam,
-                        const int4 *gmap, float *out, float *gout, int epw, hipStream_t s)
+                        float2 helm, const int4 *gmap, float *out, float *gout, int epw, hipStream_t s)
 {
   using C = MfmaCfg<NQ>;
   const int64_t perBlock = (int64_t)C::WAVES * std::max(1, epw);
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((E + perBlock - 1) / perBlock, 65536));
   if (ras)
-    fdm_mfma_kernel<NQ, FMT, false><<<grid, 64 * C::WAVES, 0, s>>>(E, u, S16, lam, gmap, out, gout);
+    fdm_mfma_kernel<NQ, FMT, false><<<grid, 64 * C::WAVES, 0, s>>>(E, u, S16, lam, helm, gmap, out, gout);
   else
-    fdm_mfma_kernel<NQ, FMT, true><<<grid, 64 * C::WAVES, 0, s>>>(E, u, S16, lam, gmap, out, gout);
+    fdm_mfma_kernel<NQ, FMT, true><<<grid, 64 * C::WAVES, 0, s>>>(E, u, S16, lam, helm, gmap, out, gout);
   NRS_LAUNCH_CHECK();
 }
 
 template <typename T, int NQ>
-void launch_fdm_nq(bool ras, int E, const T *u, const T *S, const double *lam, const int4 *gmap,
+void launch_fdm_nq(bool ras, int E, const T *u, const T *S, const double *lam, double2 helm, const int4 *gmap,
                    T *out, T *gout, hipStream_t s)
 {
   using C = FdmCfg<NQ>;
   const unsigned grid = (unsigned)((E + C::EPB - 1) / C::EPB);
   if (ras)
-    fdm_kernel<T, NQ, false><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, gmap, out, gout);
+    fdm_kernel<T, NQ, false><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, helm, gmap, out, gout);
   else
-    fdm_kernel<T, NQ, true><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, gmap, out, gout);
+    fdm_kernel<T, NQ, true><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, helm, gmap, out, gout);
   NRS_LAUNCH_CHECK();
 }
 
@@ -1037,8 +1037,13 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
   });
   NRS_CHECK(std::find(bad.begin(), bad.end(), 1) == bad.end(),
             "schwarz: 1D mass matrix not positive definite");
-  if (!poisson)   // lambda0 (lx + ly + lz) + lambda1, one third of the shift per direction
-    for (auto &v : lh) v = lambda0 * v + lambda1 / 3.0;
+  // Helmholtz: the box solve inverts lambda0 (lx + ly + lz) + lambda1 in the
+  // eigenbasis (S^T B S = I per direction), except the modes with
+  // lx + ly + lz <= 1e-5 (constant on a box with free ends), dropped as in the
+  // Poisson case; inverting those by 1/lambda1 would make the smoother's
+  // largest eigenvalue grow like 1/(lambda1 h^2) under refinement.
+  helm0 = poisson ? 1.0 : lambda0;
+  helm1 = poisson ? 0.0 : lambda1;
   S.upload(Sh);
   lam.upload(lh);
   if (prec != NRS_FDM_F32) {
@@ -1105,15 +1110,15 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
 
 // the matrix-core box solves of one Schwarz level (FP32 vectors)
 static void fdm_mfma_boxes(int Nq, int fmt, bool ras, int E, const float *u, const uint16_t *S16,
-                           const float *lam, const int4 *gmap, float *out, float *gout, int epw,
+                           const float *lam, float2 helm, const int4 *gmap, float *out, float *gout, int epw,
                            hipStream_t s)
 {
 #define NRS_FDM_MFMA_CASE(NQ_)                                                                 \
   case NQ_:                                                                                    \
     if (fmt == NRS_FDM_BF16)                                                                   \
-      launch_fdm_mfma_nq<NQ_, NRS_FDM_BF16>(ras, E, u, S16, lam, gmap, out, gout, epw, s);     \
+      launch_fdm_mfma_nq<NQ_, NRS_FDM_BF16>(ras, E, u, S16, lam, helm, gmap, out, gout, epw, s);     \
     else                                                                                       \
-      launch_fdm_mfma_nq<NQ_, NRS_FDM_F16>(ras, E, u, S16, lam, gmap, out, gout, epw, s);      \
+      launch_fdm_mfma_nq<NQ_, NRS_FDM_F16>(ras, E, u, S16, lam, helm, gmap, out, gout, epw, s);      \
     break;
   switch (Nq) {
     NRS_FDM_MFMA_CASE(2) NRS_FDM_MFMA_CASE(3) NRS_FDM_MFMA_CASE(4) NRS_FDM_MFMA_CASE(5)
@@ -1133,7 +1138,7 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
   if constexpr (std::is_same<T, float>::value) {
     if (fdmPrecision != NRS_FDM_F32) {
       const int epw = tuned(TUNE_FDM, N, fdmPrecision, E, 4);
-      fdm_mfma_boxes(Nq, fdmPrecision, ras, E, u, S16.get(), lam32.get(), gmap.get(), out, gout, epw, s);
+      fdm_mfma_boxes(Nq, fdmPrecision, ras, E, u, S16.get(), lam32.get(), helm32(), gmap.get(), out, gout, epw, s);
       if (!ras) {
 #define NRS_FDM_POST_CASE(NQ_)                                                                 \
   case NQ_: launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); break;
@@ -1151,7 +1156,7 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
   }
 #define NRS_FDM_CASE(NQ_)                                                                      \
   case NQ_:                                                                                    \
-    launch_fdm_nq<T, NQ_>(ras, E, u, S.get(), lam.get(), gmap.get(), out, gout, s);           \
+    launch_fdm_nq<T, NQ_>(ras, E, u, S.get(), lam.get(), make_double2(helm0, helm1), gmap.get(), out, gout, s);           \
     if (!ras) launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); \
     break;
   switch (Nq) {
@@ -1190,7 +1195,7 @@ int Schwarz<T>::autotune(hipStream_t s)
   int vals[8];
   const int nv = tune_variants(TUNE_FDM, vals, 8);
   auto run = [&](int epw, float *o) {
-    fdm_mfma_boxes(Nq, fdmPrecision, ras, E, u.get(), S16.get(), lam32.get(), gmap.get(), o,
+    fdm_mfma_boxes(Nq, fdmPrecision, ras, E, u.get(), S16.get(), lam32.get(), helm32(), gmap.get(), o,
                    ras ? nullptr : o + L, epw, s);
   };
   run(4, ref.get());

@@ -51,6 +51,8 @@ struct Schwarz {
   const nrs_ogs *ogs = nullptr;
   int64_t Nmasked = 0;
   const int *maskIds = nullptr;
+  double helm0 = 1.0, helm1 = 0.0;   // box solve inverts helm0 (lx + ly + lz) + helm1
+  float2 helm32() const { return make_float2((float)helm0, (float)helm1); }
 
   // lengths: [9][E] of the finest level; EToB [E][6]; links of the mesh.
   // helmholtz: operator lambda0 * (stiffness) + lambda1 * (mass) with constant

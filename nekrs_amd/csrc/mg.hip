@@ -378,6 +378,7 @@ struct MGImpl final : MGSolver {
   int schwarzKind = 0;            // 0 Jacobi, 1 ASM, 2 RAS
   std::vector<T> betasDown, betasUp, betasCoarse;
   bool coarseSmoothing = false;      // coarse level smoothed (degree 3), no coarse solve
+  bool coarseJacobi = false;         // ... with Chebyshev-Jacobi whatever the finer levels use
 
   // ---- level operators (pMGLevel::Ax / residual / smoother)
   void op(Level<T> &l, const T *q, T *Aq, hipStream_t s)
@@ -668,9 +669,13 @@ struct MGImpl final : MGSolver {
       case NRS_SMOOTHER_RAS: cheby = NO_CHEBY; schwarzKind = 2; break;
       default: throw Error(NRS_ERR_INVALID, "multigrid: unknown smoother");
     }
-    NRS_CHECK(o.coarseSolver == NRS_COARSE_PCG_F64 || o.coarseSolver == NRS_COARSE_SMOOTH,
+    NRS_CHECK(o.coarseSolver == NRS_COARSE_PCG_F64 || o.coarseSolver == NRS_COARSE_SMOOTH ||
+                  o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI,
               "multigrid: unknown coarse solver");
-    coarseSmoothing = o.coarseSolver == NRS_COARSE_SMOOTH;
+    coarseSmoothing = o.coarseSolver != NRS_COARSE_PCG_F64;
+    coarseJacobi = o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI;
+    NRS_CHECK(!(coarseJacobi && cheby == NO_CHEBY),
+              "multigrid: a Jacobi-smoothed coarse level needs a Chebyshev smoother");
     NRS_CHECK(o.fdmPrecision == NRS_FDM_F32 || o.fdmPrecision == NRS_FDM_BF16 ||
                   o.fdmPrecision == NRS_FDM_F16,
               "multigrid: unknown FDM precision");
@@ -762,7 +767,7 @@ struct MGImpl final : MGSolver {
         l.rhs = l.rhsOwn.get(); l.x = l.xOwn.get();
       }
       if (!l.isCoarse || coarseSmoothing) {
-        if (schwarzKind) {
+        if (schwarzKind && !(l.isCoarse && coarseJacobi)) {
           // Schwarz smoother (pMGLevel::build, ellipticMultiGridSchwarz.cpp:837-1054)
           l.schwarz = std::make_unique<Schwarz<T>>();
           l.schwarz->setup(l.N, l.E, schwarzKind == 2, o.fdmPrecision, lengths, d->EToB, links,

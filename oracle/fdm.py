@@ -340,15 +340,16 @@ class Schwarz:
             for d in range(3):
                 S[d, e], lam[d, e] = matrices_1d(D, w, bc[2 * d], bc[2 * d + 1],
                                                  self.left[d, e], self.mid[d, e], self.right[d, e])
-        if not poisson:
-            # Helmholtz (an extension: the reference's Schwarz smoother is
-            # Poisson-only, ellipticSetup.cpp:59-64): invert lambda0 (Lx + Ly + Lz)
-            # + lambda1, the mass matrix being the identity in the eigenbasis
-            lam = lambda0 * lam + lambda1 / 3.0
+        # Helmholtz (an extension: the reference's Schwarz smoother is
+        # Poisson-only, ellipticSetup.cpp:59-64): invert lambda0 (Lx + Ly + Lz)
+        # + lambda1, the mass matrix being the identity in the eigenbasis; the
+        # modes with Lx + Ly + Lz <= 1e-5 (constant on a box with free ends)
+        # stay dropped as in the Poisson case
+        l0, l1 = (1.0, 0.0) if poisson else (lambda0, lambda1)
         self.S64, self.lam = S, lam
         diag = lam[2][:, :, None, None] + lam[1][:, None, :, None] + lam[0][:, None, None, :]
         with np.errstate(divide="ignore"):
-            invL = np.where(diag > EPS_DIAG, 1.0 / diag, 0.0)
+            invL = np.where(diag > EPS_DIAG, 1.0 / (l0 * diag + l1), 0.0)
         self.invL = invL.astype(T)                          # [e][k][j][i]
         self.S = S.astype(T)
         # extended mesh: ids from an order N+2 box of the same elements (or the

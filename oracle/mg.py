@@ -75,11 +75,12 @@ class MG:
         fmt: None, "bf16" or "fp16" for the FDM contractions; lambda0/lambda1/poisson:
         the (constant-coefficient) operator; coarse: "pcg" (FP64 Jacobi-PCG) or
         "smooth" (MULTIGRID COARSE SOLVE = FALSE: the coarsest level is smoothed,
-        degree 3, ellipticMultiGridSetup.cpp:315-320)."""
+        degree 3, ellipticMultiGridSetup.cpp:315-320) or "smooth_jacobi" (smoothed
+        with Chebyshev-Jacobi whatever S is; an extension)."""
         assert orders[0] == mesh.N
         assert cheby in ("cheby1", "opt4", None) and S in ("jacobi", "asm", "ras")
         assert cheby is not None or S != "jacobi"
-        assert coarse in ("pcg", "smooth")
+        assert coarse in ("pcg", "smooth", "smooth_jacobi")
         self.T = T
         self.cheby, self.S = cheby, S
         self.lambda0, self.lambda1, self.poisson = lambda0, lambda1, poisson
@@ -110,8 +111,8 @@ class MG:
                 L.R = transfer_R(Nprev, N).astype(T)
             if li == 0 and S != "jacobi":
                 fine = (N, x, y, z, wq, L.el.ogs_full)
-            if not L.isCoarse or coarse == "smooth":
-                if S == "jacobi":
+            if not L.isCoarse or coarse != "pcg":
+                if S == "jacobi" or (L.isCoarse and coarse == "smooth_jacobi"):
                     d = build_diag(N, m.ggeo, m.D, lambda0, lambda1, poisson)
                     d = L.el.ogs.gs(d, "add")
                     L.invDiagT = (1.0 / d).astype(T)
@@ -123,7 +124,7 @@ class MG:
             self.levels.append(L)
             Nprev = N
         for L in self.levels:
-            if (not L.isCoarse or coarse == "smooth") and cheby is not None:
+            if (not L.isCoarse or coarse != "pcg") and cheby is not None:
                 L.rho = self._max_eig(L, seed)
                 L.lmax, L.lmin = max_factor * L.rho, min_factor * L.rho
         cl = self.levels[-1]
@@ -176,7 +177,7 @@ class MG:
 
     def smoother_S(self, L, v):
         """pMGLevel::smoother (ellipticMultiGridLevel.cpp:104-111)."""
-        if self.S == "jacobi":
+        if getattr(L, "schwarz", None) is None:
             return (L.invDiagT * v).astype(self.T)
         return L.schwarz.apply(v)
 
@@ -262,7 +263,7 @@ class MG:
     def vcycle(self, k, rhs):
         L = self.levels[k]
         if L.isCoarse:
-            if self.coarse == "smooth":
+            if self.coarse != "pcg":
                 x = np.zeros(L.E * L.Np, dtype=self.T)
                 return self.smooth(L, rhs, x, True)
             x, self.last_coarse_it = self.coarse_solve(L, rhs)

@@ -71,25 +71,33 @@ def test_batch_equals_standalone(gpu, torch_mod):
         assert torch.equal(xb2[p], xb[p])
 
 
-@pytest.mark.parametrize("N,lam,S", [(5, 3.0, "ras"), (7, 8.0, "ras"), (5, 0.5, "jacobi")])
-def test_helmholtz_mg_smoothed_coarse_vs_oracle(gpu, torch_mod, orc, N, lam, S):
+@pytest.mark.parametrize("N,lam,S,coarse,eps", [(5, 3.0, "ras", "smooth", 0.25),
+                                                 (7, 8.0, "ras", "smooth", 0.25),
+                                                 (5, 0.5, "jacobi", "smooth", 0.25),
+                                                 (5, 4.0, "ras", "smooth_jacobi", 0.1),
+                                                 (7, 0.0, "asm", "smooth_jacobi", 0.3)])
+def test_helmholtz_mg_smoothed_coarse_vs_oracle(gpu, torch_mod, orc, N, lam, S, coarse, eps):
     import oracle.mg as omg
     torch = torch_mod
     shape = (4, 3, 3)
-    mg = gpu.BoxMesh(*shape, N, "kershaw", 0.25)
-    smoother = "opt4chebyshev+ras" if S == "ras" else "opt4chebyshev+jacobi"
-    kw = dict(MG, smoother=smoother, tol=1e-10, fdmPrecision=1 if S == "ras" else 0)
+    mg = gpu.BoxMesh(*shape, N, "kershaw", eps)
+    smoother = "opt4chebyshev+" + S
+    kw = dict(MG, smoother=smoother, tol=1e-10, fdmPrecision=1 if S != "jacobi" else 0,
+              coarse=coarse)
     eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=lam, poisson=False, levels=[N, 3, 1], **kw)
-    mo = orc.BoxMesh(*shape, N, "kershaw", 0.25)
+    mo = orc.BoxMesh(*shape, N, "kershaw", eps)
     eo = orc.Elliptic(mo, 1.0, lam, False)
     om = omg.MG(mo, [N, 3, 1], T=np.float32, degree=3, cheby="opt4", S=S, lambda0=1.0,
-                lambda1=lam, poisson=False, coarse="smooth", fmt="bf16" if S == "ras" else None)
+                lambda1=lam, poisson=False, coarse=coarse, fmt="bf16" if S != "jacobi" else None)
     rhs = np.random.default_rng(N).uniform(-1, 1, mo.Nlocal)
     xo, info = omg.fcg_solve(eo, rhs, om.apply, flexible=True, tol=1e-10, maxit=300)
     batch = gpu.Batch([(mg, dict(kw, lambda0=1.0, lambda1=lam, poisson=False, levels=[N, 3, 1]))])
     x = [torch.zeros(mo.Nlocal, dtype=torch.float64, device="cuda")]
     st = batch.solve([dev(torch, rhs)], x)[0]
-    print(N, lam, S, "gpu it", st["iterations"], "oracle it", info["iterations"])
+    lv, _ = eg.mg_info()
+    for L, g in zip(om.levels, lv):
+        assert abs(g["rho"] - L.rho) <= 2e-3 * L.rho, (g["rho"], L.rho)
+    print(N, lam, S, coarse, "gpu it", st["iterations"], "oracle it", info["iterations"])
     assert st["converged"]
     assert abs(st["iterations"] - info["iterations"]) <= 1
     assert rel(x[0].cpu().numpy(), xo) <= 1e-8

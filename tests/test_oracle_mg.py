@@ -146,3 +146,44 @@ def test_arnoldi_breakdown_on_tiny_levels(orc, shape, unknowns):
     eo = orc.Elliptic(mo, 1.0, 2.5, False)
     z = om.apply(eo.ogs.gs(np.where(np.isin(np.arange(mo.Nlocal), eo.maskIds), 0.0, rhs), "add"))
     assert np.all(np.isfinite(z))
+
+
+def test_helmholtz_schwarz_coarse_rho_mesh_independent(orc):
+    """The Helmholtz box solve drops the constant mode of a box with free ends
+    (as the Poisson one does): on undeformed meshes the order-1 level's
+    rho(S A) then stays bounded under refinement instead of growing like
+    1/(lambda1 h^2); with lambda1 = 0 it equals the Poisson smoother."""
+    import oracle.mg as omg
+    rhos = []
+    for n in (4, 8):
+        mo = orc.BoxMesh(n, n, n, 3, "kershaw", 1.0)
+        om = omg.MG(mo, [3, 1], T=np.float64, degree=3, cheby="opt4", S="ras", coarse="smooth",
+                    lambda0=1.0, lambda1=4.0, poisson=False)
+        rhos.append(om.levels[-1].rho)
+    assert max(rhos) < 2.5 and abs(rhos[1] - rhos[0]) < 0.2, rhos
+    mo = orc.BoxMesh(4, 4, 4, 3, "kershaw", 0.5)
+    a = omg.MG(mo, [3, 1], T=np.float64, degree=3, cheby="opt4", S="asm", coarse="smooth")
+    b = omg.MG(mo, [3, 1], T=np.float64, degree=3, cheby="opt4", S="asm", coarse="smooth",
+               lambda0=1.0, lambda1=0.0, poisson=False)
+    for La, Lb in zip(a.levels, b.levels):
+        assert np.array_equal(La.schwarz.invL, Lb.schwarz.invL)
+
+
+def test_jacobi_smoothed_coarse_level(orc):
+    """coarse="smooth_jacobi": Schwarz on the finer levels, Chebyshev-Jacobi on
+    the order-1 level (whose rho is then that of D^-1 A, below 2), and the
+    preconditioned solve converges to the Jacobi-PCG solution."""
+    import oracle.mg as omg
+    mo = orc.BoxMesh(6, 5, 5, 3, "kershaw", 0.1)
+    om = omg.MG(mo, [3, 1], T=np.float32, degree=3, cheby="opt4", S="ras",
+                coarse="smooth_jacobi", lambda0=1.0, lambda1=4.0, poisson=False)
+    Lc = om.levels[-1]
+    assert getattr(Lc, "schwarz", None) is None and Lc.rho < 2.0
+    assert om.levels[0].schwarz is not None
+    eo = orc.Elliptic(mo, 1.0, 4.0, False)
+    rhs = np.random.default_rng(2).uniform(-1, 1, mo.Nlocal)
+    x, info = omg.fcg_solve(eo, rhs, om.apply, flexible=True, tol=1e-10, maxit=300)
+    xj, ij = omg.fcg_solve(eo, rhs, lambda r: eo.inv_diag() * r, flexible=False, tol=1e-12,
+                           maxit=5000)
+    assert info["iterations"] < ij["iterations"] // 4
+    assert np.max(np.abs(x - xj)) <= 1e-7 * np.max(np.abs(xj))
