@@ -135,8 +135,8 @@ def solve_cases(n, N, seed=0):
     return out
 
 
-BATCH_KW = dict(solver="fcg", precon="multigrid", smoother="opt4chebyshev+ras", cheby_degree=3,
-                mgPrecision=0, fdmPrecision=1, coarse="smooth", tol=1e-8, max_iter=500)
+BATCH_KW = dict(solver="fcg", precon="multigrid", smoother="opt4chebyshev+asm", cheby_degree=3,
+                mgPrecision=0, fdmPrecision=1, coarse="smooth_jacobi", tol=1e-8, max_iter=500)
 
 
 def box_dims(E):
@@ -159,16 +159,29 @@ def run_batch(probs, iters_per_launch, seed):
     t_setup = time.perf_counter() - t0
     rng = np.random.default_rng(seed)
     rhs = [torch.from_numpy(rng.uniform(-1, 1, m.Nlocal)).cuda() for m, _ in probs]
-    for rep in range(2):
-        r = [t.clone() for t in rhs]
-        x = [torch.zeros_like(t) for t in rhs]
-        torch.cuda.synchronize()
+    t_mode = {}
+    for mode in ("one_graph", "per_problem"):
         t0 = time.perf_counter()
-        st = b.solve(r, x)
-        torch.cuda.synchronize()
-        t_batch = time.perf_counter() - t0
-    xb = [t.cpu().numpy() for t in x]
-    info = b.info()
+        b.set_mode(mode)
+        t_capture = time.perf_counter() - t0
+        for rep in range(2):
+            r = [t.clone() for t in rhs]
+            x = [torch.zeros_like(t) for t in rhs]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = b.solve(r, x)
+            torch.cuda.synchronize()
+            t_mode[mode] = time.perf_counter() - t0
+        info = b.info()
+        t_mode[mode + "_launches"] = info["last_launches"]
+        t_mode[mode + "_capture_seconds"] = t_capture
+        if mode == "one_graph":
+            xb = [t.cpu().numpy() for t in x]
+            st0 = st
+        else:
+            assert [s_["iterations"] for s_ in st] == [s_["iterations"] for s_ in st0]
+            assert all(np.array_equal(t.cpu().numpy(), xb[p]) for p, t in enumerate(x))
+    t_batch = min(t_mode["one_graph"], t_mode["per_problem"])
     # the same handles, one problem at a time (host loop per iteration)
     t_seq, seq_it, diff = 0.0, [], 0.0
     for p in range(len(probs)):
@@ -192,7 +205,7 @@ def run_batch(probs, iters_per_launch, seed):
             "iterations": its, "iterations_sequential": seq_it,
             "all_converged": all(s["converged"] for s in st),
             "max_abs_diff_batched_vs_sequential": diff,
-            "graph_nodes": info["graph_nodes"], "graph_launches": info["last_launches"],
+            "graph_nodes": info["graph_nodes"], "modes": t_mode,
             "iters_per_launch": iters_per_launch, "dofs": dofs}
 
 
@@ -215,8 +228,10 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
     out = {"uneven": run_batch(probs, iters_per_launch, seed + 1)}
     out["uneven"]["cases"] = desc
     u = out["uneven"]
-    log("batch: %.3f s batched vs %.3f s one by one (x%.2f), %.2f G dof-iterations/s, its %d..%d" %
-        (u["seconds_batched"], u["seconds_sequential"], u["speedup_batched_vs_sequential"],
+    log("batch: %.3f s batched (one graph %.3f s, graph per problem %.3f s) vs %.3f s one by one "
+        "(x%.2f), %.2f G dof-iterations/s, its %d..%d" %
+        (u["seconds_batched"], u["modes"]["one_graph"], u["modes"]["per_problem"],
+         u["seconds_sequential"], u["speedup_batched_vs_sequential"],
          u["dof_iterations_per_s_batched"] / 1e9, min(u["iterations"]), max(u["iterations"])))
     del probs
     if block_n:
@@ -229,8 +244,8 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
     out["workload"] = ("%d Helmholtz problems (seed %d): N in {3,5,7,9}, E ~ U[512,8192] as "
                        "near-cubic boxes, lambda1 ~ U[0,10], Kershaw eps ~ U[0.05,0.3]; plus 3 "
                        "velocity components (lambda1=10) at %d^3 N=7 eps 0.3; FP32 p-MG, opt. "
-                       "4th-kind Chebyshev + RAS with BF16 FDM, smoothed coarse level, FCG to "
-                       "1e-8 relative" % (count, seed, block_n))
+                       "4th-kind Chebyshev + ASM with BF16 FDM, Chebyshev-Jacobi smoothed "
+                       "coarse level, FCG to 1e-8 relative" % (count, seed, block_n))
     return out
 
 

@@ -297,22 +297,32 @@ int nrs_coarse_global_ids(int Nf, int Nelements, const long long *idsFine, int N
 /* No counterpart in the reference (one elliptic_t per field and rank there).
  * n independent problems (any order, size, lambda, options), each set up as by
  * nrs_elliptic_setup; multigrid problems must use NRS_COARSE_SMOOTH (the solve
- * is device-driven).  Setup captures itersPerLaunch iterations of all problems
- * into one HIP graph (problems as parallel branches). */
+ * is device-driven).  Setup captures itersPerLaunch iterations of each problem
+ * into a HIP graph per problem (NRS_BATCH_GRAPH_PER_PROBLEM, the default; see
+ * nrs_batch_set_mode for the single-graph form). */
 typedef struct nrs_batch nrs_batch;
 int nrs_batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic_opts *opts,
                     int itersPerLaunch, nrs_batch **out);
 /* Solve all problems: rhs[p], x[p] device arrays as in nrs_elliptic_solve
  * (x: initial guess in, solution out; rhs: final residual out); stats[n].
- * The graph is replayed until every problem has converged or reached maxIter. */
+ * Rounds of graph launches run until every problem has converged or reached
+ * maxIter. */
 int nrs_batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_stats *stats,
                     void *stream);
-/* Problem count, nodes of the captured graph, graph launches of the last solve. */
+/* Problem count, nodes of the graph(s) of the current mode, rounds of graph
+ * launches of the last solve. */
 int nrs_batch_info(nrs_batch *b, int *n, int64_t *graphNodes, int *lastLaunches);
 int nrs_batch_free(nrs_batch *b);
 /* The handle of problem p (owned by the batch: do not free it).  A stand-alone
  * nrs_elliptic_solve on it runs the same iteration one problem at a time. */
 int nrs_batch_problem(nrs_batch *b, int p, nrs_elliptic **el);
+/* How solves advance the problems: NRS_BATCH_ONE_GRAPH replays the single
+ * graph of all problems (converged problems' nodes switched off);
+ * NRS_BATCH_GRAPH_PER_PROBLEM captures one graph per problem (at the first
+ * solve in this mode) and launches only the active problems' graphs, spread
+ * over a pool of streams, per round.  Results are identical. */
+enum nrs_batch_mode { NRS_BATCH_ONE_GRAPH = 0, NRS_BATCH_GRAPH_PER_PROBLEM = 1 };
+int nrs_batch_set_mode(nrs_batch *b, int mode);
 
 /* ------------------------------------------------------------------------- */
 /* Kernel launch variants (KERNEL AUTOTUNING; the reference times p_knl       */

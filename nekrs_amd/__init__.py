@@ -308,6 +308,11 @@ class Batch:
         _lib.call("nrs_batch_info", self._h, C.byref(n), C.byref(nodes), C.byref(launches))
         return {"problems": n.value, "graph_nodes": nodes.value, "last_launches": launches.value}
 
+    def set_mode(self, mode):
+        """"one_graph" (all problems in one graph) or "per_problem" (a graph per
+        problem, active problems only)."""
+        _lib.call("nrs_batch_set_mode", self._h, {"one_graph": 0, "per_problem": 1}[mode])
+
     def problem(self, p):
         """Problem p's solver handle (borrowed: valid while the batch lives)."""
         h = C.c_void_p()

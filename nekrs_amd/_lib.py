@@ -83,6 +83,7 @@ SIGNATURES = {
     "nrs_batch_info": ([_vp, C.POINTER(_i), C.POINTER(_i64), C.POINTER(_i)], _i),
     "nrs_batch_free": ([_vp], _i),
     "nrs_batch_problem": ([_vp, _i, C.POINTER(_vp)], _i),
+    "nrs_batch_set_mode": ([_vp, _i], _i),
     "nrs_tuned_variant": ([_i, _i, _i, _i64, C.POINTER(_i)], _i),
     "nrs_set_variant": ([_i, _i, _i, _i64, _i], _i),
     "nrs_elliptic_profile_enable": ([_vp, _i], _i),

@@ -23,8 +23,20 @@ struct nrs_batch {
   std::vector<std::vector<hipGraphNode_t>> nodes;
   std::vector<char> disabled;
   bool canDisable = false;
+  // mode 1: one graph per problem, launched for the active problems only on a
+  // pool of streams (captured at the first solve in that mode)
+  int mode = 0;
+  std::vector<hipGraph_t> pgraph;
+  std::vector<hipGraphExec_t> pexec;
+  std::vector<hipEvent_t> poolDone;
+  hipEvent_t begun = nullptr;
+  size_t perProblemNodes = 0;
   ~nrs_batch()
   {
+    for (auto g : pexec) if (g) (void)hipGraphExecDestroy(g);
+    for (auto g : pgraph) if (g) (void)hipGraphDestroy(g);
+    for (auto e : poolDone) (void)hipEventDestroy(e);
+    if (begun) (void)hipEventDestroy(begun);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto e : joins) (void)hipEventDestroy(e);

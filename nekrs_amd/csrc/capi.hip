@@ -388,7 +388,7 @@ int nrs_batch_info(nrs_batch *b, int *n, int64_t *graphNodes, int *lastLaunches)
   NRS_API_BEGIN
   NRS_CHECK(b, "batch info: null handle");
   if (n) *n = (int)b->probs.size();
-  if (graphNodes) *graphNodes = (int64_t)b->graphNodes;
+  if (graphNodes) *graphNodes = (int64_t)(b->mode == 0 ? b->graphNodes : b->perProblemNodes);
   if (lastLaunches) *lastLaunches = b->lastLaunches;
   NRS_API_END
 }
@@ -397,6 +397,14 @@ int nrs_batch_free(nrs_batch *b)
 {
   NRS_API_BEGIN
   delete b;
+  NRS_API_END
+}
+
+int nrs_batch_set_mode(nrs_batch *b, int mode)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(b, "batch mode: null handle");
+  nrs::batch_set_mode(b, mode);
   NRS_API_END
 }
 

@@ -64,6 +64,7 @@ void elliptic_solve_end(nrs_elliptic *el, double *x, hipStream_t s);
 nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic_opts *opts,
                        int itersPerLaunch);
 int batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_stats *st, hipStream_t s);
+void batch_set_mode(nrs_batch *b, int mode);
 
 // Shared helpers for the multigrid levels.
 std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds);

@@ -69,6 +69,16 @@ def test_batch_equals_standalone(gpu, torch_mod):
     for p in range(len(probs)):
         assert st2[p]["iterations"] == st[p]["iterations"]
         assert torch.equal(xb2[p], xb[p])
+    # all problems in one graph (converged ones switched off): the same results
+    batch.set_mode("one_graph")
+    assert batch.info()["graph_nodes"] > info["graph_nodes"] // 2
+    rb3 = [dev(torch, r) for r in rhs]
+    xb3 = [torch.zeros(m.Nlocal, dtype=torch.float64, device="cuda") for m, _ in probs]
+    st3 = batch.solve(rb3, xb3)
+    for p in range(len(probs)):
+        assert st3[p]["iterations"] == st[p]["iterations"]
+        assert torch.equal(xb3[p], xb[p])
+        assert torch.equal(rb3[p], rb[p])
 
 
 @pytest.mark.parametrize("N,lam,S,coarse,eps", [(5, 3.0, "ras", "smooth", 0.25),
