@@ -263,7 +263,19 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--batch-iters-per-launch", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 512 MiB (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 5 x %d bytes" % (a.numel() * 8))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the multi-GPU halo exchange of this path is "
               "designed but not built, so --gpus %d is not supported" % args.gpus, file=sys.stderr)

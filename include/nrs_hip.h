@@ -332,10 +332,13 @@ int nrs_batch_set_mode(nrs_batch *b, int mode);
 /* Variants: NRS_TUNE_AX = elements-per-block multiplier {1, 2, 4} of the
  * operator kernel, keyed by dtype (nrs_dtype); NRS_TUNE_FDM = elements per
  * wavefront {1, 2, 4, 8} of the matrix-core Schwarz kernel, keyed by the
- * nrs_fdm_precision.  Keys are (kernel, N, type, floor(log2 Nelements)); a
+ * nrs_fdm_precision; NRS_TUNE_OPERATOR = element chunks {1, 2, 4, 8, 16} of
+ * the solver operator (Ax of chunk c+1 overlapped with the gather-scatter of
+ * the rows chunk c completes), keyed by dtype.
+ * Keys are (kernel, N, type, floor(log2 Nelements)); a
  * setup with opts.autotune records the fastest variant whose result matches
  * the default's.  variant 0 = none recorded (the default is used). */
-enum nrs_tune_kernel { NRS_TUNE_AX = 0, NRS_TUNE_FDM = 1 };
+enum nrs_tune_kernel { NRS_TUNE_AX = 0, NRS_TUNE_FDM = 1, NRS_TUNE_OPERATOR = 2 };
 int nrs_tuned_variant(int kernel, int N, int type, int64_t Nelements, int *variant);
 /* Record a variant by hand (0 clears it); later launches with this key use it. */
 int nrs_set_variant(int kernel, int N, int type, int64_t Nelements, int variant);

@@ -256,13 +256,13 @@ def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=5
     return o
 
 
-TUNE_KERNELS = {"ax": 0, "fdm": 1}
+TUNE_KERNELS = {"ax": 0, "fdm": 1, "op": 2}
 
 
 def tuned_variant(kernel, N, dtype, nelements):
     """Recorded launch variant for (kernel, N, type, E), 0 if none (nrs_tuned_variant).
-    type: a DTYPES name for "ax", an FDM_PRECISIONS name for "fdm"."""
-    t = DTYPES[dtype] if kernel == "ax" else FDM_PRECISIONS[dtype]
+    type: a DTYPES name for "ax" and "op", an FDM_PRECISIONS name for "fdm"."""
+    t = FDM_PRECISIONS[dtype] if kernel == "fdm" else DTYPES[dtype]
     v = C.c_int()
     _lib.call("nrs_tuned_variant", TUNE_KERNELS[kernel], int(N), t, int(nelements), C.byref(v))
     return v.value
@@ -270,7 +270,7 @@ def tuned_variant(kernel, N, dtype, nelements):
 
 def set_variant(kernel, N, dtype, nelements, variant):
     """Force a launch variant (0 clears) for later launches with this key."""
-    t = DTYPES[dtype] if kernel == "ax" else FDM_PRECISIONS[dtype]
+    t = FDM_PRECISIONS[dtype] if kernel == "fdm" else DTYPES[dtype]
     _lib.call("nrs_set_variant", TUNE_KERNELS[kernel], int(N), t, int(nelements), int(variant))
 
 

@@ -2,6 +2,7 @@
 // ellipticSolve/pcg driver (host C++ enqueueing HIP kernels on the caller's stream).
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 #include "elliptic.hpp"
 #include "tune.hpp"
@@ -17,6 +18,9 @@ nrs_elliptic::~nrs_elliptic()
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   for (auto e : eventPool) (void)hipEventDestroy(e);
+  for (auto e : chunkEv) if (e) (void)hipEventDestroy(e);
+  if (auxDone) (void)hipEventDestroy(auxDone);
+  if (aux) (void)hipStreamDestroy(aux);
 }
 
 hipEvent_t nrs_elliptic::next_event()
@@ -140,10 +144,23 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     el->ogs = ogs_setup(el->Nlocal, mg.data());
   }
 
+  // operator chunks: rows of the masked handle grouped by element chunk
+  {
+    constexpr int F = nrs_elliptic::kOpChunks;
+    const int64_t epc = (el->E + F - 1) / F;
+    ogs_set_chunks(el->ogs, epc * el->Np, F);
+    NRS_HIP(hipStreamCreateWithFlags(&el->aux, hipStreamNonBlocking));
+    for (auto &e : el->chunkEv) NRS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    NRS_HIP(hipEventCreateWithFlags(&el->auxDone, hipEventDisableTiming));
+  }
+
   // operator launch variants (KERNEL AUTOTUNING, registerEllipticKernels.cpp:170-183)
-  if (o->autotune)
+  if (o->autotune) {
     autotune_ax<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(), el->lam1.get(),
                         el->poisson, s);
+    autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                              el->lam1.get(), el->poisson, el->maskBits.get(), el->ogs, s);
+  }
 
   // workspace
   el->p.alloc(el->Nlocal); el->zv.alloc(el->Nlocal); el->Ap.alloc(el->Nlocal); el->x0.alloc(el->Nlocal);
@@ -165,22 +182,100 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   return el.release();
 }
 
-void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s)
+int operator_chunks(int N, int dtype, int64_t E) { return tuned(TUNE_OP, N, dtype, E, 1); }
+
+template <typename T>
+void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
+                      const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
+                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record)
 {
+  constexpr int F = nrs_elliptic::kOpChunks;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-  if (el->profiling) {
+  record = record && el->profiling;
+  if (record) {
     e0 = el->next_event(); e1 = el->next_event(); e2 = el->next_event();
     NRS_HIP(hipEventRecord(e0, s));
   }
-  launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                    el->lam1.get(), el->poisson, false, q, Aq, masked ? el->maskBits.get() : nullptr, s);
-  if (el->profiling) NRS_HIP(hipEventRecord(e1, s));
-  ogs_gather_scatter<double>(el->ogs, Aq, 1, el->Nlocal, NRS_ADD, s);
-  if (el->profiling) {
+  const int64_t L = (int64_t)E * (N + 1) * (N + 1) * (N + 1);
+  if (nChunks <= 1 || ogs->nChunks != F || !el->aux) {
+    launch_ax<T>(N, E, nullptr, ggeo, D, l0, l1, poisson, false, q, Aq, maskBits, s);
+    if (record) NRS_HIP(hipEventRecord(e1, s));
+    ogs_gather_scatter<T>(ogs, Aq, 1, L, NRS_ADD, s);
+  } else {
+    NRS_CHECK(F % nChunks == 0, "operator: chunk count must divide 16");
+    const int Np = (N + 1) * (N + 1) * (N + 1), NW = (Np + 31) / 32;
+    const int64_t epc = ogs->nodesPerChunk / Np;
+    const int per = F / nChunks;
+    for (int c = 0; c < nChunks; ++c) {
+      const int64_t a = std::min<int64_t>(E, (int64_t)c * per * epc);
+      const int64_t b = std::min<int64_t>(E, (int64_t)(c + 1) * per * epc);
+      if (b > a)
+        launch_ax<T>(N, (int)(b - a), nullptr, ggeo + a * 7 * Np, D, l0, l1, poisson, false, q + a * Np,
+                     Aq + a * Np, maskBits ? maskBits + a * NW : nullptr, s);
+      NRS_HIP(hipEventRecord(el->chunkEv[c], s));
+      NRS_HIP(hipStreamWaitEvent(el->aux, el->chunkEv[c], 0));
+      ogs_gather_scatter_chunks<T>(ogs, Aq, c * per, (c + 1) * per, el->aux);
+    }
+    if (record) NRS_HIP(hipEventRecord(e1, s));
+    NRS_HIP(hipEventRecord(el->auxDone, el->aux));
+    NRS_HIP(hipStreamWaitEvent(s, el->auxDone, 0));
+  }
+  if (record) {
     NRS_HIP(hipEventRecord(e2, s));
     el->spans.push_back({NRS_PHASE_AX, e0, e1});
     el->spans.push_back({NRS_PHASE_GS, e1, e2});
   }
+}
+template void chunked_operator<double>(nrs_elliptic *, int, int, const double *, const double *,
+                                       const double *, const double *, bool, const uint32_t *,
+                                       const nrs_ogs *, const double *, double *, int, hipStream_t, bool);
+template void chunked_operator<float>(nrs_elliptic *, int, int, const float *, const float *,
+                                      const float *, const float *, bool, const uint32_t *,
+                                      const nrs_ogs *, const float *, float *, int, hipStream_t, bool);
+
+template <typename T>
+int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
+                      const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
+                      hipStream_t s)
+{
+  const int64_t L = (int64_t)E * (N + 1) * (N + 1) * (N + 1);
+  DevBuf<T> q(L), ref(L), out(L);
+  fill_pattern<T>(L, q.get(), s);
+  auto run = [&](int k, T *o) {
+    chunked_operator<T>(el, N, E, ggeo, D, l0, l1, poisson, maskBits, ogs, q.get(), o, k, s, false);
+  };
+  run(1, ref.get());
+  std::vector<T> hr(L), ho(L);
+  NRS_HIP(hipMemcpyAsync(hr.data(), ref.get(), L * sizeof(T), hipMemcpyDeviceToHost, s));
+  NRS_HIP(hipStreamSynchronize(s));
+  int vals[8];
+  const int nv = tune_variants(TUNE_OP, vals, 8);
+  int best = 1;
+  double bestMs = 1e300;
+  for (int v = 0; v < nv; ++v) {
+    run(vals[v], out.get());
+    NRS_HIP(hipMemcpyAsync(ho.data(), out.get(), L * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    // the same arithmetic in every form: bit-identical results required
+    if (std::memcmp(ho.data(), hr.data(), L * sizeof(T)) != 0) continue;
+    const double ms = time_launches([&] { run(vals[v], out.get()); }, s);
+    if (ms < bestMs) { bestMs = ms; best = vals[v]; }
+  }
+  set_tuned(TUNE_OP, N, dtype_of<T>::value, E, best);
+  return best;
+}
+template int autotune_operator<double>(nrs_elliptic *, int, int, const double *, const double *,
+                                       const double *, const double *, bool, const uint32_t *,
+                                       const nrs_ogs *, hipStream_t);
+template int autotune_operator<float>(nrs_elliptic *, int, int, const float *, const float *,
+                                      const float *, const float *, bool, const uint32_t *,
+                                      const nrs_ogs *, hipStream_t);
+
+void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s)
+{
+  chunked_operator<double>(el, el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                           el->lam1.get(), el->poisson, masked ? el->maskBits.get() : nullptr,
+                           el->ogs, q, Aq, operator_chunks(el->N, NRS_F64, el->E), s, true);
 }
 
 void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s)

@@ -38,6 +38,15 @@ struct nrs_elliptic {
   std::unique_ptr<nrs::MGSolver> mg;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
+  // Chunked operator: the masked handle's rows are grouped into kOpChunks
+  // element chunks; Ax of chunk c+1 runs on the caller's stream while the
+  // gather-scatter of the rows completed by chunk c runs on `aux`, reading
+  // values that are still in cache.
+  static constexpr int kOpChunks = 16;
+  hipStream_t aux = nullptr;
+  hipEvent_t chunkEv[kOpChunks] = {};
+  hipEvent_t auxDone = nullptr;
+
   // Per-phase device timing (platform->timer analogue, src/core/timer.cpp:131-230):
   // when enabled, events bracket each Ax and gather-scatter launch of the
   // operator on the caller's stream; totals are read after a synchronisation.
@@ -65,6 +74,23 @@ nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic
                        int itersPerLaunch);
 int batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_stats *st, hipStream_t s);
 void batch_set_mode(nrs_batch *b, int mode);
+
+// Aq = gs(mask(Ax(q))) with the Ax split into nChunks (1, 2, 4, 8, 16) element
+// chunks overlapped with the gather-scatter of completed rows (ogs must have
+// been chunked into kOpChunks chunks of ceil(E / kOpChunks) elements); nChunks
+// = 1 or an unchunked handle runs Ax then gs.  Same results in every form.
+// record: add the Ax / gs spans to the handle's device timers when enabled.
+template <typename T>
+void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
+                      const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
+                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record = false);
+// the operator's chunk count for (N, type, E): tuned, else 1
+int operator_chunks(int N, int dtype, int64_t E);
+// time nChunks = 1, 2, 4, 8, 16 on this operator, record the fastest
+template <typename T>
+int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
+                      const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
+                      hipStream_t s);
 
 // Shared helpers for the multigrid levels.
 std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds);

@@ -381,11 +381,13 @@ struct MGImpl final : MGSolver {
   bool coarseJacobi = false;         // ... with Chebyshev-Jacobi whatever the finer levels use
 
   // ---- level operators (pMGLevel::Ax / residual / smoother)
+  // the level operator; the finest level shares the solver's chunked handle,
+  // so its Ax overlaps its gather-scatter as the outer operator's does
   void op(Level<T> &l, const T *q, T *Aq, hipStream_t s)
   {
-    launch_ax<T>(l.N, l.E, nullptr, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, false, q, Aq,
-                 l.maskBits, s);
-    ogs_gather_scatter<T>(l.ogs, Aq, 1, l.L, NRS_ADD, s);
+    const int k = (l.ogs->nChunks == nrs_elliptic::kOpChunks) ? operator_chunks(l.N, dtype_of<T>::value, l.E) : 1;
+    chunked_operator<T>(el, l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, l.maskBits,
+                        l.ogs, q, Aq, k, s);
   }
 
   // pMGLevel::smoother (ellipticMultiGridLevel.cpp:104-111); out != in
@@ -786,7 +788,12 @@ struct MGImpl final : MGSolver {
         if (o.autotune && l.schwarz) l.schwarz->autotune(s);
       }
       // level operator launch variants (registerEllipticPreconditionerKernels.cpp:52-66)
-      if (o.autotune) autotune_ax<T>(l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, s);
+      if (o.autotune) {
+        autotune_ax<T>(l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, s);
+        if (l.ogs->nChunks == nrs_elliptic::kOpChunks)
+          autotune_operator<T>(el, l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson,
+                               l.maskBits, l.ogs, s);
+      }
       Nprev = l.N;
       levels.push_back(std::move(lp));
     }

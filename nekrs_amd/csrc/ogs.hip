@@ -193,6 +193,74 @@ template void ogs_gather_scatter<double>(const nrs_ogs *, double *, int, int64_t
 template void ogs_gather_scatter<float>(const nrs_ogs *, float *, int, int64_t, int, hipStream_t);
 template void ogs_gather_scatter<int>(const nrs_ogs *, int *, int, int64_t, int, hipStream_t);
 
+template <typename T>
+void ogs_gather_scatter_chunks(const nrs_ogs *o, T *v, int c0, int c1, hipStream_t s)
+{
+  NRS_CHECK(o != nullptr, "ogs: null handle");
+  NRS_CHECK(o->nChunks > 0 && c0 >= 0 && c0 <= c1 && c1 <= o->nChunks, "ogs: bad chunk range");
+  const int64_t r2 = o->cOff[0][c0], r4 = o->cOff[1][c0], r8 = o->cOff[2][c0], rx = o->cOff[3][c0];
+  const int n2 = (int)(o->cOff[0][c1] - r2), n4 = (int)(o->cOff[1][c1] - r4);
+  const int n8 = (int)(o->cOff[2][c1] - r8), nx = (int)(o->cOff[3][c1] - rx);
+  const int b2 = (n2 + kGsBlock - 1) / kGsBlock;
+  const int b4 = b2 + (n4 + kGsBlock - 1) / kGsBlock;
+  const int b8 = b4 + (n8 + kGsBlock - 1) / kGsBlock;
+  const int bt = b8 + (nx + kGsBlock - 1) / kGsBlock;
+  if (bt == 0) return;
+  gs_kernel<T, NRS_ADD><<<bt, kGsBlock, 0, s>>>(n2, n4, n8, nx, b2, b4, b8, o->pairs.get() + 2 * r2,
+                                                o->quads.get() + 4 * r4, o->octs.get() + 8 * r8,
+                                                o->xoff.get() + rx, o->xids.get(), v, 1, 0);
+  NRS_LAUNCH_CHECK();
+}
+template void ogs_gather_scatter_chunks<double>(const nrs_ogs *, double *, int, int, hipStream_t);
+template void ogs_gather_scatter_chunks<float>(const nrs_ogs *, float *, int, int, hipStream_t);
+
+void ogs_set_chunks(nrs_ogs *o, int64_t nodesPerChunk, int nChunks)
+{
+  NRS_CHECK(o != nullptr && nodesPerChunk >= 1 && nChunks >= 1, "ogs: bad chunking");
+  auto chunk_of = [&](int last) { return (int)std::min<int64_t>(last / nodesPerChunk, nChunks - 1); };
+  // fixed-width buckets: stable sort of rows by chunk
+  auto regroup = [&](std::vector<int> &h, int W, std::vector<int64_t> &off) {
+    const int64_t n = (int64_t)h.size() / W;
+    off.assign(nChunks + 1, 0);
+    for (int64_t r = 0; r < n; ++r) off[chunk_of(h[r * W + W - 1]) + 1]++;
+    for (int c = 0; c < nChunks; ++c) off[c + 1] += off[c];
+    std::vector<int> out(h.size());
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t d = pos[chunk_of(h[r * W + W - 1])]++;
+      std::copy(h.begin() + r * W, h.begin() + (r + 1) * W, out.begin() + d * W);
+    }
+    h.swap(out);
+  };
+  regroup(o->h2, 2, o->cOff[0]);
+  regroup(o->h4, 4, o->cOff[1]);
+  regroup(o->h8, 8, o->cOff[2]);
+  {   // CSR rows
+    const int64_t n = (int64_t)o->hxo.size() - 1;
+    auto &off = o->cOff[3];
+    off.assign(nChunks + 1, 0);
+    for (int64_t r = 0; r < n; ++r) off[chunk_of(o->hxi[o->hxo[r + 1] - 1]) + 1]++;
+    for (int c = 0; c < nChunks; ++c) off[c + 1] += off[c];
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    std::vector<int64_t> dst(n);
+    for (int64_t r = 0; r < n; ++r) dst[r] = pos[chunk_of(o->hxi[o->hxo[r + 1] - 1])]++;
+    std::vector<int> len(n), xo(n + 1, 0), xi(o->hxi.size());
+    for (int64_t r = 0; r < n; ++r) len[dst[r]] = o->hxo[r + 1] - o->hxo[r];
+    for (int64_t r = 0; r < n; ++r) xo[r + 1] = xo[r] + len[r];
+    for (int64_t r = 0; r < n; ++r)
+      std::copy(o->hxi.begin() + o->hxo[r], o->hxi.begin() + o->hxo[r + 1], xi.begin() + xo[dst[r]]);
+    o->hxo.swap(xo);
+    o->hxi.swap(xi);
+  }
+  o->pairs.upload(o->h2);
+  o->quads.upload(o->h4);
+  o->octs.upload(o->h8);
+  o->xoff.upload(o->hxo);
+  o->xids.upload(o->hxi);
+  o->nChunks = nChunks;
+  o->nodesPerChunk = nodesPerChunk;
+}
+
 nrs_ogs *ogs_setup(int64_t N, const long long *ids)
 {
   NRS_CHECK(N >= 0 && N < (int64_t)INT32_MAX, "ogs: N must fit in int32 (dlong)");
@@ -280,6 +348,7 @@ nrs_ogs *ogs_setup(int64_t N, const long long *ids)
   o->xoff.upload(hxo);
   o->xids.upload(hxi);
   o->invDegree.upload(o->invDegreeHost);
+  o->h2.swap(h2); o->h4.swap(h4); o->h8.swap(h8); o->hxo.swap(hxo); o->hxi.swap(hxi);
   return o;
 }
 

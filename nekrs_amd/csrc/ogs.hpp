@@ -18,6 +18,15 @@ struct nrs_ogs {
   nrs::DevBuf<int> xids;
   nrs::DevBuf<double> invDegree;  // [N]
   std::vector<double> invDegreeHost;
+  // Chunks (optional, nrs::ogs_set_chunks): inside each bucket the rows are
+  // grouped by the chunk of their last member (local index / nodesPerChunk),
+  // keeping their order inside a group; chunk c's rows of bucket b are
+  // [cOff[b][c], cOff[b][c+1]).  A row of chunk c only touches nodes below
+  // the end of chunk c.
+  int nChunks = 0;
+  int64_t nodesPerChunk = 0;
+  std::vector<int64_t> cOff[4];
+  std::vector<int> h2, h4, h8, hxo, hxi;   // host copies of the buckets
 };
 
 namespace nrs {
@@ -30,4 +39,12 @@ nrs_ogs *ogs_setup(int64_t N, const long long *ids);
 // are (their weight is 1).
 template <typename T>
 void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s);
+// Group rows by chunk (chunk = last member / nodesPerChunk, at most nChunks - 1).
+// Reorders rows inside buckets only: every gather-scatter result is unchanged.
+void ogs_set_chunks(nrs_ogs *o, int64_t nodesPerChunk, int nChunks);
+// v <- gs_add(v) on the rows of chunks [c0, c1) only (one vector).  After the
+// nodes of those chunks are final, this completes their rows; rows of later
+// chunks are untouched.
+template <typename T>
+void ogs_gather_scatter_chunks(const nrs_ogs *o, T *v, int c0, int c1, hipStream_t s);
 }  // namespace nrs
