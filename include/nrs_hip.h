@@ -296,8 +296,8 @@ int nrs_coarse_global_ids(int Nf, int Nelements, const long long *idsFine, int N
 /* ------------------------------------------------------------------------- */
 /* No counterpart in the reference (one elliptic_t per field and rank there).
  * n independent problems (any order, size, lambda, options), each set up as by
- * nrs_elliptic_setup; multigrid problems must use NRS_COARSE_SMOOTH (the solve
- * is device-driven).  Setup captures itersPerLaunch iterations of each problem
+ * nrs_elliptic_setup; multigrid problems must use a smoothed coarse level,
+ * NRS_COARSE_SMOOTH or NRS_COARSE_SMOOTH_JACOBI (the solve is device-driven).  Setup captures itersPerLaunch iterations of each problem
  * into a HIP graph per problem (NRS_BATCH_GRAPH_PER_PROBLEM, the default; see
  * nrs_batch_set_mode for the single-graph form). */
 typedef struct nrs_batch nrs_batch;
