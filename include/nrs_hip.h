@@ -181,7 +181,15 @@ enum nrs_smoother {
  * smoothed with Chebyshev-Jacobi even when the finer levels use Schwarz (no
  * counterpart in the reference: its Schwarz boxes on an order-1 level with
  * free box ends make a poor smoother there). */
-enum nrs_coarse { NRS_COARSE_PCG_F64 = 0, NRS_COARSE_SMOOTH = 1, NRS_COARSE_SMOOTH_JACOBI = 2 };
+enum nrs_coarse {
+  NRS_COARSE_PCG_F64 = 0,        /* FP64 Jacobi-PCG to coarseTol                       */
+  NRS_COARSE_SMOOTH = 1,
+  NRS_COARSE_SMOOTH_JACOBI = 2,
+  NRS_COARSE_AMG = 3             /* FP64 PCG on the assembled order-1 matrix,
+                                  * preconditioned by one smoothed-aggregation AMG
+                                  * V-cycle (in place of BoomerAMG,
+                                  * coarseLevel.cpp:47-222), to coarseTol          */
+};
 enum nrs_mg_precision { NRS_MG_F32 = 0, NRS_MG_F64 = 1 };
 enum nrs_fdm_precision { NRS_FDM_F32 = 0, NRS_FDM_BF16 = 1, NRS_FDM_F16 = 2 };
 

@@ -219,7 +219,7 @@ def default_opts():
     return o
 
 
-COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2}
+COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2, "amg": 3}
 
 
 def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True):

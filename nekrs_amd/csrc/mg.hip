@@ -20,6 +20,7 @@
 #include <map>
 #include <type_traits>
 
+#include "amg.hpp"
 #include "fdm.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
@@ -297,6 +298,15 @@ struct CoarsePCG {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   static constexpr int kChunk = 8;
+  // AMG mode: PCG on the assembled matrix of the unmasked unique nodes
+  // (rows numbered by first appearance), preconditioned by one
+  // smoothed-aggregation V-cycle; E-vector <-> rows through rowOf / first.
+  bool amgMode = false;
+  int n = 0;
+  Amg amg;
+  DevBuf<int> rowOf, first, arp, aci;
+  DevBuf<double> av, ones, ru, xu, zu, pu, Apu, tmp;
+  static constexpr int kChunkAmg = 2;
   ~CoarsePCG()
   {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -307,6 +317,18 @@ struct CoarsePCG {
 
   void chunk(hipStream_t s)
   {
+    if (amgMode) {
+      for (int c = 0; c < kChunkAmg; ++c) {
+        amg.apply(ru.get(), zu.get(), tmp.get(), s);
+        pcg_precon_dots(*red, n, ones.get(), nullptr, ru.get(), zu.get(), Apu.get(), false, false, false,
+                        scal.get(), s);
+        pcg_update_p(n, zu.get(), pu.get(), scal.get(), s);
+        csr_spmv(SPMV_Y, n, arp.get(), aci.get(), av.get(), pu.get(), Apu.get(), nullptr, nullptr, 0.0, s);
+        pcg_alpha(*red, n, ones.get(), pu.get(), Apu.get(), scal.get(), s);
+        pcg_update_rx(*red, n, ones.get(), Apu.get(), pu.get(), ru.get(), xu.get(), 1.0, scal.get(), s);
+      }
+      return;
+    }
     const double *w = ogs->invDegree.get();
     for (int c = 0; c < kChunk; ++c) {
       pcg_precon_dots(*red, L, w, invDiag.get(), r.get(), z.get(), Ap.get(), true, false, false,
@@ -342,6 +364,7 @@ struct CoarsePCG {
   template <typename T>
   void solve(const T *rhs, T *xout, hipStream_t s)
   {
+    if (amgMode) { solve_amg(rhs, xout, s); return; }
     const double *w = ogs->invDegree.get();
     k_cast<T, double>(L, rhs, r.get(), s);
     if (std::is_same<T, double>::value && (const void *)rhs == (const void *)r.get()) {}
@@ -363,6 +386,113 @@ struct CoarsePCG {
       lastIters = (int)scalHost[S_ITER];
     }
     k_cast<double, T>(L, x.get(), xout, s);
+  }
+
+  template <typename T>
+  void solve_amg(const T *rhs, T *xout, hipStream_t s)
+  {
+    rows_gather<T>(n, first.get(), rhs, ru.get(), s);
+    fill<double>(n, 0.0, xu.get(), s);
+    fill<double>(n, 0.0, pu.get(), s);
+    wnorm(*red, n, ones.get(), ru.get(), 1.0, scal.get(), S_TMP0, s);
+    NRS_HIP(hipMemcpyAsync(scalHost, scal.get(), S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    const double r0 = scalHost[S_TMP0];
+    lastIters = 0;
+    if (r0 > 0.0 && !std::isnan(r0)) {
+      pcg_init(scal.get(), r0, tol * r0, maxit, s);
+      for (int done = 0; done < maxit; done += kChunkAmg) {
+        run_chunk(s);
+        NRS_HIP(hipMemcpyAsync(scalHost, scal.get(), S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        NRS_HIP(hipStreamSynchronize(s));
+        if (scalHost[S_ACTIVE] == 0.0) break;
+      }
+      lastIters = (int)scalHost[S_ITER];
+    }
+    rows_scatter<T>(L, rowOf.get(), xu.get(), xout, s);
+  }
+
+  // Assemble the level operator on the unique unmasked nodes (element
+  // matrices = the operator applied to unit vectors, one per element node)
+  // and build the AMG hierarchy on it.
+  void setup_amg(const std::vector<long long> &maskedIds, hipStream_t s)
+  {
+    const int Np = (N + 1) * (N + 1) * (N + 1);
+    std::vector<int> rows(L, -1), firstH;
+    {
+      std::vector<std::pair<long long, int64_t>> ids;
+      ids.reserve(L);
+      for (int64_t i = 0; i < L; ++i)
+        if (maskedIds[i] != 0) ids.emplace_back(maskedIds[i], i);
+      std::sort(ids.begin(), ids.end());
+      // rows by first appearance: the smallest local index of each id
+      std::vector<std::pair<int64_t, long long>> firsts;
+      for (size_t a = 0; a < ids.size();) {
+        size_t b = a;
+        while (b < ids.size() && ids[b].first == ids[a].first) ++b;
+        firsts.emplace_back(ids[a].second, ids[a].first);
+        a = b;
+      }
+      std::sort(firsts.begin(), firsts.end());
+      std::vector<std::pair<long long, int>> rowOfId(firsts.size());
+      for (size_t r = 0; r < firsts.size(); ++r) rowOfId[r] = {firsts[r].second, (int)r};
+      std::sort(rowOfId.begin(), rowOfId.end());
+      for (auto &pr : ids) {
+        auto it = std::lower_bound(rowOfId.begin(), rowOfId.end(), std::make_pair(pr.first, -1));
+        rows[pr.second] = it->second;
+      }
+      firstH.resize(firsts.size());
+      for (size_t r = 0; r < firsts.size(); ++r) firstH[r] = (int)firsts[r].first;
+    }
+    n = (int)firstH.size();
+    NRS_CHECK(n > 0, "coarse AMG: no unknowns on the coarse level");
+    // element matrices: column b of every element = operator on unit vectors at node b
+    std::vector<std::vector<double>> cols(Np);
+    {
+      DevBuf<double> u(L), Au(L);
+      std::vector<double> uh(L);
+      for (int b = 0; b < Np; ++b) {
+        for (int64_t i = 0; i < L; ++i) uh[i] = ((i % Np) == b) ? 1.0 : 0.0;
+        u.upload(uh);
+        launch_ax<double>(N, E, nullptr, ggeo.get(), D.get(), lam0.get(), lam1.get(), poisson, false,
+                          u.get(), Au.get(), nullptr, s);
+        NRS_HIP(hipStreamSynchronize(s));
+        cols[b] = Au.download();
+      }
+    }
+    // assemble rows (columns ascending, duplicates summed in element order)
+    std::vector<std::vector<std::pair<int, double>>> rowsAcc(n);
+    for (int e = 0; e < E; ++e)
+      for (int b = 0; b < Np; ++b) {
+        const int rb = rows[(int64_t)e * Np + b];
+        if (rb < 0) continue;
+        for (int a = 0; a < Np; ++a) {
+          const int ra = rows[(int64_t)e * Np + a];
+          if (ra < 0) continue;
+          const double v = cols[b][(int64_t)e * Np + a];
+          auto &rw = rowsAcc[ra];
+          bool found = false;
+          for (auto &pr : rw)
+            if (pr.first == rb) { pr.second += v; found = true; break; }
+          if (!found) rw.emplace_back(rb, v);
+        }
+      }
+    CsrHost A;
+    A.n = A.m = n;
+    A.rp.assign(n + 1, 0);
+    for (int r = 0; r < n; ++r) {
+      auto &rw = rowsAcc[r];
+      std::sort(rw.begin(), rw.end());
+      for (auto &pr : rw) { A.ci.push_back(pr.first); A.v.push_back(pr.second); }
+      A.rp[r + 1] = (int)A.ci.size();
+    }
+    rowOf.upload(rows);
+    first.upload(firstH);
+    arp.upload(A.rp); aci.upload(A.ci); av.upload(A.v);
+    ones.upload(std::vector<double>(n, 1.0));
+    ru.alloc(n); xu.alloc(n); zu.alloc(n); pu.alloc(n); Apu.alloc(n); tmp.alloc(n);
+    amg.setup(A, AmgParams{});
+    amgMode = true;
   }
 };
 
@@ -652,6 +782,7 @@ struct MGImpl final : MGSolver {
       NRS_HIP(hipMemset(c.scal.get(), 0, S_COUNT * sizeof(double)));
       NRS_HIP(hipHostMalloc((void **)&c.scalHost, S_COUNT * sizeof(double), hipHostMallocDefault));
       c.red = std::make_unique<Reducer>();
+      if (el->opts.coarseSolver == NRS_COARSE_AMG) c.setup_amg(mg, s);
     }
     NRS_HIP(hipStreamSynchronize(s));
   }
@@ -672,9 +803,9 @@ struct MGImpl final : MGSolver {
       default: throw Error(NRS_ERR_INVALID, "multigrid: unknown smoother");
     }
     NRS_CHECK(o.coarseSolver == NRS_COARSE_PCG_F64 || o.coarseSolver == NRS_COARSE_SMOOTH ||
-                  o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI,
+                  o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI || o.coarseSolver == NRS_COARSE_AMG,
               "multigrid: unknown coarse solver");
-    coarseSmoothing = o.coarseSolver != NRS_COARSE_PCG_F64;
+    coarseSmoothing = o.coarseSolver == NRS_COARSE_SMOOTH || o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI;
     coarseJacobi = o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI;
     NRS_CHECK(!(coarseJacobi && cheby == NO_CHEBY),
               "multigrid: a Jacobi-smoothed coarse level needs a Chebyshev smoother");

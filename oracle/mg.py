@@ -80,7 +80,7 @@ class MG:
         assert orders[0] == mesh.N
         assert cheby in ("cheby1", "opt4", None) and S in ("jacobi", "asm", "ras")
         assert cheby is not None or S != "jacobi"
-        assert coarse in ("pcg", "smooth", "smooth_jacobi")
+        assert coarse in ("pcg", "smooth", "smooth_jacobi", "amg")
         self.T = T
         self.cheby, self.S = cheby, S
         self.lambda0, self.lambda1, self.poisson = lambda0, lambda1, poisson
@@ -111,7 +111,7 @@ class MG:
                 L.R = transfer_R(Nprev, N).astype(T)
             if li == 0 and S != "jacobi":
                 fine = (N, x, y, z, wq, L.el.ogs_full)
-            if not L.isCoarse or coarse != "pcg":
+            if not L.isCoarse or coarse in ("smooth", "smooth_jacobi"):
                 if S == "jacobi" or (L.isCoarse and coarse == "smooth_jacobi"):
                     d = build_diag(N, m.ggeo, m.D, lambda0, lambda1, poisson)
                     d = L.el.ogs.gs(d, "add")
@@ -124,13 +124,37 @@ class MG:
             self.levels.append(L)
             Nprev = N
         for L in self.levels:
-            if (not L.isCoarse or coarse != "pcg") and cheby is not None:
+            if (not L.isCoarse or coarse in ("smooth", "smooth_jacobi")) and cheby is not None:
                 L.rho = self._max_eig(L, seed)
                 L.lmax, L.lmin = max_factor * L.rho, min_factor * L.rho
         cl = self.levels[-1]
         cl.invDiag64 = cl.el.inv_diag()
+        if coarse == "amg":
+            from . import amg as _amg
+            g = cl.mesh.gid.copy()
+            g[cl.el.maskIds] = 0
+            cl.amg_rows = _amg.row_numbers(g)
+            n = int(cl.amg_rows.max()) + 1
+            cl.amg_first = np.zeros(n, dtype=np.int64)
+            seen = np.zeros(n, dtype=bool)
+            for i, r in enumerate(cl.amg_rows.tolist()):
+                if r >= 0 and not seen[r]:
+                    seen[r] = True
+                    cl.amg_first[r] = i
+            cl.amg_A = _amg.assemble(cl, lambda u: self.op_local64(cl, u), cl.amg_rows)
+            cl.amg = _amg.AMG(cl.amg_A)
 
     # ---- level operators
+    def op_local64(self, L, q):
+        """FP64 element operator of the level (no mask, no gather-scatter)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        Aq = np.zeros_like(q)
+        l0 = np.full(1, self.lambda0)
+        l1 = np.full(1, self.lambda1)
+        lib().orc_ax_f64(L.N, L.E, None, L.mesh.ggeo, L.mesh.D.ravel().copy(), l0, l1,
+                         int(self.poisson), 0, q, Aq)
+        return Aq
+
     def op(self, L, q):
         q = np.ascontiguousarray(q, dtype=self.T)
         fn = lib().orc_ax_f64 if self.T == np.float64 else lib().orc_ax_f32
@@ -246,7 +270,39 @@ class MG:
         x[L.el.maskIds] = 0
         return x
 
+    def coarse_solve_amg(self, L, rhs):
+        """PCG on the assembled order-1 matrix, preconditioned by one AMG
+        V-cycle, from 0 to coarse_tol relative (the library's coarse AMG
+        mode; scalars as in PCG.cpp:85-203)."""
+        A, H = L.amg_A, L.amg
+        b = rhs.astype(np.float64)[L.amg_first]
+        x = np.zeros_like(b)
+        r = b.copy()
+        r0 = np.sqrt(r @ r)
+        if not r0 > 0:
+            return np.zeros(L.E * L.Np, dtype=self.T), 0
+        tol = self.coarse_tol * r0
+        p = np.zeros_like(b)
+        rdotz_old, it = 0.0, 0
+        while it < self.coarse_maxit:
+            z = H.vcycle(r)
+            rdotz = r @ z
+            beta = rdotz / rdotz_old if it > 0 else 0.0
+            p = z + beta * p
+            Ap = A @ p
+            alpha = rdotz / (p @ Ap + 1e-300)
+            r = r - alpha * Ap
+            x = x + alpha * p
+            rdotz_old = rdotz
+            it += 1
+            if np.sqrt(r @ r) <= tol:
+                break
+        xe = np.where(L.amg_rows >= 0, x[np.maximum(L.amg_rows, 0)], 0.0)
+        return xe.astype(self.T), it
+
     def coarse_solve(self, L, rhs):
+        if self.coarse == "amg":
+            return self.coarse_solve_amg(L, rhs)
         el = L.el
         r = rhs.astype(np.float64).copy()
         x = np.zeros_like(r)
@@ -263,7 +319,7 @@ class MG:
     def vcycle(self, k, rhs):
         L = self.levels[k]
         if L.isCoarse:
-            if self.coarse != "pcg":
+            if self.coarse in ("smooth", "smooth_jacobi"):
                 x = np.zeros(L.E * L.Np, dtype=self.T)
                 return self.smooth(L, rhs, x, True)
             x, self.last_coarse_it = self.coarse_solve(L, rhs)

@@ -123,3 +123,40 @@ def test_setup_on_recycled_memory(gpu, torch_mod):
     rhs = np.random.default_rng(7).uniform(-1, 1, m.Nlocal)
     st = el.solve(dev(torch, rhs), torch.zeros(m.Nlocal, dtype=torch.float64, device="cuda"))
     assert st["converged"]
+
+
+@pytest.mark.parametrize("shape,N,smoother,lam", [((10, 9, 8), 3, "chebyshev+jacobi", 0.0),
+                                                  ((9, 8, 8), 5, "opt4chebyshev+ras", 2.0)])
+def test_mg_amg_coarse_vs_oracle(gpu, torch_mod, orc, shape, N, smoother, lam):
+    """Coarse level by FP64 PCG on the assembled order-1 matrix with one
+    smoothed-aggregation V-cycle as preconditioner (more than 256 coarse
+    unknowns, so the hierarchy has several levels): outer iterations within
+    +-1 of the oracle, coarse iterations within +-1, solution 1e-8."""
+    import oracle.mg as omg
+    torch = torch_mod
+    eps = 0.3
+    poisson = lam == 0.0
+    mg = gpu.BoxMesh(*shape, N, "kershaw", eps)
+    S = "ras" if "ras" in smoother else "jacobi"
+    cheby = "opt4" if smoother.startswith("opt4") else "cheby1"
+    deg = 3 if cheby == "opt4" else 2
+    eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=lam, poisson=poisson, precon="multigrid",
+                      solver="fcg", levels=[N, 1], smoother=smoother, cheby_degree=deg,
+                      coarse="amg", tol=1e-10, max_iter=200)
+    mo = orc.BoxMesh(*shape, N, "kershaw", eps)
+    eo = orc.Elliptic(mo, 1.0, lam, poisson)
+    om = omg.MG(mo, [N, 1], T=np.float32, degree=deg, cheby=cheby, S=S, coarse="amg",
+                lambda0=1.0, lambda1=lam, poisson=poisson)
+    assert len(om.levels[-1].amg.levels) >= 2
+    rhs = np.random.default_rng(N).uniform(-1, 1, mo.Nlocal)
+    xo, info = omg.fcg_solve(eo, rhs, om.apply, flexible=True, tol=1e-10, maxit=200)
+    x = torch.zeros(mo.Nlocal, dtype=torch.float64, device="cuda")
+    st = eg.solve(torch.from_numpy(rhs).cuda(), x)
+    _, cit = eg.mg_info()
+    print(shape, N, smoother, "gpu it", st["iterations"], "oracle it", info["iterations"],
+          "coarse it", cit, om.last_coarse_it)
+    assert st["converged"]
+    assert abs(st["iterations"] - info["iterations"]) <= 1
+    assert abs(cit - om.last_coarse_it) <= 1
+    xh = x.cpu().numpy()
+    assert np.max(np.abs(xh - xo)) <= 1e-8 * np.max(np.abs(xo))
