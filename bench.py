@@ -210,6 +210,28 @@ def run_batch(probs, iters_per_launch, seed):
             "iters_per_launch": iters_per_launch, "dofs": dofs}
 
 
+def block_velocity(m, seed):
+    """Three velocity components as ONE block solve (the reference's velocity
+    solve: Nfields = 3, block Jacobi-PCG, the block operator reading the
+    geometric factors once for the three fields), lambda1 = 10, to 1e-8."""
+    import numpy as np
+    import torch
+    import nekrs_amd
+    el = nekrs_amd.Elliptic(m, lambda0=1.0, lambda1=10.0, poisson=False, nfields=3,
+                            solver="pcg", precon="jacobi", tol=1e-8, max_iter=3000)
+    rng = np.random.default_rng(seed)
+    rhs = torch.from_numpy(rng.uniform(-1, 1, 3 * el.Nlocal)).cuda()
+    for rep in range(2):
+        x = torch.zeros_like(rhs)
+        torch.cuda.synchronize()
+        st = el.solve(rhs.clone(), x)
+    dofs = 3 * m.E * m.N ** 3
+    out = {"iterations": st["iterations"], "seconds": st["seconds"], "converged": st["converged"],
+           "dof_iterations_per_s": dofs * st["iterations"] / st["seconds"]}
+    del el
+    return out
+
+
 def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
     """The uneven Helmholtz batch and the 3-component velocity Helmholtz solve."""
     import numpy as np
@@ -242,9 +264,14 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
         v = out["velocity3"]
         log("velocity x3 at %d^3 N=7: %.3f s batched vs %.3f s one by one, its %s" %
             (block_n, v["seconds_batched"], v["seconds_sequential"], v["iterations"]))
+        out["velocity3_block"] = block_velocity(m, seed + 2)
+        vb = out["velocity3_block"]
+        log("velocity block (3 fields, one operator, Jacobi-PCG): %d it, %.3f s" %
+            (vb["iterations"], vb["seconds"]))
     out["workload"] = ("%d Helmholtz problems (seed %d): N in {3,5,7,9}, E ~ U[512,8192] as "
                        "near-cubic boxes, lambda1 ~ U[0,10], Kershaw eps ~ U[0.05,0.3]; plus 3 "
-                       "velocity components (lambda1=10) at %d^3 N=7 eps 0.3; FP32 p-MG, opt. "
+                       "velocity components (lambda1=10) at %d^3 N=7 eps 0.3, as a batch and as "
+                       "one 3-field block Jacobi-PCG solve; FP32 p-MG, opt. "
                        "4th-kind Chebyshev + ASM with BF16 FDM, Chebyshev-Jacobi smoothed "
                        "coarse level, FCG to 1e-8 relative" % (count, seed, block_n))
     return out

@@ -162,6 +162,10 @@ typedef struct {
   const int *EToB;               /* host [E*6], 1 = DIRICHLET (elliptic.h:45)   */
   int poisson;                   /* 1: Poisson (lambda1 ignored)                */
   double lambda0, lambda1;       /* constant coefficients (p_lambda = 0)        */
+  int Nfields;                   /* 0 or 1: one field; 3: block solve of three
+                                  * fields [3][E*Np] with one operator (the
+                                  * velocity solve, elliptic_t::Nfields = 3,
+                                  * elliptic.h:77; Jacobi or no preconditioner) */
 } nrs_elliptic_desc;
 
 enum nrs_solver { NRS_SOLVER_PCG = 0, NRS_SOLVER_FCG = 1 };  /* SOLVER=PCG[+FLEXIBLE] */

@@ -222,11 +222,11 @@ def default_opts():
 COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2, "amg": 3}
 
 
-def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True):
+def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1):
     """nrs_elliptic_desc of a mesh (host arrays stay owned by the mesh)."""
     return _lib.EllipticDesc(mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
                              mesh.z.ctypes.data, mesh.gid.ctypes.data, mesh.EToB.ctypes.data,
-                             int(poisson), float(lambda0), float(lambda1))
+                             int(poisson), float(lambda0), float(lambda1), int(nfields))
 
 
 def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=500, levels=None,
@@ -333,9 +333,12 @@ class Batch:
 class Elliptic:
     """elliptic_t for one scalar field (ellipticSolveSetup / ellipticSolve)."""
 
-    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, **kw):
+    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1, **kw):
+        """nfields=3: block solve of three fields (vectors [3][Nlocal]), Jacobi or
+        no preconditioner; Nlocal stays the per-field length."""
         self.mesh = mesh
-        d = make_desc(mesh, lambda0, lambda1, poisson)
+        self.nfields = nfields
+        d = make_desc(mesh, lambda0, lambda1, poisson, nfields)
         o = make_opts(**kw)
         self.opts = o
         h = C.c_void_p()

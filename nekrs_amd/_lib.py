@@ -26,7 +26,8 @@ class EllipticDesc(C.Structure):
     _fields_ = [("N", C.c_int), ("Nelements", C.c_int),
                 ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
                 ("globalIds", C.c_void_p), ("EToB", C.c_void_p),
-                ("poisson", C.c_int), ("lambda0", C.c_double), ("lambda1", C.c_double)]
+                ("poisson", C.c_int), ("lambda0", C.c_double), ("lambda1", C.c_double),
+                ("Nfields", C.c_int)]
 
 
 class EllipticOpts(C.Structure):

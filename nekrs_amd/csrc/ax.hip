@@ -241,7 +241,7 @@ template void launch_ax<float>(int, int, const int *, const float *, const float
 // ellipticBlockPartialAxCoeffHex3D (kernels/elliptic/ellipticBlockPartialAxCoeffHex3D.c:2-147);
 // the geometric factors are read once for the three fields.
 // -------------------------------------------------------------------------------------
-template <int NQ>
+template <int NQ, bool LFIELD>
 __global__ void __launch_bounds__(AxShape<NQ>::THREADS)
 block_ax_kernel(int E, int64_t offset, int64_t loffset, const double *__restrict__ ggeo,
                 const double *__restrict__ Dg, const double *__restrict__ lam0,
@@ -294,7 +294,7 @@ block_ax_kernel(int E, int64_t offset, int64_t loffset, const double *__restrict
         qs += s_D[j][m] * s_q[f][le][m][i];
         qt += s_D[k][m] * r_q[f][m];
       }
-      const double l0 = active ? lam0[f * loffset + id] : 0.0;
+      const double l0 = LFIELD ? (active ? lam0[f * loffset + id] : 0.0) : lam0[0];
       s_Gr[f][le][j][i] = l0 * (G00 * qr + G01 * qs + G02 * qt);
       s_Gs[f][le][j][i] = l0 * (G01 * qr + G11 * qs + G12 * qt);
       Gqt[f] = l0 * (G02 * qr + G12 * qs + G22 * qt);
@@ -302,7 +302,7 @@ block_ax_kernel(int E, int64_t offset, int64_t loffset, const double *__restrict
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      const double l1 = active ? lam1[f * loffset + id] : 0.0;
+      const double l1 = LFIELD ? (active ? lam1[f * loffset + id] : 0.0) : lam1[0];
       double a = 0, b = 0;
 #pragma unroll
       for (int m = 0; m < NQ; ++m) {
@@ -324,12 +324,15 @@ block_ax_kernel(int E, int64_t offset, int64_t loffset, const double *__restrict
 
 void launch_block_ax(int N, int E, int64_t offset, int64_t loffset, const double *ggeo,
                      const double *D, const double *l0, const double *l1, const double *q,
-                     double *Aq, hipStream_t s)
+                     double *Aq, hipStream_t s, bool lfield)
 {
 #define BLK(NQ)                                                                              \
   case NQ: {                                                                                 \
     const unsigned g = (unsigned)((E + AxShape<NQ>::NE - 1) / AxShape<NQ>::NE);              \
-    if (g) block_ax_kernel<NQ><<<g, AxShape<NQ>::THREADS, 0, s>>>(E, offset, loffset, ggeo, D, l0, l1, q, Aq); \
+    if (g && lfield)                                                                         \
+      block_ax_kernel<NQ, true><<<g, AxShape<NQ>::THREADS, 0, s>>>(E, offset, loffset, ggeo, D, l0, l1, q, Aq); \
+    else if (g)                                                                              \
+      block_ax_kernel<NQ, false><<<g, AxShape<NQ>::THREADS, 0, s>>>(E, offset, loffset, ggeo, D, l0, l1, q, Aq); \
     break;                                                                                   \
   }
   switch (N + 1) {

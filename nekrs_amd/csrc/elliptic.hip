@@ -100,6 +100,12 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   // are Poisson-only in the reference; kept as in the reference.
   auto el = std::make_unique<nrs_elliptic>();
   el->N = d->N; el->Nq = d->N + 1; el->Np = el->Nq * el->Nq * el->Nq; el->E = d->Nelements;
+  el->Nf = d->Nfields <= 1 ? 1 : d->Nfields;
+  NRS_CHECK(el->Nf == 1 || el->Nf == 3, "elliptic setup: Nfields must be 1 or 3");
+  // the block (velocity) solve of the reference is preconditioned by Jacobi
+  // (ellipticSolveSetup with blockSolver): multigrid is single-field here too
+  NRS_CHECK(el->Nf == 1 || o->preconditioner != NRS_PRECON_MULTIGRID,
+            "elliptic setup: a 3-field block solve takes the Jacobi or no preconditioner");
   el->Nlocal = (int64_t)el->E * el->Np;
   NRS_CHECK(el->Nlocal < (int64_t)INT32_MAX, "E*Np must fit in int32 (dlong)");
   el->poisson = d->poisson != 0;
@@ -163,17 +169,27 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   }
 
   // workspace
-  el->p.alloc(el->Nlocal); el->zv.alloc(el->Nlocal); el->Ap.alloc(el->Nlocal); el->x0.alloc(el->Nlocal);
+  const int64_t Lt = (int64_t)el->Nf * el->Nlocal;
+  el->p.alloc(Lt); el->zv.alloc(Lt); el->Ap.alloc(Lt); el->x0.alloc(Lt);
+  if (el->Nf > 1) {
+    el->wRep.alloc(Lt);
+    for (int f = 0; f < el->Nf; ++f)
+      NRS_HIP(hipMemcpyAsync(el->wRep.get() + f * el->Nlocal, el->ogs->invDegree.get(),
+                             el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
   NRS_HIP(hipEventCreate(&el->ev0));
   NRS_HIP(hipEventCreate(&el->ev1));
 
   // Jacobi: ellipticUpdateJacobi (src/solvers/elliptic/ellipticUpdateJacobi.cpp:62-85)
   if (o->preconditioner == NRS_PRECON_JACOBI) {
-    el->invDiagA.alloc(el->Nlocal);
+    el->invDiagA.alloc(Lt);
     launch_build_diag<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
                               el->lam1.get(), el->poisson, false, el->invDiagA.get(), s);
     ogs_gather_scatter<double>(el->ogs, el->invDiagA.get(), 1, el->Nlocal, NRS_ADD, s);
     reciprocal<double>(el->Nlocal, el->invDiagA.get(), s);
+    for (int f = 1; f < el->Nf; ++f)   // the same operator for every field
+      NRS_HIP(hipMemcpyAsync(el->invDiagA.get() + f * el->Nlocal, el->invDiagA.get(),
+                             el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   if (o->preconditioner == NRS_PRECON_MULTIGRID) {
     el->mg = mg_create(el.get(), d, s);
@@ -271,8 +287,33 @@ template int autotune_operator<float>(nrs_elliptic *, int, int, const float *, c
                                       const float *, const float *, bool, const uint32_t *,
                                       const nrs_ogs *, hipStream_t);
 
+// element part of the operator, all fields (no mask, no gather-scatter)
+static void element_operator(nrs_elliptic *el, const double *q, double *Aq, hipStream_t s)
+{
+  if (el->Nf > 1)
+    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                    el->lam1.get(), q, Aq, s, false);
+  else
+    launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                      el->lam1.get(), el->poisson, false, q, Aq, nullptr, s);
+}
+
+static void mask_fields(nrs_elliptic *el, double *v, hipStream_t s)
+{
+  for (int f = 0; f < el->Nf; ++f) launch_mask<double>(el->Nmasked, el->maskIds.get(), v + f * el->Nlocal, s);
+}
+
 void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s)
 {
+  if (el->Nf > 1) {
+    // three fields: the block apply reads the geometric factors once
+    // (ellipticBlockPartialAxCoeffHex3D), then mask and one gather-scatter of
+    // the three vectors (oogs::startFinish with k = 3)
+    element_operator(el, q, Aq, s);
+    if (masked) mask_fields(el, Aq, s);
+    ogs_gather_scatter<double>(el->ogs, Aq, el->Nf, el->Nlocal, NRS_ADD, s);
+    return;
+  }
   chunked_operator<double>(el, el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
                            el->lam1.get(), el->poisson, masked ? el->maskBits.get() : nullptr,
                            el->ogs, q, Aq, operator_chunks(el->N, NRS_F64, el->E), s, true);
@@ -281,10 +322,10 @@ void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool maske
 void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s)
 {
   switch (el->opts.preconditioner) {
-    case NRS_PRECON_JACOBI: axmyz<double>(el->Nlocal, el->invDiagA.get(), r, z, s); break;
+    case NRS_PRECON_JACOBI: axmyz<double>(el->Nf * el->Nlocal, el->invDiagA.get(), r, z, s); break;
     case NRS_PRECON_MULTIGRID: el->mg->apply(r, z, s); break;
     default:
-      NRS_HIP(hipMemcpyAsync(z, r, el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
+      NRS_HIP(hipMemcpyAsync(z, r, el->Nf * el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
       break;
   }
 }
@@ -292,8 +333,8 @@ void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipSt
 // ellipticSolve (src/solvers/elliptic/ellipticSolve.cpp:32-190) + pcg (PCG.cpp:85-203).
 int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s)
 {
-  const int64_t L = el->Nlocal;
-  const double *w = el->ogs->invDegree.get();
+  const int64_t L = (int64_t)el->Nf * el->Nlocal;   // all fields
+  const double *w = el->Nf > 1 ? el->wRep.get() : el->ogs->invDegree.get();
   double *r = rhs;
   double *scal = el->scal.get();
   Reducer &R = *el->red;
@@ -306,11 +347,10 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   NRS_HIP(hipEventRecord(el->ev0, s));
 
   // r = rhs - A x0 (element apply only), mask, gs   (ellipticSolve.cpp:91-103)
-  launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                    el->lam1.get(), el->poisson, false, x, el->Ap.get(), nullptr, s);
+  element_operator(el, x, el->Ap.get(), s);
   axpby<double>(L, -1.0, el->Ap.get(), 1.0, r, s);
-  launch_mask<double>(el->Nmasked, el->maskIds.get(), r, s);
-  ogs_gather_scatter<double>(el->ogs, r, 1, L, NRS_ADD, s);
+  mask_fields(el, r, s);
+  ogs_gather_scatter<double>(el->ogs, r, el->Nf, el->Nlocal, NRS_ADD, s);
   // x0 = x, x = 0   (:105-106)
   NRS_HIP(hipMemcpyAsync(el->x0.get(), x, L * sizeof(double), hipMemcpyDeviceToDevice, s));
   fill<double>(L, 0.0, x, s);
@@ -378,6 +418,7 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
 // smoothed coarse level, Jacobi, or none).
 void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t s)
 {
+  NRS_CHECK(el->Nf == 1, "device-driven solve: one field");
   const int64_t L = el->Nlocal;
   NRS_CHECK(!(el->mg && !el->mg->device_only()),
             "device-driven solve: the multigrid coarse level must be smoothed, not solved");

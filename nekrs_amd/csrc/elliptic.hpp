@@ -13,7 +13,8 @@ struct MGSolver;
 // Mirrors elliptic_t (src/solvers/elliptic/elliptic.h:73-181) for one field.
 struct nrs_elliptic {
   int N = 0, Nq = 0, Np = 0, E = 0;
-  int64_t Nlocal = 0;
+  int Nf = 1;                        // fields (3: block solve, vectors [3][Nlocal])
+  int64_t Nlocal = 0;                // per field
   bool poisson = true;
   double lambda0 = 1.0, lambda1 = 0.0;
   double volume = 0.0;
@@ -28,7 +29,8 @@ struct nrs_elliptic {
   int64_t Nmasked = 0;
   nrs::DevBuf<int> maskIds;
   nrs::DevBuf<uint32_t> maskBits;    // [E][ceil(Np/32)]
-  nrs::DevBuf<double> invDiagA;      // Jacobi (assembled), fine level
+  nrs::DevBuf<double> invDiagA;      // Jacobi (assembled), fine level, [Nf][Nlocal]
+  nrs::DevBuf<double> wRep;          // invDegree repeated per field (Nf > 1)
 
   // workspace (ellipticUpdateWorkspace, elliptic.h:240-262)
   nrs::DevBuf<double> p, zv, Ap, x0;

@@ -18,7 +18,8 @@ void launch_ax_variant(int N, int mult, int E, const int *elist, const T *ggeo, 
                        const uint32_t *maskBits, hipStream_t s);
 void launch_block_ax(int N, int E, int64_t offset, int64_t loffset, const double *ggeo,
                      const double *D, const double *l0, const double *l1, const double *q,
-                     double *Aq, hipStream_t s);
+                     double *Aq, hipStream_t s,
+                     bool lfield = true);
 template <typename T>
 void launch_build_diag(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1,
                        bool poisson, bool lfield, T *diag, hipStream_t s);

@@ -177,3 +177,74 @@ def test_c1_shape_properties(gpu, torch_mod):
     el.operator(one, A1, masked=False)
     # constants are in the kernel of every element apply, so the assembled result is 0
     assert (torch.max(torch.abs(A1)) / torch.max(torch.abs(Ap))).item() < 1e-12
+
+
+def _block_pcg(eo, rhs3, tol, maxit):
+    """Jacobi-PCG over three fields with one operator (ellipticSolve /
+    PCG.cpp:85-203 with Nfields = 3): inner products summed over the fields
+    with invDegree weights, norm scaled by 1/volume, relative tolerance."""
+    L = rhs3.size // 3
+    w = np.tile(eo.invDegree, 3)
+    inv = np.tile(eo.inv_diag(), 3)
+    A = lambda v: np.concatenate([eo.operator(v[f * L:(f + 1) * L]) for f in range(3)])
+    r = np.concatenate([eo.ogs.gs(np.where(np.isin(np.arange(L), eo.maskIds), 0.0,
+                                           rhs3[f * L:(f + 1) * L]), "add") for f in range(3)])
+    x = np.zeros_like(r)
+    res0 = np.sqrt(np.sum(w * r * r) / eo.volume)
+    tol = tol * res0
+    p = np.zeros_like(r)
+    rdotz_old, it = 0.0, 0
+    while it < maxit:
+        z = inv * r
+        rdotz = np.sum(w * r * z)
+        beta = rdotz / rdotz_old if it > 0 else 0.0
+        p = z + beta * p
+        Ap = A(p)
+        alpha = rdotz / (np.sum(w * p * Ap) + 1e-300)
+        r = r - alpha * Ap
+        x = x + alpha * p
+        rdotz_old = rdotz
+        it += 1
+        if np.sqrt(np.sum(w * r * r) / eo.volume) <= tol:
+            break
+    return x, it
+
+
+@pytest.mark.parametrize("N,lam1", [(3, 10.0), (5, 0.0), (7, 2.5)])
+def test_block_three_fields_vs_oracle(gpu, torch_mod, orc, N, lam1):
+    """Block (velocity-type) solve of three fields sharing one Helmholtz
+    operator: the block operator equals the oracle's operator on every field
+    (1e-12); the block Jacobi-PCG matches the oracle's three-field PCG in
+    iterations (+-1) and solution (1e-8)."""
+    torch = torch_mod
+    shape = (4, 3, 3)
+    mg = gpu.BoxMesh(*shape, N, "kershaw", 0.3)
+    mo = orc.BoxMesh(*shape, N, "kershaw", 0.3)
+    eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=lam1, poisson=False, nfields=3, precon="jacobi",
+                      tol=1e-9, max_iter=1000)
+    eo = orc.Elliptic(mo, 1.0, lam1, False)
+    L = mo.Nlocal
+    assert eg.Nlocal == L
+    q = np.random.default_rng(N).uniform(-1, 1, 3 * L)
+    Aq = torch.zeros(3 * L, dtype=torch.float64, device="cuda")
+    eg.operator(dev(torch, q), Aq)
+    torch.cuda.synchronize()
+    got = host(Aq)
+    for f in range(3):
+        ref = eo.operator(q[f * L:(f + 1) * L])
+        assert rel(got[f * L:(f + 1) * L], ref) < 1e-12
+    rhs = np.random.default_rng(N + 1).uniform(-1, 1, 3 * L)
+    xo, it = _block_pcg(eo, rhs, 1e-9, 1000)
+    x = torch.zeros(3 * L, dtype=torch.float64, device="cuda")
+    st = eg.solve(dev(torch, rhs), x)
+    print(N, lam1, "gpu it", st["iterations"], "oracle it", it)
+    assert st["converged"] and abs(st["iterations"] - it) <= 1
+    assert rel(host(x), xo) <= 1e-8
+
+
+def test_block_rejects_multigrid(gpu):
+    m = gpu.BoxMesh(2, 2, 2, 3)
+    with pytest.raises(gpu.NrsError):
+        gpu.Elliptic(m, nfields=3, precon="multigrid")
+    with pytest.raises(gpu.NrsError):
+        gpu.Elliptic(m, nfields=2, precon="jacobi")
