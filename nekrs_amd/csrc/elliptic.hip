@@ -102,10 +102,8 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   el->N = d->N; el->Nq = d->N + 1; el->Np = el->Nq * el->Nq * el->Nq; el->E = d->Nelements;
   el->Nf = d->Nfields <= 1 ? 1 : d->Nfields;
   NRS_CHECK(el->Nf == 1 || el->Nf == 3, "elliptic setup: Nfields must be 1 or 3");
-  // the block (velocity) solve of the reference is preconditioned by Jacobi
-  // (ellipticSolveSetup with blockSolver): multigrid is single-field here too
-  NRS_CHECK(el->Nf == 1 || o->preconditioner != NRS_PRECON_MULTIGRID,
-            "elliptic setup: a 3-field block solve takes the Jacobi or no preconditioner");
+  // a block solve with multigrid applies the (single-field) hierarchy to each
+  // field; the reference's velocity solve uses Jacobi there
   el->Nlocal = (int64_t)el->E * el->Np;
   NRS_CHECK(el->Nlocal < (int64_t)INT32_MAX, "E*Np must fit in int32 (dlong)");
   el->poisson = d->poisson != 0;
@@ -287,15 +285,16 @@ template int autotune_operator<float>(nrs_elliptic *, int, int, const float *, c
                                       const float *, const float *, bool, const uint32_t *,
                                       const nrs_ogs *, hipStream_t);
 
-// element part of the operator, all fields (no mask, no gather-scatter)
+// element part of the operator, all fields (no mask, no gather-scatter).
+// The fields share the coefficients, so each goes through the single-field
+// kernel (its register-resident pencils leave no room for three fields at
+// once); the geometric factors are read once per field.
 static void element_operator(nrs_elliptic *el, const double *q, double *Aq, hipStream_t s)
 {
-  if (el->Nf > 1)
-    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                    el->lam1.get(), q, Aq, s, false);
-  else
+  for (int f = 0; f < el->Nf; ++f)
     launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                      el->lam1.get(), el->poisson, false, q, Aq, nullptr, s);
+                      el->lam1.get(), el->poisson, false, q + f * el->Nlocal, Aq + f * el->Nlocal,
+                      nullptr, s);
 }
 
 static void mask_fields(nrs_elliptic *el, double *v, hipStream_t s)
@@ -306,11 +305,12 @@ static void mask_fields(nrs_elliptic *el, double *v, hipStream_t s)
 void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s)
 {
   if (el->Nf > 1) {
-    // three fields: the block apply reads the geometric factors once
-    // (ellipticBlockPartialAxCoeffHex3D), then mask and one gather-scatter of
-    // the three vectors (oogs::startFinish with k = 3)
-    element_operator(el, q, Aq, s);
-    if (masked) mask_fields(el, Aq, s);
+    // three fields: the element apply of each field (mask fused), then one
+    // gather-scatter of the three vectors (oogs::startFinish with k = 3)
+    for (int f = 0; f < el->Nf; ++f)
+      launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                        el->lam1.get(), el->poisson, false, q + f * el->Nlocal, Aq + f * el->Nlocal,
+                        masked ? el->maskBits.get() : nullptr, s);
     ogs_gather_scatter<double>(el->ogs, Aq, el->Nf, el->Nlocal, NRS_ADD, s);
     return;
   }
@@ -323,7 +323,9 @@ void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipSt
 {
   switch (el->opts.preconditioner) {
     case NRS_PRECON_JACOBI: axmyz<double>(el->Nf * el->Nlocal, el->invDiagA.get(), r, z, s); break;
-    case NRS_PRECON_MULTIGRID: el->mg->apply(r, z, s); break;
+    case NRS_PRECON_MULTIGRID:   // one V-cycle per field (the hierarchy is single-field)
+      for (int f = 0; f < el->Nf; ++f) el->mg->apply(r + f * el->Nlocal, z + f * el->Nlocal, s);
+      break;
     default:
       NRS_HIP(hipMemcpyAsync(z, r, el->Nf * el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
       break;

@@ -179,13 +179,16 @@ def test_c1_shape_properties(gpu, torch_mod):
     assert (torch.max(torch.abs(A1)) / torch.max(torch.abs(Ap))).item() < 1e-12
 
 
-def _block_pcg(eo, rhs3, tol, maxit):
-    """Jacobi-PCG over three fields with one operator (ellipticSolve /
+def _block_pcg(eo, rhs3, tol, maxit, precon=None, flexible=False):
+    """(Flexible) PCG over three fields with one operator (ellipticSolve /
     PCG.cpp:85-203 with Nfields = 3): inner products summed over the fields
-    with invDegree weights, norm scaled by 1/volume, relative tolerance."""
+    with invDegree weights, norm scaled by 1/volume, relative tolerance;
+    Jacobi unless precon (one call per field) is given."""
     L = rhs3.size // 3
     w = np.tile(eo.invDegree, 3)
     inv = np.tile(eo.inv_diag(), 3)
+    M = (lambda v: inv * v) if precon is None else \
+        (lambda v: np.concatenate([precon(v[f * L:(f + 1) * L]) for f in range(3)]))
     A = lambda v: np.concatenate([eo.operator(v[f * L:(f + 1) * L]) for f in range(3)])
     r = np.concatenate([eo.ogs.gs(np.where(np.isin(np.arange(L), eo.maskIds), 0.0,
                                            rhs3[f * L:(f + 1) * L]), "add") for f in range(3)])
@@ -193,11 +196,17 @@ def _block_pcg(eo, rhs3, tol, maxit):
     res0 = np.sqrt(np.sum(w * r * r) / eo.volume)
     tol = tol * res0
     p = np.zeros_like(r)
-    rdotz_old, it = 0.0, 0
+    Ap = np.zeros_like(r)
+    rdotz_old, alpha, it = 0.0, 0.0, 0
     while it < maxit:
-        z = inv * r
+        z = M(r)
         rdotz = np.sum(w * r * z)
-        beta = rdotz / rdotz_old if it > 0 else 0.0
+        if it == 0:
+            beta = 0.0
+        elif flexible:             # PCG.cpp:141-147: beta = -alpha <z, Ap_old> / <r,z>_old
+            beta = -alpha * np.sum(w * z * Ap) / rdotz_old
+        else:
+            beta = rdotz / rdotz_old
         p = z + beta * p
         Ap = A(p)
         alpha = rdotz / (np.sum(w * p * Ap) + 1e-300)
@@ -242,9 +251,32 @@ def test_block_three_fields_vs_oracle(gpu, torch_mod, orc, N, lam1):
     assert rel(host(x), xo) <= 1e-8
 
 
-def test_block_rejects_multigrid(gpu):
+def test_block_rejects_two_fields(gpu):
     m = gpu.BoxMesh(2, 2, 2, 3)
     with pytest.raises(gpu.NrsError):
-        gpu.Elliptic(m, nfields=3, precon="multigrid")
-    with pytest.raises(gpu.NrsError):
         gpu.Elliptic(m, nfields=2, precon="jacobi")
+
+
+def test_block_multigrid_vs_oracle(gpu, torch_mod, orc):
+    """Three fields with the p-multigrid hierarchy applied to each field and
+    flexible CG over the block: iterations +-1 and solution 1e-8 against the
+    oracle's block FCG with its multigrid per field."""
+    import oracle.mg as omg
+    torch = torch_mod
+    shape, N, lam1 = (4, 3, 3), 5, 3.0
+    mg = gpu.BoxMesh(*shape, N, "kershaw", 0.3)
+    mo = orc.BoxMesh(*shape, N, "kershaw", 0.3)
+    eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=lam1, poisson=False, nfields=3, solver="fcg",
+                      precon="multigrid", levels=[N, 3, 1], smoother="opt4chebyshev+jacobi",
+                      cheby_degree=3, coarse="smooth", tol=1e-9, max_iter=200)
+    eo = orc.Elliptic(mo, 1.0, lam1, False)
+    om = omg.MG(mo, [N, 3, 1], T=np.float32, degree=3, cheby="opt4", S="jacobi", coarse="smooth",
+                lambda0=1.0, lambda1=lam1, poisson=False)
+    L = mo.Nlocal
+    rhs = np.random.default_rng(9).uniform(-1, 1, 3 * L)
+    xo, it = _block_pcg(eo, rhs, 1e-9, 200, precon=om.apply, flexible=True)
+    x = torch.zeros(3 * L, dtype=torch.float64, device="cuda")
+    st = eg.solve(dev(torch, rhs), x)
+    print("block MG gpu it", st["iterations"], "oracle it", it)
+    assert st["converged"] and abs(st["iterations"] - it) <= 1
+    assert rel(host(x), xo) <= 1e-8
