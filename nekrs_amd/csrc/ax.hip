@@ -241,8 +241,12 @@ template void launch_ax<float>(int, int, const int *, const float *, const float
 // ellipticBlockPartialAxCoeffHex3D (kernels/elliptic/ellipticBlockPartialAxCoeffHex3D.c:2-147);
 // the geometric factors are read once for the three fields.
 // -------------------------------------------------------------------------------------
+// Same scheme as ax_kernel: one (i,j) column per lane, the k-pencils of q and
+// of the result in registers with the k loop rolled (static register
+// indexing through selects), r/s contractions through LDS slabs, geometric
+// factors prefetched one plane ahead and shared by the three fields.
 template <int NQ, bool LFIELD>
-__global__ void __launch_bounds__(AxShape<NQ>::THREADS)
+__global__ void __launch_bounds__(AxShape<NQ>::THREADS, LFIELD ? 1 : 2)
 block_ax_kernel(int E, int64_t offset, int64_t loffset, const double *__restrict__ ggeo,
                 const double *__restrict__ Dg, const double *__restrict__ lam0,
                 const double *__restrict__ lam1, const double *__restrict__ q,
@@ -262,64 +266,81 @@ block_ax_kernel(int E, int64_t offset, int64_t loffset, const double *__restrict
   const int64_t e = active ? eidx : 0;
   if (t < NQ * NQ) s_D[t / NQ][t % NQ] = Dg[t];
   __syncthreads();
+  const int lid = j * NQ + i;
   double r_q[NF][NQ], r_Aq[NF][NQ];
 #pragma unroll
   for (int f = 0; f < NF; ++f)
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      r_q[f][k] = active ? q[f * offset + e * NP + k * NQ * NQ + j * NQ + i] : 0.0;
+      r_q[f][k] = active ? q[f * offset + e * NP + k * NQ * NQ + lid] : 0.0;
       r_Aq[f][k] = 0.0;
     }
-  const double *G = ggeo + e * 7 * NP + j * NQ + i;
+  const double *G = ggeo + e * 7 * NP + lid;
+  double g[7];
 #pragma unroll
+  for (int c = 0; c < 7; ++c) g[c] = active ? stream_load(G + c * NP) : 0.0;
+  const double l0c = LFIELD ? 0.0 : lam0[0];
+  const double l1c = LFIELD ? 0.0 : lam1[0];
+
+#pragma unroll 1
   for (int k = 0; k < NQ; ++k) {
     const int n = k * NQ * NQ;
-    const int64_t id = e * NP + n + j * NQ + i;
-    double G00 = 0, G01 = 0, G02 = 0, G11 = 0, G12 = 0, G22 = 0, GWJ = 0;
-    if (active) {
-      G00 = G[kG00 * NP + n]; G01 = G[kG01 * NP + n]; G02 = G[kG02 * NP + n];
-      G11 = G[kG11 * NP + n]; G12 = G[kG12 * NP + n]; G22 = G[kG22 * NP + n];
-      GWJ = G[kGWJ * NP + n];
-    }
-    double Gqt[NF];
+    const int64_t id = e * NP + n + lid;
+    double gn[7];
+    const bool more = active && (k + 1 < NQ);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) s_q[f][le][j][i] = r_q[f][k];
-    __syncthreads();
+    for (int c = 0; c < 7; ++c) gn[c] = more ? stream_load(G + c * NP + n + NQ * NQ) : 0.0;
+    double qk[NF], qt[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      double qr = 0, qs = 0, qt = 0;
+      qk[f] = r_q[f][0];
+#pragma unroll
+      for (int m = 1; m < NQ; ++m) qk[f] = (k == m) ? r_q[f][m] : qk[f];
+      s_q[f][le][j][i] = qk[f];
+      double a = 0.0;
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) a += s_D[k][m] * r_q[f][m];
+      qt[f] = a;
+    }
+    __syncthreads();
+    double Gqt[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double qr = 0.0, qs = 0.0;
 #pragma unroll
       for (int m = 0; m < NQ; ++m) {
         qr += s_D[i][m] * s_q[f][le][j][m];
         qs += s_D[j][m] * s_q[f][le][m][i];
-        qt += s_D[k][m] * r_q[f][m];
       }
-      const double l0 = LFIELD ? (active ? lam0[f * loffset + id] : 0.0) : lam0[0];
-      s_Gr[f][le][j][i] = l0 * (G00 * qr + G01 * qs + G02 * qt);
-      s_Gs[f][le][j][i] = l0 * (G01 * qr + G11 * qs + G12 * qt);
-      Gqt[f] = l0 * (G02 * qr + G12 * qs + G22 * qt);
+      const double l0 = LFIELD ? (active ? lam0[f * loffset + id] : 0.0) : l0c;
+      s_Gr[f][le][j][i] = l0 * (g[kG00] * qr + g[kG01] * qs + g[kG02] * qt[f]);
+      s_Gs[f][le][j][i] = l0 * (g[kG01] * qr + g[kG11] * qs + g[kG12] * qt[f]);
+      Gqt[f] = l0 * (g[kG02] * qr + g[kG12] * qs + g[kG22] * qt[f]);
     }
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      const double l1 = LFIELD ? (active ? lam1[f * loffset + id] : 0.0) : lam1[0];
-      double a = 0, b = 0;
+      const double l1 = LFIELD ? (active ? lam1[f * loffset + id] : 0.0) : l1c;
+      double a = 0.0, b = 0.0;
 #pragma unroll
       for (int m = 0; m < NQ; ++m) {
         a += s_D[m][i] * s_Gr[f][le][j][m];
         b += s_D[m][j] * s_Gs[f][le][m][i];
       }
+      const double ab = a + b + g[kGWJ] * l1 * qk[f];
 #pragma unroll
       for (int m = 0; m < NQ; ++m) r_Aq[f][m] += s_D[k][m] * Gqt[f];
-      r_Aq[f][k] += a + b + GWJ * l1 * r_q[f][k];
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) r_Aq[f][m] += (k == m) ? ab : 0.0;
     }
-    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 7; ++c) g[c] = gn[c];
   }
   if (!active) return;
 #pragma unroll
   for (int f = 0; f < NF; ++f)
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) Aq[f * offset + e * NP + k * NQ * NQ + j * NQ + i] = r_Aq[f][k];
+    for (int k = 0; k < NQ; ++k) Aq[f * offset + e * NP + k * NQ * NQ + lid] = r_Aq[f][k];
 }
 
 void launch_block_ax(int N, int E, int64_t offset, int64_t loffset, const double *ggeo,

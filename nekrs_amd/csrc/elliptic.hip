@@ -285,16 +285,16 @@ template int autotune_operator<float>(nrs_elliptic *, int, int, const float *, c
                                       const float *, const float *, bool, const uint32_t *,
                                       const nrs_ogs *, hipStream_t);
 
-// element part of the operator, all fields (no mask, no gather-scatter).
-// The fields share the coefficients, so each goes through the single-field
-// kernel (its register-resident pencils leave no room for three fields at
-// once); the geometric factors are read once per field.
+// element part of the operator, all fields (no mask, no gather-scatter); three
+// fields go through the block kernel, which reads the geometric factors once
 static void element_operator(nrs_elliptic *el, const double *q, double *Aq, hipStream_t s)
 {
-  for (int f = 0; f < el->Nf; ++f)
+  if (el->Nf > 1)
+    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                    el->lam1.get(), q, Aq, s, false);
+  else
     launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                      el->lam1.get(), el->poisson, false, q + f * el->Nlocal, Aq + f * el->Nlocal,
-                      nullptr, s);
+                      el->lam1.get(), el->poisson, false, q, Aq, nullptr, s);
 }
 
 static void mask_fields(nrs_elliptic *el, double *v, hipStream_t s)
@@ -305,12 +305,12 @@ static void mask_fields(nrs_elliptic *el, double *v, hipStream_t s)
 void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s)
 {
   if (el->Nf > 1) {
-    // three fields: the element apply of each field (mask fused), then one
-    // gather-scatter of the three vectors (oogs::startFinish with k = 3)
-    for (int f = 0; f < el->Nf; ++f)
-      launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                        el->lam1.get(), el->poisson, false, q + f * el->Nlocal, Aq + f * el->Nlocal,
-                        masked ? el->maskBits.get() : nullptr, s);
+    // three fields: the block apply (ellipticBlockPartialAxCoeffHex3D), the
+    // mask of each field, one gather-scatter of the three vectors
+    // (oogs::startFinish with k = 3)
+    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                    el->lam1.get(), q, Aq, s, false);
+    if (masked) mask_fields(el, Aq, s);
     ogs_gather_scatter<double>(el->ogs, Aq, el->Nf, el->Nlocal, NRS_ADD, s);
     return;
   }
