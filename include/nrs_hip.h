@@ -192,7 +192,10 @@ enum nrs_coarse {
   NRS_COARSE_AMG = 3             /* FP64 PCG on the assembled order-1 matrix,
                                   * preconditioned by one smoothed-aggregation AMG
                                   * V-cycle (in place of BoomerAMG,
-                                  * coarseLevel.cpp:47-222), to coarseTol          */
+                                  * coarseLevel.cpp:47-222), to coarseTol; with
+                                  * coarseTol <= 0 exactly coarseMaxIter (1..64)
+                                  * iterations on the device, no host round trip,
+                                  * so the whole V-cycle replays as one graph      */
 };
 enum nrs_mg_precision { NRS_MG_F32 = 0, NRS_MG_F64 = 1 };
 enum nrs_fdm_precision { NRS_FDM_F32 = 0, NRS_FDM_BF16 = 1, NRS_FDM_F16 = 2 };
@@ -308,8 +311,9 @@ int nrs_coarse_global_ids(int Nf, int Nelements, const long long *idsFine, int N
 /* ------------------------------------------------------------------------- */
 /* No counterpart in the reference (one elliptic_t per field and rank there).
  * n independent problems (any order, size, lambda, options), each set up as by
- * nrs_elliptic_setup; multigrid problems must use a smoothed coarse level,
- * NRS_COARSE_SMOOTH or NRS_COARSE_SMOOTH_JACOBI (the solve is device-driven).  Setup captures itersPerLaunch iterations of each problem
+ * nrs_elliptic_setup; multigrid problems need a coarse level without host
+ * round trips: NRS_COARSE_SMOOTH, NRS_COARSE_SMOOTH_JACOBI, or NRS_COARSE_AMG with
+ * coarseTol <= 0 (the solve is device-driven).  Setup captures itersPerLaunch iterations of each problem
  * into a HIP graph per problem (NRS_BATCH_GRAPH_PER_PROBLEM, the default; see
  * nrs_batch_set_mode for the single-graph form). */
 typedef struct nrs_batch nrs_batch;

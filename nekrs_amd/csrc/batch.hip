@@ -156,10 +156,11 @@ nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic
   auto b = std::make_unique<nrs_batch>();
   b->itersPerLaunch = itersPerLaunch;
   for (int p = 0; p < n; ++p) {
-    NRS_CHECK(!(opts[p].preconditioner == NRS_PRECON_MULTIGRID && opts[p].coarseSolver == NRS_COARSE_PCG_F64),
-              "batch: multigrid problems need a smoothed coarse level (no host round trip)");
     NRS_CHECK(descs[p].Nfields <= 1, "batch: one field per problem (a block solve is not device-driven)");
     b->probs.push_back(elliptic_setup(&descs[p], &opts[p]));
+    NRS_CHECK(!b->probs.back()->mg || b->probs.back()->mg->device_only(),
+              "batch: multigrid problems need a coarse level without host round trips (a smoothed "
+              "coarse level, or the AMG coarse solve with coarseTol <= 0)");
     const int64_t L = b->probs.back()->Nlocal;
     b->r.emplace_back(L);
     b->x.emplace_back(L);

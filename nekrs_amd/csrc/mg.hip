@@ -302,6 +302,7 @@ struct CoarsePCG {
   // (rows numbered by first appearance), preconditioned by one
   // smoothed-aggregation V-cycle; E-vector <-> rows through rowOf / first.
   bool amgMode = false;
+  bool deviceOnly = false;   // AMG with tol <= 0: exactly maxit iterations, no host round trip
   int n = 0;
   Amg amg;
   DevBuf<int> rowOf, first, arp, aci;
@@ -391,6 +392,28 @@ struct CoarsePCG {
   template <typename T>
   void solve_amg(const T *rhs, T *xout, hipStream_t s)
   {
+    if (deviceOnly) {
+      // a fixed number of AMG-preconditioned PCG iterations, all on the
+      // device (capturable): the reference's coarse solve is likewise a fixed
+      // BoomerAMG cycle count (coarseLevel.cpp:80-93)
+      rows_gather<T>(n, first.get(), rhs, ru.get(), s);
+      fill<double>(n, 0.0, xu.get(), s);
+      fill<double>(n, 0.0, pu.get(), s);
+      wnorm(*red, n, ones.get(), ru.get(), 1.0, scal.get(), S_RDOTR, s);
+      pcg_init_device(scal.get(), 0.0, false, maxit, s);
+      for (int it = 0; it < maxit; ++it) {
+        amg.apply(ru.get(), zu.get(), tmp.get(), s);
+        pcg_precon_dots(*red, n, ones.get(), nullptr, ru.get(), zu.get(), Apu.get(), false, false, false,
+                        scal.get(), s);
+        pcg_update_p(n, zu.get(), pu.get(), scal.get(), s);
+        csr_spmv(SPMV_Y, n, arp.get(), aci.get(), av.get(), pu.get(), Apu.get(), nullptr, nullptr, 0.0, s);
+        pcg_alpha(*red, n, ones.get(), pu.get(), Apu.get(), scal.get(), s);
+        pcg_update_rx(*red, n, ones.get(), Apu.get(), pu.get(), ru.get(), xu.get(), 1.0, scal.get(), s);
+      }
+      lastIters = maxit;
+      rows_scatter<T>(L, rowOf.get(), xu.get(), xout, s);
+      return;
+    }
     rows_gather<T>(n, first.get(), rhs, ru.get(), s);
     fill<double>(n, 0.0, xu.get(), s);
     fill<double>(n, 0.0, pu.get(), s);
@@ -493,6 +516,9 @@ struct CoarsePCG {
     ru.alloc(n); xu.alloc(n); zu.alloc(n); pu.alloc(n); Apu.alloc(n); tmp.alloc(n);
     amg.setup(A, AmgParams{});
     amgMode = true;
+    deviceOnly = !(tol > 0.0);
+    NRS_CHECK(!deviceOnly || (maxit >= 1 && maxit <= 64),
+              "coarse AMG with coarseTol <= 0 runs exactly coarseMaxIter (1..64) iterations");
   }
 };
 
@@ -509,6 +535,20 @@ struct MGImpl final : MGSolver {
   std::vector<T> betasDown, betasUp, betasCoarse;
   bool coarseSmoothing = false;      // coarse level smoothed (degree 3), no coarse solve
   bool coarseJacobi = false;         // ... with Chebyshev-Jacobi whatever the finer levels use
+  // Replay of the whole V-cycle as one HIP graph (every argument lives in
+  // the handle: level vectors, device scalars, host-fixed Chebyshev
+  // coefficients), used when the cycle never waits on the host
+  bool cycleGraphs = false;
+  DevBuf<double> rIn, zOut;          // fixed FP64 cycle input / output (FP64 multigrid)
+  hipStream_t cycCap = nullptr;
+  hipGraph_t cycGraph = nullptr;
+  hipGraphExec_t cycExec = nullptr;
+  ~MGImpl() override
+  {
+    if (cycExec) (void)hipGraphExecDestroy(cycExec);
+    if (cycGraph) (void)hipGraphDestroy(cycGraph);
+    if (cycCap) (void)hipStreamDestroy(cycCap);
+  }
 
   // ---- level operators (pMGLevel::Ax / residual / smoother)
   // the level operator; the finest level shares the solver's chunked handle,
@@ -635,6 +675,30 @@ struct MGImpl final : MGSolver {
   void apply(const double *r, double *z, hipStream_t s) override
   {
     Level<T> &l0 = *levels[0];
+    hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+    NRS_HIP(hipStreamIsCapturing(s, &cst));
+    if (cycleGraphs && device_only() && cst == hipStreamCaptureStatusNone) {
+      // input and output through fixed buffers, the cycle itself replayed
+      if (std::is_same<T, double>::value) {
+        k_axpby<double>(l0.L, 1.0, r, 0.0, rIn.get(), s);
+        l0.rhs = (T *)rIn.get();
+        l0.x = (T *)zOut.get();
+      } else {
+        k_cast<double, T>(l0.L, r, rP.get(), s);
+        l0.rhs = rP.get();
+        l0.x = zP.get();
+      }
+      if (!cycExec) {
+        if (!cycCap) NRS_HIP(hipStreamCreateWithFlags(&cycCap, hipStreamNonBlocking));
+        NRS_HIP(hipStreamBeginCapture(cycCap, hipStreamCaptureModeThreadLocal));
+        vcycle(0, cycCap);
+        NRS_HIP(hipStreamEndCapture(cycCap, &cycGraph));
+        NRS_HIP(hipGraphInstantiate(&cycExec, cycGraph, nullptr, nullptr, 0));
+      }
+      NRS_HIP(hipGraphLaunch(cycExec, s));
+      k_cast<T, double>(l0.L, l0.x, z, s);
+      return;
+    }
     if (std::is_same<T, double>::value) {
       l0.rhs = (T *)r;   // read-only inside the cycle
       l0.x = (T *)z;
@@ -654,7 +718,7 @@ struct MGImpl final : MGSolver {
     return out;
   }
   int coarse_iterations() const override { return coarse ? coarse->lastIters : 0; }
-  bool device_only() const override { return coarse == nullptr; }
+  bool device_only() const override { return coarse == nullptr || coarse->deviceOnly; }
   void level_smoother(int level, const void *in, void *out, hipStream_t s) override
   {
     NRS_CHECK(level >= 0 && level < (int)levels.size() && (level + 1 < (int)levels.size() || !coarse),
@@ -930,6 +994,8 @@ struct MGImpl final : MGSolver {
     }
     const int64_t L0 = levels[0]->L;
     sRes.alloc(L0); sAd.alloc(L0); sD.alloc(L0); sTmp.alloc(L0);
+    cycleGraphs = o.useGraphs != 0;
+    if (cycleGraphs && std::is_same<T, double>::value) { rIn.alloc(L0); zOut.alloc(L0); }
     if (!std::is_same<T, double>::value) { rP.alloc(L0); zP.alloc(L0); }
     for (auto &lp : levels) {
       Level<T> &l = *lp;
@@ -942,15 +1008,19 @@ struct MGImpl final : MGSolver {
   }
 
   // optimal fourth-kind Chebyshev betas (MG/optimalCoeffs.cpp:4-215; Lottes,
-  // arXiv:2202.08830), degrees 1..4 tabulated here
+  // arXiv:2202.08830), degrees 1..8 tabulated here
   template <typename U> static std::vector<U> opt4_betas(int deg)
   {
     static const std::vector<std::vector<double>> t = {
         {1.12500000000000},
         {1.02387287570313, 1.26408905371085},
         {1.00842544782028, 1.08867839208730, 1.33753125909618},
-        {1.00391310427285, 1.04035811188593, 1.14863498546254, 1.38268869241000}};
-    NRS_CHECK(deg >= 1 && deg <= (int)t.size(), "opt. 4th-kind Chebyshev: degree 1..4 built");
+        {1.00391310427285, 1.04035811188593, 1.14863498546254, 1.38268869241000},
+        {1.00212930146164, 1.02173711549260, 1.07872433192603, 1.19810065292663, 1.41322542791682},
+        {1.00128517255940, 1.01304293035233, 1.04678215124113, 1.11616489419675, 1.23829020218444, 1.43524297106744},
+        {1.00083464397912, 1.00843949430122, 1.03008707768713, 1.07408384092003, 1.15036186707366, 1.27116474046139, 1.45186658649364},
+        {1.00057246631197, 1.00577427662415, 1.02050187922941, 1.05019803444565, 1.10115572984941, 1.18086042806856, 1.29838585382576, 1.46486073151099}};
+    NRS_CHECK(deg >= 1 && deg <= (int)t.size(), "opt. 4th-kind Chebyshev: degree 1..8 built");
     std::vector<U> b;
     for (double v : t[deg - 1]) b.push_back((U)v);
     return b;

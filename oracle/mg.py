@@ -31,13 +31,17 @@ from . import (BoxMesh, Elliptic, GWJ, Ogs, gll, interp, lib, mask_ids, masked_g
                transfer_R, build_diag)
 from . import fdm as _fdm
 
-# optimal 4th-kind Chebyshev betas, degrees 1..4 (MG/optimalCoeffs.cpp:4-215;
+# optimal 4th-kind Chebyshev betas, degrees 1..8 (MG/optimalCoeffs.cpp:4-215;
 # Lottes, arXiv:2202.08830)
 OPT4_BETAS = {
     1: [1.12500000000000],
     2: [1.02387287570313, 1.26408905371085],
     3: [1.00842544782028, 1.08867839208730, 1.33753125909618],
-    4: [1.00391310427285, 1.04035811188593, 1.14863498546254, 1.38268869241000]}
+    4: [1.00391310427285, 1.04035811188593, 1.14863498546254, 1.38268869241000],
+    5: [1.00212930146164, 1.02173711549260, 1.07872433192603, 1.19810065292663, 1.41322542791682],
+    6: [1.00128517255940, 1.01304293035233, 1.04678215124113, 1.11616489419675, 1.23829020218444, 1.43524297106744],
+    7: [1.00083464397912, 1.00843949430122, 1.03008707768713, 1.07408384092003, 1.15036186707366, 1.27116474046139, 1.45186658649364],
+    8: [1.00057246631197, 1.00577427662415, 1.02050187922941, 1.05019803444565, 1.10115572984941, 1.18086042806856, 1.29838585382576, 1.46486073151099]}
 
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 

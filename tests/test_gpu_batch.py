@@ -36,6 +36,10 @@ def problems(gpu):
         lam = float(rng.uniform(0.0, 10.0))
         kw = dict(MG, lambda0=1.0, lambda1=lam, poisson=False, fdmPrecision=1 if i % 2 else 0)
         out.append((m, kw))
+    # multigrid with the device-side AMG coarse solve (fixed iteration count)
+    out.append((gpu.BoxMesh(9, 8, 8, 3, "kershaw", 0.25),
+                dict(MG, coarse="amg", coarse_tol=0.0, coarse_max_iter=2, lambda0=1.0, lambda1=1.5,
+                     poisson=False, fdmPrecision=1)))
     # plain Jacobi-PCG Poisson problems ride along
     out.append((gpu.BoxMesh(3, 3, 3, 4, "sine"), dict(solver="pcg", precon="jacobi", tol=1e-9,
                                                       max_iter=500)))

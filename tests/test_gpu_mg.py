@@ -160,3 +160,38 @@ def test_mg_amg_coarse_vs_oracle(gpu, torch_mod, orc, shape, N, smoother, lam):
     assert abs(cit - om.last_coarse_it) <= 1
     xh = x.cpu().numpy()
     assert np.max(np.abs(xh - xo)) <= 1e-8 * np.max(np.abs(xo))
+
+
+def test_mg_amg_fixed_iterations_graph_vs_oracle(gpu, torch_mod, orc):
+    """AMG coarse solve with a fixed iteration count (coarseTol = 0, two
+    AMG-preconditioned PCG iterations, no host round trip): the whole V-cycle
+    is replayed as one HIP graph; the result equals the eager cycle bit for
+    bit and the solve matches the oracle (iterations +-1, solution 1e-8)."""
+    import oracle.mg as omg
+    torch = torch_mod
+    shape, N = (10, 9, 8), 3
+    mg = gpu.BoxMesh(*shape, N, "kershaw", 0.3)
+    kw = dict(precon="multigrid", solver="fcg", levels=[N, 1], smoother="opt4chebyshev+ras",
+              cheby_degree=3, fdmPrecision=1, coarse="amg", coarse_tol=0.0, coarse_max_iter=2,
+              tol=1e-10, max_iter=200)
+    eg = gpu.Elliptic(mg, **kw)
+    ee = gpu.Elliptic(mg, useGraphs=0, **kw)
+    mo = orc.BoxMesh(*shape, N, "kershaw", 0.3)
+    eo = orc.Elliptic(mo)
+    om = omg.MG(mo, [N, 1], T=np.float32, degree=3, cheby="opt4", S="ras", fmt="bf16",
+                coarse="amg", coarse_tol=0.0, coarse_maxit=2)
+    rhs = np.random.default_rng(4).uniform(-1, 1, mo.Nlocal)
+    r = torch.from_numpy(rhs).cuda()
+    z1 = torch.zeros_like(r)
+    z2 = torch.zeros_like(r)
+    for _ in range(3):                      # replays after the capture
+        eg.preconditioner(r, z1)
+        ee.preconditioner(r, z2)
+        torch.cuda.synchronize()
+        assert torch.equal(z1, z2)
+    xo, info = omg.fcg_solve(eo, rhs, om.apply, flexible=True, tol=1e-10, maxit=200)
+    x = torch.zeros_like(r)
+    st = eg.solve(r.clone(), x)
+    print("fixed AMG gpu it", st["iterations"], "oracle it", info["iterations"])
+    assert st["converged"] and abs(st["iterations"] - info["iterations"]) <= 1
+    assert np.max(np.abs(x.cpu().numpy() - xo)) <= 1e-8 * np.max(np.abs(xo))
