@@ -16,12 +16,16 @@ times 100 steps after 10 warm-up steps, inputs resident in HBM.
   solves     the second half of the metric: time to a 1e-8 relative residual of
              the 48^3 N=7 Kershaw pressure-type Poisson solve (manufactured sine
              solution plus seeded 1e-3 Gaussian noise in the forcing), flexible
-             CG preconditioned by p-multigrid 7-3-1 with an FP64 Jacobi-PCG coarse
-             solve: all-FP64 Chebyshev-Jacobi (degree 2) against FP32 multigrid
-             with optimal 4th-kind Chebyshev + RAS/FDM smoothing (degree 3), the FDM
-             contractions in BF16 on the matrix cores (FP32 accumulation).
-             Each case is set up once, solved once untimed, then timed over one
-             solve from x = 0 (setup excluded).
+             CG preconditioned by p-multigrid 7-3-1 with an FP64 coarse solve (two
+             PCG iterations on the assembled order-1 matrix, each preconditioned by
+             one smoothed-aggregation AMG V-cycle, on the device; the V-cycle is
+             replayed as one HIP graph): all-FP64 Chebyshev-Jacobi (degree 2)
+             against FP32 multigrid with optimal 4th-kind Chebyshev + RAS/FDM
+             smoothing (degree 4), the FDM contractions in BF16 on the matrix cores
+             (FP32 accumulation).  Each case is set up once, solved once untimed,
+             then timed over one solve from x = 0 (setup excluded).
+  batch      64 uneven Helmholtz problems advanced by captured graphs, and the
+             3-component velocity Helmholtz solve (batch of 3, and one block solve)
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
