@@ -168,7 +168,8 @@ typedef struct {
                                   * elliptic.h:77; Jacobi or no preconditioner) */
 } nrs_elliptic_desc;
 
-enum nrs_solver { NRS_SOLVER_PCG = 0, NRS_SOLVER_FCG = 1 };  /* SOLVER=PCG[+FLEXIBLE] */
+/* SOLVER = PCG, PCG+FLEXIBLE, PGMRES, PGMRES+FLEXIBLE (PGMRES.cpp:111-340) */
+enum nrs_solver { NRS_SOLVER_PCG = 0, NRS_SOLVER_FCG = 1, NRS_SOLVER_GMRES = 2, NRS_SOLVER_FGMRES = 3 };
 enum nrs_precon { NRS_PRECON_NONE = 0, NRS_PRECON_JACOBI = 1, NRS_PRECON_MULTIGRID = 2 };
 enum nrs_smoother {
   NRS_SMOOTHER_CHEBYSHEV_JACOBI = 0,  /* DAMPEDJACOBI,CHEBYSHEV (1st kind)       */
@@ -221,6 +222,7 @@ typedef struct {
   unsigned long long seed;       /* Arnoldi start vector seed                   */
   int useGraphs;                 /* replay captured HIP graphs for the coarse PCG */
   int autotune;                  /* KERNEL AUTOTUNING: time launch variants at setup */
+  int gmresRestart;              /* PGMRES RESTART (default 15, at most 32)     */
 } nrs_elliptic_opts;
 
 typedef struct {

@@ -25,7 +25,8 @@ __all__ = ["NrsError", "init", "box_mesh", "gll", "ax", "block_ax", "build_diago
 
 OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
 MAPS = {"none": 0, "sine": 1, "kershaw": 2}
-SOLVERS = {"pcg": 0, "fcg": 1, "pcg+flexible": 1}
+SOLVERS = {"pcg": 0, "fcg": 1, "pcg+flexible": 1, "gmres": 2, "pgmres": 2, "fgmres": 3,
+           "pgmres+flexible": 3}
 PRECONS = {"none": 0, "jacobi": 1, "multigrid": 2}
 SMOOTHERS = {"chebyshev+jacobi": 0, "opt4chebyshev+jacobi": 1, "chebyshev+asm": 2,
              "opt4chebyshev+asm": 3, "asm": 4, "ras": 5, "chebyshev+ras": 6,

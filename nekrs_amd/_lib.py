@@ -37,7 +37,7 @@ class EllipticOpts(C.Structure):
                 ("chebyMinFactor", C.c_double), ("chebyMaxFactor", C.c_double),
                 ("coarseSolver", C.c_int), ("coarseTol", C.c_double), ("coarseMaxIter", C.c_int),
                 ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong),
-                ("useGraphs", C.c_int), ("autotune", C.c_int)]
+                ("useGraphs", C.c_int), ("autotune", C.c_int), ("gmresRestart", C.c_int)]
 
 
 class SolveStats(C.Structure):

@@ -157,6 +157,8 @@ nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic
   b->itersPerLaunch = itersPerLaunch;
   for (int p = 0; p < n; ++p) {
     NRS_CHECK(descs[p].Nfields <= 1, "batch: one field per problem (a block solve is not device-driven)");
+    NRS_CHECK(opts[p].solver == NRS_SOLVER_PCG || opts[p].solver == NRS_SOLVER_FCG,
+              "batch: PCG or flexible PCG (PGMRES needs the host for its Hessenberg system)");
     b->probs.push_back(elliptic_setup(&descs[p], &opts[p]));
     NRS_CHECK(!b->probs.back()->mg || b->probs.back()->mg->device_only(),
               "batch: multigrid problems need a coarse level without host round trips (a smoothed "

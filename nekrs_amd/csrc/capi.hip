@@ -268,6 +268,7 @@ void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
   o->seed = 0;
   o->useGraphs = 1;
   o->autotune = 1;
+  o->gmresRestart = 15;
 }
 
 int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts, nrs_elliptic **out)

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "elliptic.hpp"
+#include "gmres.hpp"
 #include "tune.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
@@ -92,7 +93,9 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   NRS_CHECK(d->N >= 1 && d->N <= 9, "elliptic setup: N must be 1..9");
   NRS_CHECK(d->Nelements > 0, "elliptic setup: no elements");
   NRS_CHECK(d->x && d->y && d->z && d->globalIds && d->EToB, "elliptic setup: null mesh arrays");
-  NRS_CHECK(o->solver == NRS_SOLVER_PCG || o->solver == NRS_SOLVER_FCG, "unknown solver");
+  NRS_CHECK(o->solver >= NRS_SOLVER_PCG && o->solver <= NRS_SOLVER_FGMRES, "unknown solver");
+  NRS_CHECK(o->solver < NRS_SOLVER_GMRES || (o->gmresRestart >= 1 && o->gmresRestart <= kGmresMaxVectors),
+            "PGMRES restart must be 1..32");
   NRS_CHECK(o->preconditioner >= NRS_PRECON_NONE && o->preconditioner <= NRS_PRECON_MULTIGRID,
             "unknown preconditioner");
   NRS_CHECK(o->maxIter >= 1, "maxIter must be >= 1");
@@ -169,6 +172,15 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   // workspace
   const int64_t Lt = (int64_t)el->Nf * el->Nlocal;
   el->p.alloc(Lt); el->zv.alloc(Lt); el->Ap.alloc(Lt); el->x0.alloc(Lt);
+  if (o->solver == NRS_SOLVER_GMRES || o->solver == NRS_SOLVER_FGMRES) {
+    // PGMRES workspace (GmresData, PGMRES.cpp:30-57): V [m][Lt], Z [m or 1][Lt]
+    const int m = o->gmresRestart;
+    el->gV.alloc((size_t)m * Lt);
+    el->gZ.alloc((size_t)(o->solver == NRS_SOLVER_FGMRES ? m : 1) * Lt);
+    el->gB.alloc(Lt);
+    el->gY.alloc(kGmresMaxVectors + 2);
+    el->gPart.alloc(kGmresPartials);
+  }
   if (el->Nf > 1) {
     el->wRep.alloc(Lt);
     for (int f = 0; f < el->Nf; ++f)
@@ -332,6 +344,116 @@ void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipSt
   }
 }
 
+// Restarted (flexible) PGMRES (src/solvers/elliptic/linearSolver/PGMRES.cpp:111-340):
+// classical Gram-Schmidt against all basis vectors in one pass, Givens
+// rotations and the small triangular solve on the host; one readback of
+// (y, |w|) per iteration.  r: the masked, assembled initial residual; x = 0;
+// tol: absolute on the 1/volume-scaled norm.
+static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, double tol, bool timed,
+                        int &iter, double &rdotr, double &precon_s, int &status, hipStream_t s)
+{
+  const int64_t L = (int64_t)el->Nf * el->Nlocal;
+  const double *wgt = el->Nf > 1 ? el->wRep.get() : el->ogs->invDegree.get();
+  const auto &o = el->opts;
+  const bool flexible = o.solver == NRS_SOLVER_FGMRES;
+  const int m = o.gmresRestart, ldH = m + 1;
+  const double sqrtRNF = std::sqrt(1.0 / el->volume);
+  double *V = el->gV.get(), *Z = el->gZ.get(), *b = el->gB.get(), *w = el->p.get(), *Ax = el->Ap.get();
+  double *yd = el->gY.get(), *part = el->gPart.get();
+  std::vector<double> H((size_t)ldH * ldH, 0.0), cs(m, 0.0), sn(m, 0.0), sv(m + 1, 0.0), y(m + 1, 0.0);
+  NRS_HIP(hipMemcpyAsync(b, r, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+  double nr = res0 / sqrtRNF;
+  double error = res0;
+  iter = 0;
+  rdotr = res0;
+  if (!(nr > 0.0)) return;   // zero residual: x = 0 is the solution
+  auto precon = [&](const double *in, double *out) {
+    auto p0 = std::chrono::steady_clock::now();
+    elliptic_preconditioner(el, in, out, s);
+    if (timed) {
+      NRS_HIP(hipStreamSynchronize(s));
+      precon_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
+    }
+  };
+  // gmresUpdate (PGMRES.cpp:59-104): back substitution, then x += Z y (flexible)
+  // or x += M^-1 (V y)
+  auto update = [&](int k) {
+    for (int a = k - 1; a >= 0; --a) {
+      y[a] = sv[a];
+      for (int c = a + 1; c < k; ++c) y[a] -= H[a + c * ldH] * y[c];
+      y[a] /= H[a + a * ldH];
+    }
+    NRS_HIP(hipMemcpyAsync(yd, y.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+    if (flexible) {
+      gmres_update(L, k, L, yd, Z, x, s);
+    } else {
+      double *zt = el->zv.get();
+      fill<double>(L, 0.0, zt, s);
+      gmres_update(L, k, L, yd, V, zt, s);
+      precon(zt, w);
+      axpby<double>(L, 1.0, w, 1.0, x, s);
+    }
+  };
+  std::vector<double> hy(kGmresMaxVectors + 2);
+  while (iter < o.maxIter) {
+    sv.assign(m + 1, 0.0);
+    sv[0] = nr;
+    axpby<double>(L, 1.0 / nr, r, 0.0, V, s);                       // V_0 = r / |r|
+    int used = m;
+    bool done = false;
+    for (int i = 0; i < m; ++i) {
+      double *Mv = flexible ? Z + (int64_t)i * L : Z;
+      precon(V + (int64_t)i * L, Mv);                               // z = M^-1 V_i
+      elliptic_operator(el, Mv, w, true, s);                        // w = A z
+      gmres_multi_dot(L, i + 1, L, wgt, V, w, part, yd, s);        // y_j = <V_j, w>
+      gmres_gram_schmidt(L, i + 1, L, wgt, yd, V, w, part, yd + kGmresMaxVectors, s);
+      NRS_HIP(hipMemcpyAsync(hy.data(), yd, (kGmresMaxVectors + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+      NRS_HIP(hipStreamSynchronize(s));
+      const double nw = std::sqrt(hy[kGmresMaxVectors]);
+      H[i + 1 + i * ldH] = nw;
+      if (i < m - 1) axpby<double>(L, 1.0 / nw, w, 0.0, V + (int64_t)(i + 1) * L, s);
+      for (int k = 0; k <= i; ++k) H[k + i * ldH] = hy[k];
+      for (int k = 0; k < i; ++k) {                                 // previous rotations
+        const double h1 = H[k + i * ldH], h2 = H[k + 1 + i * ldH];
+        H[k + i * ldH] = cs[k] * h1 + sn[k] * h2;
+        H[k + 1 + i * ldH] = -sn[k] * h1 + cs[k] * h2;
+      }
+      const double h1 = H[i + i * ldH], h2 = H[i + 1 + i * ldH];
+      const double hr = std::sqrt(h1 * h1 + h2 * h2);
+      cs[i] = h1 / hr;
+      sn[i] = h2 / hr;
+      H[i + i * ldH] = cs[i] * h1 + sn[i] * h2;
+      H[i + 1 + i * ldH] = 0.0;
+      sv[i + 1] = -sn[i] * sv[i];
+      sv[i] = cs[i] * sv[i];
+      ++iter;
+      error = std::fabs(sv[i + 1]) * sqrtRNF;
+      rdotr = error;
+      if (std::isnan(error)) { status = NRS_ERR_NAN; return; }
+      if (error < tol || iter == o.maxIter) {
+        update(i + 1);
+        used = i + 1;
+        done = true;
+        break;
+      }
+    }
+    (void)used;
+    if (done) break;
+    update(m);
+    // restart: r = b - A x (fusedResidualAndNorm)
+    elliptic_operator(el, x, Ax, true, s);
+    gmres_residual(L, wgt, b, Ax, r, part, yd + kGmresMaxVectors, s);
+    NRS_HIP(hipMemcpyAsync(hy.data() + kGmresMaxVectors, yd + kGmresMaxVectors, sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    nr = std::sqrt(hy[kGmresMaxVectors]);
+    error = nr * sqrtRNF;
+    rdotr = error;
+    if (std::isnan(error)) { status = NRS_ERR_NAN; return; }
+    if (error <= tol) return;
+  }
+}
+
 // ellipticSolve (src/solvers/elliptic/ellipticSolve.cpp:32-190) + pcg (PCG.cpp:85-203).
 int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s)
 {
@@ -364,6 +486,24 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   if (std::isnan(res0)) return NRS_ERR_NAN;
   double tol = o.tol;
   if (o.relative) tol *= res0;
+
+  if (o.solver == NRS_SOLVER_GMRES || o.solver == NRS_SOLVER_FGMRES) {
+    int iter = 0, status = NRS_OK;
+    double rdotr = res0, precon_s = 0.0;
+    gmres_solve(el, r, x, res0, tol, st != nullptr, iter, rdotr, precon_s, status, s);
+    axpby<double>(L, 1.0, el->x0.get(), 1.0, x, s);
+    NRS_HIP(hipEventRecord(el->ev1, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    if (st) {
+      st->iterations = iter;
+      st->converged = (rdotr <= tol) ? 1 : 0;
+      st->res0 = res0;
+      st->res = rdotr;
+      st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      st->precon_seconds = precon_s;
+    }
+    return status;
+  }
 
   // pcg
   double *p = el->p.get(), *Ap = el->Ap.get();

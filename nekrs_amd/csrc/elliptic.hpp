@@ -31,6 +31,7 @@ struct nrs_elliptic {
   nrs::DevBuf<uint32_t> maskBits;    // [E][ceil(Np/32)]
   nrs::DevBuf<double> invDiagA;      // Jacobi (assembled), fine level, [Nf][Nlocal]
   nrs::DevBuf<double> wRep;          // invDegree repeated per field (Nf > 1)
+  nrs::DevBuf<double> gV, gZ, gB, gY, gPart;   // PGMRES basis, preconditioned basis, rhs, y, partials
 
   // workspace (ellipticUpdateWorkspace, elliptic.h:240-262)
   nrs::DevBuf<double> p, zv, Ap, x0;

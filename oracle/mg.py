@@ -394,3 +394,86 @@ def fcg_solve(el, rhs, precon, flexible=True, tol=1e-8, maxit=500, x0=None):
         if not (rdotr > tol_abs and it < maxit):
             break
     return xs + x, {"iterations": it, "res0": res0, "res": rdotr, "history": np.array(hist)}
+
+
+def gmres_solve(el, rhs, precon, flexible=True, restart=15, tol=1e-8, maxit=500, nfields=1):
+    """ellipticSolve + pgmres (PGMRES.cpp:111-340) with numpy: classical
+    Gram-Schmidt with invDegree-weighted inner products, Givens rotations,
+    restart; tol relative to the 1/volume-scaled initial norm.  nfields > 1:
+    the same operator on each field, inner products summed over the fields."""
+    m = el.mesh
+    L = m.Nlocal
+    A = lambda v: np.concatenate([el.operator(v[f * L:(f + 1) * L]) for f in range(nfields)])
+    M = lambda v: np.concatenate([precon(v[f * L:(f + 1) * L]) for f in range(nfields)])
+    mask = np.isin(np.arange(L), el.maskIds)
+    r = np.concatenate([el.ogs.gs(np.where(mask, 0.0, rhs[f * L:(f + 1) * L]), "add")
+                        for f in range(nfields)])
+    w = np.tile(el.invDegree, nfields)
+    rnf = 1.0 / el.volume
+    dot = lambda a, b: float(np.sum(w * a * b))
+    res0 = np.sqrt(dot(r, r) * rnf)
+    tol_abs = tol * res0
+    x = np.zeros_like(r)
+    b = r.copy()
+    nr = res0 / np.sqrt(rnf)
+    err = res0
+    it = 0
+
+    def update(k, H, s, V, Z):
+        y = np.zeros(k)
+        for a in range(k - 1, -1, -1):
+            y[a] = s[a]
+            for c in range(a + 1, k):
+                y[a] -= H[a, c] * y[c]
+            y[a] /= H[a, a]
+        if flexible:
+            return sum(y[j] * Z[j] for j in range(k))
+        return M(sum(y[j] * V[j] for j in range(k)))
+
+    while it < maxit:
+        H = np.zeros((restart + 1, restart + 1))
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        s = np.zeros(restart + 1)
+        s[0] = nr
+        V = [r / nr]
+        Z = []
+        done = False
+        for i in range(restart):
+            z = M(V[i])
+            Z.append(z)
+            wv = A(z)
+            y = np.array([dot(V[j], wv) for j in range(i + 1)])
+            for j in range(i + 1):
+                wv = wv - y[j] * V[j]
+            nw = np.sqrt(dot(wv, wv))
+            H[i + 1, i] = nw
+            if i < restart - 1:
+                V.append(wv / nw)
+            H[:i + 1, i] = y
+            for k in range(i):
+                h1, h2 = H[k, i], H[k + 1, i]
+                H[k, i] = cs[k] * h1 + sn[k] * h2
+                H[k + 1, i] = -sn[k] * h1 + cs[k] * h2
+            h1, h2 = H[i, i], H[i + 1, i]
+            hr = np.sqrt(h1 * h1 + h2 * h2)
+            cs[i], sn[i] = h1 / hr, h2 / hr
+            H[i, i] = cs[i] * h1 + sn[i] * h2
+            H[i + 1, i] = 0.0
+            s[i + 1] = -sn[i] * s[i]
+            s[i] = cs[i] * s[i]
+            it += 1
+            err = abs(s[i + 1]) * np.sqrt(rnf)
+            if err < tol_abs or it == maxit:
+                x = x + update(i + 1, H, s, V, Z)
+                done = True
+                break
+        if done:
+            break
+        x = x + update(restart, H, s, V, Z)
+        r = b - A(x)
+        nr = np.sqrt(dot(r, r))
+        err = nr * np.sqrt(rnf)
+        if err <= tol_abs:
+            break
+    return x, {"iterations": it, "res0": res0, "res": err}
+

@@ -280,3 +280,57 @@ def test_block_multigrid_vs_oracle(gpu, torch_mod, orc):
     print("block MG gpu it", st["iterations"], "oracle it", it)
     assert st["converged"] and abs(st["iterations"] - it) <= 1
     assert rel(host(x), xo) <= 1e-8
+
+
+@pytest.mark.parametrize("solver,precon,restart", [("fgmres", "multigrid", 15), ("gmres", "jacobi", 15),
+                                                   ("gmres", "jacobi", 5), ("fgmres", "none", 8)])
+def test_pgmres_vs_oracle(gpu, torch_mod, orc, solver, precon, restart):
+    """Restarted (flexible) PGMRES (PGMRES.cpp:111-340) against the oracle's
+    restatement with the same preconditioner: iterations +-1, solution 1e-8;
+    restart 5 and 8 exercise the restart path."""
+    import oracle.mg as omg
+    torch = torch_mod
+    shape, N, lam1 = (4, 3, 3), 5, 1.5
+    mg = gpu.BoxMesh(*shape, N, "kershaw", 0.3)
+    mo = orc.BoxMesh(*shape, N, "kershaw", 0.3)
+    kw = dict(solver=solver, precon=precon, gmresRestart=restart, tol=1e-9, max_iter=600)
+    if precon == "multigrid":
+        kw.update(levels=[N, 3, 1], smoother="opt4chebyshev+jacobi", cheby_degree=3, coarse="smooth")
+    eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=lam1, poisson=False, **kw)
+    eo = orc.Elliptic(mo, 1.0, lam1, False)
+    if precon == "multigrid":
+        om = omg.MG(mo, [N, 3, 1], T=np.float32, degree=3, cheby="opt4", S="jacobi",
+                    coarse="smooth", lambda0=1.0, lambda1=lam1, poisson=False)
+        M = om.apply
+    elif precon == "jacobi":
+        inv = eo.inv_diag()
+        M = lambda r: inv * r
+    else:
+        M = lambda r: r
+    rhs = np.random.default_rng(restart).uniform(-1, 1, mo.Nlocal)
+    xo, info = omg.gmres_solve(eo, rhs, M, flexible=(solver == "fgmres"), restart=restart,
+                               tol=1e-9, maxit=600)
+    x = torch.zeros(mo.Nlocal, dtype=torch.float64, device="cuda")
+    st = eg.solve(dev(torch, rhs), x)
+    print(solver, precon, restart, "gpu it", st["iterations"], "oracle it", info["iterations"])
+    assert st["converged"] and abs(st["iterations"] - info["iterations"]) <= 1
+    assert rel(host(x), xo) <= 1e-8
+
+
+def test_pgmres_block_three_fields(gpu, torch_mod, orc):
+    import oracle.mg as omg
+    torch = torch_mod
+    shape, N = (4, 3, 3), 3
+    mg = gpu.BoxMesh(*shape, N, "kershaw", 0.3)
+    mo = orc.BoxMesh(*shape, N, "kershaw", 0.3)
+    eg = gpu.Elliptic(mg, lambda0=1.0, lambda1=4.0, poisson=False, nfields=3, solver="gmres",
+                      precon="jacobi", gmresRestart=10, tol=1e-9, max_iter=600)
+    eo = orc.Elliptic(mo, 1.0, 4.0, False)
+    inv = eo.inv_diag()
+    rhs = np.random.default_rng(3).uniform(-1, 1, 3 * mo.Nlocal)
+    xo, info = omg.gmres_solve(eo, rhs, lambda r: inv * r, flexible=False, restart=10, tol=1e-9,
+                               maxit=600, nfields=3)
+    x = torch.zeros(3 * mo.Nlocal, dtype=torch.float64, device="cuda")
+    st = eg.solve(dev(torch, rhs), x)
+    assert st["converged"] and abs(st["iterations"] - info["iterations"]) <= 1
+    assert rel(host(x), xo) <= 1e-8
