@@ -94,11 +94,17 @@ def solve_cases(n, N, seed=0):
     cases = [("fp64_chebyshev_jacobi", dict(smoother="chebyshev+jacobi", cheby_degree=2,
                                             mgPrecision=1, **amg)),
              ("mixed_opt4chebyshev_ras", dict(smoother="opt4chebyshev+ras", cheby_degree=4,
-                                              mgPrecision=0, fdmPrecision=1, **amg))]
+                                              mgPrecision=0, fdmPrecision=1, **amg)),
+             # the reference's default outer solver (PGMRES+FLEXIBLE, restart 15) on the
+             # mixed preconditioner
+             ("mixed_fgmres", dict(smoother="opt4chebyshev+ras", cheby_degree=4, mgPrecision=0,
+                                   fdmPrecision=1, solver="fgmres", **amg))]
     out, xs = {}, {}
     for name, kw in cases:
         t0 = time.perf_counter()
-        el = nekrs_amd.Elliptic(mesh, precon="multigrid", solver="fcg", tol=1e-8, max_iter=1000,
+        kw = dict(kw)
+        solver = kw.pop("solver", "fcg")
+        el = nekrs_amd.Elliptic(mesh, precon="multigrid", solver=solver, tol=1e-8, max_iter=1000,
                                 levels=[N, 3, 1], **kw)
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t0
