@@ -284,9 +284,10 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
     out["workload"] = ("%d Helmholtz problems (seed %d): N in {3,5,7,9}, E ~ U[512,8192] as "
                        "near-cubic boxes, lambda1 ~ U[0,10], Kershaw eps ~ U[0.05,0.3]; plus 3 "
                        "velocity components (lambda1=10) at %d^3 N=7 eps 0.3, as a batch and as "
-                       "one 3-field block Jacobi-PCG solve; FP32 p-MG, opt. "
-                       "4th-kind Chebyshev + ASM with BF16 FDM, Chebyshev-Jacobi smoothed "
-                       "coarse level, FCG to 1e-8 relative" % (count, seed, block_n))
+                       "one 3-field block solve; FP32 p-MG, opt. 4th-kind Chebyshev + ASM with "
+                       "BF16 FDM, Chebyshev-Jacobi smoothed coarse level (the block solve: AMG "
+                       "coarse solve, p-MG applied per field), FCG to 1e-8 relative" %
+                       (count, seed, block_n))
     return out
 
 

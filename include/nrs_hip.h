@@ -199,6 +199,11 @@ enum nrs_coarse {
                                   * so the whole V-cycle replays as one graph      */
 };
 enum nrs_mg_precision { NRS_MG_F32 = 0, NRS_MG_F64 = 1 };
+/* INITIAL GUESS (parReader.cpp:1195-1246, ellipticSetup.cpp:306-321): the
+ * caller's x (PREVIOUS), or the previous solutions' projection
+ * (SolutionProjection, ellipticSolutionProjection.cpp): CLASSIC keeps A x_k for
+ * every basis vector, ACONJ keeps an A-orthonormal basis and applies A once. */
+enum nrs_initial_guess { NRS_GUESS_PREVIOUS = 0, NRS_GUESS_PROJECTION = 1, NRS_GUESS_PROJECTION_ACONJ = 2 };
 enum nrs_fdm_precision { NRS_FDM_F32 = 0, NRS_FDM_BF16 = 1, NRS_FDM_F16 = 2 };
 
 typedef struct {
@@ -223,6 +228,10 @@ typedef struct {
   int useGraphs;                 /* replay captured HIP graphs for the coarse PCG */
   int autotune;                  /* KERNEL AUTOTUNING: time launch variants at setup */
   int gmresRestart;              /* PGMRES RESTART (default 15, at most 32)     */
+  int initialGuess;              /* nrs_initial_guess (default PREVIOUS)        */
+  int projVectors;               /* RESIDUAL PROJECTION VECTORS (default 8, 1..32) */
+  int projStart;                 /* RESIDUAL PROJECTION START: solve index from
+                                  * which the projection runs (default 5)       */
 } nrs_elliptic_opts;
 
 typedef struct {
@@ -232,6 +241,9 @@ typedef struct {
   double res;                    /* final normalised residual norm              */
   double seconds;                /* wall time of the solve                      */
   double precon_seconds;
+  double res00;                  /* normalised residual before the projection
+                                  * (= res0 without one)                       */
+  int projVectors;               /* basis vectors the projection used (0: none) */
 } nrs_solve_stats;
 
 /* Fill opts with the reference's defaults for this path (PCG, Jacobi,
@@ -268,8 +280,13 @@ int nrs_elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, vo
  * mask and gather-scatter to rhs - A x0 itself); x holds the initial guess on
  * entry and the solution on exit.  rhs is overwritten with the final residual.
  * Replaces ellipticSolve (src/solvers/elliptic/ellipticSolve.cpp:32-190) with
- * pcg (linearSolver/PCG.cpp:85-203).  Returns NRS_OK also when maxIter is hit
- * (stats->converged = 0), NRS_ERR_NAN on a NaN residual. */
+ * pcg (linearSolver/PCG.cpp:85-203) or PGMRES.  With opts.initialGuess =
+ * PROJECTION or PROJECTION_ACONJ the residual's projection on the previous
+ * solutions is removed before the iteration and added back after it, and the
+ * new solution extends the basis (SolutionProjection::pre/post,
+ * ellipticSolve.cpp:121-174); every call counts as one time step.  Returns
+ * NRS_OK also when maxIter is hit (stats->converged = 0), NRS_ERR_NAN on a NaN
+ * residual. */
 int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *stats,
                        void *stream);
 

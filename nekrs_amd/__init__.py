@@ -221,6 +221,7 @@ def default_opts():
 
 
 COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2, "amg": 3}
+INITIAL_GUESS = {"previous": 0, "projection": 1, "projection-aconj": 2}
 
 
 def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1):
@@ -232,8 +233,11 @@ def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1):
 
 def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=500, levels=None,
               smoother="chebyshev+jacobi", cheby_degree=2, cheby_min=0.1, cheby_max=1.1,
-              coarse_tol=1e-2, coarse_max_iter=200, seed=0, coarse="pcg", **extra):
-    """nrs_elliptic_opts from keyword options (extra: raw field names)."""
+              coarse_tol=1e-2, coarse_max_iter=200, seed=0, coarse="pcg",
+              initial_guess="previous", proj_vectors=8, proj_start=5, **extra):
+    """nrs_elliptic_opts from keyword options (extra: raw field names).
+    initial_guess: "previous", "projection" or "projection-aconj" (INITIAL GUESS);
+    proj_vectors / proj_start: RESIDUAL PROJECTION VECTORS / START."""
     o = default_opts()
     o.solver = SOLVERS[solver]
     o.preconditioner = PRECONS[precon]
@@ -252,6 +256,9 @@ def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=5
     o.coarseMaxIter = coarse_max_iter
     o.coarseSolver = COARSE[coarse]
     o.seed = seed
+    o.initialGuess = INITIAL_GUESS[initial_guess]
+    o.projVectors = proj_vectors
+    o.projStart = proj_start
     for k, v in extra.items():
         setattr(o, k, v)
     return o
@@ -434,4 +441,5 @@ class Elliptic:
         st = _lib.SolveStats()
         _lib.call("nrs_elliptic_solve", self._h, _ptr(rhs), _ptr(x), C.byref(st), _stream(stream))
         return {"iterations": st.iterations, "converged": bool(st.converged), "res0": st.res0,
-                "res": st.res, "seconds": st.seconds, "precon_seconds": st.precon_seconds}
+                "res": st.res, "seconds": st.seconds, "precon_seconds": st.precon_seconds,
+                "res00": st.res00, "proj_vectors": st.projVectors}

@@ -37,12 +37,14 @@ class EllipticOpts(C.Structure):
                 ("chebyMinFactor", C.c_double), ("chebyMaxFactor", C.c_double),
                 ("coarseSolver", C.c_int), ("coarseTol", C.c_double), ("coarseMaxIter", C.c_int),
                 ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong),
-                ("useGraphs", C.c_int), ("autotune", C.c_int), ("gmresRestart", C.c_int)]
+                ("useGraphs", C.c_int), ("autotune", C.c_int), ("gmresRestart", C.c_int),
+                ("initialGuess", C.c_int), ("projVectors", C.c_int), ("projStart", C.c_int)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("res0", C.c_double),
-                ("res", C.c_double), ("seconds", C.c_double), ("precon_seconds", C.c_double)]
+                ("res", C.c_double), ("seconds", C.c_double), ("precon_seconds", C.c_double),
+                ("res00", C.c_double), ("projVectors", C.c_int)]
 
 
 # exported symbols and their signatures (all return int status unless noted)

@@ -157,6 +157,8 @@ nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic
   b->itersPerLaunch = itersPerLaunch;
   for (int p = 0; p < n; ++p) {
     NRS_CHECK(descs[p].Nfields <= 1, "batch: one field per problem (a block solve is not device-driven)");
+    NRS_CHECK(opts[p].initialGuess == NRS_GUESS_PREVIOUS,
+              "batch: the projection initial guess needs host readbacks (not device-driven)");
     NRS_CHECK(opts[p].solver == NRS_SOLVER_PCG || opts[p].solver == NRS_SOLVER_FCG,
               "batch: PCG or flexible PCG (PGMRES needs the host for its Hessenberg system)");
     b->probs.push_back(elliptic_setup(&descs[p], &opts[p]));
@@ -266,6 +268,8 @@ int batch_solve(nrs_batch *b, double *const *rhs, double *const *x, nrs_solve_st
       st[p].converged = (h[S_RDOTR] <= h[S_TOL]) ? 1 : 0;
       st[p].seconds = secs;
       st[p].precon_seconds = 0.0;
+      st[p].res00 = h[S_TMP3];
+      st[p].projVectors = 0;
     }
   }
   b->lastLaunches = launches;

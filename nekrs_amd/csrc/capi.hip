@@ -269,6 +269,9 @@ void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
   o->useGraphs = 1;
   o->autotune = 1;
   o->gmresRestart = 15;
+  o->initialGuess = NRS_GUESS_PREVIOUS;
+  o->projVectors = 8;
+  o->projStart = 5;
 }
 
 int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts, nrs_elliptic **out)

@@ -98,6 +98,11 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
             "PGMRES restart must be 1..32");
   NRS_CHECK(o->preconditioner >= NRS_PRECON_NONE && o->preconditioner <= NRS_PRECON_MULTIGRID,
             "unknown preconditioner");
+  NRS_CHECK(o->initialGuess >= NRS_GUESS_PREVIOUS && o->initialGuess <= NRS_GUESS_PROJECTION_ACONJ,
+            "unknown initial guess");
+  NRS_CHECK(o->initialGuess == NRS_GUESS_PREVIOUS ||
+                (o->projVectors >= 1 && o->projVectors <= kGmresMaxVectors),
+            "residual projection vectors must be 1..32");
   NRS_CHECK(o->maxIter >= 1, "maxIter must be >= 1");
   // ellipticSetup.cpp:59-73: multigrid with non-Jacobi smoothers and its coarse solve
   // are Poisson-only in the reference; kept as in the reference.
@@ -178,8 +183,19 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     el->gV.alloc((size_t)m * Lt);
     el->gZ.alloc((size_t)(o->solver == NRS_SOLVER_FGMRES ? m : 1) * Lt);
     el->gB.alloc(Lt);
+  }
+  if (o->solver == NRS_SOLVER_GMRES || o->solver == NRS_SOLVER_FGMRES ||
+      o->initialGuess != NRS_GUESS_PREVIOUS) {
     el->gY.alloc(kGmresMaxVectors + 2);
     el->gPart.alloc(kGmresPartials);
+  }
+  if (o->initialGuess != NRS_GUESS_PREVIOUS) {
+    // SolutionProjection constructor (ellipticSolutionProjection.cpp:237-267)
+    const int mv = o->projVectors;
+    el->pjXX.alloc((size_t)mv * Lt);
+    el->pjBB.alloc((size_t)(o->initialGuess == NRS_GUESS_PROJECTION ? mv : 1) * Lt);
+    el->pjXbar.alloc(Lt);
+    el->pjAlpha.alloc(mv);
   }
   if (el->Nf > 1) {
     el->wRep.alloc(Lt);
@@ -478,30 +494,48 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   // x0 = x, x = 0   (:105-106)
   NRS_HIP(hipMemcpyAsync(el->x0.get(), x, L * sizeof(double), hipMemcpyDeviceToDevice, s));
   fill<double>(L, 0.0, x, s);
+  // projection of r on the previous solutions (:107-128); res00 before it
+  const bool project = el->opts.initialGuess != NRS_GUESS_PREVIOUS;
+  bool projected = false;
+  double res00 = 0.0;
+  if (project) {
+    wnorm(R, L, w, r, resNormFactor, scal, S_TMP0, s);
+    projected = projection_pre(el, r, s);
+  }
   // res0 = ||r||_w / sqrt(volume)   (:130-139)
   wnorm(R, L, w, r, resNormFactor, scal, S_RDOTR, s);
   NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
   NRS_HIP(hipStreamSynchronize(s));
   const double res0 = el->scalHost[S_RDOTR];
-  if (std::isnan(res0)) return NRS_ERR_NAN;
+  res00 = project ? el->scalHost[S_TMP0] : res0;
+  if (std::isnan(res0) || std::isnan(res00)) return NRS_ERR_NAN;
   double tol = o.tol;
   if (o.relative) tol *= res0;
+  // after the iteration: x += xbar and the basis update (:171-176), then x += x0
+  auto finish = [&](int status) {
+    if (project && status == NRS_OK) projection_post(el, x, s);
+    axpby<double>(L, 1.0, el->x0.get(), 1.0, x, s);
+  };
+  auto fill_stats = [&](int iter, double rdotr, double precon_s) {
+    if (!st) return;
+    st->iterations = iter;
+    st->converged = (rdotr <= tol) ? 1 : 0;
+    st->res0 = res0;
+    st->res = rdotr;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->precon_seconds = precon_s;
+    st->res00 = res00;
+    st->projVectors = projected ? el->pjPrev : 0;
+  };
 
   if (o.solver == NRS_SOLVER_GMRES || o.solver == NRS_SOLVER_FGMRES) {
     int iter = 0, status = NRS_OK;
     double rdotr = res0, precon_s = 0.0;
     gmres_solve(el, r, x, res0, tol, st != nullptr, iter, rdotr, precon_s, status, s);
-    axpby<double>(L, 1.0, el->x0.get(), 1.0, x, s);
+    finish(status);
     NRS_HIP(hipEventRecord(el->ev1, s));
     NRS_HIP(hipStreamSynchronize(s));
-    if (st) {
-      st->iterations = iter;
-      st->converged = (rdotr <= tol) ? 1 : 0;
-      st->res0 = res0;
-      st->res = rdotr;
-      st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      st->precon_seconds = precon_s;
-    }
+    fill_stats(iter, rdotr, precon_s);
     return status;
   }
 
@@ -536,18 +570,11 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
     if (el->scalHost[S_ACTIVE] == 0.0) break;     // rdotr <= tol or iter == maxIter (PCG.cpp:200)
   }
 
-  // x += x0   (:178-186)
-  axpby<double>(L, 1.0, el->x0.get(), 1.0, x, s);
+  // x += xbar, basis update, x += x0   (:171-186)
+  finish(status);
   NRS_HIP(hipEventRecord(el->ev1, s));
   NRS_HIP(hipStreamSynchronize(s));
-  if (st) {
-    st->iterations = iter;
-    st->converged = (rdotr <= tol) ? 1 : 0;
-    st->res0 = res0;
-    st->res = rdotr;
-    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    st->precon_seconds = precon_s;
-  }
+  fill_stats(iter, rdotr, precon_s);
   return status;
 }
 
@@ -561,6 +588,8 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
 void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t s)
 {
   NRS_CHECK(el->Nf == 1, "device-driven solve: one field");
+  NRS_CHECK(el->opts.initialGuess == NRS_GUESS_PREVIOUS,
+            "device-driven solve: the projection initial guess needs host readbacks");
   const int64_t L = el->Nlocal;
   NRS_CHECK(!(el->mg && !el->mg->device_only()),
             "device-driven solve: the multigrid coarse level must be smoothed, not solved");

@@ -33,6 +33,15 @@ struct nrs_elliptic {
   nrs::DevBuf<double> wRep;          // invDegree repeated per field (Nf > 1)
   nrs::DevBuf<double> gV, gZ, gB, gY, gPart;   // PGMRES basis, preconditioned basis, rhs, y, partials
 
+  // Solution projection (SolutionProjection, ellipticSolutionProjection.h):
+  // basis xx [projVectors][Nf*Nlocal], A xx ([projVectors] for CLASSIC, one
+  // vector for ACONJ), xbar = the projected guess, alpha [projVectors].
+  nrs::DevBuf<double> pjXX, pjBB, pjXbar, pjAlpha;
+  int pjM = 0;                       // numVecsProjection
+  int pjStep = 0;                    // solves so far (timestep)
+  int pjPrev = 0;                    // basis size used by the last pre()
+  std::vector<double> pjAlphaHost;
+
   // workspace (ellipticUpdateWorkspace, elliptic.h:240-262)
   nrs::DevBuf<double> p, zv, Ap, x0;
   nrs::DevBuf<double> scal;          // nrs::S_COUNT device scalars
@@ -72,6 +81,11 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
 void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t s);
 void elliptic_iterate(nrs_elliptic *el, double *r, double *x, hipStream_t s);
 void elliptic_solve_end(nrs_elliptic *el, double *x, hipStream_t s);
+// SolutionProjection::pre / post (ellipticSolutionProjection.cpp:269-288):
+// pre removes the projection of the assembled residual r (and returns whether
+// it ran); post adds it back to the solution correction x and extends the basis.
+bool projection_pre(nrs_elliptic *el, double *r, hipStream_t s);
+void projection_post(nrs_elliptic *el, double *x, hipStream_t s);
 
 nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic_opts *opts,
                        int itersPerLaunch);

@@ -363,6 +363,13 @@ def fcg_solve(el, rhs, precon, flexible=True, tol=1e-8, maxit=500, x0=None):
     r = rhs - Ax0
     r[el.maskIds] = 0
     r = el.ogs.gs(r, "add")
+    xs, st = pcg(el, r, precon, flexible, tol, maxit)
+    return xs + x, st
+
+
+def pcg(el, r, precon, flexible=True, tol=1e-8, maxit=500):
+    """pcg (PCG.cpp:85-203) from x = 0 on the assembled, masked residual r;
+    tol relative to its 1/volume-scaled weighted norm."""
     w = el.invDegree
     vol = el.volume
     res0 = np.sqrt(np.sum(w * r * r) / vol)
@@ -393,7 +400,7 @@ def fcg_solve(el, rhs, precon, flexible=True, tol=1e-8, maxit=500, x0=None):
         hist.append(rdotr)
         if not (rdotr > tol_abs and it < maxit):
             break
-    return xs + x, {"iterations": it, "res0": res0, "res": rdotr, "history": np.array(hist)}
+    return xs, {"iterations": it, "res0": res0, "res": rdotr, "history": np.array(hist)}
 
 
 def gmres_solve(el, rhs, precon, flexible=True, restart=15, tol=1e-8, maxit=500, nfields=1):
@@ -401,13 +408,18 @@ def gmres_solve(el, rhs, precon, flexible=True, restart=15, tol=1e-8, maxit=500,
     Gram-Schmidt with invDegree-weighted inner products, Givens rotations,
     restart; tol relative to the 1/volume-scaled initial norm.  nfields > 1:
     the same operator on each field, inner products summed over the fields."""
-    m = el.mesh
-    L = m.Nlocal
-    A = lambda v: np.concatenate([el.operator(v[f * L:(f + 1) * L]) for f in range(nfields)])
-    M = lambda v: np.concatenate([precon(v[f * L:(f + 1) * L]) for f in range(nfields)])
+    L = el.mesh.Nlocal
     mask = np.isin(np.arange(L), el.maskIds)
     r = np.concatenate([el.ogs.gs(np.where(mask, 0.0, rhs[f * L:(f + 1) * L]), "add")
                         for f in range(nfields)])
+    return gmres(el, r, precon, flexible, restart, tol, maxit, nfields)
+
+
+def gmres(el, r, precon, flexible=True, restart=15, tol=1e-8, maxit=500, nfields=1):
+    """pgmres from x = 0 on the assembled, masked residual r (all fields)."""
+    L = el.mesh.Nlocal
+    A = lambda v: np.concatenate([el.operator(v[f * L:(f + 1) * L]) for f in range(nfields)])
+    M = lambda v: np.concatenate([precon(v[f * L:(f + 1) * L]) for f in range(nfields)])
     w = np.tile(el.invDegree, nfields)
     rnf = 1.0 / el.volume
     dot = lambda a, b: float(np.sum(w * a * b))
@@ -418,6 +430,8 @@ def gmres_solve(el, rhs, precon, flexible=True, restart=15, tol=1e-8, maxit=500,
     nr = res0 / np.sqrt(rnf)
     err = res0
     it = 0
+    if not nr > 0:      # zero residual: x = 0 (the library's early return; the reference divides by 0)
+        return x, {"iterations": 0, "res0": res0, "res": res0}
 
     def update(k, H, s, V, Z):
         y = np.zeros(k)
