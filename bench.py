@@ -46,6 +46,40 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+class Ranges:
+    """roctx ranges around each timed solve, so that a `rocprofv3 --marker-trace`
+    run can attribute kernels to them; no-ops when the roctx library is absent."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+                break
+            except (OSError, AttributeError):
+                self.lib = None
+
+    def push(self, name):
+        if self.lib is not None:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib is not None:
+            self.lib.roctxRangePop()
+
+
+RANGES = None
+
+
+def ranges():
+    global RANGES
+    if RANGES is None:
+        RANGES = Ranges()
+    return RANGES
+
+
 def traffic_from_profiles():
     """HBM bytes per Ax launch from the committed rocprofv3 --pmc summary."""
     p = os.path.join(ROOT, "profiles", "ax_f64_n7_e32768_traffic.json")
@@ -115,7 +149,11 @@ def solve_cases(n, N, seed=0):
             r = rhs.clone()
             x = torch.zeros_like(r)
             torch.cuda.synchronize()
+            if rep == 1:
+                ranges().push("solve:" + name)
             st = el.solve(r, x)
+            if rep == 1:
+                ranges().pop()
         # true residual of the returned solution, in the solver's norm
         b = rhs.clone()
         mk = el.mask_ids.long()

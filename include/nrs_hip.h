@@ -391,6 +391,45 @@ int nrs_elliptic_profile_enable(nrs_elliptic *el, int on);
 int nrs_elliptic_profile_read(nrs_elliptic *el, double ms[NRS_PHASE_COUNT],
                               int64_t launches[NRS_PHASE_COUNT]);
 
+/* ------------------------------------------------------------------------- */
+/* Pressure right-hand side of the TOMBO splitting (the step before the solve) */
+/* ------------------------------------------------------------------------- */
+typedef struct nrs_tombo nrs_tombo;
+
+/* Geometry for tombo::pressureSolve's right-hand side: volume factors
+ * (geometricFactorsHex3D.okl:27-143: rx..tz, JW), face normals and WSJ
+ * (surfaceGeometricFactorsHex3D.okl) of the faces that carry a flux, the
+ * unmasked gather-scatter (nrs->gsh) and invLMM = 1/gs(JW)
+ * (meshComputeInvLMM.cpp).  x, y, z, globalIds: host [E*Np] as in
+ * nrs_elliptic_desc; bcType: host [E*6] velocity boundary type of each face
+ * with the ids of src/bdry/bcType.h (1 W, 2 INT, 3 V, 4-7 SYM*, 16 O, ...;
+ * 0 = interior face).  Replaces the mesh and boundary setup those kernels read
+ * (meshSetup.cpp, meshSurfaceGeometricFactorsHex3D.cpp, bcMap). */
+int nrs_tombo_setup(int N, int Nelements, const double *x, const double *y, const double *z,
+                    const long long *globalIds, const int *bcType, nrs_tombo **out);
+int nrs_tombo_free(nrs_tombo *h);
+
+/* rhs = the pressure Poisson right-hand side of tombo::pressureSolve
+ * (src/navierStokes/tombo.cpp:11-121, VELOCITY STRESSFORMULATION off):
+ *   W = gs(JW curl Ue) / gs(JW);  rhsU = BF - mue irho (JW curl W - 4/3 JW grad div)
+ *   (rhsU = BF when viscContribution = 0, i.e. nBDF = 1);  rhsU = gs(rhsU) / gs(JW);
+ *   rhs = wdiv(rhsU) + g0idt JW div - WSJ n.(g0idt U) on V/INT faces - WSJ n.rhsU on SYM faces.
+ * Ue, U, BF: device [3][fieldOffset]; mue, irho, div: device [E*Np]; rhs: device
+ * [E*Np] (the input of nrs_elliptic_solve's rhs).  FP64. */
+int nrs_tombo_pressure_rhs(nrs_tombo *h, int64_t fieldOffset, const double *Ue, const double *U,
+                           const double *BF, const double *mue, const double *irho,
+                           const double *div, double g0idt, int viscContribution, double *rhs,
+                           void *stream);
+
+/* W = JW curl(U) on [3][fieldOffset] device fields (curlHex3D.okl, scaleJW = 1). */
+int nrs_tombo_curl(nrs_tombo *h, int64_t fieldOffset, const double *U, double *W, void *stream);
+
+/* Device buffers owned by the handle: 0 vgeo (f64 [E][10][Np]: rx ry rz sx sy
+ * sz tx ty tz JW), 1 invLMM (f64 [E*Np]). */
+int nrs_tombo_buffer(nrs_tombo *h, int which, void **ptr, int64_t *count);
+/* Copy `count` entries of buffer `which` into the caller's device array dst. */
+int nrs_tombo_copy_buffer(nrs_tombo *h, int which, double *dst, int64_t count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,8 @@ from ._lib import NrsError
 
 __all__ = ["NrsError", "init", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
            "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh", "fdm_1d",
-           "fdm_element_lengths", "Batch", "make_desc", "make_opts", "tuned_variant", "set_variant"]
+           "fdm_element_lengths", "Batch", "make_desc", "make_opts", "tuned_variant", "set_variant",
+           "PressureRhs", "BC"]
 
 OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
 MAPS = {"none": 0, "sine": 1, "kershaw": 2}
@@ -191,6 +192,76 @@ class Ogs:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         _lib.call("nrs_ogs_info", self._h, C.byref(a), C.byref(b), C.byref(c))
         return {"rows": a.value, "members": b.value, "singletons": c.value}
+
+
+# velocity boundary types of src/bdry/bcType.h (0: interior face)
+BC = {"interior": 0, "wall": 1, "interface": 2, "inlet": 3, "symx": 4, "symy": 5, "symz": 6,
+      "sym": 7, "outflow": 16}
+
+
+class PressureRhs:
+    """The pressure right-hand side of the TOMBO splitting (tombo::pressureSolve,
+    src/navierStokes/tombo.cpp:11-121) on a mesh; bc_types: [E*6] face types
+    (BC ids; default: every boundary face of a box mesh an inlet, 0 inside)."""
+
+    def __init__(self, mesh, bc_types=None):
+        self.mesh = mesh
+        self.N, self.E = mesh.N, mesh.E
+        self.Nlocal = mesh.E * (mesh.N + 1) ** 3
+        if bc_types is None:
+            bc_types = np.where(np.asarray(mesh.EToB) != 0, BC["inlet"], 0)
+        self.bc_types = np.ascontiguousarray(bc_types, dtype=np.int32).reshape(-1)
+        assert self.bc_types.size == 6 * mesh.E
+        h = C.c_void_p()
+        _lib.call("nrs_tombo_setup", mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
+                  mesh.z.ctypes.data, mesh.gid.ctypes.data, self.bc_types.ctypes.data, C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().nrs_tombo_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _copy(self, which):
+        torch = _torch()
+        p, n = C.c_void_p(), C.c_int64()
+        _lib.call("nrs_tombo_buffer", self._h, which, C.byref(p), C.byref(n))
+        out = torch.empty(n.value, dtype=torch.float64, device="cuda")
+        _lib.call("nrs_tombo_copy_buffer", self._h, which, _ptr(out), n.value, _stream())
+        return out
+
+    @property
+    def vgeo(self):
+        """[E][10][Np]: rx ry rz sx sy sz tx ty tz JW (a device copy)."""
+        return self._copy(0)
+
+    @property
+    def inv_lmm(self):
+        return self._copy(1)
+
+    def curl(self, U, W=None, field_offset=None, stream=None):
+        torch = _torch()
+        off = self.Nlocal if field_offset is None else field_offset
+        if W is None:
+            W = torch.empty_like(U)
+        _lib.call("nrs_tombo_curl", self._h, off, _ptr(U), _ptr(W), _stream(stream))
+        return W
+
+    def compute(self, Ue, U, BF, mue, irho, div, g0idt, visc=True, rhs=None, field_offset=None,
+                stream=None):
+        torch = _torch()
+        off = self.Nlocal if field_offset is None else field_offset
+        if rhs is None:
+            rhs = torch.empty(self.Nlocal, dtype=torch.float64, device="cuda")
+        _lib.call("nrs_tombo_pressure_rhs", self._h, off, _ptr(Ue) if Ue is not None else None,
+                  _ptr(U), _ptr(BF), _ptr(mue) if mue is not None else None,
+                  _ptr(irho) if irho is not None else None, _ptr(div), float(g0idt), int(visc),
+                  _ptr(rhs), _stream(stream))
+        return rhs
 
 
 def fdm_1d(N, lbc, rbc, ll, lm, lr):

@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "mg.hpp"
+#include "tombo.hpp"
 
 namespace nrs {
 static thread_local std::string g_last_error;
@@ -533,3 +534,70 @@ int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats
 }
 
 }  // extern "C"
+
+int nrs_tombo_setup(int N, int Nelements, const double *x, const double *y, const double *z,
+                    const long long *globalIds, const int *bcType, nrs_tombo **out)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(out != nullptr, "tombo setup: null output handle");
+  *out = nullptr;
+  *out = nrs::tombo_setup(N, Nelements, x, y, z, globalIds, bcType);
+  NRS_API_END
+}
+
+int nrs_tombo_free(nrs_tombo *h)
+{
+  NRS_API_BEGIN
+  delete h;
+  NRS_API_END
+}
+
+int nrs_tombo_pressure_rhs(nrs_tombo *h, int64_t fieldOffset, const double *Ue, const double *U,
+                           const double *BF, const double *mue, const double *irho,
+                           const double *div, double g0idt, int viscContribution, double *rhs,
+                           void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(h, "tombo: null handle");
+  NRS_CHECK(fieldOffset >= h->Nlocal, "tombo: fieldOffset smaller than E*Np");
+  NRS_CHECK(U && BF && div && rhs, "tombo: null array");
+  NRS_CHECK(!viscContribution || (Ue && mue && irho), "tombo: null array for the viscous term");
+  nrs::tombo_pressure_rhs(h, fieldOffset, Ue, U, BF, mue, irho, div, g0idt, viscContribution != 0,
+                          rhs, as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_tombo_curl(nrs_tombo *h, int64_t fieldOffset, const double *U, double *W, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(h && U && W, "tombo curl: null argument");
+  NRS_CHECK(fieldOffset >= h->Nlocal, "tombo curl: fieldOffset smaller than E*Np");
+  NRS_CHECK(U != W, "tombo curl: in place is not supported");
+  nrs::tombo_curl(h, fieldOffset, U, W, as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_tombo_buffer(nrs_tombo *h, int which, void **ptr, int64_t *count)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(h && ptr && count, "tombo buffer: null argument");
+  switch (which) {
+    case 0: *ptr = h->vgeo.get(); *count = (int64_t)h->vgeo.n; break;
+    case 1: *ptr = h->invLMM.get(); *count = (int64_t)h->invLMM.n; break;
+    default: throw nrs::Error(NRS_ERR_INVALID, "tombo buffer: which must be 0 or 1");
+  }
+  NRS_API_END
+}
+
+int nrs_tombo_copy_buffer(nrs_tombo *h, int which, double *dst, int64_t count, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(h && dst, "tombo copy buffer: null argument");
+  const nrs::DevBuf<double> *b = which == 0 ? &h->vgeo : which == 1 ? &h->invLMM : nullptr;
+  NRS_CHECK(b != nullptr, "tombo copy buffer: which must be 0 or 1");
+  NRS_CHECK(count >= 0 && (size_t)count <= b->n, "tombo copy buffer: count exceeds the buffer");
+  NRS_HIP(hipMemcpyAsync(dst, b->get(), count * sizeof(double), hipMemcpyDeviceToDevice,
+                         as_stream(stream)));
+  NRS_HIP(hipStreamSynchronize(as_stream(stream)));
+  NRS_API_END
+}

@@ -1,0 +1,119 @@
+"""GPU parity of the pressure right-hand side of the TOMBO splitting
+(nrs_tombo_*, tombo::pressureSolve) against oracle/tombo.py.  Bars: geometry
+and curl 1e-12, the full right-hand side 1e-11 relative to its largest entry
+(FP64, different summation order), closed forms on an affine mesh to 1e-10."""
+import types
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+A = np.array([[1.3, 0.2, -0.1], [0.1, 0.9, 0.25], [-0.2, 0.15, 1.1]])
+B = np.array([0.3, -0.2, 0.5])
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+def meshes(gpu, orc, shape, N, mp="kershaw", eps=0.3, affine=False):
+    mg = gpu.BoxMesh(*shape, N, mp, eps)
+    mo = orc.BoxMesh(*shape, N, mp, eps)
+    x, y, z = mo.x, mo.y, mo.z
+    if affine:
+        X = A @ np.stack([x, y, z]) + B[:, None]
+        x, y, z = X[0].copy(), X[1].copy(), X[2].copy()
+    m = types.SimpleNamespace(N=N, E=mo.E, x=x, y=y, z=z, gid=mg.gid, EToB=mg.EToB)
+    return m, mo
+
+
+def bc_mix(m, seed):
+    """Boundary faces: inflow, wall, symmetry, outflow at random; 0 inside."""
+    rng = np.random.default_rng(seed)
+    kinds = np.array([3, 1, 7, 4, 16, 2])
+    return np.where(m.EToB != 0, kinds[rng.integers(0, len(kinds), m.EToB.size)], 0).astype(np.int32)
+
+
+@pytest.mark.parametrize("N", [1, 3, 7])
+def test_geometry_and_inv_lmm(gpu, torch_mod, orc, N):
+    import oracle.tombo as ot
+    m, mo = meshes(gpu, orc, (3, 2, 2), N)
+    t = gpu.PressureRhs(m, bc_mix(m, 0))
+    g = ot.Geometry(N, m.x, m.y, m.z, m.gid)
+    assert rel(t.vgeo.cpu().numpy().reshape(g.E, 10, -1), g.vgeo()) <= 1e-12
+    assert rel(t.inv_lmm.cpu().numpy(), g.invLMM) <= 1e-13
+
+
+@pytest.mark.parametrize("N", [2, 5, 7, 9])
+def test_curl_vs_oracle(gpu, torch_mod, orc, N):
+    import oracle.tombo as ot
+    torch = torch_mod
+    m, mo = meshes(gpu, orc, (3, 3, 2), N)
+    t = gpu.PressureRhs(m)
+    g = ot.Geometry(N, m.x, m.y, m.z, m.gid)
+    L = m.x.size
+    off = L + 37                                     # a padded field offset
+    U = np.random.default_rng(N).uniform(-1, 1, 3 * off)
+    W = t.curl(torch.from_numpy(U).cuda(), field_offset=off).cpu().numpy()
+    ref = g.curl(U, off)
+    got = np.concatenate([W[c * off:c * off + L] for c in range(3)])
+    assert rel(got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("N,visc,seed", [(7, True, 0), (7, False, 1), (4, True, 2), (1, True, 3)])
+def test_pressure_rhs_vs_oracle(gpu, torch_mod, orc, N, visc, seed):
+    import oracle.tombo as ot
+    torch = torch_mod
+    m, mo = meshes(gpu, orc, (4, 3, 3), N)
+    bc = bc_mix(m, seed)
+    t = gpu.PressureRhs(m, bc)
+    g = ot.Geometry(N, m.x, m.y, m.z, m.gid)
+    L = m.x.size
+    off = L + 64
+    rng = np.random.default_rng(seed)
+    Ue, U, BF = (rng.uniform(-1, 1, 3 * off) for _ in range(3))
+    mue = rng.uniform(0.5, 1.5, L)
+    irho = rng.uniform(0.5, 1.5, L)
+    div = rng.uniform(-0.1, 0.1, L)
+    g0idt = 1.5 / 0.01
+    ref = g.pressure_rhs(bc, Ue, U, BF, mue, irho, div, g0idt, visc, off)
+    dv = lambda a: torch.from_numpy(a).cuda()
+    got = t.compute(dv(Ue), dv(U), dv(BF), dv(mue), dv(irho), dv(div), g0idt, visc,
+                    field_offset=off).cpu().numpy()
+    print("N", N, "visc", visc, "flux elements", int(np.sum(np.any(np.isin(bc.reshape(-1, 6), [2, 3, 4, 7]), 1))),
+          "rel", rel(got, ref), flush=True)
+    assert rel(got, ref) <= 1e-11
+
+
+def test_pressure_rhs_closed_form_affine(gpu, torch_mod, orc):
+    """Walls everywhere: sum_i x_i rhs_i = sum JW (x + nu (2 + 4/3 a_x)) + g0idt sum x JW div
+    for Ue = (y^2, z^2, x^2), BF = JW (x, 1, z), div = c + a.x (the oracle's closed form)."""
+    import oracle.tombo as ot
+    torch = torch_mod
+    N = 4
+    m, mo = meshes(gpu, orc, (3, 2, 2), N, mp="none", eps=0.0, affine=True)
+    bc = np.where(m.EToB != 0, 1, 0).astype(np.int32)
+    t = gpu.PressureRhs(m, bc)
+    x, y, z = m.x, m.y, m.z
+    L = x.size
+    JW = ot.Geometry(N, x, y, z, m.gid).JW.ravel()
+    nu = 0.7 * 1.5
+    div = 0.3 + 0.5 * x - 0.2 * y
+    dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    rhs = t.compute(dv(np.concatenate([y * y, z * z, x * x])), dv(np.zeros(3 * L)),
+                    dv(np.concatenate([JW * x, JW, JW * z])), dv(np.full(L, 0.7)),
+                    dv(np.full(L, 1.5)), dv(div), 4.0, True).cpu().numpy()
+    expect = np.sum(JW * (x + nu * (2 + 4.0 / 3.0 * 0.5))) + 4.0 * np.sum(x * JW * div)
+    assert abs(np.sum(x * rhs) - expect) <= 1e-10 * abs(expect)
+
+
+def test_pressure_rhs_rejects_short_offset(gpu, torch_mod, orc):
+    torch = torch_mod
+    m, mo = meshes(gpu, orc, (2, 2, 2), 3)
+    t = gpu.PressureRhs(m)
+    L = m.x.size
+    a = torch.zeros(3 * L, dtype=torch.float64, device="cuda")
+    s = torch.zeros(L, dtype=torch.float64, device="cuda")
+    with pytest.raises(gpu.NrsError):
+        t.compute(a, a, a, s, s, s, 1.0, True, field_offset=L - 1)
