@@ -24,6 +24,9 @@ times 100 steps after 10 warm-up steps, inputs resident in HBM.
              smoothing (degree 4), the FDM contractions in BF16 on the matrix cores
              (FP32 accumulation).  Each case is set up once, solved once untimed,
              then timed over one solve from x = 0 (setup excluded).
+  pressure_rhs  the step before the solve: tombo::pressureSolve's right-hand
+             side on the same 32^3 mesh (channel boundary faces), ms per call and
+             algorithmic GB/s
   batch      64 uneven Helmholtz problems advanced by captured graphs, and the
              3-component velocity Helmholtz solve (batch of 3, and one block solve)
 
@@ -150,7 +153,7 @@ def solve_cases(n, N, seed=0):
             x = torch.zeros_like(r)
             torch.cuda.synchronize()
             if rep == 1:
-                ranges().push("solve:" + name)
+                ranges().push("nrs:solve:" + name)
             st = el.solve(r, x)
             if rep == 1:
                 ranges().pop()
@@ -183,6 +186,72 @@ def solve_cases(n, N, seed=0):
                        "solve (2 AMG-preconditioned PCG iterations), FCG to 1e-8 relative, rhs "
                        "sine + 1e-3 noise (seed %d)" % (n, n ** 3, N, N, seed))
     return out
+
+
+def channel_bc(mesh):
+    """Face types of a channel on the box: inflow at x = 0 (face 4), outflow at
+    x = 1 (face 2), walls at y = 0, 1 (faces 1, 3), symmetry at z = 0, 1 (0, 5)."""
+    import numpy as np
+    kind = np.array([7, 1, 16, 1, 3, 7], dtype=np.int32)          # by face id
+    e2b = np.asarray(mesh.EToB).reshape(-1, 6)
+    return np.where(e2b != 0, kind[None, :], 0).astype(np.int32).ravel()
+
+
+def pressure_rhs_case(mesh, N, steps, warmup, seed=0):
+    """tombo::pressureSolve's right-hand side on the operator mesh: one call =
+    curl, gs, curl + gradient + combination, gs, weak divergence + mass term,
+    surface terms.  Algorithmic bytes (FP64): curl 128 B/node (Ue 24, rx..tz and
+    JW 80, W 24), fused rhsU 184 B/node (W 24, invLMM 8, div 8, geometry 80, BF
+    24, mue 8, irho 8, rhsU 24), weak divergence 128 B/node (rhsU 24, invLMM 8,
+    geometry 80, div 8, rhs 8); each 3-field gather-scatter 52 B per member of
+    a shared row (3 x 16 B values + 4 B index) + 4 B per row; the surface
+    kernel 16 B per node of each element with a flux face (rhs read and
+    written) + 32 B geometry and 24 B field (+8 B invLMM on symmetry faces) per
+    node of each flux face."""
+    import numpy as np
+    import torch
+    import nekrs_amd
+    t0 = time.perf_counter()
+    bc = channel_bc(mesh)
+    pr = nekrs_amd.PressureRhs(mesh, bc)
+    setup = time.perf_counter() - t0
+    L = pr.Nlocal
+    rng = np.random.default_rng(seed)
+    dv = lambda a: torch.from_numpy(a).cuda()
+    Ue, U, BF = (dv(rng.uniform(-1, 1, 3 * L)) for _ in range(3))
+    mue, irho = dv(rng.uniform(0.5, 1.5, L)), dv(rng.uniform(0.5, 1.5, L))
+    div = dv(rng.uniform(-0.1, 0.1, L))
+    rhs = torch.empty(L, dtype=torch.float64, device="cuda")
+    g0idt = 1.5 / 1e-3
+    for _ in range(warmup):
+        pr.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ranges().push("nrs:pressure_rhs")
+    a.record()
+    for _ in range(steps):
+        pr.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
+    b.record()
+    torch.cuda.synchronize()
+    ranges().pop()
+    ms = a.elapsed_time(b) / steps
+    info = nekrs_amd.Ogs(mesh.gid).info()
+    gs3 = 52.0 * info["members"] + 4.0 * (info["rows"] + 1)
+    Np, Nfp = (N + 1) ** 3, (N + 1) ** 2
+    e2b = bc.reshape(-1, 6)
+    flux = np.isin(e2b, [2, 3, 4, 5, 6, 7])
+    sym = np.isin(e2b, [4, 5, 6, 7])
+    surf = (16.0 * Np * np.sum(flux.any(1)) + 56.0 * Nfp * flux.sum() + 8.0 * Nfp * sym.sum())
+    alg = 440.0 * L + 2 * gs3 + surf
+    gbs = alg / (ms * 1e-3) / 1e9
+    log("pressure rhs (%d elements N=%d, channel): %.4f ms per call, %.2f GB algorithmic, "
+        "%.0f GB/s (%.1f %% of HBM peak)" % (mesh.E, N, ms, alg / 1e9, gbs, 100 * gbs / PEAK_HBM_GBS))
+    return {"ms_per_call": ms, "setup_seconds": setup, "algorithmic_bytes": alg,
+            "achieved_GBps": gbs, "roofline_frac": gbs / PEAK_HBM_GBS, "steps": steps,
+            "flux_elements": int(flux.any(1).sum()),
+            "workload": "tombo::pressureSolve right-hand side (viscous term on), %d hex elements "
+                        "N=%d Kershaw eps=0.3, channel faces (inflow, outflow, walls, symmetry), "
+                        "fields U(-1,1) seed %d" % (mesh.E, N, seed)}
 
 
 BATCH_KW = dict(solver="fcg", precon="multigrid", smoother="opt4chebyshev+asm", cheby_degree=3,
@@ -340,6 +409,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solve-n", type=int, default=48, help="elements per direction of the solves")
     ap.add_argument("--no-solves", action="store_true")
+    ap.add_argument("--no-pressure-rhs", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--batch-iters-per-launch", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
@@ -405,6 +475,9 @@ def main():
 
     Nlocal = el.Nlocal
     del el
+    prhs = None
+    if not args.no_pressure_rhs:
+        prhs = pressure_rhs_case(mesh, N, 20, 3)
     solves = None
     if not args.no_solves:
         log("p-MG solves at %d^3 ..." % args.solve_n)
@@ -446,6 +519,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles()},
         "cpu_baseline": cpu,
+        "pressure_rhs": prhs,
         "solves": solves,
         "batch": batch,
     }

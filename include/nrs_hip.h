@@ -430,6 +430,32 @@ int nrs_tombo_buffer(nrs_tombo *h, int which, void **ptr, int64_t *count);
 /* Copy `count` entries of buffer `which` into the caller's device array dst. */
 int nrs_tombo_copy_buffer(nrs_tombo *h, int which, double *dst, int64_t count, void *stream);
 
+/* ------------------------------------------------------------------------- */
+/* Mesh files: .re2 and the numbering of unstructured hexahedral meshes        */
+/* ------------------------------------------------------------------------- */
+/* Header of a .re2 file (re2::nelg, src/io/re2Reader.cpp:3-41, versions
+ * #v002/#v003/#v004): element counts, curved-side and boundary record counts. */
+int nrs_re2_info(const char *path, int *nelgt, int *nelgv, int *ncurve, int *nbc);
+/* Element corners and boundary codes (3rd_party/nek5000/core/reader_re2.f:
+ * readp_re2_mesh/buf_to_xyz :68-157, :430-471, readp_re2_bc/buf_to_bc
+ * :503-541).  vx, vy, vz: host [nelgt*8], corners in lexicographic order
+ * i + 2j + 4k (the file's cyclic order mapped as meshNekReader.cpp:30);
+ * group: host [nelgt] or NULL; cbc: host [nelgt*6*3] three-character codes of
+ * the first field ("W  ", "v  ", "O  ", "E  ", ...; blanks where the file has
+ * none) in the solver's face order (meshNekReader.cpp:31). Curved sides are
+ * not applied (nrs_re2_info reports their count). */
+int nrs_re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc);
+/* GLL node coordinates (trilinear in the corners) and 1-based global ids of
+ * an order-N conforming hexahedral mesh from its corners [E*8] (lexicographic):
+ * corners closer than tol are one vertex (tol <= 0: 1e-3 of the shortest
+ * edge; replaces nek's set_glo_num / parCon connectivity), then every node is
+ * numbered by its vertex, edge (ordered by its end vertices), face (ordered
+ * from its smallest vertex) or element interior.  Periodic faces are not
+ * connected.  x, y, z, globalIds: host [E*(N+1)^3]. */
+int nrs_mesh_from_vertices(int N, int Nelements, const double *vx, const double *vy,
+                           const double *vz, double tol, double *x, double *y, double *z,
+                           long long *globalIds);
+
 #ifdef __cplusplus
 }
 #endif

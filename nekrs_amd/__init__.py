@@ -22,7 +22,7 @@ from ._lib import NrsError
 __all__ = ["NrsError", "init", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
            "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh", "fdm_1d",
            "fdm_element_lengths", "Batch", "make_desc", "make_opts", "tuned_variant", "set_variant",
-           "PressureRhs", "BC"]
+           "PressureRhs", "BC", "read_re2", "mesh_from_vertices", "Re2Mesh"]
 
 OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
 MAPS = {"none": 0, "sine": 1, "kershaw": 2}
@@ -102,6 +102,65 @@ class BoxMesh:
         self.Nlocal = self.E * self.Np
         self.map, self.eps = map, eps
         self.x, self.y, self.z, self.gid, self.EToB = box_mesh(nx, ny, nz, N, map, eps)
+
+
+def read_re2(path):
+    """Corners [E][8] (lexicographic), groups [E] and boundary codes [E][6] (the
+    solver's face order) of a .re2 file, plus its header counts."""
+    path_b = str(path).encode()
+    nelgt, nelgv, ncurve, nbc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _lib.call("nrs_re2_info", path_b, C.byref(nelgt), C.byref(nelgv), C.byref(ncurve), C.byref(nbc))
+    E = nelgt.value
+    vx, vy, vz = (np.zeros(E * 8) for _ in range(3))
+    group = np.zeros(E, dtype=np.int32)
+    cbc = np.zeros(E * 6 * 3, dtype=np.uint8)
+    _lib.call("nrs_re2_read", path_b, vx.ctypes.data, vy.ctypes.data, vz.ctypes.data,
+              group.ctypes.data, cbc.ctypes.data)
+    codes = np.array([bytes(cbc[3 * n:3 * n + 3]).decode("ascii", "replace")
+                      for n in range(E * 6)]).reshape(E, 6)
+    return {"nelgt": E, "nelgv": nelgv.value, "ncurve": ncurve.value, "nbc": nbc.value,
+            "vx": vx, "vy": vy, "vz": vz, "group": group, "cbc": codes}
+
+
+def mesh_from_vertices(N, vx, vy, vz, tol=0.0):
+    """(x, y, z, globalIds) of the order-N GLL nodes of a conforming hex mesh
+    from its corners [E*8] (lexicographic), nrs_mesh_from_vertices."""
+    vx, vy, vz = (np.ascontiguousarray(a, dtype=np.float64) for a in (vx, vy, vz))
+    E = vx.size // 8
+    L = E * (N + 1) ** 3
+    x, y, z = np.zeros(L), np.zeros(L), np.zeros(L)
+    gid = np.zeros(L, dtype=np.int64)
+    _lib.call("nrs_mesh_from_vertices", N, E, vx.ctypes.data, vy.ctypes.data, vz.ctypes.data,
+              float(tol), x.ctypes.data, y.ctypes.data, z.ctypes.data, gid.ctypes.data)
+    return x, y, z, gid
+
+
+# .re2 boundary codes -> velocity boundary types (src/bdry/bcType.h)
+RE2_BC = {"W": 1, "v": 3, "V": 3, "O": 16, "o": 16, "ON": 15, "SYM": 7, "int": 2, "E": 0, "": 0}
+
+
+class Re2Mesh:
+    """A mesh read from a .re2 file at order N: coordinates (trilinear in the
+    corners, curved sides not applied), global ids from the corner topology,
+    EToB = 1 on faces whose code is in `dirichlet` (for the pressure solve the
+    outflow faces), and bc_types for the pressure right-hand side."""
+
+    def __init__(self, path, N, dirichlet=("O", "o", "ON"), scale=1.0):
+        d = read_re2(path)
+        self.N, self.Nq = N, N + 1
+        self.Np = self.Nq ** 3
+        self.E = d["nelgt"]
+        self.Nlocal = self.E * self.Np
+        self.cbc = d["cbc"]
+        self.ncurve = d["ncurve"]
+        self.x, self.y, self.z, self.gid = mesh_from_vertices(N, d["vx"] * scale, d["vy"] * scale,
+                                                              d["vz"] * scale)
+        codes = np.char.strip(self.cbc.astype(str))
+        self.EToB = np.isin(codes, list(dirichlet)).astype(np.int32).ravel()
+        unknown = sorted(set(codes.ravel()) - set(RE2_BC))
+        if unknown:
+            raise ValueError("unsupported boundary codes in %s: %s" % (path, unknown))
+        self.bc_types = np.vectorize(RE2_BC.get)(codes).astype(np.int32).ravel()
 
 
 def coarse_global_ids(Nf, E, ids_fine, Nc):

@@ -97,6 +97,9 @@ SIGNATURES = {
     "nrs_tombo_curl": ([_vp, _i64, _vp, _vp, _vp], _i),
     "nrs_tombo_buffer": ([_vp, _i, C.POINTER(_vp), C.POINTER(_i64)], _i),
     "nrs_tombo_copy_buffer": ([_vp, _i, _vp, _i64, _vp], _i),
+    "nrs_re2_info": ([C.c_char_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
+    "nrs_re2_read": ([C.c_char_p, _vp, _vp, _vp, _vp, _vp], _i),
+    "nrs_mesh_from_vertices": ([_i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp], _i),
 }
 PHASES = ("ax", "gs")
 

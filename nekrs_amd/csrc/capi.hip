@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "mg.hpp"
+#include "re2.hpp"
 #include "tombo.hpp"
 
 namespace nrs {
@@ -599,5 +600,34 @@ int nrs_tombo_copy_buffer(nrs_tombo *h, int which, double *dst, int64_t count, v
   NRS_HIP(hipMemcpyAsync(dst, b->get(), count * sizeof(double), hipMemcpyDeviceToDevice,
                          as_stream(stream)));
   NRS_HIP(hipStreamSynchronize(as_stream(stream)));
+  NRS_API_END
+}
+
+int nrs_re2_info(const char *path, int *nelgt, int *nelgv, int *ncurve, int *nbc)
+{
+  NRS_API_BEGIN
+  const nrs::Re2Header h = nrs::re2_header(path);
+  if (nelgt) *nelgt = h.nelgt;
+  if (nelgv) *nelgv = h.nelgv;
+  if (ncurve) *ncurve = h.ncurve;
+  if (nbc) *nbc = h.nbc;
+  NRS_API_END
+}
+
+int nrs_re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(vx && vy && vz, "re2 read: null corner arrays");
+  nrs::re2_read(path, vx, vy, vz, group, cbc);
+  NRS_API_END
+}
+
+int nrs_mesh_from_vertices(int N, int Nelements, const double *vx, const double *vy,
+                           const double *vz, double tol, double *x, double *y, double *z,
+                           long long *globalIds)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(vx && vy && vz && x && y && z && globalIds, "mesh from vertices: null array");
+  nrs::mesh_from_vertices(N, Nelements, vx, vy, vz, tol, x, y, z, globalIds);
   NRS_API_END
 }

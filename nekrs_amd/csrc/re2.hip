@@ -1,0 +1,224 @@
+// re2.hip -- the .re2 binary mesh format and the numbering of an unstructured
+// hexahedral mesh (host code).
+//
+// Format (src/io/re2Reader.cpp:3-41; 3rd_party/nek5000/core/reader_re2.f):
+//   80-byte ASCII header "#v002 nelgt ndim nelgv ..." (#v002/#v003/#v004),
+//   4-byte float 6.54321 (byte-order test), then in 8-byte words:
+//   nelgt element records  [group, x[8], y[8], z[8]]            (buf_to_xyz)
+//   ncurve, then ncurve    [element, face, curve[5], type]       (buf_to_curve)
+//   per field: nbc, then   [element, face, bc[5], "cbc"]         (buf_to_bc)
+//   vertices in the preprocessor's cyclic order (bottom face counter-clockwise,
+//   then top), faces 1..6 = s-, r+, s+, r-, t-, t+ (1-based element ids).
+// The numbering: vertices that coincide within a tolerance share an id
+// (nek's set_glo_num from coordinates), then every GLL node is numbered by the
+// vertex, edge, face or interior it belongs to (the same topology rule as the
+// multigrid levels' ids).
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <unordered_map>
+
+#include "mesh.hpp"
+#include "mg.hpp"
+#include "re2.hpp"
+
+namespace nrs {
+
+namespace {
+
+struct File {
+  FILE *f = nullptr;
+  explicit File(const char *p) : f(std::fopen(p, "rb")) {}
+  ~File() { if (f) std::fclose(f); }
+};
+
+void read_exact(FILE *f, void *dst, size_t n, const char *what)
+{
+  if (std::fread(dst, 1, n, f) != n) throw Error(NRS_ERR_INVALID, std::string("re2: truncated file (") + what + ")");
+}
+
+double swap8(double v)
+{
+  unsigned char b[8];
+  std::memcpy(b, &v, 8);
+  for (int i = 0; i < 4; ++i) std::swap(b[i], b[7 - i]);
+  std::memcpy(&v, b, 8);
+  return v;
+}
+
+// cyclic (file) vertex j -> lexicographic corner i + 2 j + 4 k (meshNekReader.cpp:30)
+constexpr int kVtxMap[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+// file face 1..6 -> solver face (0 t-, 1 s-, 2 r+, 3 s+, 4 r-, 5 t+) (meshNekReader.cpp:31)
+constexpr int kFaceMap[6] = {1, 2, 3, 4, 0, 5};
+
+}  // namespace
+
+Re2Header re2_header(const char *path)
+{
+  NRS_CHECK(path != nullptr, "re2: null path");
+  File fh(path);
+  NRS_CHECK(fh.f != nullptr, std::string("re2: cannot open ") + path);
+  char hdr[81] = {0};
+  read_exact(fh.f, hdr, 80, "header");
+  Re2Header h;
+  char ver[6] = {0};
+  std::memcpy(ver, hdr, 5);
+  int ndim = 0;
+  if (std::strcmp(ver, "#v004") == 0) {
+    NRS_CHECK(std::sscanf(hdr, "%5s %d %d %d", ver, &h.nelgt, &ndim, &h.nelgv) == 4, "re2: bad #v004 header");
+  } else if (std::strcmp(ver, "#v002") == 0 || std::strcmp(ver, "#v003") == 0) {
+    NRS_CHECK(std::sscanf(hdr, "%5s %9d %1d %9d", ver, &h.nelgt, &ndim, &h.nelgv) == 4, "re2: bad header");
+  } else {
+    throw Error(NRS_ERR_UNSUPPORTED, std::string("re2: unsupported version ") + ver);
+  }
+  NRS_CHECK(ndim == 3, "re2: only 3D meshes");
+  NRS_CHECK(h.nelgt > 0 && h.nelgv > 0 && h.nelgv <= h.nelgt, "re2: invalid element counts");
+  float test = 0.0f;
+  read_exact(fh.f, &test, 4, "byte-order word");
+  if (std::fabs(test - 6.54321f) < 1e-5f) {
+    h.swap = false;
+  } else {
+    unsigned char b[4];
+    std::memcpy(b, &test, 4);
+    std::swap(b[0], b[3]); std::swap(b[1], b[2]);
+    std::memcpy(&test, b, 4);
+    NRS_CHECK(std::fabs(test - 6.54321f) < 1e-5f, "re2: bad byte-order word");
+    h.swap = true;
+  }
+  // skip the elements, read the curve and boundary counts
+  NRS_CHECK(std::fseek(fh.f, 84 + (long)h.nelgt * 25 * 8, SEEK_SET) == 0, "re2: seek");
+  double c = 0.0;
+  read_exact(fh.f, &c, 8, "curve count");
+  h.ncurve = (int)(h.swap ? swap8(c) : c);
+  NRS_CHECK(h.ncurve >= 0, "re2: bad curve count");
+  NRS_CHECK(std::fseek(fh.f, (long)h.ncurve * 8 * 8, SEEK_CUR) == 0, "re2: seek");
+  double nb = 0.0;
+  read_exact(fh.f, &nb, 8, "boundary count");
+  h.nbc = (int)(h.swap ? swap8(nb) : nb);
+  NRS_CHECK(h.nbc >= 0, "re2: bad boundary count");
+  return h;
+}
+
+void re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc)
+{
+  const Re2Header h = re2_header(path);
+  File fh(path);
+  NRS_CHECK(fh.f != nullptr, std::string("re2: cannot open ") + path);
+  NRS_CHECK(std::fseek(fh.f, 84, SEEK_SET) == 0, "re2: seek");
+  std::vector<double> rec(25);
+  for (int e = 0; e < h.nelgt; ++e) {
+    read_exact(fh.f, rec.data(), 25 * 8, "element record");
+    if (h.swap) for (auto &v : rec) v = swap8(v);
+    if (group) group[e] = (int)rec[0];
+    for (int j = 0; j < 8; ++j) {
+      const int p = kVtxMap[j];
+      vx[e * 8 + p] = rec[1 + j];
+      vy[e * 8 + p] = rec[9 + j];
+      vz[e * 8 + p] = rec[17 + j];
+    }
+  }
+  NRS_CHECK(std::fseek(fh.f, 8 + (long)h.ncurve * 8 * 8, SEEK_CUR) == 0, "re2: seek");
+  double nb = 0.0;
+  read_exact(fh.f, &nb, 8, "boundary count");
+  if (cbc) {
+    std::memset(cbc, ' ', (size_t)h.nelgt * 6 * 3);
+    std::vector<double> b(8);
+    for (int r = 0; r < h.nbc; ++r) {
+      read_exact(fh.f, b.data(), 64, "boundary record");
+      double eg = b[0], f = b[1];
+      char code[8];
+      std::memcpy(code, &b[7], 8);
+      if (h.swap) { eg = swap8(eg); f = swap8(f); }
+      const int e = (int)eg - 1, face = (int)f - 1;
+      NRS_CHECK(e >= 0 && e < h.nelgt && face >= 0 && face < 6, "re2: boundary record out of range");
+      std::memcpy(cbc + ((size_t)e * 6 + kFaceMap[face]) * 3, code, 3);
+    }
+  }
+}
+
+std::vector<long long> vertex_ids(int E, const double *vx, const double *vy, const double *vz, double tol)
+{
+  // tolerance: a fraction of the shortest element edge unless given
+  if (!(tol > 0.0)) {
+    double hmin = 1e300;
+    const int edges[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3},
+                              {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    for (int e = 0; e < E; ++e)
+      for (auto &ed : edges) {
+        const int a = e * 8 + ed[0], b = e * 8 + ed[1];
+        const double d = std::sqrt((vx[a] - vx[b]) * (vx[a] - vx[b]) + (vy[a] - vy[b]) * (vy[a] - vy[b]) +
+                                   (vz[a] - vz[b]) * (vz[a] - vz[b]));
+        hmin = std::min(hmin, d);
+      }
+    NRS_CHECK(hmin > 0.0, "mesh: degenerate element (an edge of length zero)");
+    tol = 1e-3 * hmin;
+  }
+  // spatial hash with cells of size 4 tol; search the 27 neighbouring cells
+  const double cell = 4.0 * tol;
+  struct KeyHash {
+    size_t operator()(const std::array<long long, 3> &k) const
+    {
+      return (size_t)(k[0] * 73856093LL ^ k[1] * 19349663LL ^ k[2] * 83492791LL);
+    }
+  };
+  std::unordered_map<std::array<long long, 3>, std::vector<int>, KeyHash> grid;
+  std::vector<double> px, py, pz;
+  std::vector<long long> ids((size_t)E * 8);
+  for (int64_t n = 0; n < (int64_t)E * 8; ++n) {
+    const long long cx = (long long)std::floor(vx[n] / cell), cy = (long long)std::floor(vy[n] / cell),
+                    cz = (long long)std::floor(vz[n] / cell);
+    int found = -1;
+    for (int dx = -1; dx <= 1 && found < 0; ++dx)
+      for (int dy = -1; dy <= 1 && found < 0; ++dy)
+        for (int dz = -1; dz <= 1 && found < 0; ++dz) {
+          auto it = grid.find({cx + dx, cy + dy, cz + dz});
+          if (it == grid.end()) continue;
+          for (int u : it->second) {
+            const double d2 = (px[u] - vx[n]) * (px[u] - vx[n]) + (py[u] - vy[n]) * (py[u] - vy[n]) +
+                              (pz[u] - vz[n]) * (pz[u] - vz[n]);
+            if (d2 <= tol * tol) { found = u; break; }
+          }
+        }
+    if (found < 0) {
+      found = (int)px.size();
+      px.push_back(vx[n]); py.push_back(vy[n]); pz.push_back(vz[n]);
+      grid[{cx, cy, cz}].push_back(found);
+    }
+    ids[n] = found + 1;
+  }
+  return ids;
+}
+
+void mesh_from_vertices(int N, int E, const double *vx, const double *vy, const double *vz, double tol,
+                        double *x, double *y, double *z, long long *gids)
+{
+  NRS_CHECK(N >= 1 && N <= 9, "mesh: N must be 1..9");
+  NRS_CHECK(E > 0, "mesh: no elements");
+  const std::vector<long long> vid = vertex_ids(E, vx, vy, vz, tol);
+  const std::vector<long long> g = coarse_global_ids(1, E, vid.data(), N);
+  std::vector<double> r, w;
+  gll_nodes(N, r, w);
+  const int Nq = N + 1, Np = Nq * Nq * Nq;
+  for (int64_t e = 0; e < E; ++e)
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const double a[2] = {0.5 * (1 - r[i]), 0.5 * (1 + r[i])};
+          const double b[2] = {0.5 * (1 - r[j]), 0.5 * (1 + r[j])};
+          const double c[2] = {0.5 * (1 - r[k]), 0.5 * (1 + r[k])};
+          double X = 0, Y = 0, Z = 0;
+          for (int cc = 0; cc < 2; ++cc)
+            for (int bb = 0; bb < 2; ++bb)
+              for (int aa = 0; aa < 2; ++aa) {
+                const double wgt = a[aa] * b[bb] * c[cc];
+                const int64_t v = e * 8 + aa + 2 * bb + 4 * cc;
+                X += wgt * vx[v]; Y += wgt * vy[v]; Z += wgt * vz[v];
+              }
+          const int64_t n = e * Np + i + Nq * (j + Nq * k);
+          x[n] = X; y[n] = Y; z[n] = Z;
+          gids[n] = g[n];
+        }
+}
+
+}  // namespace nrs

@@ -1,0 +1,27 @@
+// re2.hpp -- .re2 mesh files and the numbering of unstructured hexahedral meshes.
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace nrs {
+
+struct Re2Header {
+  int nelgt = 0, nelgv = 0;      // elements (all, fluid)
+  int ncurve = 0, nbc = 0;       // curved-side records, boundary records of the first field
+  bool swap = false;             // file written with the other byte order
+};
+
+Re2Header re2_header(const char *path);
+// vx, vy, vz [nelgt][8] corners in lexicographic order (i + 2 j + 4 k); group
+// [nelgt] (may be null); cbc [nelgt][6][3] boundary codes of the first field in
+// the solver's face order (0 t-, 1 s-, 2 r+, 3 s+, 4 r-, 5 t+; blanks inside)
+void re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc);
+// 1-based ids of the corners [E][8]: corners closer than tol share an id
+// (tol <= 0: 1e-3 of the shortest element edge)
+std::vector<long long> vertex_ids(int E, const double *vx, const double *vy, const double *vz, double tol);
+// GLL node coordinates (trilinear map of the corners) and global ids at order N
+void mesh_from_vertices(int N, int E, const double *vx, const double *vy, const double *vz, double tol,
+                        double *x, double *y, double *z, long long *gids);
+
+}  // namespace nrs
