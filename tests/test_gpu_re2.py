@@ -1,10 +1,14 @@
 """Solves on a mesh read from a .re2 file whose elements have their corners in
 randomly rotated orders (written by oracle/re2.py), against the same box from
 the structured mesher: the assembled problem is the same, so Jacobi-PCG takes
-the same iterations (+-1) and the solutions agree node by node (matched by
-coordinates) to 1e-8; multigrid with Schwarz (RAS) smoothing, whose face
+the same iterations (+-1) and the solutions, both solved to 1e-12, agree node
+by node (matched by coordinates) to 1e-8; multigrid with Schwarz (RAS) smoothing, whose face
 exchange must follow each element's orientation, converges in the same number
-of iterations (+-2) to the same solution."""
+of iterations (+-2) to the same solution within 2e-7: its FP32 prolongation is
+elementwise, as in the reference (ellipticMultiGridLevel.cpp:70-83, no
+gather-scatter after it), so elements of different orientation round the
+values of a shared node differently, and the FP64 solution inherits those
+FP32-rounding-level jumps (measured 3.3e-8)."""
 import numpy as np
 import pytest
 
@@ -31,7 +35,7 @@ def rotated_re2(tmp_path, nx, ny, nz, seed):
 
 
 def solve(gpu, torch, mesh, f, **kw):
-    el = gpu.Elliptic(mesh, tol=1e-10, max_iter=1000, **kw)
+    el = gpu.Elliptic(mesh, tol=1e-12, max_iter=2000, **kw)
     gw = el.ggeo.cpu().numpy().reshape(-1, 7, mesh.Np)[:, 6].ravel()
     x = torch.zeros(mesh.Nlocal, dtype=torch.float64, device="cuda")
     st = el.solve(torch.from_numpy(gw * f(mesh.x, mesh.y, mesh.z)).cuda(), x)
@@ -64,4 +68,5 @@ def test_rotated_re2_mesh_solves_like_the_box(gpu, torch_mod, tmp_path, kw):
     assert abs(sm["iterations"] - sb["iterations"]) <= tol_it
     um, ub = by_coordinates(m, xm), by_coordinates(b, xb)
     assert um.shape == ub.shape
-    assert np.max(np.abs(um - ub)) <= 1e-8 * np.max(np.abs(ub))
+    bar = 1e-8 if kw["precon"] == "jacobi" else 2e-7
+    assert np.max(np.abs(um - ub)) <= bar * np.max(np.abs(ub))
