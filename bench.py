@@ -246,7 +246,9 @@ def pressure_rhs_case(mesh, N, steps, warmup, seed=0):
     gbs = alg / (ms * 1e-3) / 1e9
     log("pressure rhs (%d elements N=%d, channel): %.4f ms per call, %.2f GB algorithmic, "
         "%.0f GB/s (%.1f %% of HBM peak)" % (mesh.E, N, ms, alg / 1e9, gbs, 100 * gbs / PEAK_HBM_GBS))
+    variants = {k: nekrs_amd.tuned_variant("tombo", N, k, mesh.E) for k in ("curl", "rhs")}
     return {"ms_per_call": ms, "setup_seconds": setup, "algorithmic_bytes": alg,
+            "load_schedule_variants": variants,
             "achieved_GBps": gbs, "roofline_frac": gbs / PEAK_HBM_GBS, "steps": steps,
             "flux_elements": int(flux.any(1).sum()),
             "workload": "tombo::pressureSolve right-hand side (viscous term on), %d hex elements "

@@ -369,11 +369,14 @@ int nrs_batch_set_mode(nrs_batch *b, int mode);
  * wavefront {1, 2, 4, 8} of the matrix-core Schwarz kernel, keyed by the
  * nrs_fdm_precision; NRS_TUNE_OPERATOR = element chunks {1, 2, 4, 8, 16} of
  * the solver operator (Ax of chunk c+1 overlapped with the gather-scatter of
- * the rows chunk c completes), keyed by dtype.
+ * the rows chunk c completes), keyed by dtype; NRS_TUNE_TOMBO = load schedule
+ * {1: at use, 2: one plane ahead} of the pressure right-hand side's curl
+ * kernels, type 0 = JW curl, 1 = the fused second curl + gradient (timed at
+ * nrs_tombo_setup).
  * Keys are (kernel, N, type, floor(log2 Nelements)); a
  * setup with opts.autotune records the fastest variant whose result matches
  * the default's.  variant 0 = none recorded (the default is used). */
-enum nrs_tune_kernel { NRS_TUNE_AX = 0, NRS_TUNE_FDM = 1, NRS_TUNE_OPERATOR = 2 };
+enum nrs_tune_kernel { NRS_TUNE_AX = 0, NRS_TUNE_FDM = 1, NRS_TUNE_OPERATOR = 2, NRS_TUNE_TOMBO = 3 };
 int nrs_tuned_variant(int kernel, int N, int type, int64_t Nelements, int *variant);
 /* Record a variant by hand (0 clears it); later launches with this key use it. */
 int nrs_set_variant(int kernel, int N, int type, int64_t Nelements, int variant);

@@ -394,13 +394,22 @@ def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=5
     return o
 
 
-TUNE_KERNELS = {"ax": 0, "fdm": 1, "op": 2}
+TUNE_KERNELS = {"ax": 0, "fdm": 1, "op": 2, "tombo": 3}
+
+
+def _tune_type(kernel, dtype):
+    if kernel == "fdm":
+        return FDM_PRECISIONS[dtype]
+    if kernel == "tombo":
+        return {"curl": 0, "rhs": 1}[dtype]
+    return DTYPES[dtype]
 
 
 def tuned_variant(kernel, N, dtype, nelements):
     """Recorded launch variant for (kernel, N, type, E), 0 if none (nrs_tuned_variant).
-    type: a DTYPES name for "ax" and "op", an FDM_PRECISIONS name for "fdm"."""
-    t = FDM_PRECISIONS[dtype] if kernel == "fdm" else DTYPES[dtype]
+    type: a DTYPES name for "ax" and "op", an FDM_PRECISIONS name for "fdm", "curl" or
+    "rhs" for "tombo"."""
+    t = _tune_type(kernel, dtype)
     v = C.c_int()
     _lib.call("nrs_tuned_variant", TUNE_KERNELS[kernel], int(N), t, int(nelements), C.byref(v))
     return v.value
@@ -408,7 +417,7 @@ def tuned_variant(kernel, N, dtype, nelements):
 
 def set_variant(kernel, N, dtype, nelements, variant):
     """Force a launch variant (0 clears) for later launches with this key."""
-    t = FDM_PRECISIONS[dtype] if kernel == "fdm" else DTYPES[dtype]
+    t = _tune_type(kernel, dtype)
     _lib.call("nrs_set_variant", TUNE_KERNELS[kernel], int(N), t, int(nelements), int(variant))
 
 

@@ -427,8 +427,7 @@ int nrs_tuned_variant(int kernel, int N, int type, int64_t Nelements, int *varia
 {
   NRS_API_BEGIN
   NRS_CHECK(variant != nullptr, "tuned variant: null output");
-  NRS_CHECK(kernel == NRS_TUNE_AX || kernel == NRS_TUNE_FDM || kernel == NRS_TUNE_OPERATOR,
-            "tuned variant: unknown kernel");
+  NRS_CHECK(kernel >= NRS_TUNE_AX && kernel <= NRS_TUNE_TOMBO, "tuned variant: unknown kernel");
   NRS_CHECK(N >= 1 && N <= 9 && type >= 0 && type < nrs::TUNE_TYPES && Nelements >= 1,
             "tuned variant: bad key");
   *variant = nrs::tuned(kernel, N, type, Nelements, 0);
@@ -438,8 +437,7 @@ int nrs_tuned_variant(int kernel, int N, int type, int64_t Nelements, int *varia
 int nrs_set_variant(int kernel, int N, int type, int64_t Nelements, int variant)
 {
   NRS_API_BEGIN
-  NRS_CHECK(kernel == NRS_TUNE_AX || kernel == NRS_TUNE_FDM || kernel == NRS_TUNE_OPERATOR,
-            "set variant: unknown kernel");
+  NRS_CHECK(kernel >= NRS_TUNE_AX && kernel <= NRS_TUNE_TOMBO, "set variant: unknown kernel");
   NRS_CHECK(N >= 1 && N <= 9 && type >= 0 && type < nrs::TUNE_TYPES && Nelements >= 1,
             "set variant: bad key");
   nrs::set_tuned(kernel, N, type, Nelements, variant);

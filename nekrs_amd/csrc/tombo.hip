@@ -17,6 +17,7 @@
 
 #include "mesh.hpp"
 #include "tombo.hpp"
+#include "tune.hpp"
 
 nrs_tombo::~nrs_tombo()
 {
@@ -102,7 +103,7 @@ vgeo_kernel(int E, const double *__restrict__ Dg, const double *__restrict__ wg,
 // JW grad(div) (gradientVolumeHex3D.okl) and the combination of
 // pressureRhsHex3D.okl: out = BF - mue irho (JW curl - 4/3 JW grad(div)).
 // One block per element, thread (i,j), k-pencils of the fields in registers.
-template <int NQ, bool PRESCALE, bool RHS>
+template <int NQ, bool PRESCALE, bool RHS, bool PF>
 __global__ void __launch_bounds__(NQ * NQ)
 curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ Dg, int64_t offIn,
             int64_t offOut, const double *__restrict__ invLMM, const double *__restrict__ U,
@@ -125,7 +126,26 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
     for (int c = 0; c < 3; ++c) pf[c][l] = PRESCALE ? sc * U[id + c * offIn] : U[id + c * offIn];
     if (RHS) pf[NF - 1][l] = div[id];
   }
+  // per-plane inputs, loaded one plane ahead: geometry (and BF, mue, irho)
+  constexpr int NX = RHS ? NV + 5 : NV;
+  const double *Ge = vgeo + (int64_t)e * NV * NP + j * NQ + i;
+  auto load_plane = [&](int k, double (&g)[NX]) {
+#pragma unroll
+    for (int f = 0; f < NV; ++f) g[f] = __builtin_nontemporal_load(Ge + f * NP + k * NFP);
+    if (RHS) {
+      const int64_t id = base + k * NFP;
+      g[NV + 0] = BF[id];
+      g[NV + 1] = BF[id + offBF];
+      g[NV + 2] = BF[id + 2 * offBF];
+      g[NV + 3] = mue[id];
+      g[NV + 4] = irho[id];
+    }
+  };
+  double gc[NX], gn[NX];
+  if (PF) load_plane(0, gc);
+#pragma unroll
   for (int k = 0; k < NQ; ++k) {
+    if (PF && k + 1 < NQ) load_plane(k + 1, gn);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NF; ++c) s_f[c][j][i] = pf[c][k];
@@ -144,11 +164,11 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
       }
     }
     const int64_t id = base + k * NFP;
-    const double *G = vgeo + (int64_t)e * NV * NP + k * NFP + j * NQ + i;
-    const double rx = G[0 * NP], ry = G[1 * NP], rz = G[2 * NP];
-    const double sx = G[3 * NP], sy = G[4 * NP], sz = G[5 * NP];
-    const double tx = G[6 * NP], ty = G[7 * NP], tz = G[8 * NP];
-    const double JW = G[9 * NP];
+    if (!PF) load_plane(k, gc);
+    const double rx = gc[0], ry = gc[1], rz = gc[2];
+    const double sx = gc[3], sy = gc[4], sz = gc[5];
+    const double tx = gc[6], ty = gc[7], tz = gc[8];
+    const double JW = gc[9];
     const double uy = ry * dr[0] + sy * ds[0] + ty * dt[0];
     const double uz = rz * dr[0] + sz * ds[0] + tz * dt[0];
     const double vx = rx * dr[1] + sx * ds[1] + tx * dt[1];
@@ -162,14 +182,18 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
       const double gy = JW * (ry * dr[P] + sy * ds[P] + ty * dt[P]);
       const double gz = JW * (rz * dr[P] + sz * ds[P] + tz * dt[P]);
       const double factor = 4. / 3;
-      const double nu = mue[id] * irho[id];
-      c0 = BF[id] - nu * (c0 - factor * gx);
-      c1 = BF[id + offBF] - nu * (c1 - factor * gy);
-      c2 = BF[id + 2 * offBF] - nu * (c2 - factor * gz);
+      const double nu = gc[NV + 3] * gc[NV + 4];
+      c0 = gc[NV + 0] - nu * (c0 - factor * gx);
+      c1 = gc[NV + 1] - nu * (c1 - factor * gy);
+      c2 = gc[NV + 2] - nu * (c2 - factor * gz);
     }
-    out[id] = c0;
-    out[id + offOut] = c1;
-    out[id + 2 * offOut] = c2;
+    __builtin_nontemporal_store(c0, out + id);
+    __builtin_nontemporal_store(c1, out + id + offOut);
+    __builtin_nontemporal_store(c2, out + id + 2 * offOut);
+    if (PF && k + 1 < NQ) {
+#pragma unroll
+      for (int f = 0; f < NX; ++f) gc[f] = gn[f];
+    }
   }
 }
 
@@ -315,26 +339,31 @@ template <int NQ> struct VgeoL {
   }
 };
 template <int NQ> struct CurlL {
-  static void run(const nrs_tombo *t, int64_t offIn, int64_t offOut, bool prescale, const double *U,
-                  double *W, hipStream_t s)
+  static void run(const nrs_tombo *t, int64_t offIn, int64_t offOut, bool prescale, bool pf,
+                  const double *U, double *W, hipStream_t s)
   {
-    if (prescale)
-      curl_kernel<NQ, true, false><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), offIn,
-                                                             offOut, t->invLMM.get(), U, nullptr,
-                                                             nullptr, 0, nullptr, nullptr, W);
-    else
-      curl_kernel<NQ, false, false><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), offIn,
-                                                              offOut, nullptr, U, nullptr, nullptr, 0,
-                                                              nullptr, nullptr, W);
+    const dim3 g(t->E), b(NQ * NQ);
+    const double *G = t->vgeo.get(), *D = t->D.get(), *il = t->invLMM.get();
+#define CURL(P, F) curl_kernel<NQ, P, false, F><<<g, b, 0, s>>>(t->E, G, D, offIn, offOut, P ? il : nullptr, \
+                                                                U, nullptr, nullptr, 0, nullptr, nullptr, W)
+    if (prescale) { if (pf) CURL(true, true); else CURL(true, false); }
+    else { if (pf) CURL(false, true); else CURL(false, false); }
+#undef CURL
   }
 };
 template <int NQ> struct RhsL {
-  static void run(const nrs_tombo *t, const double *W, const double *div, const double *BF,
+  static void run(const nrs_tombo *t, bool pf, const double *W, const double *div, const double *BF,
                   int64_t offBF, const double *mue, const double *irho, double *out, hipStream_t s)
   {
-    curl_kernel<NQ, true, true><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
-                                                          t->Nlocal, t->invLMM.get(), W, div, BF,
-                                                          offBF, mue, irho, out);
+    const dim3 g(t->E), b(NQ * NQ);
+    if (pf)
+      curl_kernel<NQ, true, true, true><<<g, b, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
+                                                         t->Nlocal, t->invLMM.get(), W, div, BF, offBF,
+                                                         mue, irho, out);
+    else
+      curl_kernel<NQ, true, true, false><<<g, b, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
+                                                          t->Nlocal, t->invLMM.get(), W, div, BF, offBF,
+                                                          mue, irho, out);
   }
 };
 template <int NQ> struct WdivL {
@@ -373,6 +402,44 @@ __global__ void copy_jw_kernel(int E, int Np, const double *__restrict__ vgeo, d
 bool flux_face(int t)
 {
   return t == BC_V || t == BC_INT || t == BC_SYMX || t == BC_SYMY || t == BC_SYMZ || t == BC_SYM;
+}
+
+// Time the two load schedules of each curl form on this mesh with patterned
+// inputs, require results within 400 eps of each other (the same arithmetic;
+// the compiler may contract multiply-adds differently), record the faster
+// (type 0: JW curl, type 1: the fused right-hand-side kernel).
+void autotune(nrs_tombo *t, hipStream_t s)
+{
+  const int64_t L = t->Nlocal;
+  DevBuf<double> U(3 * L), div(L), BF(3 * L), mue(L), irho(L), out2(3 * L);
+  fill_pattern<double>(3 * L, U.get(), s);
+  fill_pattern<double>(L, div.get(), s);
+  fill_pattern<double>(3 * L, BF.get(), s);
+  fill_pattern<double>(L, mue.get(), s);
+  fill_pattern<double>(L, irho.get(), s);
+  for (int type = 0; type < 2; ++type) {
+    double ms[2];
+    double *outs[2] = {t->W.get(), out2.get()};
+    for (int v = 0; v < 2; ++v) {
+      auto f = [&]() {
+        if (type == 0) by_order<CurlL>(t->N, t, L, L, false, v == 1, U.get(), outs[v], s);
+        else by_order<RhsL>(t->N, t, v == 1, U.get(), div.get(), BF.get(), L, mue.get(), irho.get(), outs[v], s);
+      };
+      ms[v] = time_launches(f, s);
+    }
+    NRS_LAUNCH_CHECK();
+    std::vector<double> a(3 * L), b(3 * L);
+    NRS_HIP(hipMemcpyAsync(a.data(), outs[0], 3 * L * sizeof(double), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipMemcpyAsync(b.data(), outs[1], 3 * L * sizeof(double), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    double amax = 0.0, dmax = 0.0;
+    for (int64_t n = 0; n < 3 * L; ++n) {
+      amax = std::max(amax, std::fabs(a[n]));
+      dmax = std::max(dmax, std::fabs(a[n] - b[n]));
+    }
+    const bool same = dmax <= 400 * 2.220446049250313e-16 * amax;   // NaN fails
+    set_tuned(TUNE_TOMBO, t->N, type, t->E, (same && ms[1] < ms[0]) ? 2 : 1);
+  }
 }
 
 }  // namespace
@@ -431,12 +498,13 @@ nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const dou
   t->W.alloc(3 * (size_t)t->Nlocal);
   t->F.alloc(3 * (size_t)t->Nlocal);
   NRS_HIP(hipStreamSynchronize(s));
+  autotune(t.get(), s);
   return t.release();
 }
 
 void tombo_curl(const nrs_tombo *t, int64_t off, const double *U, double *W, hipStream_t s)
 {
-  by_order<CurlL>(t->N, t, off, off, false, U, W, s);
+  by_order<CurlL>(t->N, t, off, off, false, tuned(TUNE_TOMBO, t->N, 0, t->E, 1) == 2, U, W, s);
   NRS_LAUNCH_CHECK();
 }
 
@@ -447,9 +515,11 @@ void tombo_pressure_rhs(nrs_tombo *t, int64_t off, const double *Ue, const doubl
   const int64_t L = t->Nlocal;
   double *W = t->W.get(), *F = t->F.get();
   if (visc) {
-    by_order<CurlL>(t->N, t, off, L, false, Ue, W, s);                  // W = JW curl(Ue)
+    const bool pf0 = tuned(TUNE_TOMBO, t->N, 0, t->E, 1) == 2;
+    const bool pf1 = tuned(TUNE_TOMBO, t->N, 1, t->E, 1) == 2;
+    by_order<CurlL>(t->N, t, off, L, false, pf0, Ue, W, s);            // W = JW curl(Ue)
     ogs_gather_scatter<double>(t->ogs, W, 3, L, NRS_ADD, s);
-    by_order<RhsL>(t->N, t, W, div, BF, off, mue, irho, F, s);          // F = rhsU
+    by_order<RhsL>(t->N, t, pf1, W, div, BF, off, mue, irho, F, s);    // F = rhsU
   } else {
     for (int c = 0; c < 3; ++c)                                         // F = BF
       NRS_HIP(hipMemcpyAsync(F + c * L, BF + c * off, L * sizeof(double), hipMemcpyDeviceToDevice, s));

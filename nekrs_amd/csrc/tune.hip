@@ -16,6 +16,7 @@ std::atomic<int> g_table[TUNE_KERNELS][10][TUNE_TYPES][TUNE_ECLASS];
 const int kAxVariants[] = {1, 2, 4};         // elements-per-block multipliers
 const int kFdmVariants[] = {1, 2, 4, 8};     // elements per wavefront
 const int kOpVariants[] = {1, 2, 4, 8, 16};  // operator chunks (Ax overlapped with gs)
+const int kTomboVariants[] = {1, 2};        // pressure-rhs volume kernels: inputs loaded at use / one plane ahead
 
 bool key_ok(int kernel, int N, int type)
 {
@@ -64,8 +65,9 @@ void set_tuned(int kernel, int N, int type, int64_t E, int variant)
 
 int tune_variants(int kernel, int *values, int max)
 {
-  const int *v = kernel == TUNE_AX ? kAxVariants : kernel == TUNE_FDM ? kFdmVariants : kOpVariants;
-  const int n = kernel == TUNE_AX ? 3 : kernel == TUNE_FDM ? 4 : 5;
+  const int *v = kernel == TUNE_AX ? kAxVariants : kernel == TUNE_FDM ? kFdmVariants
+               : kernel == TUNE_OP ? kOpVariants : kTomboVariants;
+  const int n = kernel == TUNE_AX ? 3 : kernel == TUNE_FDM ? 4 : kernel == TUNE_OP ? 5 : 2;
   for (int i = 0; i < n && i < max; ++i) values[i] = v[i];
   return n;
 }

@@ -19,7 +19,7 @@
 
 namespace nrs {
 
-enum TuneKernel { TUNE_AX = 0, TUNE_FDM = 1, TUNE_OP = 2, TUNE_KERNELS = 3 };
+enum TuneKernel { TUNE_AX = 0, TUNE_FDM = 1, TUNE_OP = 2, TUNE_TOMBO = 3, TUNE_KERNELS = 4 };
 constexpr int TUNE_TYPES = 8;      // data-type slots (nrs_dtype / nrs_fdm_precision codes)
 constexpr int TUNE_ECLASS = 32;    // floor(log2 E)
 

@@ -117,3 +117,29 @@ def test_pressure_rhs_rejects_short_offset(gpu, torch_mod, orc):
     s = torch.zeros(L, dtype=torch.float64, device="cuda")
     with pytest.raises(gpu.NrsError):
         t.compute(a, a, a, s, s, s, 1.0, True, field_offset=L - 1)
+
+
+def test_load_schedules_agree(gpu, torch_mod, orc):
+    """Both load schedules of the curl kernels (NRS_TUNE_TOMBO 1 and 2) give
+    the same right-hand side to rounding (the compiler may contract the
+    multiply-adds of the two schedules differently): 400 eps of the largest entry."""
+    torch = torch_mod
+    N = 7
+    m, mo = meshes(gpu, orc, (4, 3, 3), N)
+    t = gpu.PressureRhs(m, bc_mix(m, 5))
+    assert gpu.tuned_variant("tombo", N, "curl", m.E) in (1, 2)
+    assert gpu.tuned_variant("tombo", N, "rhs", m.E) in (1, 2)
+    L = m.x.size
+    rng = np.random.default_rng(5)
+    dv = lambda a: torch.from_numpy(a).cuda()
+    args = [dv(rng.uniform(-1, 1, 3 * L)) for _ in range(3)] + [dv(rng.uniform(0.5, 1.5, L)) for _ in range(2)]
+    args.append(dv(rng.uniform(-0.1, 0.1, L)))
+    outs = []
+    for v in (1, 2):
+        gpu.set_variant("tombo", N, "curl", m.E, v)
+        gpu.set_variant("tombo", N, "rhs", m.E, v)
+        outs.append(t.compute(*args, 75.0, True).cpu().numpy())
+    gpu.set_variant("tombo", N, "curl", m.E, 0)
+    gpu.set_variant("tombo", N, "rhs", m.E, 0)
+    print("schedules differ by", rel(outs[1], outs[0]), flush=True)
+    assert rel(outs[1], outs[0]) <= 400 * np.finfo(np.float64).eps
