@@ -8,7 +8,11 @@ one ellipticOperator application Aq = gs(mask(Ax(q))) in FP64; the default run
 times 100 steps after 10 warm-up steps, inputs resident in HBM.
 
   value      GDOF/s = E*N^3 / (seconds per step) / 1e9   (reference convention,
-             src/bench/axHelm/benchmarkAx.cpp:312)
+             src/bench/axHelm/benchmarkAx.cpp:312); the operator as set up by
+             default, i.e. with the face fusion (the interior nodes of faces
+             shared by consecutive elements of a block of four summed inside
+             the Ax kernel, bit-identical results); the same loop without it
+             is timed on the same box and reported in config
   roofline   dominant kernel = FP64 Ax: 64 algorithmic bytes per local node
              (q 8 + Aq 8 + 6 geometric factors 48) / average launch time
              measured with HIP events around every Ax launch in the timed loop
@@ -476,7 +480,27 @@ def main():
         (ms_step, gdofs, ax_ms, achieved, gs_ms))
 
     Nlocal = el.Nlocal
+    fused_pairs = el.fused_pairs
     del el
+    # the same loop without the operator face fusion, on the same box, for comparison
+    el0 = nekrs_amd.Elliptic(mesh, precon="none", operatorFusion=0)
+    for _ in range(args.warmup):
+        el0.operator(q, Aq)
+    el0.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        el0.operator(q, Aq)
+    torch.cuda.synchronize()
+    ms0 = (time.perf_counter() - t0) / args.steps * 1e3
+    p0 = el0.profile_read()
+    el0.profile(False)
+    del el0
+    fusion = {"fused_pairs": fused_pairs, "fused_ms_per_step": ms_step, "unfused_ms_per_step": ms0,
+              "unfused_ax_ms": p0["ax"][0] / max(p0["ax"][1], 1),
+              "unfused_gs_ms": p0["gs"][0] / max(p0["gs"][1], 1)}
+    log("without the face fusion: step %.4f ms (%.2f GDOF/s); Ax %.4f ms, gs %.4f ms" %
+        (ms0, dofs / (ms0 * 1e-3) / 1e9, fusion["unfused_ax_ms"], fusion["unfused_gs_ms"]))
     prhs = None
     if not args.no_pressure_rhs:
         prhs = pressure_rhs_case(mesh, N, 20, 3)
@@ -517,7 +541,8 @@ def main():
                    "gs_achieved_GBps": gs_bytes / (gs_ms * 1e-3) / 1e9,
                    "axgs_algorithmic_GBps": (ax_bytes + gs_bytes) / (ms_step * 1e-3) / 1e9,
                    "axgs_roofline_frac": (ax_bytes + gs_bytes) / (ms_step * 1e-3) / 1e9 /
-                   PEAK_HBM_GBS},
+                   PEAK_HBM_GBS,
+                   "operator_face_fusion": fusion},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles()},
         "cpu_baseline": cpu,

@@ -232,6 +232,11 @@ typedef struct {
   int projVectors;               /* RESIDUAL PROJECTION VECTORS (default 8, 1..32) */
   int projStart;                 /* RESIDUAL PROJECTION START: solve index from
                                   * which the projection runs (default 5)       */
+  int operatorFusion;            /* 1 (default): at N = 7, sum the shared face
+                                  * of consecutive elements that meet i = N to
+                                  * i = 0 inside the Ax kernel and leave those
+                                  * nodes out of the operator's gather-scatter
+                                  * (same results bit for bit); 0: off          */
 } nrs_elliptic_opts;
 
 typedef struct {
@@ -259,6 +264,10 @@ int nrs_elliptic_free(nrs_elliptic *el);
 
 /* Sizes: Nlocal = E*Np, Nmasked, volume = sum(GwJ). */
 int nrs_elliptic_info(nrs_elliptic *el, int64_t *Nlocal, int64_t *Nmasked, double *volume);
+
+/* Face pairs summed inside the Ax kernel by the operator face fusion (0 when
+ * off or not applicable). */
+int nrs_elliptic_fusion_info(nrs_elliptic *el, int64_t *fusedPairs);
 
 /* Device buffers owned by the handle: 0 ggeo (f64), 1 D (f64), 2 invDegree
  * (f64), 3 maskIds (int32), 4 invDiagA (f64). */

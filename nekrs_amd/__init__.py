@@ -545,6 +545,13 @@ class Elliptic:
     def operator(self, q, Aq, masked=True, stream=None):
         _lib.call("nrs_elliptic_operator", self._h, _ptr(q), _ptr(Aq), int(masked), _stream(stream))
 
+    @property
+    def fused_pairs(self):
+        """Face pairs summed inside the Ax kernel (operator face fusion)."""
+        n = C.c_int64()
+        _lib.call("nrs_elliptic_fusion_info", self._h, C.byref(n))
+        return n.value
+
     def preconditioner(self, r, z, stream=None):
         _lib.call("nrs_elliptic_preconditioner", self._h, _ptr(r), _ptr(z), _stream(stream))
 

@@ -38,7 +38,8 @@ class EllipticOpts(C.Structure):
                 ("coarseSolver", C.c_int), ("coarseTol", C.c_double), ("coarseMaxIter", C.c_int),
                 ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong),
                 ("useGraphs", C.c_int), ("autotune", C.c_int), ("gmresRestart", C.c_int),
-                ("initialGuess", C.c_int), ("projVectors", C.c_int), ("projStart", C.c_int)]
+                ("initialGuess", C.c_int), ("projVectors", C.c_int), ("projStart", C.c_int),
+                ("operatorFusion", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -71,6 +72,7 @@ SIGNATURES = {
     "nrs_elliptic_setup": ([C.POINTER(EllipticDesc), C.POINTER(EllipticOpts), C.POINTER(_vp)], _i),
     "nrs_elliptic_free": ([_vp], _i),
     "nrs_elliptic_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_d)], _i),
+    "nrs_elliptic_fusion_info": ([_vp, C.POINTER(_i64)], _i),
     "nrs_elliptic_buffer": ([_vp, _i, C.POINTER(_vp), C.POINTER(_i64)], _i),
     "nrs_elliptic_copy_buffer": ([_vp, _i, _vp, _i64, _vp], _i),
     "nrs_elliptic_operator": ([_vp, _vp, _vp, _i, _vp], _i),

@@ -44,11 +44,17 @@ template <int NQ, int MULT = 1> struct AxShape {
 //   [Gqr,Gqs,Gqt] = lambda0 * G * [qr,qs,qt]
 //   Aq = sum_m D[m][i] Gqr(m,j,k) + D[m][j] Gqs(i,m,k) + D[m][k] Gqt(i,j,m) (+ GwJ lambda1 q)
 // -------------------------------------------------------------------------------------
-template <typename T, int NQ, int MULT, bool POISSON, bool LFIELD>
+//
+// FUSEX: the block's consecutive elements e, e+1 with fx[e] = 1 share their
+// face i = N / i = 0 node for node; the interior nodes of that face are summed
+// here through LDS (both copies get v_e + v_e+1, the gather-scatter's pair sum,
+// bit for bit) and the caller's gather-scatter handle leaves them out.
+template <typename T, int NQ, int MULT, bool POISSON, bool LFIELD, bool FUSEX = false>
 __global__ void __launch_bounds__((AxShape<NQ, MULT>::THREADS), AX_WAVES_PER_EU)
 ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
           const T *__restrict__ Dg, const T *__restrict__ lam0, const T *__restrict__ lam1,
-          const T *__restrict__ q, T *__restrict__ Aq, const uint32_t *__restrict__ maskBits)
+          const T *__restrict__ q, T *__restrict__ Aq, const uint32_t *__restrict__ maskBits,
+          const uint8_t *__restrict__ fx = nullptr)
 {
   constexpr int NE = AxShape<NQ, MULT>::NE;
   constexpr int NP = NQ * NQ * NQ;
@@ -151,6 +157,26 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
     for (int c = 0; c < 7; ++c) g[c] = gn[c];
   }
 
+  if constexpr (FUSEX) {
+    static_assert(NE >= 2, "face fusion needs at least two elements per block");
+    __shared__ T s_x[NE][2][NQ][NQ];     // [element][side: i = 0, i = N][j][k]
+    const bool face = (i == 0 || i == NQ - 1) && j > 0 && j < NQ - 1;
+    if (face) {
+      const int side = (i == 0) ? 0 : 1;
+#pragma unroll
+      for (int k = 1; k < NQ - 1; ++k) s_x[le][side][j][k] = r_Aq[k];
+    }
+    __syncthreads();
+    if (active && face) {
+      if (i == NQ - 1 && le + 1 < NE && eidx + 1 < E && fx[eidx]) {
+#pragma unroll
+        for (int k = 1; k < NQ - 1; ++k) r_Aq[k] = r_Aq[k] + s_x[le + 1][0][j][k];
+      } else if (i == 0 && le >= 1 && fx[eidx - 1]) {
+#pragma unroll
+        for (int k = 1; k < NQ - 1; ++k) r_Aq[k] = s_x[le - 1][1][j][k] + r_Aq[k];
+      }
+    }
+  }
   if (!active) return;
   T *Ae = Aq + e * NP + lid;
   if (maskBits) {
@@ -212,6 +238,30 @@ void launch_ax_variant(int N, int mult, int E, const int *elist, const T *ggeo, 
     default: throw Error(NRS_ERR_UNSUPPORTED, "Ax: order N must be 1..9");
   }
 }
+
+// Ax with the face fusion (N = 7, four elements per block); fx [E] relative to q
+template <typename T>
+void launch_ax_fused(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1, bool poisson,
+                     const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s)
+{
+  NRS_CHECK(N == 7 && fx != nullptr, "Ax face fusion: order 7 only");
+  using S = AxShape<8, 4>;
+  const unsigned grid = (unsigned)((E + S::NE - 1) / S::NE);
+  if (grid == 0) return;
+  if (poisson)
+    ax_kernel<T, 8, 4, true, false, true><<<grid, S::THREADS, 0, s>>>(E, nullptr, ggeo, D, l0, l1, q, Aq,
+                                                                      maskBits, fx);
+  else
+    ax_kernel<T, 8, 4, false, false, true><<<grid, S::THREADS, 0, s>>>(E, nullptr, ggeo, D, l0, l1, q, Aq,
+                                                                       maskBits, fx);
+  NRS_LAUNCH_CHECK();
+}
+template void launch_ax_fused<double>(int, int, const double *, const double *, const double *,
+                                      const double *, bool, const double *, double *, const uint32_t *,
+                                      const uint8_t *, hipStream_t);
+template void launch_ax_fused<float>(int, int, const float *, const float *, const float *,
+                                     const float *, bool, const float *, float *, const uint32_t *,
+                                     const uint8_t *, hipStream_t);
 
 // elements-per-block multiplier: the autotuner's choice for (N, type), else 1
 template <typename T>

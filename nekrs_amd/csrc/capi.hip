@@ -274,6 +274,7 @@ void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
   o->initialGuess = NRS_GUESS_PREVIOUS;
   o->projVectors = 8;
   o->projStart = 5;
+  o->operatorFusion = 1;
 }
 
 int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts, nrs_elliptic **out)
@@ -627,5 +628,13 @@ int nrs_mesh_from_vertices(int N, int Nelements, const double *vx, const double 
   NRS_API_BEGIN
   NRS_CHECK(vx && vy && vz && x && y && z && globalIds, "mesh from vertices: null array");
   nrs::mesh_from_vertices(N, Nelements, vx, vy, vz, tol, x, y, z, globalIds);
+  NRS_API_END
+}
+
+int nrs_elliptic_fusion_info(nrs_elliptic *el, int64_t *fusedPairs)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && fusedPairs, "fusion info: null argument");
+  *fusedPairs = el->fusedPairs;
   NRS_API_END
 }

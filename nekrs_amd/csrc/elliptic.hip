@@ -14,6 +14,7 @@
 nrs_elliptic::~nrs_elliptic()
 {
   if (ogs) delete ogs;
+  if (ogsOp) delete ogsOp;
   if (ogsFull) delete ogsFull;
   if (scalHost) (void)hipHostFree(scalHost);
   if (ev0) (void)hipEventDestroy(ev0);
@@ -87,6 +88,37 @@ std::vector<int> dirichlet_mask(int N, int E, const int *EToB, nrs_ogs *ogsFull,
   return ids;
 }
 
+std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids)
+{
+  std::vector<uint8_t> fx;
+  if (N != 7 || E < 2) return fx;
+  constexpr int Nq = 8, Np = 512;
+  fx.assign(E, 0);
+  int64_t pairs = 0;
+  for (int64_t e = 0; e + 1 < E; ++e) {
+    if (e % 4 == 3) continue;                       // partner in the next block
+    bool ok = true;
+    for (int k = 1; k < Nq - 1 && ok; ++k)
+      for (int j = 1; j < Nq - 1 && ok; ++j) {
+        const long long a = ids[e * Np + k * 64 + j * 8 + 7], b = ids[(e + 1) * Np + k * 64 + j * 8];
+        ok = a != 0 && a == b;
+      }
+    if (!ok) continue;
+    fx[e] = 1;
+    ++pairs;
+  }
+  if (!pairs) { fx.clear(); return fx; }
+  for (int64_t e = 0; e + 1 < E; ++e) {
+    if (!fx[e]) continue;
+    for (int k = 1; k < Nq - 1; ++k)
+      for (int j = 1; j < Nq - 1; ++j) {
+        ids[e * Np + k * 64 + j * 8 + 7] = 0;
+        ids[(e + 1) * Np + k * 64 + j * 8] = 0;
+      }
+  }
+  return fx;
+}
+
 nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts *o)
 {
   NRS_CHECK(d && o, "elliptic setup: null descriptor/options");
@@ -154,13 +186,23 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     std::vector<long long> mg(d->globalIds, d->globalIds + el->Nlocal);
     for (int id : mids) mg[id] = 0;
     el->ogs = ogs_setup(el->Nlocal, mg.data());
+    if (o->operatorFusion && el->Nf == 1) {
+      const std::vector<uint8_t> fx = face_fusion(el->N, el->E, mg);
+      if (!fx.empty()) {
+        el->ogsOp = ogs_setup(el->Nlocal, mg.data());
+        el->fx.upload(fx);
+        for (uint8_t f : fx) el->fusedPairs += f;
+      }
+    }
   }
 
-  // operator chunks: rows of the masked handle grouped by element chunk
+  // operator chunks: rows of the operator's handle grouped by element chunk
+  // (chunks of a multiple of four elements, the fused kernel's block)
   {
     constexpr int F = nrs_elliptic::kOpChunks;
-    const int64_t epc = (el->E + F - 1) / F;
+    const int64_t epc = ((el->E + F - 1) / F + 3) / 4 * 4;
     ogs_set_chunks(el->ogs, epc * el->Np, F);
+    if (el->ogsOp) ogs_set_chunks(el->ogsOp, epc * el->Np, F);
     NRS_HIP(hipStreamCreateWithFlags(&el->aux, hipStreamNonBlocking));
     for (auto &e : el->chunkEv) NRS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     NRS_HIP(hipEventCreateWithFlags(&el->auxDone, hipEventDisableTiming));
@@ -171,7 +213,8 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     autotune_ax<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(), el->lam1.get(),
                         el->poisson, s);
     autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                              el->lam1.get(), el->poisson, el->maskBits.get(), el->ogs, s);
+                              el->lam1.get(), el->poisson, el->maskBits.get(),
+                              el->ogsOp ? el->ogsOp : el->ogs, s, el->ogsOp ? el->fx.get() : nullptr);
   }
 
   // workspace
@@ -229,8 +272,18 @@ int operator_chunks(int N, int dtype, int64_t E) { return tuned(TUNE_OP, N, dtyp
 template <typename T>
 void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
-                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record)
+                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record, const uint8_t *fx)
 {
+  auto ax = [&](int64_t a, int64_t n) {
+    const int Np = (N + 1) * (N + 1) * (N + 1), NW = (Np + 31) / 32;
+    const uint32_t *mb = maskBits ? maskBits + a * NW : nullptr;
+    if (fx)
+      launch_ax_fused<T>(N, (int)n, ggeo + a * 7 * Np, D, l0, l1, poisson, q + a * Np, Aq + a * Np, mb,
+                         fx + a, s);
+    else
+      launch_ax<T>(N, (int)n, nullptr, ggeo + a * 7 * Np, D, l0, l1, poisson, false, q + a * Np,
+                   Aq + a * Np, mb, s);
+  };
   constexpr int F = nrs_elliptic::kOpChunks;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   record = record && el->profiling;
@@ -240,20 +293,19 @@ void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
   }
   const int64_t L = (int64_t)E * (N + 1) * (N + 1) * (N + 1);
   if (nChunks <= 1 || ogs->nChunks != F || !el->aux) {
-    launch_ax<T>(N, E, nullptr, ggeo, D, l0, l1, poisson, false, q, Aq, maskBits, s);
+    ax(0, E);
     if (record) NRS_HIP(hipEventRecord(e1, s));
     ogs_gather_scatter<T>(ogs, Aq, 1, L, NRS_ADD, s);
   } else {
     NRS_CHECK(F % nChunks == 0, "operator: chunk count must divide 16");
-    const int Np = (N + 1) * (N + 1) * (N + 1), NW = (Np + 31) / 32;
+    const int Np = (N + 1) * (N + 1) * (N + 1);
     const int64_t epc = ogs->nodesPerChunk / Np;
+    NRS_CHECK(!fx || epc % 4 == 0, "operator: fused chunks must hold whole blocks of four elements");
     const int per = F / nChunks;
     for (int c = 0; c < nChunks; ++c) {
       const int64_t a = std::min<int64_t>(E, (int64_t)c * per * epc);
       const int64_t b = std::min<int64_t>(E, (int64_t)(c + 1) * per * epc);
-      if (b > a)
-        launch_ax<T>(N, (int)(b - a), nullptr, ggeo + a * 7 * Np, D, l0, l1, poisson, false, q + a * Np,
-                     Aq + a * Np, maskBits ? maskBits + a * NW : nullptr, s);
+      if (b > a) ax(a, b - a);
       NRS_HIP(hipEventRecord(el->chunkEv[c], s));
       NRS_HIP(hipStreamWaitEvent(el->aux, el->chunkEv[c], 0));
       ogs_gather_scatter_chunks<T>(ogs, Aq, c * per, (c + 1) * per, el->aux);
@@ -270,21 +322,23 @@ void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
 }
 template void chunked_operator<double>(nrs_elliptic *, int, int, const double *, const double *,
                                        const double *, const double *, bool, const uint32_t *,
-                                       const nrs_ogs *, const double *, double *, int, hipStream_t, bool);
+                                       const nrs_ogs *, const double *, double *, int, hipStream_t, bool,
+                                       const uint8_t *);
 template void chunked_operator<float>(nrs_elliptic *, int, int, const float *, const float *,
                                       const float *, const float *, bool, const uint32_t *,
-                                      const nrs_ogs *, const float *, float *, int, hipStream_t, bool);
+                                      const nrs_ogs *, const float *, float *, int, hipStream_t, bool,
+                                      const uint8_t *);
 
 template <typename T>
 int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
-                      hipStream_t s)
+                      hipStream_t s, const uint8_t *fx)
 {
   const int64_t L = (int64_t)E * (N + 1) * (N + 1) * (N + 1);
   DevBuf<T> q(L), ref(L), out(L);
   fill_pattern<T>(L, q.get(), s);
   auto run = [&](int k, T *o) {
-    chunked_operator<T>(el, N, E, ggeo, D, l0, l1, poisson, maskBits, ogs, q.get(), o, k, s, false);
+    chunked_operator<T>(el, N, E, ggeo, D, l0, l1, poisson, maskBits, ogs, q.get(), o, k, s, false, fx);
   };
   run(1, ref.get());
   std::vector<T> hr(L), ho(L);
@@ -308,10 +362,10 @@ int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
 }
 template int autotune_operator<double>(nrs_elliptic *, int, int, const double *, const double *,
                                        const double *, const double *, bool, const uint32_t *,
-                                       const nrs_ogs *, hipStream_t);
+                                       const nrs_ogs *, hipStream_t, const uint8_t *);
 template int autotune_operator<float>(nrs_elliptic *, int, int, const float *, const float *,
                                       const float *, const float *, bool, const uint32_t *,
-                                      const nrs_ogs *, hipStream_t);
+                                      const nrs_ogs *, hipStream_t, const uint8_t *);
 
 // element part of the operator, all fields (no mask, no gather-scatter); three
 // fields go through the block kernel, which reads the geometric factors once
@@ -344,7 +398,8 @@ void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool maske
   }
   chunked_operator<double>(el, el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
                            el->lam1.get(), el->poisson, masked ? el->maskBits.get() : nullptr,
-                           el->ogs, q, Aq, operator_chunks(el->N, NRS_F64, el->E), s, true);
+                           el->ogsOp ? el->ogsOp : el->ogs, q, Aq, operator_chunks(el->N, NRS_F64, el->E),
+                           s, true, el->ogsOp ? el->fx.get() : nullptr);
 }
 
 void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s)

@@ -26,6 +26,12 @@ struct nrs_elliptic {
 
   nrs_ogs *ogsFull = nullptr;        // unmasked (mesh->ogs)
   nrs_ogs *ogs = nullptr;            // masked   (elliptic->ogs)
+  // Operator face fusion (N = 7): fx[e] = 1 when elements e, e+1 (same block
+  // of four) share face i = N / i = 0 node for node; their face-interior nodes
+  // are summed inside Ax and left out of ogsOp, the operator's handle.
+  nrs_ogs *ogsOp = nullptr;
+  nrs::DevBuf<uint8_t> fx;
+  int64_t fusedPairs = 0;
   int64_t Nmasked = 0;
   nrs::DevBuf<int> maskIds;
   nrs::DevBuf<uint32_t> maskBits;    // [E][ceil(Np/32)]
@@ -97,17 +103,24 @@ void batch_set_mode(nrs_batch *b, int mode);
 // been chunked into kOpChunks chunks of ceil(E / kOpChunks) elements); nChunks
 // = 1 or an unchunked handle runs Ax then gs.  Same results in every form.
 // record: add the Ax / gs spans to the handle's device timers when enabled.
+// fx: the face-fusion flags (launch_ax_fused; ogs then is the reduced handle) or null.
 template <typename T>
 void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
-                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record = false);
+                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record = false,
+                      const uint8_t *fx = nullptr);
 // the operator's chunk count for (N, type, E): tuned, else 1
 int operator_chunks(int N, int dtype, int64_t E);
 // time nChunks = 1, 2, 4, 8, 16 on this operator, record the fastest
 template <typename T>
 int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
-                      hipStream_t s);
+                      hipStream_t s, const uint8_t *fx = nullptr);
+// Face fusion of an order-7 operator: from the masked ids [E*512] find the
+// pairs (e, e+1), e % 4 != 3, that share face i = N / i = 0 node for node,
+// return their flags [E] and the ids with those face-interior nodes zeroed (the
+// operator's gather-scatter ids).  Empty flags when no pair qualifies.
+std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids);
 
 // Shared helpers for the multigrid levels.
 std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds);

@@ -25,6 +25,11 @@ void launch_build_diag(int N, int E, const T *ggeo, const T *D, const T *l0, con
                        bool poisson, bool lfield, T *diag, hipStream_t s);
 template <typename T>
 void launch_mask(int64_t Nm, const int *ids, T *v, hipStream_t s);
+// Ax whose consecutive elements e, e+1 with fx[e] = 1 sum their shared face i = N /
+// i = 0 interior nodes in the kernel (N = 7); blocks are 4 elements from q's start
+template <typename T>
+void launch_ax_fused(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1, bool poisson,
+                     const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s);
 
 // mesh.hip
 void launch_geofacs(int N, int E, const double *D, const double *w, const double *x,

@@ -1,0 +1,80 @@
+"""The operator face fusion (N = 7: the shared face of consecutive elements
+meeting i = N to i = 0 summed inside the Ax kernel, those nodes left out of the
+operator's gather-scatter) gives the unfused operator bit for bit: masked and
+unmasked, every chunk variant, on the structured box, on a box whose element
+row length leaves partial blocks, and on a .re2 mesh with rotated elements
+(where only identically oriented faces fuse); solves match iteration for
+iteration and bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(gpu, mesh, **kw):
+    on = gpu.Elliptic(mesh, operatorFusion=1, **kw)
+    off = gpu.Elliptic(mesh, operatorFusion=0, **kw)
+    return on, off
+
+
+@pytest.mark.parametrize("shape", [(8, 3, 2), (6, 2, 3), (5, 5, 2)])
+def test_operator_fused_equals_unfused(gpu, torch_mod, shape):
+    torch = torch_mod
+    m = gpu.BoxMesh(*shape, 7, "kershaw", 0.3)
+    on, off = pair(gpu, m, precon="jacobi")
+    assert off.fused_pairs == 0
+    assert on.fused_pairs > 0
+    print(shape, "fused pairs", on.fused_pairs, flush=True)
+    q = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, m.Nlocal)).cuda()
+    for masked in (True, False):
+        a, b = torch.empty_like(q), torch.empty_like(q)
+        off.operator(q, b, masked=masked)
+        for chunks in (1, 2, 4, 8, 16):
+            gpu.set_variant("op", 7, "f64", m.E, chunks)
+            on.operator(q, a, masked=masked)
+            assert torch.equal(a, b), (masked, chunks)
+        gpu.set_variant("op", 7, "f64", m.E, 0)
+
+
+def test_helmholtz_and_solve_bitwise(gpu, torch_mod):
+    torch = torch_mod
+    m = gpu.BoxMesh(8, 4, 3, 7, "kershaw", 0.25)
+    kw = dict(lambda0=1.0, lambda1=3.0, poisson=False, precon="multigrid", solver="fcg",
+              levels=[7, 3, 1], smoother="chebyshev+jacobi", mgPrecision=1, coarse="smooth_jacobi",
+              tol=1e-10, max_iter=200)
+    on, off = pair(gpu, m, **kw)
+    rhs = np.random.default_rng(2).uniform(-1, 1, m.Nlocal)
+    xs, sts = [], []
+    for el in (on, off):
+        x = torch.zeros(m.Nlocal, dtype=torch.float64, device="cuda")
+        sts.append(el.solve(torch.from_numpy(rhs).cuda(), x))
+        xs.append(x)
+    assert sts[0]["iterations"] == sts[1]["iterations"] and sts[0]["converged"]
+    assert torch.equal(xs[0], xs[1])
+
+
+def test_rotated_mesh_partial_fusion(gpu, torch_mod, tmp_path):
+    import oracle.re2 as r2
+    torch = torch_mod
+    c = r2.box_corners(8, 2, 2)
+    rots = r2.rotations()
+    rng = np.random.default_rng(3)
+    for e in range(0, c.shape[0], 3):                # every third element rotated
+        c[e] = c[e][rots[rng.integers(0, 24)]]
+    codes = {}
+    for e in range(c.shape[0]):
+        for f, cs in enumerate(r2.FILE_FACE_CORNERS):
+            p = c[e][list(cs)]
+            if any(np.all(np.abs(p[:, ax] - v) < 1e-12) for ax in range(3) for v in (0.0, 1.0)):
+                codes[(e, f)] = "O  "
+    path = tmp_path / "part.re2"
+    r2.write(path, c, codes)
+    m = gpu.Re2Mesh(path, 7)
+    on, off = pair(gpu, m, precon="jacobi")
+    full = gpu.Elliptic(gpu.BoxMesh(8, 2, 2, 7), precon="jacobi", operatorFusion=1)
+    assert 0 < on.fused_pairs < full.fused_pairs
+    q = torch.from_numpy(np.random.default_rng(4).uniform(-1, 1, m.Nlocal)).cuda()
+    a, b = torch.empty_like(q), torch.empty_like(q)
+    on.operator(q, a)
+    off.operator(q, b)
+    assert torch.equal(a, b)
