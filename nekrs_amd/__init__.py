@@ -261,12 +261,15 @@ BC = {"interior": 0, "wall": 1, "interface": 2, "inlet": 3, "symx": 4, "symy": 5
 class PressureRhs:
     """The pressure right-hand side of the TOMBO splitting (tombo::pressureSolve,
     src/navierStokes/tombo.cpp:11-121) on a mesh; bc_types: [E*6] face types
-    (BC ids; default: every boundary face of a box mesh an inlet, 0 inside)."""
+    (BC ids; default: the mesh's own bc_types when it has them (Re2Mesh), else every
+    boundary face an inlet, 0 inside)."""
 
     def __init__(self, mesh, bc_types=None):
         self.mesh = mesh
         self.N, self.E = mesh.N, mesh.E
         self.Nlocal = mesh.E * (mesh.N + 1) ** 3
+        if bc_types is None:
+            bc_types = getattr(mesh, "bc_types", None)       # a Re2Mesh carries its own
         if bc_types is None:
             bc_types = np.where(np.asarray(mesh.EToB) != 0, BC["inlet"], 0)
         self.bc_types = np.ascontiguousarray(bc_types, dtype=np.int32).reshape(-1)
