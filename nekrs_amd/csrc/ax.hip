@@ -31,6 +31,22 @@ template <typename T> __device__ __forceinline__ T stream_load(const T *p)
   return __builtin_nontemporal_load(p);
 }
 
+// Synchronisation between the stages of one plane.  When an element is exactly
+// one wave64 (NQ = 8) its LDS slabs are private to that wave, and a wave's LDS
+// operations complete in issue order, so a wave-level barrier (compiler fence,
+// as HIP's __syncwarp) is enough; other orders share or split waves and use
+// the workgroup barrier.
+template <int NQ> __device__ __forceinline__ void plane_sync()
+{
+  if constexpr (NQ * NQ == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <int NQ, int MULT = 1> struct AxShape {
   // elements per block: fill at least one wave64, times the tuned multiplier
   static constexpr int NE = ((NQ * NQ >= 64) ? 1 : (64 + NQ * NQ - 1) / (NQ * NQ)) * MULT;
@@ -125,7 +141,7 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
     T qt = 0;
 #pragma unroll
     for (int m = 0; m < NQ; ++m) qt += Dg[k * NQ + m] * r_q[m];
-    __syncthreads();
+    plane_sync<NQ>();
 
     T qr = 0, qs = 0;
 #pragma unroll
@@ -139,7 +155,7 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
     s_Gr[le][j][i] = l0 * Gqr;
     s_Gs[le][j][i] = l0 * Gqs;
     Gqt *= l0;
-    __syncthreads();
+    plane_sync<NQ>();
 
     T a = 0;
 #pragma unroll
