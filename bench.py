@@ -479,6 +479,22 @@ def main():
     log("step %.4f ms (%.2f GDOF/s); Ax %.4f ms (%.0f GB/s), gs %.4f ms" %
         (ms_step, gdofs, ax_ms, achieved, gs_ms))
 
+    # the device copy bandwidth on this box (512 MiB read + 512 MiB written per copy):
+    # the practical ceiling a streaming kernel can reach, reported beside the nominal peak
+    ca = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    cb = torch.empty_like(ca)
+    for _ in range(3):
+        cb.copy_(ca)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(20):
+        cb.copy_(ca)
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * ca.numel() * 8 / (ev0.elapsed_time(ev1) / 20 * 1e-3) / 1e9
+    del ca, cb
+    log("device copy: %.0f GB/s" % copy_gbs)
+
     Nlocal = el.Nlocal
     fused_pairs = el.fused_pairs
     del el
@@ -542,7 +558,9 @@ def main():
                    "axgs_algorithmic_GBps": (ax_bytes + gs_bytes) / (ms_step * 1e-3) / 1e9,
                    "axgs_roofline_frac": (ax_bytes + gs_bytes) / (ms_step * 1e-3) / 1e9 /
                    PEAK_HBM_GBS,
-                   "operator_face_fusion": fusion},
+                   "operator_face_fusion": fusion,
+                   "device_copy_GBps": copy_gbs,
+                   "ax_frac_of_device_copy": achieved / copy_gbs},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles()},
         "cpu_baseline": cpu,

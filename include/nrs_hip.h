@@ -236,7 +236,8 @@ typedef struct {
                                   * of consecutive elements that meet i = N to
                                   * i = 0 inside the Ax kernel and leave those
                                   * nodes out of the operator's gather-scatter
-                                  * (same results bit for bit); 0: off          */
+                                  * (same results bit for bit); with autotune on
+                                  * it is kept only where it times faster; 0: off */
 } nrs_elliptic_opts;
 
 typedef struct {

@@ -212,9 +212,34 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   if (o->autotune) {
     autotune_ax<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(), el->lam1.get(),
                         el->poisson, s);
-    autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                              el->lam1.get(), el->poisson, el->maskBits.get(),
-                              el->ogsOp ? el->ogsOp : el->ogs, s, el->ogsOp ? el->fx.get() : nullptr);
+    // operator chunks for the unfused form, then -- where the face fusion applies -- for the
+    // fused one; the fusion is kept only where it is faster (the results are the same)
+    auto op_ms = [&](const nrs_ogs *h, const uint8_t *fx, int chunks) {
+      DevBuf<double> q(el->Nlocal), out(el->Nlocal);
+      fill_pattern<double>(el->Nlocal, q.get(), s);
+      return time_launches([&] {
+        chunked_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
+                                 el->lam1.get(), el->poisson, el->maskBits.get(), h, q.get(), out.get(),
+                                 chunks, s, false, fx);
+      }, s);
+    };
+    const int cUnf = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
+                                               el->lam0.get(), el->lam1.get(), el->poisson,
+                                               el->maskBits.get(), el->ogs, s);
+    if (el->ogsOp) {
+      const double tUnf = op_ms(el->ogs, nullptr, cUnf);
+      const int cFus = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
+                                                 el->lam0.get(), el->lam1.get(), el->poisson,
+                                                 el->maskBits.get(), el->ogsOp, s, el->fx.get());
+      const double tFus = op_ms(el->ogsOp, el->fx.get(), cFus);
+      if (!(tFus < tUnf)) {
+        delete el->ogsOp;
+        el->ogsOp = nullptr;
+        el->fx.release();
+        el->fusedPairs = 0;
+        set_tuned(TUNE_OP, el->N, NRS_F64, el->E, cUnf);
+      }
+    }
   }
 
   // workspace

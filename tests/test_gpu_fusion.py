@@ -12,8 +12,9 @@ pytestmark = pytest.mark.gpu
 
 
 def pair(gpu, mesh, **kw):
-    on = gpu.Elliptic(mesh, operatorFusion=1, **kw)
-    off = gpu.Elliptic(mesh, operatorFusion=0, **kw)
+    """Fused and unfused handles; autotune off, so the fusion is kept whatever it times."""
+    on = gpu.Elliptic(mesh, operatorFusion=1, autotune=0, **kw)
+    off = gpu.Elliptic(mesh, operatorFusion=0, autotune=0, **kw)
     return on, off
 
 
@@ -71,10 +72,25 @@ def test_rotated_mesh_partial_fusion(gpu, torch_mod, tmp_path):
     r2.write(path, c, codes)
     m = gpu.Re2Mesh(path, 7)
     on, off = pair(gpu, m, precon="jacobi")
-    full = gpu.Elliptic(gpu.BoxMesh(8, 2, 2, 7), precon="jacobi", operatorFusion=1)
+    full = gpu.Elliptic(gpu.BoxMesh(8, 2, 2, 7), precon="jacobi", operatorFusion=1, autotune=0)
     assert 0 < on.fused_pairs < full.fused_pairs
     q = torch.from_numpy(np.random.default_rng(4).uniform(-1, 1, m.Nlocal)).cuda()
     a, b = torch.empty_like(q), torch.empty_like(q)
     on.operator(q, a)
+    off.operator(q, b)
+    assert torch.equal(a, b)
+
+
+def test_autotune_keeps_fusion_only_when_faster(gpu, torch_mod):
+    """With autotune on, the setup times both forms; whichever it keeps, the operator
+    equals the unfused one bit for bit."""
+    torch = torch_mod
+    m = gpu.BoxMesh(8, 4, 4, 7, "kershaw", 0.3)
+    tuned = gpu.Elliptic(m, precon="jacobi", operatorFusion=1, autotune=1)
+    off = gpu.Elliptic(m, precon="jacobi", operatorFusion=0, autotune=0)
+    print("fused pairs kept:", tuned.fused_pairs, flush=True)
+    q = torch.from_numpy(np.random.default_rng(5).uniform(-1, 1, m.Nlocal)).cuda()
+    a, b = torch.empty_like(q), torch.empty_like(q)
+    tuned.operator(q, a)
     off.operator(q, b)
     assert torch.equal(a, b)
