@@ -251,8 +251,22 @@ def pressure_rhs_case(mesh, N, steps, warmup, seed=0):
     log("pressure rhs (%d elements N=%d, channel): %.4f ms per call, %.2f GB algorithmic, "
         "%.0f GB/s (%.1f %% of HBM peak)" % (mesh.E, N, ms, alg / 1e9, gbs, 100 * gbs / PEAK_HBM_GBS))
     variants = {k: nekrs_amd.tuned_variant("tombo", N, k, mesh.E) for k in ("curl", "rhs")}
+    fused = pr.fused_pairs
+    # the same calls without the face fusion, on the same box
+    pr0 = nekrs_amd.PressureRhs(mesh, bc, fusion=False)
+    for _ in range(warmup):
+        pr0.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
+    a.record()
+    for _ in range(steps):
+        pr0.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
+    b.record()
+    torch.cuda.synchronize()
+    ms0 = a.elapsed_time(b) / steps
+    del pr0
+    log("pressure rhs: %d face pairs fused; without the fusion %.4f ms per call" % (fused, ms0))
     return {"ms_per_call": ms, "setup_seconds": setup, "algorithmic_bytes": alg,
-            "load_schedule_variants": variants,
+            "load_schedule_variants": variants, "fused_face_pairs": fused,
+            "unfused_ms_per_call": ms0,
             "achieved_GBps": gbs, "roofline_frac": gbs / PEAK_HBM_GBS, "steps": steps,
             "flux_elements": int(flux.any(1).sum()),
             "workload": "tombo::pressureSolve right-hand side (viscous term on), %d hex elements "

@@ -418,8 +418,13 @@ typedef struct nrs_tombo nrs_tombo;
  * with the ids of src/bdry/bcType.h (1 W, 2 INT, 3 V, 4-7 SYM*, 16 O, ...;
  * 0 = interior face).  Replaces the mesh and boundary setup those kernels read
  * (meshSetup.cpp, meshSurfaceGeometricFactorsHex3D.cpp, bcMap). */
+enum nrs_tombo_flags {
+  NRS_TOMBO_FUSION = 1,          /* N = 7: faces shared by consecutive elements summed in the
+                                  * curl kernels (bit-identical), gather-scatters without them */
+  NRS_TOMBO_AUTOTUNE = 2         /* time the load schedules, keep the fusion only if faster  */
+};
 int nrs_tombo_setup(int N, int Nelements, const double *x, const double *y, const double *z,
-                    const long long *globalIds, const int *bcType, nrs_tombo **out);
+                    const long long *globalIds, const int *bcType, int flags, nrs_tombo **out);
 int nrs_tombo_free(nrs_tombo *h);
 
 /* rhs = the pressure Poisson right-hand side of tombo::pressureSolve
@@ -433,6 +438,10 @@ int nrs_tombo_pressure_rhs(nrs_tombo *h, int64_t fieldOffset, const double *Ue, 
                            const double *BF, const double *mue, const double *irho,
                            const double *div, double g0idt, int viscContribution, double *rhs,
                            void *stream);
+
+/* Face pairs summed in the curl kernels (0: fusion off or dropped by the
+ * autotuner) and elements with a flux face. */
+int nrs_tombo_info(nrs_tombo *h, int64_t *fusedPairs, int64_t *fluxElements);
 
 /* W = JW curl(U) on [3][fieldOffset] device fields (curlHex3D.okl, scaleJW = 1). */
 int nrs_tombo_curl(nrs_tombo *h, int64_t fieldOffset, const double *U, double *W, void *stream);

@@ -264,7 +264,7 @@ class PressureRhs:
     (BC ids; default: the mesh's own bc_types when it has them (Re2Mesh), else every
     boundary face an inlet, 0 inside)."""
 
-    def __init__(self, mesh, bc_types=None):
+    def __init__(self, mesh, bc_types=None, fusion=True, autotune=True):
         self.mesh = mesh
         self.N, self.E = mesh.N, mesh.E
         self.Nlocal = mesh.E * (mesh.N + 1) ** 3
@@ -275,8 +275,10 @@ class PressureRhs:
         self.bc_types = np.ascontiguousarray(bc_types, dtype=np.int32).reshape(-1)
         assert self.bc_types.size == 6 * mesh.E
         h = C.c_void_p()
+        flags = (1 if fusion else 0) | (2 if autotune else 0)
         _lib.call("nrs_tombo_setup", mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
-                  mesh.z.ctypes.data, mesh.gid.ctypes.data, self.bc_types.ctypes.data, C.byref(h))
+                  mesh.z.ctypes.data, mesh.gid.ctypes.data, self.bc_types.ctypes.data, flags,
+                  C.byref(h))
         self._h = h
 
     def __del__(self):
@@ -287,6 +289,13 @@ class PressureRhs:
             except Exception:
                 pass
             self._h = None
+
+    @property
+    def fused_pairs(self):
+        """Face pairs the curl kernels sum (0: fusion off or not kept by the autotuner)."""
+        a, b = C.c_int64(), C.c_int64()
+        _lib.call("nrs_tombo_info", self._h, C.byref(a), C.byref(b))
+        return a.value
 
     def _copy(self, which):
         torch = _torch()

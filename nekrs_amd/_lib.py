@@ -93,10 +93,11 @@ SIGNATURES = {
     "nrs_set_variant": ([_i, _i, _i, _i64, _i], _i),
     "nrs_elliptic_profile_enable": ([_vp, _i], _i),
     "nrs_elliptic_profile_read": ([_vp, C.POINTER(_d), C.POINTER(_i64)], _i),
-    "nrs_tombo_setup": ([_i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)], _i),
+    "nrs_tombo_setup": ([_i, _i, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)], _i),
     "nrs_tombo_free": ([_vp], _i),
     "nrs_tombo_pressure_rhs": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp], _i),
     "nrs_tombo_curl": ([_vp, _i64, _vp, _vp, _vp], _i),
+    "nrs_tombo_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64)], _i),
     "nrs_tombo_buffer": ([_vp, _i, C.POINTER(_vp), C.POINTER(_i64)], _i),
     "nrs_tombo_copy_buffer": ([_vp, _i, _vp, _i64, _vp], _i),
     "nrs_re2_info": ([C.c_char_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),

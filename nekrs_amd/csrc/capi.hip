@@ -536,12 +536,12 @@ int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats
 }  // extern "C"
 
 int nrs_tombo_setup(int N, int Nelements, const double *x, const double *y, const double *z,
-                    const long long *globalIds, const int *bcType, nrs_tombo **out)
+                    const long long *globalIds, const int *bcType, int flags, nrs_tombo **out)
 {
   NRS_API_BEGIN
   NRS_CHECK(out != nullptr, "tombo setup: null output handle");
   *out = nullptr;
-  *out = nrs::tombo_setup(N, Nelements, x, y, z, globalIds, bcType);
+  *out = nrs::tombo_setup(N, Nelements, x, y, z, globalIds, bcType, flags);
   NRS_API_END
 }
 
@@ -636,5 +636,14 @@ int nrs_elliptic_fusion_info(nrs_elliptic *el, int64_t *fusedPairs)
   NRS_API_BEGIN
   NRS_CHECK(el && fusedPairs, "fusion info: null argument");
   *fusedPairs = el->fusedPairs;
+  NRS_API_END
+}
+
+int nrs_tombo_info(nrs_tombo *h, int64_t *fusedPairs, int64_t *fluxElements)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(h, "tombo info: null handle");
+  if (fusedPairs) *fusedPairs = h->fusedPairs;
+  if (fluxElements) *fluxElements = h->Nb;
   NRS_API_END
 }

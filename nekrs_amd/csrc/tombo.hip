@@ -15,6 +15,7 @@
 #include <cmath>
 #include <memory>
 
+#include "elliptic.hpp"
 #include "mesh.hpp"
 #include "tombo.hpp"
 #include "tune.hpp"
@@ -22,6 +23,7 @@
 nrs_tombo::~nrs_tombo()
 {
   if (ogs) delete ogs;
+  if (ogsOp) delete ogsOp;
 }
 
 namespace nrs {
@@ -102,97 +104,145 @@ vgeo_kernel(int E, const double *__restrict__ Dg, const double *__restrict__ wg,
 // JW curl(s U) (curlHex3D.okl), s = invLMM when PRESCALE; with RHS also
 // JW grad(div) (gradientVolumeHex3D.okl) and the combination of
 // pressureRhsHex3D.okl: out = BF - mue irho (JW curl - 4/3 JW grad(div)).
-// One block per element, thread (i,j), k-pencils of the fields in registers.
-template <int NQ, bool PRESCALE, bool RHS, bool PF>
-__global__ void __launch_bounds__(NQ * NQ)
+// Thread (i,j) per element, k-pencils of the fields in registers.  FX (N = 7):
+// four elements per block, and the interior nodes of a face i = N / i = 0 shared
+// by consecutive elements with fx[e] = 1 get the pair sum here (the lower
+// element's value first, as the gather-scatter adds them), so the following
+// gather-scatter leaves them out.
+template <int NQ, bool PRESCALE, bool RHS, bool PF, bool FX = false>
+__global__ void __launch_bounds__(NQ * NQ * (FX ? 4 : 1))
 curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ Dg, int64_t offIn,
             int64_t offOut, const double *__restrict__ invLMM, const double *__restrict__ U,
             const double *__restrict__ div, const double *__restrict__ BF, int64_t offBF,
-            const double *__restrict__ mue, const double *__restrict__ irho, double *__restrict__ out)
+            const double *__restrict__ mue, const double *__restrict__ irho, double *__restrict__ out,
+            const uint8_t *__restrict__ fx = nullptr)
 {
   constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
   constexpr int NF = RHS ? 4 : 3;                 // u, v, w (+ div)
-  __shared__ double s_D[NQ][NQ], s_f[NF][NQ][NQ];
-  const int e = blockIdx.x;
-  const int i = threadIdx.x % NQ, j = threadIdx.x / NQ;
-  s_D[j][i] = Dg[j * NQ + i];
-  const int64_t base = (int64_t)e * NP + j * NQ + i;
-  double pf[NF][NQ];
-#pragma unroll
-  for (int l = 0; l < NQ; ++l) {
-    const int64_t id = base + l * NFP;
-    const double sc = PRESCALE ? invLMM[id] : 1.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) pf[c][l] = PRESCALE ? sc * U[id + c * offIn] : U[id + c * offIn];
-    if (RHS) pf[NF - 1][l] = div[id];
-  }
-  // per-plane inputs, loaded one plane ahead: geometry (and BF, mue, irho)
-  constexpr int NX = RHS ? NV + 5 : NV;
-  const double *Ge = vgeo + (int64_t)e * NV * NP + j * NQ + i;
-  auto load_plane = [&](int k, double (&g)[NX]) {
-#pragma unroll
-    for (int f = 0; f < NV; ++f) g[f] = __builtin_nontemporal_load(Ge + f * NP + k * NFP);
-    if (RHS) {
-      const int64_t id = base + k * NFP;
-      g[NV + 0] = BF[id];
-      g[NV + 1] = BF[id + offBF];
-      g[NV + 2] = BF[id + 2 * offBF];
-      g[NV + 3] = mue[id];
-      g[NV + 4] = irho[id];
+  constexpr int NE = FX ? 4 : 1;
+  static_assert(!FX || NQ * NQ == 64, "face fusion: one element per wave64");
+  __shared__ double s_D[NQ][NQ], s_f[NE][NF][NQ][NQ];
+  __shared__ double s_x[FX ? NE : 1][2][3][NQ][NQ];   // face pencils [element][i = 0, N][field][j][k]
+  const int t = threadIdx.x;
+  const int le = FX ? t / NFP : 0;
+  const int i = (t % NFP) % NQ, j = (t % NFP) / NQ;
+  const int64_t e = (int64_t)blockIdx.x * NE + le;
+  const bool active = e < E;
+  if (le == 0) s_D[j][i] = Dg[j * NQ + i];
+  __syncthreads();
+  auto sync_plane = [&]() {
+    if constexpr (FX) {            // each element is one wave with its own slabs
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
     }
   };
-  double gc[NX], gn[NX];
-  if (PF) load_plane(0, gc);
+  const bool faceLane = FX && (i == 0 || i == NQ - 1) && j > 0 && j < NQ - 1;
+  const bool withNext = faceLane && active && i == NQ - 1 && le + 1 < NE && e + 1 < E && fx[e];
+  const bool withPrev = faceLane && active && i == 0 && le >= 1 && fx[e - 1];
+  const int64_t base = e * NP + j * NQ + i;
+  if (active) {
+    double pf[NF][NQ];
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) {
-    if (PF && k + 1 < NQ) load_plane(k + 1, gn);
-    __syncthreads();
+    for (int l = 0; l < NQ; ++l) {
+      const int64_t id = base + l * NFP;
+      const double sc = PRESCALE ? invLMM[id] : 1.0;
 #pragma unroll
-    for (int c = 0; c < NF; ++c) s_f[c][j][i] = pf[c][k];
-    __syncthreads();
-    double dr[NF], ds[NF], dt[NF];
+      for (int c = 0; c < 3; ++c) pf[c][l] = PRESCALE ? sc * U[id + c * offIn] : U[id + c * offIn];
+      if (RHS) pf[NF - 1][l] = div[id];
+    }
+    // per-plane inputs: geometry (and BF, mue, irho), at use or one plane ahead
+    constexpr int NX = RHS ? NV + 5 : NV;
+    const double *Ge = vgeo + e * NV * NP + j * NQ + i;
+    auto load_plane = [&](int k, double (&g)[NX]) {
 #pragma unroll
-    for (int c = 0; c < NF; ++c) { dr[c] = 0.0; ds[c] = 0.0; dt[c] = 0.0; }
+      for (int f = 0; f < NV; ++f) g[f] = __builtin_nontemporal_load(Ge + f * NP + k * NFP);
+      if (RHS) {
+        const int64_t id = base + k * NFP;
+        g[NV + 0] = BF[id];
+        g[NV + 1] = BF[id + offBF];
+        g[NV + 2] = BF[id + 2 * offBF];
+        g[NV + 3] = mue[id];
+        g[NV + 4] = irho[id];
+      }
+    };
+    double gc[NX], gn[NX];
+    if (PF) load_plane(0, gc);
 #pragma unroll
-    for (int n = 0; n < NQ; ++n) {
-      const double Dr = s_D[i][n], Ds = s_D[j][n], Dt = s_D[k][n];
+    for (int k = 0; k < NQ; ++k) {
+      if (PF && k + 1 < NQ) load_plane(k + 1, gn);
+      sync_plane();
 #pragma unroll
-      for (int c = 0; c < NF; ++c) {
-        dr[c] += Dr * s_f[c][j][n];
-        ds[c] += Ds * s_f[c][n][i];
-        dt[c] += Dt * pf[c][n];
+      for (int c = 0; c < NF; ++c) s_f[le][c][j][i] = pf[c][k];
+      sync_plane();
+      double dr[NF], ds[NF], dt[NF];
+#pragma unroll
+      for (int c = 0; c < NF; ++c) { dr[c] = 0.0; ds[c] = 0.0; dt[c] = 0.0; }
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) {
+        const double Dr = s_D[i][n], Ds = s_D[j][n], Dt = s_D[k][n];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+          dr[c] += Dr * s_f[le][c][j][n];
+          ds[c] += Ds * s_f[le][c][n][i];
+          dt[c] += Dt * pf[c][n];
+        }
+      }
+      const int64_t id = base + k * NFP;
+      if (!PF) load_plane(k, gc);
+      const double rx = gc[0], ry = gc[1], rz = gc[2];
+      const double sx = gc[3], sy = gc[4], sz = gc[5];
+      const double tx = gc[6], ty = gc[7], tz = gc[8];
+      const double JW = gc[9];
+      const double uy = ry * dr[0] + sy * ds[0] + ty * dt[0];
+      const double uz = rz * dr[0] + sz * ds[0] + tz * dt[0];
+      const double vx = rx * dr[1] + sx * ds[1] + tx * dt[1];
+      const double vz = rz * dr[1] + sz * ds[1] + tz * dt[1];
+      const double wx = rx * dr[2] + sx * ds[2] + tx * dt[2];
+      const double wy = ry * dr[2] + sy * ds[2] + ty * dt[2];
+      double c0 = (wy - vz) * JW, c1 = (uz - wx) * JW, c2 = (vx - uy) * JW;
+      if (RHS) {
+        constexpr int P = NF - 1;
+        const double gx = JW * (rx * dr[P] + sx * ds[P] + tx * dt[P]);
+        const double gy = JW * (ry * dr[P] + sy * ds[P] + ty * dt[P]);
+        const double gz = JW * (rz * dr[P] + sz * ds[P] + tz * dt[P]);
+        const double factor = 4. / 3;
+        const double nu = gc[NV + 3] * gc[NV + 4];
+        c0 = gc[NV + 0] - nu * (c0 - factor * gx);
+        c1 = gc[NV + 1] - nu * (c1 - factor * gy);
+        c2 = gc[NV + 2] - nu * (c2 - factor * gz);
+      }
+      const bool inner = k > 0 && k < NQ - 1;
+      if (faceLane && inner) {
+        const int side = (i == 0) ? 0 : 1;
+        s_x[le][side][0][j][k] = c0;
+        s_x[le][side][1][j][k] = c1;
+        s_x[le][side][2][j][k] = c2;
+      }
+      if (!((withNext || withPrev) && inner)) {
+        __builtin_nontemporal_store(c0, out + id);
+        __builtin_nontemporal_store(c1, out + id + offOut);
+        __builtin_nontemporal_store(c2, out + id + 2 * offOut);
+      }
+      if (PF && k + 1 < NQ) {
+#pragma unroll
+        for (int f = 0; f < NX; ++f) gc[f] = gn[f];
       }
     }
-    const int64_t id = base + k * NFP;
-    if (!PF) load_plane(k, gc);
-    const double rx = gc[0], ry = gc[1], rz = gc[2];
-    const double sx = gc[3], sy = gc[4], sz = gc[5];
-    const double tx = gc[6], ty = gc[7], tz = gc[8];
-    const double JW = gc[9];
-    const double uy = ry * dr[0] + sy * ds[0] + ty * dt[0];
-    const double uz = rz * dr[0] + sz * ds[0] + tz * dt[0];
-    const double vx = rx * dr[1] + sx * ds[1] + tx * dt[1];
-    const double vz = rz * dr[1] + sz * ds[1] + tz * dt[1];
-    const double wx = rx * dr[2] + sx * ds[2] + tx * dt[2];
-    const double wy = ry * dr[2] + sy * ds[2] + ty * dt[2];
-    double c0 = (wy - vz) * JW, c1 = (uz - wx) * JW, c2 = (vx - uy) * JW;
-    if (RHS) {
-      constexpr int P = NF - 1;
-      const double gx = JW * (rx * dr[P] + sx * ds[P] + tx * dt[P]);
-      const double gy = JW * (ry * dr[P] + sy * ds[P] + ty * dt[P]);
-      const double gz = JW * (rz * dr[P] + sz * ds[P] + tz * dt[P]);
-      const double factor = 4. / 3;
-      const double nu = gc[NV + 3] * gc[NV + 4];
-      c0 = gc[NV + 0] - nu * (c0 - factor * gx);
-      c1 = gc[NV + 1] - nu * (c1 - factor * gy);
-      c2 = gc[NV + 2] - nu * (c2 - factor * gz);
-    }
-    __builtin_nontemporal_store(c0, out + id);
-    __builtin_nontemporal_store(c1, out + id + offOut);
-    __builtin_nontemporal_store(c2, out + id + 2 * offOut);
-    if (PF && k + 1 < NQ) {
+  }
+  if constexpr (FX) {
+    __syncthreads();
+    if (withNext || withPrev) {
+      const int lo = withNext ? le : le - 1;      // the lower element of the pair
 #pragma unroll
-      for (int f = 0; f < NX; ++f) gc[f] = gn[f];
+      for (int k = 1; k < NQ - 1; ++k) {
+        const int64_t id = base + k * NFP;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          out[id + c * offOut] = s_x[lo][1][c][j][k] + s_x[lo + 1][0][c][j][k];
+      }
     }
   }
 }
@@ -340,10 +390,21 @@ template <int NQ> struct VgeoL {
 };
 template <int NQ> struct CurlL {
   static void run(const nrs_tombo *t, int64_t offIn, int64_t offOut, bool prescale, bool pf,
-                  const double *U, double *W, hipStream_t s)
+                  const double *U, double *W, hipStream_t s, const uint8_t *fx = nullptr)
   {
-    const dim3 g(t->E), b(NQ * NQ);
     const double *G = t->vgeo.get(), *D = t->D.get(), *il = t->invLMM.get();
+    if constexpr (NQ == 8) {
+      if (fx) {
+        const dim3 g((t->E + 3) / 4), b(256);
+#define CURLF(P, F) curl_kernel<NQ, P, false, F, true><<<g, b, 0, s>>>(t->E, G, D, offIn, offOut, P ? il : nullptr, \
+                                                                       U, nullptr, nullptr, 0, nullptr, nullptr, W, fx)
+        if (prescale) { if (pf) CURLF(true, true); else CURLF(true, false); }
+        else { if (pf) CURLF(false, true); else CURLF(false, false); }
+#undef CURLF
+        return;
+      }
+    }
+    const dim3 g(t->E), b(NQ * NQ);
 #define CURL(P, F) curl_kernel<NQ, P, false, F><<<g, b, 0, s>>>(t->E, G, D, offIn, offOut, P ? il : nullptr, \
                                                                 U, nullptr, nullptr, 0, nullptr, nullptr, W)
     if (prescale) { if (pf) CURL(true, true); else CURL(true, false); }
@@ -353,17 +414,28 @@ template <int NQ> struct CurlL {
 };
 template <int NQ> struct RhsL {
   static void run(const nrs_tombo *t, bool pf, const double *W, const double *div, const double *BF,
-                  int64_t offBF, const double *mue, const double *irho, double *out, hipStream_t s)
+                  int64_t offBF, const double *mue, const double *irho, double *out, hipStream_t s,
+                  const uint8_t *fx = nullptr)
   {
+    const double *G = t->vgeo.get(), *D = t->D.get(), *il = t->invLMM.get();
+    const int64_t L = t->Nlocal;
+    if constexpr (NQ == 8) {
+      if (fx) {
+        const dim3 g((t->E + 3) / 4), b(256);
+        if (pf)
+          curl_kernel<NQ, true, true, true, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue,
+                                                                  irho, out, fx);
+        else
+          curl_kernel<NQ, true, true, false, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue,
+                                                                   irho, out, fx);
+        return;
+      }
+    }
     const dim3 g(t->E), b(NQ * NQ);
     if (pf)
-      curl_kernel<NQ, true, true, true><<<g, b, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
-                                                         t->Nlocal, t->invLMM.get(), W, div, BF, offBF,
-                                                         mue, irho, out);
+      curl_kernel<NQ, true, true, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out);
     else
-      curl_kernel<NQ, true, true, false><<<g, b, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
-                                                          t->Nlocal, t->invLMM.get(), W, div, BF, offBF,
-                                                          mue, irho, out);
+      curl_kernel<NQ, true, true, false><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out);
   }
 };
 template <int NQ> struct WdivL {
@@ -422,8 +494,9 @@ void autotune(nrs_tombo *t, hipStream_t s)
     double *outs[2] = {t->W.get(), out2.get()};
     for (int v = 0; v < 2; ++v) {
       auto f = [&]() {
-        if (type == 0) by_order<CurlL>(t->N, t, L, L, false, v == 1, U.get(), outs[v], s);
-        else by_order<RhsL>(t->N, t, v == 1, U.get(), div.get(), BF.get(), L, mue.get(), irho.get(), outs[v], s);
+        if (type == 0) by_order<CurlL>(t->N, t, L, L, false, v == 1, U.get(), outs[v], s, t->fx.get());
+        else by_order<RhsL>(t->N, t, v == 1, U.get(), div.get(), BF.get(), L, mue.get(), irho.get(), outs[v], s,
+                            t->fx.get());
       };
       ms[v] = time_launches(f, s);
     }
@@ -440,12 +513,31 @@ void autotune(nrs_tombo *t, hipStream_t s)
     const bool same = dmax <= 400 * 2.220446049250313e-16 * amax;   // NaN fails
     set_tuned(TUNE_TOMBO, t->N, type, t->E, (same && ms[1] < ms[0]) ? 2 : 1);
   }
+  // the face fusion stays only where the whole right-hand side times faster with it
+  if (t->ogsOp) {
+    auto run = [&]() {
+      tombo_pressure_rhs(t, L, U.get(), U.get(), BF.get(), mue.get(), irho.get(), div.get(), 1.0, true,
+                         out2.get(), s);
+    };
+    const double withFusion = time_launches(run, s);
+    nrs_ogs *h = t->ogsOp;
+    DevBuf<uint8_t> f = std::move(t->fx);
+    t->ogsOp = nullptr;
+    const double without = time_launches(run, s);
+    if (withFusion < without) {
+      t->ogsOp = h;
+      t->fx = std::move(f);
+    } else {
+      delete h;
+      t->fusedPairs = 0;
+    }
+  }
 }
 
 }  // namespace
 
 nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const double *z,
-                       const long long *gids, const int *bcType)
+                       const long long *gids, const int *bcType, int flags)
 {
   NRS_CHECK(N >= 1 && N <= 9, "tombo setup: N must be 1..9");
   NRS_CHECK(E > 0, "tombo setup: no elements");
@@ -489,6 +581,15 @@ nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const dou
   }
   // invLMM = 1 / gs(JW)   (meshComputeInvLMM.cpp)
   t->ogs = ogs_setup(t->Nlocal, gids);
+  if (flags & NRS_TOMBO_FUSION) {
+    std::vector<long long> ids(gids, gids + t->Nlocal);
+    const std::vector<uint8_t> fxv = face_fusion(N, E, ids);
+    if (!fxv.empty()) {
+      t->ogsOp = ogs_setup(t->Nlocal, ids.data());
+      t->fx.upload(fxv);
+      for (uint8_t f : fxv) t->fusedPairs += f;
+    }
+  }
   t->invLMM.alloc(t->Nlocal);
   const unsigned g = (unsigned)((t->Nlocal + 255) / 256);
   copy_jw_kernel<<<g, 256, 0, s>>>(E, t->Np, t->vgeo.get(), t->invLMM.get());
@@ -498,7 +599,7 @@ nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const dou
   t->W.alloc(3 * (size_t)t->Nlocal);
   t->F.alloc(3 * (size_t)t->Nlocal);
   NRS_HIP(hipStreamSynchronize(s));
-  autotune(t.get(), s);
+  if (flags & NRS_TOMBO_AUTOTUNE) autotune(t.get(), s);
   return t.release();
 }
 
@@ -514,17 +615,20 @@ void tombo_pressure_rhs(nrs_tombo *t, int64_t off, const double *Ue, const doubl
 {
   const int64_t L = t->Nlocal;
   double *W = t->W.get(), *F = t->F.get();
+  // the fused face pairs are summed by the curl kernels: the gather-scatters after
+  // them leave those nodes out (F = BF copied in needs the full handle)
+  const nrs_ogs *gsh = t->ogsOp ? t->ogsOp : t->ogs;
   if (visc) {
     const bool pf0 = tuned(TUNE_TOMBO, t->N, 0, t->E, 1) == 2;
     const bool pf1 = tuned(TUNE_TOMBO, t->N, 1, t->E, 1) == 2;
-    by_order<CurlL>(t->N, t, off, L, false, pf0, Ue, W, s);            // W = JW curl(Ue)
-    ogs_gather_scatter<double>(t->ogs, W, 3, L, NRS_ADD, s);
-    by_order<RhsL>(t->N, t, pf1, W, div, BF, off, mue, irho, F, s);    // F = rhsU
+    by_order<CurlL>(t->N, t, off, L, false, pf0, Ue, W, s, t->fx.get());   // W = JW curl(Ue)
+    ogs_gather_scatter<double>(gsh, W, 3, L, NRS_ADD, s);
+    by_order<RhsL>(t->N, t, pf1, W, div, BF, off, mue, irho, F, s, t->fx.get());   // F = rhsU
   } else {
     for (int c = 0; c < 3; ++c)                                         // F = BF
       NRS_HIP(hipMemcpyAsync(F + c * L, BF + c * off, L * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
-  ogs_gather_scatter<double>(t->ogs, F, 3, L, NRS_ADD, s);
+  ogs_gather_scatter<double>(visc ? gsh : t->ogs, F, 3, L, NRS_ADD, s);
   by_order<WdivL>(t->N, t, F, div, g0idt, rhs, s);
   by_order<SurfL>(t->N, t, g0idt, off, U, rhs, s);
   NRS_LAUNCH_CHECK();

@@ -12,6 +12,11 @@ struct nrs_tombo {
   nrs::DevBuf<double> vgeo;          // [E][10][Np]: rx ry rz sx sy sz tx ty tz JW
   nrs::DevBuf<double> invLMM;        // [Nlocal] 1 / gs(JW)
   nrs_ogs *ogs = nullptr;            // unmasked (nrs->gsh)
+  // N = 7: faces i = N / i = 0 shared by consecutive elements (same block of
+  // four, fx[e] = 1) are summed in the curl kernels; ogsOp leaves them out
+  nrs_ogs *ogsOp = nullptr;
+  nrs::DevBuf<uint8_t> fx;
+  int64_t fusedPairs = 0;
   int Nb = 0;                        // elements with at least one flux face
   nrs::DevBuf<int> bElem, bType;     // [Nb] element, [Nb][6] face boundary type
   nrs::DevBuf<double> bGeo;          // [Nb][6][Nq*Nq][4]: nx ny nz WSJ
@@ -25,7 +30,7 @@ namespace nrs {
 enum { BC_W = 1, BC_INT = 2, BC_V = 3, BC_SYMX = 4, BC_SYMY = 5, BC_SYMZ = 6, BC_SYM = 7 };
 
 nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const double *z,
-                       const long long *gids, const int *bcType);
+                       const long long *gids, const int *bcType, int flags);
 // W[c] = JW * curl(U)[c] (curlHex3D with scaleJW = 1); fields off apart
 void tombo_curl(const nrs_tombo *t, int64_t off, const double *U, double *W, hipStream_t s);
 void tombo_pressure_rhs(nrs_tombo *t, int64_t off, const double *Ue, const double *U,

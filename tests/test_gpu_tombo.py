@@ -143,3 +143,29 @@ def test_load_schedules_agree(gpu, torch_mod, orc):
     gpu.set_variant("tombo", N, "rhs", m.E, 0)
     print("schedules differ by", rel(outs[1], outs[0]), flush=True)
     assert rel(outs[1], outs[0]) <= 400 * np.finfo(np.float64).eps
+
+
+@pytest.mark.parametrize("visc", [True, False])
+def test_face_fusion_bitwise(gpu, torch_mod, orc, visc):
+    """The curl kernels' face fusion (N = 7, consecutive elements of a block of four)
+    gives the unfused right-hand side bit for bit, and matches the oracle."""
+    import oracle.tombo as ot
+    torch = torch_mod
+    N = 7
+    m, mo = meshes(gpu, orc, (8, 3, 2), N)
+    bc = bc_mix(m, 9)
+    on = gpu.PressureRhs(m, bc, fusion=True, autotune=False)
+    off = gpu.PressureRhs(m, bc, fusion=False, autotune=False)
+    L = m.x.size
+    off_ = L + 16
+    rng = np.random.default_rng(9)
+    Ue, U, BF = (rng.uniform(-1, 1, 3 * off_) for _ in range(3))
+    mue, irho = rng.uniform(0.5, 1.5, L), rng.uniform(0.5, 1.5, L)
+    div = rng.uniform(-0.1, 0.1, L)
+    dv = lambda a: torch.from_numpy(a).cuda()
+    args = (dv(Ue), dv(U), dv(BF), dv(mue), dv(irho), dv(div), 80.0, visc)
+    a = on.compute(*args, field_offset=off_).cpu().numpy()
+    b = off.compute(*args, field_offset=off_).cpu().numpy()
+    assert np.array_equal(a, b)
+    g = ot.Geometry(N, m.x, m.y, m.z, m.gid)
+    assert rel(a, g.pressure_rhs(bc, Ue, U, BF, mue, irho, div, 80.0, visc, off_)) <= 1e-11
