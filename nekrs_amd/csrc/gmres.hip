@@ -16,7 +16,7 @@ namespace nrs {
 namespace {
 
 constexpr int GT = 256;      // threads per block
-constexpr int GB = 1024;     // max blocks of a partial-sum pass
+constexpr int GB = 4096;     // max blocks of a partial-sum pass
 
 unsigned ggrid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + GT - 1) / GT, GB)); }
 

@@ -8,7 +8,7 @@
 namespace nrs {
 
 constexpr int kGmresMaxVectors = 32;
-constexpr int kGmresPartials = kGmresMaxVectors * 1024;   // partial sums of one pass
+constexpr int kGmresPartials = kGmresMaxVectors * 4096;   // partial sums of one pass
 
 // y_j = sum_i W_i V_j[i] x_i, j < k
 void gmres_multi_dot(int64_t n, int k, int64_t ld, const double *w, const double *V, const double *x,
