@@ -170,7 +170,10 @@ typedef struct {
 
 /* SOLVER = PCG, PCG+FLEXIBLE, PGMRES, PGMRES+FLEXIBLE (PGMRES.cpp:111-340) */
 enum nrs_solver { NRS_SOLVER_PCG = 0, NRS_SOLVER_FCG = 1, NRS_SOLVER_GMRES = 2, NRS_SOLVER_FGMRES = 3 };
-enum nrs_precon { NRS_PRECON_NONE = 0, NRS_PRECON_JACOBI = 1, NRS_PRECON_MULTIGRID = 2 };
+/* PRECONDITIONER = NONE, JACOBI, MULTIGRID, USER (ellipticPreconditioner.cpp:33-84);
+ * USER calls the function given to nrs_elliptic_set_user_preconditioner. */
+enum nrs_precon { NRS_PRECON_NONE = 0, NRS_PRECON_JACOBI = 1, NRS_PRECON_MULTIGRID = 2,
+                  NRS_PRECON_USER = 3 };
 enum nrs_smoother {
   NRS_SMOOTHER_CHEBYSHEV_JACOBI = 0,  /* DAMPEDJACOBI,CHEBYSHEV (1st kind)       */
   NRS_SMOOTHER_OPT4_CHEBYSHEV_JACOBI = 1,
@@ -285,6 +288,29 @@ int nrs_elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, int mas
 /* z = M^{-1} r.  Replaces ellipticPreconditioner
  * (src/solvers/elliptic/ellipticPreconditioner.cpp:33-84). */
 int nrs_elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, void *stream);
+
+/* The user preconditioner hook, elliptic_t::userPreconditioner (elliptic.h:178),
+ * called for PRECONDITIONER = USER (ellipticPreconditioner.cpp:72-74) in place of
+ * the built-in ones: fn(user, r, z, stream) must enqueue z = M^{-1} r on
+ * `stream` (r and z are device vectors [Nfields][E*Np] of the handle; fn must not
+ * write r).  The solve calls it from the calling host thread, once per iteration
+ * (twice per PGMRES restart for the non-flexible form).  fn = NULL clears the hook;
+ * a USER handle without one fails its solve with NRS_ERR_INVALID. */
+typedef void (*nrs_user_precon_fn)(void *user, const double *r, double *z, void *stream);
+int nrs_elliptic_set_user_preconditioner(nrs_elliptic *el, nrs_user_precon_fn fn, void *user);
+
+/* The Krylov iteration alone, the seam between ellipticSolve and its linear
+ * solver: pcg(elliptic, o_r, o_x, tol, MAXIT, rdotr) (linearSolver/PCG.cpp:85-203,
+ * elliptic.h:195-196) for opts.solver = PCG / FCG, pgmres (PGMRES.cpp:111-340,
+ * elliptic.h:199-200) for GMRES / FGMRES.  r: the masked, assembled residual
+ * (overwritten); x: the increment, updated in place (PGMRES's restarts assume it
+ * starts at zero, as ellipticSolve passes it, ellipticSolve.cpp:105-106); tol:
+ * absolute, on the normalised norm; maxIter >= 1 replaces opts.maxIter; *rdotr:
+ * on entry the normalised initial norm ||r||_w / sqrt(volume), on exit the final
+ * one; *iterations: the count pcg returns.  No projection, no mask of x0.
+ * Returns NRS_OK (also when maxIter is reached) or NRS_ERR_NAN. */
+int nrs_elliptic_krylov(nrs_elliptic *el, double *r, double *x, double tol, int maxIter,
+                        double *rdotr, int *iterations, void *stream);
 
 /* Solve A x = rhs.  rhs is the unassembled right-hand side (the solve applies
  * mask and gather-scatter to rhs - A x0 itself); x holds the initial guess on

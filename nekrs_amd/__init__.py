@@ -28,7 +28,7 @@ OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
 MAPS = {"none": 0, "sine": 1, "kershaw": 2}
 SOLVERS = {"pcg": 0, "fcg": 1, "pcg+flexible": 1, "gmres": 2, "pgmres": 2, "fgmres": 3,
            "pgmres+flexible": 3}
-PRECONS = {"none": 0, "jacobi": 1, "multigrid": 2}
+PRECONS = {"none": 0, "jacobi": 1, "multigrid": 2, "user": 3}
 SMOOTHERS = {"chebyshev+jacobi": 0, "opt4chebyshev+jacobi": 1, "chebyshev+asm": 2,
              "opt4chebyshev+asm": 3, "asm": 4, "ras": 5, "chebyshev+ras": 6,
              "opt4chebyshev+ras": 7}
@@ -66,6 +66,20 @@ def _stream(stream=None):
     torch = _torch()
     s = torch.cuda.current_stream() if stream is None else stream
     return C.c_void_p(s.cuda_stream)
+
+
+class _DeviceArray:
+    """A device buffer owned by the library, for torch.as_tensor (CUDA array
+    interface; torch takes no read-only views, so r is writable but must not be
+    written)."""
+
+    def __init__(self, addr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f8", "version": 2,
+                                         "data": (int(addr), False), "strides": None}
+
+
+def _device_view(addr, n):
+    return _torch().as_tensor(_DeviceArray(addr, n), device="cuda")
 
 
 def init(device=0):
@@ -594,10 +608,62 @@ class Elliptic:
         _lib.call("nrs_elliptic_profile_read", self._h, ms, cnt)
         return {p: (ms[i], cnt[i]) for i, p in enumerate(_lib.PHASES)}
 
+    def set_user_preconditioner(self, fn, tensors=True):
+        """The PRECONDITIONER = USER hook (elliptic_t::userPreconditioner,
+        elliptic.h:178).  fn(r, z, stream) must enqueue z = M^-1 r on `stream`
+        (a torch.cuda.ExternalStream); r and z are float64 device tensors of
+        Nfields * Nlocal entries viewing the solver's own vectors (tensors=True),
+        or their device addresses as ints (tensors=False).  An exception raised
+        by fn is re-raised by the solve that called it.  fn=None clears the hook."""
+        if fn is None:
+            self._user_cb = None
+            _lib.call("nrs_elliptic_set_user_preconditioner", self._h, _lib.USER_PRECON_FN(), None)
+            return
+        torch = _torch()
+        n = getattr(self, "nfields", 1) * self.Nlocal
+        self._user_err = None
+
+        def cb(user, r, z, stream):
+            if self._user_err is not None:
+                return
+            try:
+                # a NULL stream (ctypes gives None) is the device's default stream
+                st = torch.cuda.ExternalStream(stream) if stream else torch.cuda.default_stream()
+                if tensors:
+                    fn(_device_view(r, n), _device_view(z, n), st)
+                else:
+                    fn(r, z, st)
+            except BaseException as e:       # raised again after the C call returns
+                self._user_err = e
+
+        self._user_cb = _lib.USER_PRECON_FN(cb)
+        _lib.call("nrs_elliptic_set_user_preconditioner", self._h, self._user_cb, None)
+
+    def _raise_user_error(self):
+        e = getattr(self, "_user_err", None)
+        if e is not None:
+            self._user_err = None
+            raise RuntimeError("user preconditioner failed") from e
+
+    def krylov(self, r, x, tol, max_iter, rdotr, stream=None):
+        """pcg / pgmres alone (PCG.cpp:85-203, PGMRES.cpp:111-340) on the masked,
+        assembled residual r (overwritten); x += the increment.  tol absolute on
+        the normalised norm, rdotr = ||r||_w / sqrt(volume) on entry.
+        Returns (iterations, final normalised norm)."""
+        res = C.c_double(rdotr)
+        it = C.c_int()
+        rc = _lib.load().nrs_elliptic_krylov(self._h, _ptr(r), _ptr(x), float(tol), int(max_iter),
+                                             C.byref(res), C.byref(it), _stream(stream))
+        self._raise_user_error()
+        _lib.check(rc)
+        return it.value, res.value
+
     def solve(self, rhs, x, stream=None):
         """In place: x <- solution, rhs <- final residual.  Returns a stats dict."""
         st = _lib.SolveStats()
-        _lib.call("nrs_elliptic_solve", self._h, _ptr(rhs), _ptr(x), C.byref(st), _stream(stream))
+        rc = _lib.load().nrs_elliptic_solve(self._h, _ptr(rhs), _ptr(x), C.byref(st), _stream(stream))
+        self._raise_user_error()
+        _lib.check(rc)
         return {"iterations": st.iterations, "converged": bool(st.converged), "res0": st.res0,
                 "res": st.res, "seconds": st.seconds, "precon_seconds": st.precon_seconds,
                 "res00": st.res00, "proj_vectors": st.projVectors}

@@ -50,6 +50,8 @@ class SolveStats(C.Structure):
 
 # exported symbols and their signatures (all return int status unless noted)
 _vp, _i, _i64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_double
+# nrs_user_precon_fn: fn(user, r, z, stream), no return value
+USER_PRECON_FN = C.CFUNCTYPE(None, _vp, _vp, _vp, _vp)
 SIGNATURES = {
     "nrs_last_error": ([], C.c_char_p),
     "nrs_version": ([], C.c_char_p),
@@ -78,6 +80,8 @@ SIGNATURES = {
     "nrs_elliptic_operator": ([_vp, _vp, _vp, _i, _vp], _i),
     "nrs_elliptic_preconditioner": ([_vp, _vp, _vp, _vp], _i),
     "nrs_elliptic_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats), _vp], _i),
+    "nrs_elliptic_set_user_preconditioner": ([_vp, USER_PRECON_FN, _vp], _i),
+    "nrs_elliptic_krylov": ([_vp, _vp, _vp, _d, _i, C.POINTER(_d), C.POINTER(_i), _vp], _i),
     "nrs_elliptic_mg_info": ([_vp, _i, C.POINTER(_i), _vp, _vp, _vp, _vp, C.POINTER(_i)], _i),
     "nrs_elliptic_mg_smoother": ([_vp, _i, _vp, _vp, _vp], _i),
     "nrs_fdm_1d": ([_i, _i, _i, _d, _d, _d, _vp, _vp], _i),

@@ -159,6 +159,8 @@ nrs_batch *batch_setup(int n, const nrs_elliptic_desc *descs, const nrs_elliptic
     NRS_CHECK(descs[p].Nfields <= 1, "batch: one field per problem (a block solve is not device-driven)");
     NRS_CHECK(opts[p].initialGuess == NRS_GUESS_PREVIOUS,
               "batch: the projection initial guess needs host readbacks (not device-driven)");
+    NRS_CHECK(opts[p].preconditioner != NRS_PRECON_USER,
+              "batch: a user preconditioner is a host callback (not device-driven)");
     NRS_CHECK(opts[p].solver == NRS_SOLVER_PCG || opts[p].solver == NRS_SOLVER_FCG,
               "batch: PCG or flexible PCG (PGMRES needs the host for its Hessenberg system)");
     b->probs.push_back(elliptic_setup(&descs[p], &opts[p]));

@@ -1,6 +1,7 @@
 // capi.hip -- the extern "C" boundary of libnekrs_hip (declared in include/nrs_hip.h).
 // Every entry point validates its arguments, runs, and converts exceptions into
 // status codes; nothing here aborts the process.
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -523,6 +524,39 @@ int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats
     NRS_CHECK(el && rhs && x, "solve: null argument");
     const int rc = nrs::elliptic_solve(el, rhs, x, stats, as_stream(stream));
     if (rc != NRS_OK) nrs::set_last_error("solve: NaN residual norm");
+    return rc;
+  } catch (const nrs::Error &e) {
+    nrs::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    nrs::set_last_error(e.what());
+    return NRS_ERR_INVALID;
+  }
+}
+
+int nrs_elliptic_set_user_preconditioner(nrs_elliptic *el, nrs_user_precon_fn fn, void *user)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el != nullptr, "user preconditioner: null handle");
+  el->userPrecon = fn;
+  el->userCtx = fn ? user : nullptr;
+  NRS_API_END
+}
+
+int nrs_elliptic_krylov(nrs_elliptic *el, double *r, double *x, double tol, int maxIter,
+                        double *rdotr, int *iterations, void *stream)
+{
+  try {
+    NRS_CHECK(el && r && x && rdotr, "krylov: null argument");
+    NRS_CHECK(maxIter >= 1, "krylov: maxIter must be >= 1");
+    int iter = 0;
+    double res = *rdotr, precon_s = 0.0;
+    NRS_CHECK(std::isfinite(res) && res >= 0.0, "krylov: the initial norm must be finite and >= 0");
+    const int rc = nrs::elliptic_krylov(el, r, x, res, tol, maxIter, false, iter, res, precon_s,
+                                        as_stream(stream));
+    *rdotr = res;
+    if (iterations) *iterations = iter;
+    if (rc != NRS_OK) nrs::set_last_error("krylov: NaN residual norm");
     return rc;
   } catch (const nrs::Error &e) {
     nrs::set_last_error(e.what());

@@ -128,7 +128,7 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   NRS_CHECK(o->solver >= NRS_SOLVER_PCG && o->solver <= NRS_SOLVER_FGMRES, "unknown solver");
   NRS_CHECK(o->solver < NRS_SOLVER_GMRES || (o->gmresRestart >= 1 && o->gmresRestart <= kGmresMaxVectors),
             "PGMRES restart must be 1..32");
-  NRS_CHECK(o->preconditioner >= NRS_PRECON_NONE && o->preconditioner <= NRS_PRECON_MULTIGRID,
+  NRS_CHECK(o->preconditioner >= NRS_PRECON_NONE && o->preconditioner <= NRS_PRECON_USER,
             "unknown preconditioner");
   NRS_CHECK(o->initialGuess >= NRS_GUESS_PREVIOUS && o->initialGuess <= NRS_GUESS_PROJECTION_ACONJ,
             "unknown initial guess");
@@ -434,6 +434,10 @@ void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipSt
     case NRS_PRECON_MULTIGRID:   // one V-cycle per field (the hierarchy is single-field)
       for (int f = 0; f < el->Nf; ++f) el->mg->apply(r + f * el->Nlocal, z + f * el->Nlocal, s);
       break;
+    case NRS_PRECON_USER:        // ellipticPreconditioner.cpp:72-74
+      NRS_CHECK(el->userPrecon, "PRECONDITIONER = USER without a user preconditioner");
+      el->userPrecon(el->userCtx, r, z, (void *)s);
+      break;
     default:
       NRS_HIP(hipMemcpyAsync(z, r, el->Nf * el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
       break;
@@ -445,8 +449,9 @@ void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipSt
 // rotations and the small triangular solve on the host; one readback of
 // (y, |w|) per iteration.  r: the masked, assembled initial residual; x = 0;
 // tol: absolute on the 1/volume-scaled norm.
-static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, double tol, bool timed,
-                        int &iter, double &rdotr, double &precon_s, int &status, hipStream_t s)
+static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, double tol, int maxIter,
+                        bool timed, int &iter, double &rdotr, double &precon_s, int &status,
+                        hipStream_t s)
 {
   const int64_t L = (int64_t)el->Nf * el->Nlocal;
   const double *wgt = el->Nf > 1 ? el->wRep.get() : el->ogs->invDegree.get();
@@ -491,7 +496,7 @@ static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, dou
     }
   };
   std::vector<double> hy(kGmresMaxVectors + 2);
-  while (iter < o.maxIter) {
+  while (iter < maxIter) {
     sv.assign(m + 1, 0.0);
     sv[0] = nr;
     axpby<double>(L, 1.0 / nr, r, 0.0, V, s);                       // V_0 = r / |r|
@@ -526,7 +531,7 @@ static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, dou
       error = std::fabs(sv[i + 1]) * sqrtRNF;
       rdotr = error;
       if (std::isnan(error)) { status = NRS_ERR_NAN; return; }
-      if (error < tol || iter == o.maxIter) {
+      if (error < tol || iter == maxIter) {
         update(i + 1);
         used = i + 1;
         done = true;
@@ -550,6 +555,59 @@ static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, dou
   }
 }
 
+// pcg (linearSolver/PCG.cpp:85-203) or pgmres (PGMRES.cpp:111-340): the
+// iteration on the masked, assembled residual r, x += the increment.
+int elliptic_krylov(nrs_elliptic *el, double *r, double *x, double res0, double tol, int maxIter,
+                    bool timed, int &iter, double &rdotr, double &precon_s, hipStream_t s)
+{
+  const int64_t L = (int64_t)el->Nf * el->Nlocal;
+  const double *w = el->Nf > 1 ? el->wRep.get() : el->ogs->invDegree.get();
+  double *scal = el->scal.get();
+  Reducer &R = *el->red;
+  const auto &o = el->opts;
+  const bool flexible = o.solver == NRS_SOLVER_FCG;
+  const bool none = o.preconditioner == NRS_PRECON_NONE;
+  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI;
+  const double resNormFactor = 1.0 / el->volume;
+  NRS_CHECK(maxIter >= 1, "maxIter must be >= 1");
+  NRS_CHECK(o.preconditioner != NRS_PRECON_USER || el->userPrecon,
+            "PRECONDITIONER = USER without a user preconditioner");
+  iter = 0;
+  rdotr = res0;
+  precon_s = 0.0;
+  int status = NRS_OK;
+  if (o.solver == NRS_SOLVER_GMRES || o.solver == NRS_SOLVER_FGMRES) {
+    gmres_solve(el, r, x, res0, tol, maxIter, timed, iter, rdotr, precon_s, status, s);
+    return status;
+  }
+
+  double *p = el->p.get(), *Ap = el->Ap.get();
+  double *z = none ? r : el->zv.get();
+  fill<double>(L, 0.0, p, s);
+  pcg_init(scal, res0, tol, maxIter, s);
+  for (;;) {
+    if (!none && !jacobi) {
+      auto p0 = std::chrono::steady_clock::now();
+      elliptic_preconditioner(el, r, z, s);
+      if (timed) {  // per-phase timing only when stats are requested
+        NRS_HIP(hipStreamSynchronize(s));
+        precon_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
+      }
+    }
+    pcg_precon_dots(R, L, w, el->invDiagA.get(), r, z, Ap, jacobi, flexible, none, scal, s);
+    pcg_update_p(L, z, p, scal, s);
+    elliptic_operator(el, p, Ap, true, s);
+    pcg_alpha(R, L, w, p, Ap, scal, s);
+    pcg_update_rx(R, L, w, Ap, p, r, x, resNormFactor, scal, s);
+    NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+    NRS_HIP(hipStreamSynchronize(s));
+    rdotr = el->scalHost[S_RDOTR];
+    iter = (int)el->scalHost[S_ITER];
+    if (std::isnan(rdotr)) return NRS_ERR_NAN;
+    if (el->scalHost[S_ACTIVE] == 0.0) return NRS_OK;   // rdotr <= tol or iter == maxIter (PCG.cpp:200)
+  }
+}
+
 // ellipticSolve (src/solvers/elliptic/ellipticSolve.cpp:32-190) + pcg (PCG.cpp:85-203).
 int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s)
 {
@@ -559,10 +617,9 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   double *scal = el->scal.get();
   Reducer &R = *el->red;
   const auto &o = el->opts;
-  const bool flexible = o.solver == NRS_SOLVER_FCG;
-  const bool none = o.preconditioner == NRS_PRECON_NONE;
-  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI;
   const double resNormFactor = 1.0 / el->volume;
+  NRS_CHECK(o.preconditioner != NRS_PRECON_USER || el->userPrecon,
+            "PRECONDITIONER = USER without a user preconditioner");
   auto t0 = std::chrono::steady_clock::now();
   NRS_HIP(hipEventRecord(el->ev0, s));
 
@@ -608,47 +665,10 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
     st->projVectors = projected ? el->pjPrev : 0;
   };
 
-  if (o.solver == NRS_SOLVER_GMRES || o.solver == NRS_SOLVER_FGMRES) {
-    int iter = 0, status = NRS_OK;
-    double rdotr = res0, precon_s = 0.0;
-    gmres_solve(el, r, x, res0, tol, st != nullptr, iter, rdotr, precon_s, status, s);
-    finish(status);
-    NRS_HIP(hipEventRecord(el->ev1, s));
-    NRS_HIP(hipStreamSynchronize(s));
-    fill_stats(iter, rdotr, precon_s);
-    return status;
-  }
-
-  // pcg
-  double *p = el->p.get(), *Ap = el->Ap.get();
-  double *z = none ? r : el->zv.get();
-  fill<double>(L, 0.0, p, s);
-  pcg_init(scal, res0, tol, o.maxIter, s);
-  double rdotr = res0;
   int iter = 0;
-  double precon_s = 0.0;
-  int status = NRS_OK;
-  for (;;) {
-    if (!none && !jacobi) {
-      auto p0 = std::chrono::steady_clock::now();
-      elliptic_preconditioner(el, r, z, s);
-      if (st) {  // per-phase timing only when stats are requested
-        NRS_HIP(hipStreamSynchronize(s));
-        precon_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
-      }
-    }
-    pcg_precon_dots(R, L, w, el->invDiagA.get(), r, z, Ap, jacobi, flexible, none, scal, s);
-    pcg_update_p(L, z, p, scal, s);
-    elliptic_operator(el, p, Ap, true, s);
-    pcg_alpha(R, L, w, p, Ap, scal, s);
-    pcg_update_rx(R, L, w, Ap, p, r, x, resNormFactor, scal, s);
-    NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
-    NRS_HIP(hipStreamSynchronize(s));
-    rdotr = el->scalHost[S_RDOTR];
-    iter = (int)el->scalHost[S_ITER];
-    if (std::isnan(rdotr)) { status = NRS_ERR_NAN; break; }
-    if (el->scalHost[S_ACTIVE] == 0.0) break;     // rdotr <= tol or iter == maxIter (PCG.cpp:200)
-  }
+  double rdotr = res0, precon_s = 0.0;
+  const int status = elliptic_krylov(el, r, x, res0, tol, o.maxIter, st != nullptr, iter, rdotr,
+                                     precon_s, s);
 
   // x += xbar, basis update, x += x0   (:171-186)
   finish(status);

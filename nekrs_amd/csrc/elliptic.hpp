@@ -48,6 +48,10 @@ struct nrs_elliptic {
   int pjPrev = 0;                    // basis size used by the last pre()
   std::vector<double> pjAlphaHost;
 
+  // PRECONDITIONER = USER: elliptic_t::userPreconditioner (elliptic.h:178)
+  nrs_user_precon_fn userPrecon = nullptr;
+  void *userCtx = nullptr;
+
   // workspace (ellipticUpdateWorkspace, elliptic.h:240-262)
   nrs::DevBuf<double> p, zv, Ap, x0;
   nrs::DevBuf<double> scal;          // nrs::S_COUNT device scalars
@@ -83,6 +87,10 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
 void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s);
 void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s);
 int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s);
+// pcg / pgmres on the masked, assembled residual r, x += increment; res0 the
+// normalised initial norm, tol absolute; returns NRS_OK or NRS_ERR_NAN
+int elliptic_krylov(nrs_elliptic *el, double *r, double *x, double res0, double tol, int maxIter,
+                    bool timed, int &iter, double &rdotr, double &precon_s, hipStream_t s);
 // Device-driven pieces of the same solve (no host synchronisation).
 void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t s);
 void elliptic_iterate(nrs_elliptic *el, double *r, double *x, hipStream_t s);
