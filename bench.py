@@ -31,6 +31,12 @@ times 100 steps after 10 warm-up steps, inputs resident in HBM.
   pressure_rhs  the step before the solve: tombo::pressureSolve's right-hand
              side on the same 32^3 mesh (channel boundary faces), ms per call and
              algorithmic GB/s
+  kershaw    the reference's own published benchmarks (examples/kershaw): BPS5,
+             the pressure solve with the reference's defaults (PGMRES+FLEXIBLE,
+             RAS + opt. 4th-kind Chebyshev degree 3, FP32 multigrid) on the 20^3
+             N=7 Kershaw box, with the FDM in FP32 and in BF16, and BP5
+             (unpreconditioned PCG, 1000 iterations), beside the published
+             V100 / A100 / MI250X figures
   batch      64 uneven Helmholtz problems advanced by captured graphs, and the
              3-component velocity Helmholtz solve (batch of 3, and one block solve)
 
@@ -189,6 +195,77 @@ def solve_cases(n, N, seed=0):
     out["workload"] = ("%d^3=%d hex elements N=%d Kershaw eps=0.3, p-MG %d-3-1 with an FP64 coarse "
                        "solve (2 AMG-preconditioned PCG iterations), FCG to 1e-8 relative, rhs "
                        "sine + 1e-3 noise (seed %d)" % (n, n ** 3, N, N, seed))
+    return out
+
+
+# the reference's published Kershaw results at 8000 elements per GPU, N=7
+# (examples/kershaw/README.md:13-57): solve seconds / iterations for BPS5 and
+# (DOF x iter)/s for BP5
+PUBLISHED_KERSHAW = {"V100": {"bps5_s": 0.292781, "bps5_it": 28, "bp5_dof_it_per_s": 2.39e9},
+                     "A100": {"bps5_s": 0.196772, "bps5_it": 28, "bp5_dof_it_per_s": 3.87e9},
+                     "MI250X/1": {"bps5_s": 0.279318, "bps5_it": 28, "bp5_dof_it_per_s": 3.07e9}}
+
+
+def kershaw_cases(N=7, reps=5):
+    """The reference's Kershaw benchmarks (examples/kershaw/kershaw.udf:14-140) on
+    its 20^3 box (kershaw.re2, Dirichlet pressure on every face) with eps = 0.3,
+    rhs = 3 pi^2 sin sin sin at the nodes (kershaw.udf:20-23), x0 = 0, the best of
+    `reps` timed solves after a warm-up, as the udf reports.
+    BPS5: the pressure defaults (kershaw.par, parReader.cpp:836-841): PGMRES+FLEXIBLE
+    restart 15 to 1e-8 relative, p-MG 7-3-1 with RAS + opt. 4th-kind Chebyshev
+    degree 3 in FP32; coarse level: one AMG-preconditioned PCG iteration on the
+    device (the reference: one BoomerAMG V-cycle on the host).  Run with the FDM in
+    FP32 (as the reference) and in BF16 on the matrix cores.
+    BP5: PCG, PRECONDITIONER = NONE, tolerance 1e-15 relative, 1000 iterations at
+    most (kershaw.udf:47-53)."""
+    import numpy as np
+    import torch
+    import nekrs_amd
+    mesh = nekrs_amd.BoxMesh(20, 20, 20, N, "kershaw", 0.3)
+    dofs = mesh.E * N ** 3
+    f = 3 * np.pi ** 2 * np.sin(np.pi * mesh.x) * np.sin(np.pi * mesh.y) * np.sin(np.pi * mesh.z)
+    rhs = torch.from_numpy(f).cuda()
+
+    def timed(el, name):
+        best, st_best = None, None
+        for rep in range(reps + 1):
+            r = rhs.clone()
+            x = torch.zeros_like(r)
+            torch.cuda.synchronize()
+            if rep == reps:
+                ranges().push("nrs:kershaw:" + name)
+            st = el.solve(r, x)
+            if rep == reps:
+                ranges().pop()
+            if rep > 0 and (best is None or st["seconds"] < best):
+                best, st_best = st["seconds"], st
+        return best, st_best
+
+    out = {"workload": "Kershaw BPS5/BP5 as in examples/kershaw: 20^3 = %d hex elements N=%d "
+                       "eps=0.3, rhs 3 pi^2 sin sin sin, x0 = 0, best of %d solves after a "
+                       "warm-up" % (mesh.E, N, reps),
+           "dofs": dofs, "published": PUBLISHED_KERSHAW}
+    bps5 = dict(precon="multigrid", solver="fgmres", gmresRestart=15, tol=1e-8, max_iter=200,
+                levels=[N, 3, 1], smoother="opt4chebyshev+ras", cheby_degree=3, mgPrecision=0,
+                coarse="amg", coarse_tol=0.0, coarse_max_iter=1)
+    for name, fdm in (("bps5_fdm_f32", 0), ("bps5_fdm_bf16", 1)):
+        el = nekrs_amd.Elliptic(mesh, fdmPrecision=fdm, **bps5)
+        t, st = timed(el, name)
+        out[name] = {"seconds": t, "iterations": st["iterations"], "converged": st["converged"],
+                     "precon_seconds": st["precon_seconds"], "dof_per_s": dofs / t,
+                     "dof_iter_per_s": dofs * st["iterations"] / t,
+                     "speedup_vs_published": {k: v["bps5_s"] / t for k, v in PUBLISHED_KERSHAW.items()}}
+        log("Kershaw %s: %d it, %.4f s (A100 published 0.1968 s, 28 it)" %
+            (name, st["iterations"], t))
+        del el
+    el = nekrs_amd.Elliptic(mesh, precon="none", solver="pcg", tol=1e-15, max_iter=1000)
+    t, st = timed(el, "bp5")
+    out["bp5"] = {"seconds": t, "iterations": st["iterations"],
+                  "dof_iter_per_s": dofs * st["iterations"] / t,
+                  "speedup_vs_published": {k: dofs * st["iterations"] / t / v["bp5_dof_it_per_s"]
+                                           for k, v in PUBLISHED_KERSHAW.items()}}
+    log("Kershaw bp5: %d it, %.4f s, %.3g DOF*iter/s (A100 published 3.87e9)" %
+        (st["iterations"], t, dofs * st["iterations"] / t))
     return out
 
 
@@ -430,6 +507,8 @@ def main():
     ap.add_argument("--solve-n", type=int, default=48, help="elements per direction of the solves")
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--no-pressure-rhs", action="store_true")
+    ap.add_argument("--no-kershaw", action="store_true",
+                    help="skip the reference's Kershaw BPS5/BP5 benchmarks (20^3, N=7)")
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--batch-iters-per-launch", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
@@ -534,6 +613,10 @@ def main():
     prhs = None
     if not args.no_pressure_rhs:
         prhs = pressure_rhs_case(mesh, N, 20, 3)
+    kershaw = None
+    if not args.no_kershaw:
+        log("Kershaw BPS5 / BP5 at 20^3 ...")
+        kershaw = kershaw_cases(N)
     solves = None
     if not args.no_solves:
         log("p-MG solves at %d^3 ..." % args.solve_n)
@@ -580,6 +663,7 @@ def main():
         "cpu_baseline": cpu,
         "pressure_rhs": prhs,
         "solves": solves,
+        "kershaw": kershaw,
         "batch": batch,
     }
     print(json.dumps(out), flush=True)
