@@ -241,6 +241,12 @@ typedef struct {
                                   * nodes out of the operator's gather-scatter
                                   * (same results bit for bit); with autotune on
                                   * it is kept only where it times faster; 0: off */
+  int preDegree[8];              /* per multigrid level (0: chebyDegree): Chebyshev
+                                  * degree of the smoothing before restriction
+                                  * (x = 0), DownLegChebyshevDegree               */
+  int postDegree[8];             /* ... after prolongation, UpLegChebyshevDegree
+                                  * (ellipticMultiGridLevel.cpp:146,214); the
+                                  * coarsest level, when smoothed, keeps degree 3 */
 } nrs_elliptic_opts;
 
 typedef struct {
@@ -254,6 +260,25 @@ typedef struct {
                                   * (= res0 without one)                       */
   int projVectors;               /* basis vectors the projection used (0: none) */
 } nrs_solve_stats;
+
+/* MULTIGRID SCHEDULE (the .par key pMGSchedule), e.g.
+ * "p=7+degree=2, p=3+degree=2, p=1, p=3+degree=2, p=7+degree=2": the rules of
+ * parseMultigridSchedule (MG/parseMultigridSchedule.cpp:7-131) and the levels
+ * determineMGLevels takes from it (MG/determineMGLevels.cpp:22-40).  Sets
+ * opts->nLevels and opts->levels (the distinct down-leg orders, finest first),
+ * and preDegree / postDegree as the reference's level setup reads the schedule
+ * (MG/ellipticMultiGridLevelSetup.cpp:160-167): the smoothing before
+ * restriction takes the degree written on the schedule's up leg for that order,
+ * the smoothing after prolongation the down leg's (the reference's assignment,
+ * kept).  defaultDegree fills entries of orders above the coarsest that give
+ * none; N = the mesh order, which the finest entry must equal.  A degree given
+ * for the coarsest order is not used (the coarsest level, when smoothed, keeps
+ * degree 3; the reference's lookup of the absent up-leg entry would read 0).
+ * Errors (NRS_ERR_INVALID, the reasons in nrs_last_error): unknown token,
+ * missing order or degree, different down- and up-leg orders, an order < 1,
+ * finest order != N, more than 8 levels.  As in the reference, a token is an
+ * order when it contains 'p' (so "sweeps=2" reads as order 2).  Host only. */
+int nrs_parse_mg_schedule(const char *schedule, int N, int defaultDegree, nrs_elliptic_opts *opts);
 
 /* Fill opts with the reference's defaults for this path (PCG, Jacobi,
  * tol 1e-8 relative, maxIter 500, Chebyshev factors 0.1/1.1, degree 2). */

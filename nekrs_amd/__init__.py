@@ -19,7 +19,7 @@ import numpy as np
 from . import _lib
 from ._lib import NrsError
 
-__all__ = ["NrsError", "init", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
+__all__ = ["NrsError", "init", "parse_mg_schedule", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
            "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh", "fdm_1d",
            "fdm_element_lengths", "Batch", "make_desc", "make_opts", "tuned_variant", "set_variant",
            "PressureRhs", "BC", "read_re2", "mesh_from_vertices", "Re2Mesh"]
@@ -387,13 +387,25 @@ def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1):
                              int(poisson), float(lambda0), float(lambda1), int(nfields))
 
 
+def parse_mg_schedule(schedule, N, default_degree=3, opts=None):
+    """MULTIGRID SCHEDULE (pMGSchedule), e.g. "p=7+degree=2, p=3+degree=2, p=1,
+    p=3+degree=2, p=7+degree=2", into opts (a new default one if None): levels
+    and the per-level degrees before restriction / after prolongation."""
+    o = default_opts() if opts is None else opts
+    _lib.call("nrs_parse_mg_schedule", schedule.encode(), int(N), int(default_degree), C.byref(o))
+    return o
+
+
 def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=500, levels=None,
               smoother="chebyshev+jacobi", cheby_degree=2, cheby_min=0.1, cheby_max=1.1,
               coarse_tol=1e-2, coarse_max_iter=200, seed=0, coarse="pcg",
-              initial_guess="previous", proj_vectors=8, proj_start=5, **extra):
+              initial_guess="previous", proj_vectors=8, proj_start=5, schedule=None, order=None,
+              **extra):
     """nrs_elliptic_opts from keyword options (extra: raw field names).
     initial_guess: "previous", "projection" or "projection-aconj" (INITIAL GUESS);
-    proj_vectors / proj_start: RESIDUAL PROJECTION VECTORS / START."""
+    proj_vectors / proj_start: RESIDUAL PROJECTION VECTORS / START;
+    schedule: a MULTIGRID SCHEDULE string (needs order = the mesh order; the
+    entries without a degree take cheby_degree), in place of levels."""
     o = default_opts()
     o.solver = SOLVERS[solver]
     o.preconditioner = PRECONS[precon]
@@ -415,6 +427,8 @@ def make_opts(solver="pcg", precon="jacobi", tol=1e-8, relative=True, max_iter=5
     o.initialGuess = INITIAL_GUESS[initial_guess]
     o.projVectors = proj_vectors
     o.projStart = proj_start
+    if schedule is not None:
+        parse_mg_schedule(schedule, order, cheby_degree, o)
     for k, v in extra.items():
         setattr(o, k, v)
     return o
@@ -512,6 +526,8 @@ class Elliptic:
         self.mesh = mesh
         self.nfields = nfields
         d = make_desc(mesh, lambda0, lambda1, poisson, nfields)
+        if kw.get("schedule") is not None:
+            kw.setdefault("order", mesh.N)
         o = make_opts(**kw)
         self.opts = o
         h = C.c_void_p()

@@ -39,7 +39,7 @@ class EllipticOpts(C.Structure):
                 ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong),
                 ("useGraphs", C.c_int), ("autotune", C.c_int), ("gmresRestart", C.c_int),
                 ("initialGuess", C.c_int), ("projVectors", C.c_int), ("projStart", C.c_int),
-                ("operatorFusion", C.c_int)]
+                ("operatorFusion", C.c_int), ("preDegree", C.c_int * 8), ("postDegree", C.c_int * 8)]
 
 
 class SolveStats(C.Structure):
@@ -81,6 +81,7 @@ SIGNATURES = {
     "nrs_elliptic_preconditioner": ([_vp, _vp, _vp, _vp], _i),
     "nrs_elliptic_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats), _vp], _i),
     "nrs_elliptic_set_user_preconditioner": ([_vp, USER_PRECON_FN, _vp], _i),
+    "nrs_parse_mg_schedule": ([C.c_char_p, _i, _i, C.POINTER(EllipticOpts)], _i),
     "nrs_elliptic_krylov": ([_vp, _vp, _vp, _d, _i, C.POINTER(_d), C.POINTER(_i), _vp], _i),
     "nrs_elliptic_mg_info": ([_vp, _i, C.POINTER(_i), _vp, _vp, _vp, _vp, C.POINTER(_i)], _i),
     "nrs_elliptic_mg_smoother": ([_vp, _i, _vp, _vp, _vp], _i),

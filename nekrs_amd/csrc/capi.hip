@@ -13,6 +13,7 @@
 #include "mesh.hpp"
 #include "mg.hpp"
 #include "re2.hpp"
+#include "schedule.hpp"
 #include "tombo.hpp"
 
 namespace nrs {
@@ -532,6 +533,14 @@ int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats
     nrs::set_last_error(e.what());
     return NRS_ERR_INVALID;
   }
+}
+
+int nrs_parse_mg_schedule(const char *schedule, int N, int defaultDegree, nrs_elliptic_opts *opts)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(schedule && opts, "schedule: null argument");
+  nrs::parse_mg_schedule(schedule, N, defaultDegree, opts);
+  NRS_API_END
 }
 
 int nrs_elliptic_set_user_preconditioner(nrs_elliptic *el, nrs_user_precon_fn fn, void *user)

@@ -269,7 +269,8 @@ struct Level {
   DevBuf<T> rhsOwn, xOwn, res;
   T *rhs = nullptr, *x = nullptr;
   double rho = 0, lmin = 0, lmax = 0;
-  int degDown = 0, degUp = 0;
+  int degDown = 0, degUp = 0;        // before restriction (x = 0) / after prolongation
+  std::vector<T> betasDown, betasUp;  // optimal 4th-kind Chebyshev weights of those degrees
   std::unique_ptr<Schwarz<T>> schwarz;   // ASM / RAS smoother (else Jacobi)
   ~Level()
   {
@@ -532,7 +533,6 @@ struct MGImpl final : MGSolver {
   enum Cheby { NO_CHEBY, CHEBY1, CHEBY_OPT4 };
   int cheby = CHEBY1;
   int schwarzKind = 0;            // 0 Jacobi, 1 ASM, 2 RAS
-  std::vector<T> betasDown, betasUp, betasCoarse;
   bool coarseSmoothing = false;      // coarse level smoothed (degree 3), no coarse solve
   bool coarseJacobi = false;         // ... with Chebyshev-Jacobi whatever the finer levels use
   // Replay of the whole V-cycle as one HIP graph (every argument lives in
@@ -610,7 +610,7 @@ struct MGImpl final : MGSolver {
   {
     const int deg = xIsZero ? l.degDown : l.degUp;
     if (deg == 0) return;
-    const auto &betas = l.isCoarse ? betasCoarse : (xIsZero ? betasDown : betasUp);
+    const auto &betas = xIsZero ? l.betasDown : l.betasUp;
     const double rho = l.lmax;
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get();
     if (xIsZero) {
@@ -896,11 +896,9 @@ struct MGImpl final : MGSolver {
       NRS_CHECK(orders[i] >= 1 && orders[i] < orders[i - 1], "multigrid: orders must decrease");
     NRS_CHECK(orders.size() >= 2, "multigrid: at least two levels are required");
     NRS_CHECK(o.chebyDegree >= 1 && o.chebyDegree <= 16, "multigrid: Chebyshev degree must be 1..16");
-    if (cheby == CHEBY_OPT4) {
-      betasDown = opt4_betas<T>(o.chebyDegree);
-      betasUp = betasDown;
-      betasCoarse = opt4_betas<T>(3);
-    }
+    for (int i = 0; i < 8; ++i)
+      NRS_CHECK(o.preDegree[i] >= 0 && o.preDegree[i] <= 16 && o.postDegree[i] >= 0 && o.postDegree[i] <= 16,
+                "multigrid: per-level Chebyshev degrees must be 0 (default) or 1..16");
 
     std::vector<double> xf(d->x, d->x + e->Nlocal), yf(d->y, d->y + e->Nlocal), zf(d->z, d->z + e->Nlocal);
     std::vector<long long> gf(d->globalIds, d->globalIds + e->Nlocal);
@@ -978,8 +976,20 @@ struct MGImpl final : MGSolver {
           ogs_gather_scatter<T>(l.ogs, l.invDiagA.get(), 1, l.L, NRS_ADD, s);
           reciprocal<T>(l.L, l.invDiagA.get(), s);
         }
-        // the coarsest level, when smoothed, uses degree 3 (ellipticMultiGridLevelSetup.cpp:143-150)
-        l.degDown = l.degUp = (cheby == NO_CHEBY) ? 1 : (l.isCoarse ? 3 : o.chebyDegree);
+        // the coarsest level, when smoothed, uses degree 3 (ellipticMultiGridLevelSetup.cpp:143-150);
+        // the others MULTIGRID CHEBYSHEV DEGREE or the schedule's (:154-167)
+        if (cheby == NO_CHEBY) {
+          l.degDown = l.degUp = 1;
+        } else if (l.isCoarse) {
+          l.degDown = l.degUp = 3;
+        } else {
+          l.degDown = o.preDegree[li] > 0 ? o.preDegree[li] : o.chebyDegree;
+          l.degUp = o.postDegree[li] > 0 ? o.postDegree[li] : o.chebyDegree;
+        }
+        if (cheby == CHEBY_OPT4) {
+          l.betasDown = opt4_betas<T>(l.degDown);
+          l.betasUp = opt4_betas<T>(l.degUp);
+        }
         if (o.autotune && l.schwarz) l.schwarz->autotune(s);
       }
       // level operator launch variants (registerEllipticPreconditionerKernels.cpp:52-66)

@@ -13,6 +13,11 @@ oracle's C kernels that restates the reference's multigrid:
   preconditioner casts      src/solvers/elliptic/ellipticPreconditioner.cpp:57-62
   PCG / flexible PCG        src/solvers/elliptic/linearSolver/PCG.cpp:85-203
 
+MULTIGRID SCHEDULE strings: parse_mg_schedule restates parseMultigridSchedule
+(MG/parseMultigridSchedule.cpp:7-131), the levels determineMGLevels takes from
+it (MG/determineMGLevels.cpp:22-40) and the per-level degrees the level setup
+reads (MG/ellipticMultiGridLevelSetup.cpp:160-167).
+
 Smoothers: Chebyshev of the 1st kind (ellipticMultiGridLevel.cpp:144-210) or
 the optimal 4th kind (:212-270, betas of MG/optimalCoeffs.cpp) around S =
 Jacobi (1/diag A) or the overlapping Schwarz / FDM smoother (oracle/fdm.py,
@@ -46,6 +51,94 @@ OPT4_BETAS = {
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 
 
+def _pieces(s, dlim):
+    """serializeString (src/core/nrssys.hpp:148-160): white space removed,
+    split at dlim, empty pieces dropped."""
+    s = "".join(c for c in s if not c.isspace())
+    return [p for p in s.split(dlim) if p]
+
+
+def _stoi(token):
+    """std::stoi of the second '='-piece: optional sign, leading digits."""
+    p = _pieces(token, "=")
+    if len(p) < 2:
+        raise ValueError("no value in token '%s'" % token)
+    v, i = p[1], 0
+    if i < len(v) and v[i] in "+-":
+        i += 1
+    j = i
+    while j < len(v) and v[j].isdigit():
+        j += 1
+    if j == i:
+        raise ValueError("no value in token '%s'" % token)
+    return int(v[:j])
+
+
+def parse_mg_schedule(schedule, N, default_degree):
+    """(levels, pre, post, errors): levels = distinct down-leg orders, finest
+    first; pre[i] / post[i] = Chebyshev degree before restriction / after
+    prolongation on level i (0 on the coarsest); errors = the reference's
+    messages (empty when valid).  The degree before restriction is the one on
+    the schedule's up leg (DownLegChebyshevDegree = scheduleMap[{order, false}],
+    ellipticMultiGridLevelSetup.cpp:165-166), after prolongation the down leg's."""
+    INVALID = -(2 ** 31)
+    err = []
+    entries = _pieces(schedule, ",")
+    min_order = 2 ** 31 - 1
+    for e in entries:
+        for t in _pieces(e, "+"):
+            if "p" in t:
+                min_order = min(min_order, _stoi(t))
+    sched = {}
+    down_orders = []
+    down, prev = True, 2 ** 31 - 1
+    for e in entries:
+        order, degree = INVALID, INVALID
+        for t in _pieces(e, "+"):
+            if "p" in t:                      # as there: "sweeps" contains a 'p'
+                order = _stoi(t)
+                if order > prev:
+                    down = False
+                if down:
+                    down_orders.append(order)
+                prev = order
+            elif "degree" in t:
+                degree = _stoi(t)
+            else:
+                err.append("unknown token '%s' in schedule '%s'!" % (t, schedule))
+        if order == INVALID:
+            err.append("order not specified in " + e)
+            continue
+        if degree == INVALID and order != min_order:
+            degree = default_degree
+        sched[(order, down)] = degree
+    dset = {o for (o, d) in sched if d and o > min_order}
+    uset = {o for (o, d) in sched if not d and o > min_order}
+    if dset != uset:
+        err.append("down leg and up leg orders must be identical!")
+    for (o, d), g in sched.items():
+        if o != min_order and g == INVALID:
+            err.append("degree not specified for order %d!" % o)
+    for l in down_orders:
+        if l < 1:
+            err.append("encountered level = %d < 1!" % l)
+    if not down_orders:
+        err.append("no levels")
+    elif down_orders[0] != N:
+        err.append("finest degree %d does not match polynomial degree %d!" % (down_orders[0], N))
+    for (o, d), g in sched.items():
+        if o != min_order and g != INVALID and not 1 <= g <= 16:
+            err.append("degree %d of order %d outside 1..16!" % (g, o))
+    levels = sorted({o for (o, d) in sched if d}, reverse=True)
+    if len(levels) > 8:
+        err.append("more than 8 levels")
+    if err:
+        return levels, [], [], err
+    pre = [sched[(o, False)] for o in levels[:-1]] + [0]
+    post = [sched[(o, True)] for o in levels[:-1]] + [0]
+    return levels, pre, post, []
+
+
 def splitmix_uniform(seed, n):
     """n draws of splitmix64 (state advanced by the golden-ratio increment)."""
     with np.errstate(over="ignore"):
@@ -74,13 +167,16 @@ class MG:
 
     def __init__(self, mesh, orders, T=np.float32, degree=2, min_factor=0.1, max_factor=1.1,
                  coarse_tol=1e-2, coarse_maxit=200, seed=0, cheby="cheby1", S="jacobi",
-                 fmt=None, lambda0=1.0, lambda1=0.0, poisson=True, coarse="pcg"):
+                 fmt=None, lambda0=1.0, lambda1=0.0, poisson=True, coarse="pcg",
+                 pre_degree=None, post_degree=None):
         """cheby: "cheby1", "opt4" or None (pure Schwarz); S: "jacobi", "asm", "ras";
         fmt: None, "bf16" or "fp16" for the FDM contractions; lambda0/lambda1/poisson:
         the (constant-coefficient) operator; coarse: "pcg" (FP64 Jacobi-PCG) or
         "smooth" (MULTIGRID COARSE SOLVE = FALSE: the coarsest level is smoothed,
         degree 3, ellipticMultiGridSetup.cpp:315-320) or "smooth_jacobi" (smoothed
-        with Chebyshev-Jacobi whatever S is; an extension)."""
+        with Chebyshev-Jacobi whatever S is; an extension); pre_degree /
+        post_degree: per-level Chebyshev degrees before restriction / after
+        prolongation (0 or None: `degree`), as a MULTIGRID SCHEDULE sets them."""
         assert orders[0] == mesh.N
         assert cheby in ("cheby1", "opt4", None) and S in ("jacobi", "asm", "ras")
         assert cheby is not None or S != "jacobi"
@@ -125,6 +221,12 @@ class MG:
                                              mode=S, T=T, fmt=fmt, poisson=poisson,
                                              lambda0=lambda0, lambda1=lambda1)
                 L.degree = 3 if L.isCoarse else degree
+                L.deg_pre = L.deg_post = L.degree
+                if not L.isCoarse:
+                    if pre_degree is not None and pre_degree[li]:
+                        L.deg_pre = pre_degree[li]
+                    if post_degree is not None and post_degree[li]:
+                        L.deg_post = post_degree[li]
             self.levels.append(L)
             Nprev = N
         for L in self.levels:
@@ -223,7 +325,7 @@ class MG:
     # in FP64 and stored as pfloat, as there
     def smooth_cheby1(self, L, rhs, x, x_is_zero):
         T = self.T
-        deg = L.degree
+        deg = L.deg_pre if x_is_zero else L.deg_post
         theta = T(0.5 * (L.lmax + L.lmin))
         delta = T(0.5 * (L.lmax - L.lmin))
         invTheta = T(1.0 / float(theta))
@@ -252,7 +354,7 @@ class MG:
     # smoothFourthKindChebyshev (ellipticMultiGridLevel.cpp:212-270), rho = lambda1
     def smooth_opt4(self, L, rhs, x, x_is_zero):
         T = self.T
-        deg = L.degree
+        deg = L.deg_pre if x_is_zero else L.deg_post
         betas = [T(b) for b in OPT4_BETAS[deg]]
         rho = L.lmax
         if x_is_zero:

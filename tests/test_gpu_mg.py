@@ -195,3 +195,47 @@ def test_mg_amg_fixed_iterations_graph_vs_oracle(gpu, torch_mod, orc):
     print("fixed AMG gpu it", st["iterations"], "oracle it", info["iterations"])
     assert st["converged"] and abs(st["iterations"] - info["iterations"]) <= 1
     assert np.max(np.abs(x.cpu().numpy() - xo)) <= 1e-8 * np.max(np.abs(xo))
+
+
+def test_mg_schedule_degrees_vs_oracle(gpu, torch_mod, orc):
+    """A MULTIGRID SCHEDULE with a different Chebyshev degree per level and leg
+    (parsed by the library, levels and degrees as the reference's level setup
+    reads them) against the oracle's V-cycle with the same degrees: one V-cycle
+    within 2e-4 (FP32), FCG within +-1 iteration and 1e-8; and a schedule with
+    degree 2 everywhere gives the plain levels=[7,3,1], degree-2 V-cycle bit for bit."""
+    import oracle.mg as omg
+    torch = torch_mod
+    sched = "p=7+degree=2, p=3+degree=1, p=1, p=3+degree=4, p=7+degree=3"
+    lv, pre, post, errs = omg.parse_mg_schedule(sched, 7, 2)
+    assert not errs and lv == [7, 3, 1] and pre == [3, 4, 0] and post == [2, 1, 0]
+    mesh = gpu.BoxMesh(4, 4, 4, 7, "kershaw", 0.3)
+    eg = gpu.Elliptic(mesh, precon="multigrid", solver="fcg", schedule=sched, cheby_degree=2,
+                      tol=1e-10, max_iter=400)
+    assert [l["order"] for l in eg.mg_info()[0]] == [7, 3, 1]
+    mo = orc.BoxMesh(4, 4, 4, 7, "kershaw", 0.3)
+    eo = orc.Elliptic(mo)
+    om = omg.MG(mo, lv, T=np.float32, degree=2, pre_degree=pre, post_degree=post)
+    r = assembled_random(eo, 5)
+    z = torch.zeros(eg.Nlocal, dtype=torch.float64, device="cuda")
+    eg.preconditioner(dev(torch, r), z)
+    torch.cuda.synchronize()
+    assert rel(z.cpu().numpy(), om.apply(r)) < 2e-4
+    rhs = np.random.default_rng(8).uniform(-1, 1, eg.Nlocal)
+    xo, info = omg.fcg_solve(eo, rhs, om.apply, flexible=True, tol=1e-10, maxit=400)
+    x = torch.zeros(eg.Nlocal, dtype=torch.float64, device="cuda")
+    st = eg.solve(dev(torch, rhs), x)
+    print("schedule gpu it", st["iterations"], "oracle it", info["iterations"], flush=True)
+    assert st["converged"]
+    assert abs(st["iterations"] - info["iterations"]) <= 1
+    assert rel(x.cpu().numpy(), xo) <= 1e-8
+    # uniform schedule == plain options
+    e1 = gpu.Elliptic(mesh, precon="multigrid", solver="fcg", cheby_degree=2, autotune=0,
+                      schedule="p=7+degree=2, p=3+degree=2, p=1, p=3+degree=2, p=7+degree=2")
+    e2 = gpu.Elliptic(mesh, precon="multigrid", solver="fcg", cheby_degree=2, autotune=0,
+                      levels=[7, 3, 1])
+    z1 = torch.zeros_like(z)
+    z2 = torch.zeros_like(z)
+    e1.preconditioner(dev(torch, r), z1)
+    e2.preconditioner(dev(torch, r), z2)
+    torch.cuda.synchronize()
+    assert torch.equal(z1, z2)
