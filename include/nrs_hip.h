@@ -367,6 +367,27 @@ int nrs_elliptic_mg_info(nrs_elliptic *el, int max, int *nLevels, int *orders, i
  * pMGLevel::smoothSchwarz (MG/ellipticMultiGridSchwarz.cpp:1056-1156). */
 int nrs_elliptic_mg_smoother(nrs_elliptic *el, int level, const void *in, void *out, void *stream);
 
+/* One operation of multigrid level `level` (0 = finest), the per-level
+ * interface multigridLevel::{Ax, residual, smooth, coarsen, prolongate}
+ * (MG/MGSolver.hpp:59-69, pMGLevel in MG/ellipticMultiGridLevel.cpp).  Device
+ * E-vectors of the level's order in the multigrid precision (float for
+ * NRS_MG_F32, double for NRS_MG_F64):
+ *   NRS_MG_AX          out = A in (the level operator: Ax, mask, gather-scatter)
+ *   NRS_MG_RESIDUAL    out = in - A x
+ *   NRS_MG_SMOOTH      Chebyshev / Schwarz smoothing of out (the iterate) for the
+ *                      right-hand side in; xIsZero = 1: the smoothing before
+ *                      restriction (out starts from 0), 0: after prolongation;
+ *                      not on a coarsest level that is solved
+ *   NRS_MG_COARSEN     `level` >= 1 is the coarse level: in (of level - 1) is
+ *                      weighted by that level's 1/multiplicity in place, as the
+ *                      reference does, then out = R in, gather-scatter, mask
+ *   NRS_MG_PROLONGATE  `level` >= 1 is the coarse level: out (of level - 1) += P in
+ * Enqueued on `stream`; vectors of different roles must not alias. */
+enum nrs_mg_level_op { NRS_MG_AX = 0, NRS_MG_RESIDUAL = 1, NRS_MG_SMOOTH = 2, NRS_MG_COARSEN = 3,
+                       NRS_MG_PROLONGATE = 4 };
+int nrs_elliptic_mg_level(nrs_elliptic *el, int level, int op, void *in, const void *x, void *out,
+                          int xIsZero, void *stream);
+
 /* Host-only FDM setup pieces of the Schwarz smoother (no device work):
  * the 1D operator pair of one element direction on N+3 points, with
  * boundary flags lbc/rbc (0 interior, 1 Dirichlet, 2 Neumann) and the

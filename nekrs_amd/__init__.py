@@ -612,6 +612,16 @@ class Elliptic:
         _lib.call("nrs_elliptic_mg_smoother", self._h, int(level), _ptr(x), _ptr(out),
                   _stream(stream))
 
+    MG_LEVEL_OPS = {"ax": 0, "residual": 1, "smooth": 2, "coarsen": 3, "prolongate": 4}
+
+    def mg_level(self, level, op, inp, out, x=None, x_is_zero=True, stream=None):
+        """One multigridLevel operation on `level` (tensors of the MG precision):
+        "ax" out = A inp; "residual" out = inp - A x; "smooth" smooths out for
+        the rhs inp; "coarsen" (level >= 1) out = R (inp .* 1/multiplicity of
+        level-1, inp overwritten); "prolongate" (level >= 1) out += P inp."""
+        _lib.call("nrs_elliptic_mg_level", self._h, int(level), self.MG_LEVEL_OPS[op], _ptr(inp),
+                  _ptr(x), _ptr(out), int(x_is_zero), _stream(stream))
+
     def profile(self, on=True):
         """Enable/disable per-launch device timing of the operator's kernels."""
         _lib.call("nrs_elliptic_profile_enable", self._h, int(on))

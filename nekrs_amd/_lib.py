@@ -81,6 +81,7 @@ SIGNATURES = {
     "nrs_elliptic_preconditioner": ([_vp, _vp, _vp, _vp], _i),
     "nrs_elliptic_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats), _vp], _i),
     "nrs_elliptic_set_user_preconditioner": ([_vp, USER_PRECON_FN, _vp], _i),
+    "nrs_elliptic_mg_level": ([_vp, _i, _i, _vp, _vp, _vp, _i, _vp], _i),
     "nrs_parse_mg_schedule": ([C.c_char_p, _i, _i, C.POINTER(EllipticOpts)], _i),
     "nrs_elliptic_krylov": ([_vp, _vp, _vp, _d, _i, C.POINTER(_d), C.POINTER(_i), _vp], _i),
     "nrs_elliptic_mg_info": ([_vp, _i, C.POINTER(_i), _vp, _vp, _vp, _vp, C.POINTER(_i)], _i),

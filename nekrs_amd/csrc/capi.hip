@@ -535,6 +535,15 @@ int nrs_elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats
   }
 }
 
+int nrs_elliptic_mg_level(nrs_elliptic *el, int level, int op, void *in, const void *x, void *out,
+                          int xIsZero, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el && el->mg, "mg level: handle without a multigrid preconditioner");
+  el->mg->level_op(level, op, in, x, out, xIsZero != 0, as_stream(stream));
+  NRS_API_END
+}
+
 int nrs_parse_mg_schedule(const char *schedule, int N, int defaultDegree, nrs_elliptic_opts *opts)
 {
   NRS_API_BEGIN

@@ -727,6 +727,49 @@ struct MGImpl final : MGSolver {
     smootherS(*levels[level], (const T *)in, (T *)out, s);
   }
 
+  void level_op(int level, int opk, void *in, const void *xv, void *outv, bool xIsZero,
+                hipStream_t s) override
+  {
+    const int nl = (int)levels.size();
+    NRS_CHECK(level >= 0 && level < nl, "mg level: no such level");
+    NRS_CHECK(in && outv, "mg level: null vector");
+    Level<T> &l = *levels[level];
+    T *a = (T *)in, *out = (T *)outv;
+    switch (opk) {
+      case NRS_MG_AX:   // pMGLevel::Ax (ellipticMultiGridLevel.cpp:32-35)
+        NRS_CHECK(a != out, "mg level Ax: input and output must differ");
+        op(l, a, out, s);
+        break;
+      case NRS_MG_RESIDUAL:   // :37-41, out = in - A x
+        NRS_CHECK(xv && out != a && out != xv, "mg level residual: x needed, out distinct");
+        op(l, (const T *)xv, out, s);
+        k_axpby<T>(l.L, T(1), a, T(-1), out, s);
+        break;
+      case NRS_MG_SMOOTH:     // :85-103, rhs = in, x = out
+        NRS_CHECK(!l.isCoarse || !coarse, "mg level smooth: the coarsest level is solved, not smoothed");
+        NRS_CHECK(a != out, "mg level smooth: rhs and x must differ");
+        smooth(l, a, out, xIsZero, s);
+        break;
+      case NRS_MG_COARSEN: {  // :43-68 on the coarse level: in (level-1) .*= invDegree, out = R in
+        NRS_CHECK(level >= 1, "mg level coarsen: level is the coarse level (>= 1)");
+        Level<T> &f = *levels[level - 1];
+        k_wmul<T>(f.L, f.invDegree.get(), a, a, s);
+        launch_coarsen<T>(f.N, l.N, f.E, l.R.get(), a, out, s);
+        ogs_gather_scatter<T>(l.ogs, out, 1, l.L, NRS_ADD, s);
+        launch_mask<T>(l.Nmasked, l.maskIds, out, s);
+        break;
+      }
+      case NRS_MG_PROLONGATE: {   // :70-83 on the coarse level: out (level-1) += P in
+        NRS_CHECK(level >= 1, "mg level prolongate: level is the coarse level (>= 1)");
+        Level<T> &f = *levels[level - 1];
+        launch_prolongate<T>(f.N, l.N, f.E, l.R.get(), a, out, s);
+        break;
+      }
+      default:
+        throw Error(NRS_ERR_INVALID, "mg level: unknown operation");
+    }
+  }
+
   // ---- setup helpers
 
   // lambda_max(S A) by a k-step Arnoldi in FP64 with the level operator in T

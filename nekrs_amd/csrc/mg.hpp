@@ -24,6 +24,10 @@ struct MGSolver {
   virtual int coarse_iterations() const = 0;   // iterations of the last coarse solve
   // out = S in on a smoothed level, in the level arithmetic type
   virtual void level_smoother(int level, const void *in, void *out, hipStream_t s) = 0;
+  // multigridLevel::{Ax, residual, smooth, coarsen, prolongate} on one level
+  // (op = nrs_mg_level_op), vectors in the level arithmetic type
+  virtual void level_op(int level, int op, void *in, const void *x, void *out, bool xIsZero,
+                        hipStream_t s) = 0;
   // true when apply() enqueues work only (coarse level smoothed, no host round trip)
   virtual bool device_only() const = 0;
 };

@@ -239,3 +239,45 @@ def test_mg_schedule_degrees_vs_oracle(gpu, torch_mod, orc):
     e2.preconditioner(dev(torch, r), z2)
     torch.cuda.synchronize()
     assert torch.equal(z1, z2)
+
+
+@pytest.mark.parametrize("mgp,smoother", [(0, "opt4chebyshev+ras"), (1, "chebyshev+jacobi")])
+def test_vcycle_from_level_operations(gpu, torch_mod, orc, mgp, smoother):
+    """A V-cycle composed by hand from the per-level operations
+    (nrs_elliptic_mg_level: smooth, residual, coarsen, prolongate, as
+    MGSolver_t::runVcycle, MGSolver.cpp:167-193, calls them) gives the library's
+    preconditioner bit for bit; the level Ax and residual agree with each other."""
+    torch = torch_mod
+    mesh = gpu.BoxMesh(3, 3, 2, 5, "kershaw", 0.3)
+    el = gpu.Elliptic(mesh, precon="multigrid", solver="fcg", levels=[5, 3, 1], smoother=smoother,
+                      cheby_degree=2, coarse="smooth_jacobi", mgPrecision=mgp, fdmPrecision=0,
+                      autotune=0)
+    T = torch.float32 if mgp == 0 else torch.float64
+    lv, _ = el.mg_info()
+    n = [l["Nlocal"] for l in lv]
+    eo = orc.Elliptic(orc.BoxMesh(3, 3, 2, 5, "kershaw", 0.3))
+    r = dev(torch, assembled_random(eo, 9))
+    z = torch.zeros_like(r)
+    el.preconditioner(r, z)
+    rhs = [r.to(T)] + [torch.zeros(n[k], dtype=T, device="cuda") for k in (1, 2)]
+    x = [torch.zeros(n[k], dtype=T, device="cuda") for k in range(3)]
+    res = [torch.zeros(n[k], dtype=T, device="cuda") for k in range(3)]
+    for k in (0, 1):
+        el.mg_level(k, "smooth", rhs[k], x[k], x_is_zero=True)
+        el.mg_level(k, "residual", rhs[k], res[k], x=x[k])
+        el.mg_level(k + 1, "coarsen", res[k], rhs[k + 1])
+    el.mg_level(2, "smooth", rhs[2], x[2], x_is_zero=True)
+    for k in (1, 0):
+        el.mg_level(k + 1, "prolongate", x[k + 1], x[k])
+        el.mg_level(k, "smooth", rhs[k], x[k], x_is_zero=False)
+    torch.cuda.synchronize()
+    assert torch.equal(x[0].to(torch.float64), z)
+    # residual = rhs - A x through the level Ax
+    ax = torch.zeros_like(x[1])
+    el.mg_level(1, "ax", x[1], ax)
+    rr = torch.zeros_like(x[1])
+    el.mg_level(1, "residual", rhs[1], rr, x=x[1])
+    torch.cuda.synchronize()
+    assert torch.equal(rr, rhs[1] - ax)
+    with pytest.raises(gpu.NrsError):
+        el.mg_level(0, "coarsen", res[0], rhs[1])          # level 0 has no finer level
