@@ -193,6 +193,10 @@ enum nrs_coarse {
   NRS_COARSE_PCG_F64 = 0,        /* FP64 Jacobi-PCG to coarseTol                       */
   NRS_COARSE_SMOOTH = 1,
   NRS_COARSE_SMOOTH_JACOBI = 2,
+  NRS_COARSE_USER = 4,           /* the caller's coarse solve, coarseLevel_t::solvePtr
+                                  * (MG/MGSolver.hpp:99): the function given to
+                                  * nrs_elliptic_set_user_coarse_solver; the V-cycle
+                                  * is then not replayed as one graph             */
   NRS_COARSE_AMG = 3             /* FP64 PCG on the assembled order-1 matrix,
                                   * preconditioned by one smoothed-aggregation AMG
                                   * V-cycle (in place of BoomerAMG,
@@ -323,6 +327,17 @@ int nrs_elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, vo
  * a USER handle without one fails its solve with NRS_ERR_INVALID. */
 typedef void (*nrs_user_precon_fn)(void *user, const double *r, double *z, void *stream);
 int nrs_elliptic_set_user_preconditioner(nrs_elliptic *el, nrs_user_precon_fn fn, void *user);
+
+/* The coarse-solve hook of NRS_COARSE_USER, coarseLevel_t::solvePtr
+ * (MG/MGSolver.hpp:99, called by MGSolver_t::runVcycle, MGSolver.cpp:174-177):
+ * fn(user, rhs, x, stream) must enqueue on `stream` the coarsest level's solve,
+ * writing all of x; rhs and x are device E-vectors of the coarsest level
+ * (nrs_elliptic_mg_info gives its length) in the multigrid precision.  Called
+ * from the host thread of the preconditioner, once per V-cycle.  fn = NULL
+ * clears it; a NRS_COARSE_USER handle without one fails its solve and its
+ * preconditioner with NRS_ERR_INVALID before any work. */
+typedef void (*nrs_user_coarse_fn)(void *user, const void *rhs, void *x, void *stream);
+int nrs_elliptic_set_user_coarse_solver(nrs_elliptic *el, nrs_user_coarse_fn fn, void *user);
 
 /* The Krylov iteration alone, the seam between ellipticSolve and its linear
  * solver: pcg(elliptic, o_r, o_x, tol, MAXIT, rdotr) (linearSolver/PCG.cpp:85-203,

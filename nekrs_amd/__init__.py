@@ -73,13 +73,13 @@ class _DeviceArray:
     interface; torch takes no read-only views, so r is writable but must not be
     written)."""
 
-    def __init__(self, addr, n):
-        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f8", "version": 2,
+    def __init__(self, addr, n, typestr):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "version": 2,
                                          "data": (int(addr), False), "strides": None}
 
 
-def _device_view(addr, n):
-    return _torch().as_tensor(_DeviceArray(addr, n), device="cuda")
+def _device_view(addr, n, typestr="<f8"):
+    return _torch().as_tensor(_DeviceArray(addr, n, typestr), device="cuda")
 
 
 def init(device=0):
@@ -376,7 +376,7 @@ def default_opts():
     return o
 
 
-COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2, "amg": 3}
+COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2, "amg": 3, "user": 4}
 INITIAL_GUESS = {"previous": 0, "projection": 1, "projection-aconj": 2}
 
 
@@ -595,7 +595,9 @@ class Elliptic:
         return n.value
 
     def preconditioner(self, r, z, stream=None):
-        _lib.call("nrs_elliptic_preconditioner", self._h, _ptr(r), _ptr(z), _stream(stream))
+        rc = _lib.load().nrs_elliptic_preconditioner(self._h, _ptr(r), _ptr(z), _stream(stream))
+        self._raise_user_error()
+        _lib.check(rc)
 
     def mg_info(self):
         """Multigrid levels: [{order, Nlocal, rho, degree}], coarse PCG iterations."""
@@ -664,6 +666,37 @@ class Elliptic:
 
         self._user_cb = _lib.USER_PRECON_FN(cb)
         _lib.call("nrs_elliptic_set_user_preconditioner", self._h, self._user_cb, None)
+
+    def set_user_coarse_solver(self, fn, tensors=True):
+        """The coarse-solve hook of coarse="user" (coarseLevel_t::solvePtr,
+        MG/MGSolver.hpp:99): fn(rhs, x, stream) must enqueue the coarsest level's
+        solve on `stream`, writing all of x; rhs and x are device tensors of the
+        coarsest level in the multigrid precision (tensors=True) or their
+        addresses (tensors=False).  Exceptions are re-raised by the solve or
+        preconditioner call that ran it.  fn=None clears the hook."""
+        if fn is None:
+            self._coarse_cb = None
+            _lib.call("nrs_elliptic_set_user_coarse_solver", self._h, _lib.USER_COARSE_FN(), None)
+            return
+        torch = _torch()
+        n = self.mg_info()[0][-1]["Nlocal"]
+        ts = "<f4" if self.opts.mgPrecision == 0 else "<f8"
+        self._user_err = None
+
+        def cb(user, rhs, x, stream):
+            if self._user_err is not None:
+                return
+            try:
+                st = torch.cuda.ExternalStream(stream) if stream else torch.cuda.default_stream()
+                if tensors:
+                    fn(_device_view(rhs, n, ts), _device_view(x, n, ts), st)
+                else:
+                    fn(rhs, x, st)
+            except BaseException as e:
+                self._user_err = e
+
+        self._coarse_cb = _lib.USER_COARSE_FN(cb)
+        _lib.call("nrs_elliptic_set_user_coarse_solver", self._h, self._coarse_cb, None)
 
     def _raise_user_error(self):
         e = getattr(self, "_user_err", None)

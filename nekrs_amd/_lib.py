@@ -52,6 +52,8 @@ class SolveStats(C.Structure):
 _vp, _i, _i64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_double
 # nrs_user_precon_fn: fn(user, r, z, stream), no return value
 USER_PRECON_FN = C.CFUNCTYPE(None, _vp, _vp, _vp, _vp)
+# nrs_user_coarse_fn: fn(user, rhs, x, stream)
+USER_COARSE_FN = C.CFUNCTYPE(None, _vp, _vp, _vp, _vp)
 SIGNATURES = {
     "nrs_last_error": ([], C.c_char_p),
     "nrs_version": ([], C.c_char_p),
@@ -83,6 +85,7 @@ SIGNATURES = {
     "nrs_elliptic_set_user_preconditioner": ([_vp, USER_PRECON_FN, _vp], _i),
     "nrs_elliptic_mg_level": ([_vp, _i, _i, _vp, _vp, _vp, _i, _vp], _i),
     "nrs_parse_mg_schedule": ([C.c_char_p, _i, _i, C.POINTER(EllipticOpts)], _i),
+    "nrs_elliptic_set_user_coarse_solver": ([_vp, USER_COARSE_FN, _vp], _i),
     "nrs_elliptic_krylov": ([_vp, _vp, _vp, _d, _i, C.POINTER(_d), C.POINTER(_i), _vp], _i),
     "nrs_elliptic_mg_info": ([_vp, _i, C.POINTER(_i), _vp, _vp, _vp, _vp, C.POINTER(_i)], _i),
     "nrs_elliptic_mg_smoother": ([_vp, _i, _vp, _vp, _vp], _i),

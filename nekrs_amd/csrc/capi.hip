@@ -561,6 +561,15 @@ int nrs_elliptic_set_user_preconditioner(nrs_elliptic *el, nrs_user_precon_fn fn
   NRS_API_END
 }
 
+int nrs_elliptic_set_user_coarse_solver(nrs_elliptic *el, nrs_user_coarse_fn fn, void *user)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el != nullptr, "user coarse solver: null handle");
+  el->userCoarse = fn;
+  el->userCoarseCtx = fn ? user : nullptr;
+  NRS_API_END
+}
+
 int nrs_elliptic_krylov(nrs_elliptic *el, double *r, double *x, double tol, int maxIter,
                         double *rdotr, int *iterations, void *stream)
 {

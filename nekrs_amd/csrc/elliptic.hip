@@ -620,6 +620,8 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   const double resNormFactor = 1.0 / el->volume;
   NRS_CHECK(o.preconditioner != NRS_PRECON_USER || el->userPrecon,
             "PRECONDITIONER = USER without a user preconditioner");
+  NRS_CHECK(o.preconditioner != NRS_PRECON_MULTIGRID || o.coarseSolver != NRS_COARSE_USER || el->userCoarse,
+            "multigrid: coarse solver USER without a coarse solve");
   auto t0 = std::chrono::steady_clock::now();
   NRS_HIP(hipEventRecord(el->ev0, s));
 

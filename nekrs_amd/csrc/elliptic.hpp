@@ -51,6 +51,9 @@ struct nrs_elliptic {
   // PRECONDITIONER = USER: elliptic_t::userPreconditioner (elliptic.h:178)
   nrs_user_precon_fn userPrecon = nullptr;
   void *userCtx = nullptr;
+  // MULTIGRID coarse solver = USER: coarseLevel_t::solvePtr (MG/MGSolver.hpp:99)
+  nrs_user_coarse_fn userCoarse = nullptr;
+  void *userCoarseCtx = nullptr;
 
   // workspace (ellipticUpdateWorkspace, elliptic.h:240-262)
   nrs::DevBuf<double> p, zv, Ap, x0;

@@ -535,6 +535,7 @@ struct MGImpl final : MGSolver {
   int schwarzKind = 0;            // 0 Jacobi, 1 ASM, 2 RAS
   bool coarseSmoothing = false;      // coarse level smoothed (degree 3), no coarse solve
   bool coarseJacobi = false;         // ... with Chebyshev-Jacobi whatever the finer levels use
+  bool coarseUser = false;           // the caller's coarse solve (el->userCoarse)
   // Replay of the whole V-cycle as one HIP graph (every argument lives in
   // the handle: level vectors, device scalars, host-fixed Chebyshev
   // coefficients), used when the cycle never waits on the host
@@ -653,6 +654,7 @@ struct MGImpl final : MGSolver {
     Level<T> &l = *levels[k];
     if (l.isCoarse) {
       if (coarse) coarse->solve<T>(l.rhs, l.x, s);
+      else if (coarseUser) el->userCoarse(el->userCoarseCtx, l.rhs, l.x, (void *)s);   // solvePtr
       else smooth(l, l.rhs, l.x, true, s);   // MULTIGRID COARSE SOLVE = FALSE (ellipticMultiGridSetup.cpp:315-320)
       return;
     }
@@ -674,6 +676,7 @@ struct MGImpl final : MGSolver {
 
   void apply(const double *r, double *z, hipStream_t s) override
   {
+    NRS_CHECK(!coarseUser || el->userCoarse, "multigrid: coarse solver USER without a coarse solve");
     Level<T> &l0 = *levels[0];
     hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
     NRS_HIP(hipStreamIsCapturing(s, &cst));
@@ -718,10 +721,10 @@ struct MGImpl final : MGSolver {
     return out;
   }
   int coarse_iterations() const override { return coarse ? coarse->lastIters : 0; }
-  bool device_only() const override { return coarse == nullptr || coarse->deviceOnly; }
+  bool device_only() const override { return !coarseUser && (coarse == nullptr || coarse->deviceOnly); }
   void level_smoother(int level, const void *in, void *out, hipStream_t s) override
   {
-    NRS_CHECK(level >= 0 && level < (int)levels.size() && (level + 1 < (int)levels.size() || !coarse),
+    NRS_CHECK(level >= 0 && level < (int)levels.size() && (level + 1 < (int)levels.size() || coarseSmoothing),
               "mg smoother: not a smoothed level");
     NRS_CHECK(in != out, "mg smoother: input and output must differ");
     smootherS(*levels[level], (const T *)in, (T *)out, s);
@@ -746,7 +749,7 @@ struct MGImpl final : MGSolver {
         k_axpby<T>(l.L, T(1), a, T(-1), out, s);
         break;
       case NRS_MG_SMOOTH:     // :85-103, rhs = in, x = out
-        NRS_CHECK(!l.isCoarse || !coarse, "mg level smooth: the coarsest level is solved, not smoothed");
+        NRS_CHECK(!l.isCoarse || coarseSmoothing, "mg level smooth: the coarsest level is solved, not smoothed");
         NRS_CHECK(a != out, "mg level smooth: rhs and x must differ");
         smooth(l, a, out, xIsZero, s);
         break;
@@ -865,7 +868,7 @@ struct MGImpl final : MGSolver {
     for (int id : mids) mg[id] = 0;
     l.ogs = ogs_setup(l.L, mg.data());
     l.ownOgs = true;
-    if (l.isCoarse && !coarseSmoothing) {   // FP64 operator for the coarse PCG
+    if (l.isCoarse && !coarseSmoothing && !coarseUser) {   // FP64 operator for the coarse PCG
       coarse = std::make_unique<CoarsePCG>();
       CoarsePCG &c = *coarse;
       c.N = l.N; c.E = l.E; c.L = l.L;
@@ -910,8 +913,10 @@ struct MGImpl final : MGSolver {
       default: throw Error(NRS_ERR_INVALID, "multigrid: unknown smoother");
     }
     NRS_CHECK(o.coarseSolver == NRS_COARSE_PCG_F64 || o.coarseSolver == NRS_COARSE_SMOOTH ||
-                  o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI || o.coarseSolver == NRS_COARSE_AMG,
+                  o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI || o.coarseSolver == NRS_COARSE_AMG ||
+                  o.coarseSolver == NRS_COARSE_USER,
               "multigrid: unknown coarse solver");
+    coarseUser = o.coarseSolver == NRS_COARSE_USER;
     coarseSmoothing = o.coarseSolver == NRS_COARSE_SMOOTH || o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI;
     coarseJacobi = o.coarseSolver == NRS_COARSE_SMOOTH_JACOBI;
     NRS_CHECK(!(coarseJacobi && cheby == NO_CHEBY),

@@ -173,3 +173,49 @@ def test_krylov_seam_max_iter_and_zero_residual(gpu, torch_mod):
     assert it == 1 and res == 0.0 and float(x.abs().max()) == 0.0
     with pytest.raises(gpu.NrsError):
         el.krylov(r, x, 1e-9, 0, res0)
+
+
+@pytest.mark.parametrize("tensors", [False, True])
+def test_user_coarse_solve_gives_the_builtin_coarse_smoothing(gpu, torch_mod, tensors):
+    """coarse="user" (coarseLevel_t::solvePtr, MG/MGSolver.hpp:99) with a hook that
+    runs another handle's coarsest-level smoothing reproduces that handle's
+    V-cycle and its FCG solve bit for bit; without a hook both fail before any work."""
+    torch = torch_mod
+    m = gpu.BoxMesh(*SHAPE, N, "kershaw", 0.3)
+    kw = dict(precon="multigrid", solver="fcg", levels=[5, 3, 1], smoother="opt4chebyshev+ras",
+              cheby_degree=2, autotune=0, tol=1e-10, max_iter=300)
+    a = gpu.Elliptic(m, coarse="smooth_jacobi", **kw)
+    b = gpu.Elliptic(m, coarse="user", **kw)
+    nC = b.mg_info()[0][-1]["Nlocal"]
+    r = torch.from_numpy(_rhs(m, 6)).cuda()
+    zb = torch.zeros_like(r)
+    with pytest.raises(gpu.NrsError):
+        b.preconditioner(r, zb)
+    with pytest.raises(gpu.NrsError):
+        b.solve(r.clone(), torch.zeros_like(r))
+    lib = gpu._lib.load()
+    calls = []
+    if tensors:
+        def coarse(rhs, x, st):
+            assert rhs.dtype == torch.float32 and rhs.numel() == nC and x.numel() == nC
+            calls.append(1)
+            a.mg_level(2, "smooth", rhs, x, x_is_zero=True, stream=st)
+    else:
+        def coarse(rhs, x, st):
+            calls.append(1)
+            rc = lib.nrs_elliptic_mg_level(a._h, 2, 2, C.c_void_p(rhs), None, C.c_void_p(x), 1,
+                                           C.c_void_p(st.cuda_stream))
+            assert rc == 0
+    b.set_user_coarse_solver(coarse, tensors=tensors)
+    za = torch.zeros_like(r)
+    a.preconditioner(r, za)
+    b.preconditioner(r, zb)
+    torch.cuda.synchronize()
+    assert len(calls) == 1 and torch.equal(za, zb)
+    f = _rhs(m, 7)
+    xa = torch.zeros_like(r)
+    xb = torch.zeros_like(r)
+    sa = a.solve(torch.from_numpy(f).cuda(), xa)
+    sb = b.solve(torch.from_numpy(f).cuda(), xb)
+    print("coarse hook: iterations", sa["iterations"], sb["iterations"], flush=True)
+    assert sa["converged"] and sb["iterations"] == sa["iterations"] and torch.equal(xa, xb)
