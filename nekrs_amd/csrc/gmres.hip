@@ -119,15 +119,14 @@ residual_kernel(int64_t n, const double *__restrict__ w, const double *__restric
 
 // out[j] = sum_b partials[j * nb + b], j < k
 __global__ void __launch_bounds__(GT)
-finish_multi_kernel(int nb, int k, const double *__restrict__ partials, double *__restrict__ out)
+finish_multi_kernel(int nb, const double *__restrict__ partials, double *__restrict__ out)
 {
-  for (int j = 0; j < k; ++j) {
-    double v[1] = {0.0};
-    for (int b = threadIdx.x; b < nb; b += GT) v[0] += partials[j * nb + b];
-    block_sums<1>(v);
-    if (threadIdx.x == 0) out[j] = v[0];
-    __syncthreads();
-  }
+  // one block per sum (blockIdx.x = j), partials added in a fixed order
+  const int j = blockIdx.x;
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += GT) v[0] += partials[j * nb + b];
+  block_sums<1>(v);
+  if (threadIdx.x == 0) out[j] = v[0];
 }
 
 template <template <int> class F, typename... A>
@@ -152,7 +151,7 @@ template <int K> struct MultiDot {
   {
     const unsigned g = ggrid(n);
     multi_dot_kernel<K><<<g, GT, 0, s>>>(n, k, ld, w, V, x, partials);
-    finish_multi_kernel<<<1, GT, 0, s>>>((int)g, k, partials, y);
+    finish_multi_kernel<<<k, GT, 0, s>>>((int)g, partials, y);
   }
 };
 template <int K> struct GramSchmidt {
@@ -161,7 +160,7 @@ template <int K> struct GramSchmidt {
   {
     const unsigned g = ggrid(n);
     gram_schmidt_kernel<K><<<g, GT, 0, s>>>(n, k, ld, w, y, V, x, partials);
-    finish_multi_kernel<<<1, GT, 0, s>>>((int)g, 1, partials, nw);
+    finish_multi_kernel<<<1, GT, 0, s>>>((int)g, partials, nw);
   }
 };
 template <int K> struct Update {
@@ -198,7 +197,7 @@ void gmres_residual(int64_t n, const double *w, const double *b, const double *A
 {
   const unsigned g = ggrid(n);
   residual_kernel<<<g, GT, 0, s>>>(n, w, b, Ax, r, partials);
-  finish_multi_kernel<<<1, GT, 0, s>>>((int)g, 1, partials, nr);
+  finish_multi_kernel<<<1, GT, 0, s>>>((int)g, partials, nr);
   NRS_LAUNCH_CHECK();
 }
 
