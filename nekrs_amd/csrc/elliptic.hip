@@ -19,6 +19,8 @@ nrs_elliptic::~nrs_elliptic()
   if (scalHost) (void)hipHostFree(scalHost);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+  if (evPre0) (void)hipEventDestroy(evPre0);
+  if (evPre1) (void)hipEventDestroy(evPre1);
   for (auto e : eventPool) (void)hipEventDestroy(e);
   for (auto e : chunkEv) if (e) (void)hipEventDestroy(e);
   if (auxDone) (void)hipEventDestroy(auxDone);
@@ -273,6 +275,8 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   }
   NRS_HIP(hipEventCreate(&el->ev0));
   NRS_HIP(hipEventCreate(&el->ev1));
+  NRS_HIP(hipEventCreate(&el->evPre0));
+  NRS_HIP(hipEventCreate(&el->evPre1));
 
   // Jacobi: ellipticUpdateJacobi (src/solvers/elliptic/ellipticUpdateJacobi.cpp:62-85)
   if (o->preconditioner == NRS_PRECON_JACOBI) {
@@ -468,13 +472,30 @@ static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, dou
   iter = 0;
   rdotr = res0;
   if (!(nr > 0.0)) return;   // zero residual: x = 0 is the solution
-  auto precon = [&](const double *in, double *out) {
+  // timing: inside the Arnoldi loop by events read after that iteration's
+  // readback (no extra synchronisation), elsewhere by a synchronisation
+  bool pending = false;
+  auto precon = [&](const double *in, double *out, bool deferred) {
+    if (timed && deferred) {
+      NRS_HIP(hipEventRecord(el->evPre0, s));
+      elliptic_preconditioner(el, in, out, s);
+      NRS_HIP(hipEventRecord(el->evPre1, s));
+      pending = true;
+      return;
+    }
     auto p0 = std::chrono::steady_clock::now();
     elliptic_preconditioner(el, in, out, s);
     if (timed) {
       NRS_HIP(hipStreamSynchronize(s));
       precon_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
     }
+  };
+  auto collect = [&]() {
+    if (!pending) return;
+    float ms = 0.0f;
+    NRS_HIP(hipEventElapsedTime(&ms, el->evPre0, el->evPre1));
+    precon_s += 1e-3 * ms;
+    pending = false;
   };
   // gmresUpdate (PGMRES.cpp:59-104): back substitution, then x += Z y (flexible)
   // or x += M^-1 (V y)
@@ -491,7 +512,7 @@ static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, dou
       double *zt = el->zv.get();
       fill<double>(L, 0.0, zt, s);
       gmres_update(L, k, L, yd, V, zt, s);
-      precon(zt, w);
+      precon(zt, w, false);
       axpby<double>(L, 1.0, w, 1.0, x, s);
     }
   };
@@ -504,12 +525,13 @@ static void gmres_solve(nrs_elliptic *el, double *r, double *x, double res0, dou
     bool done = false;
     for (int i = 0; i < m; ++i) {
       double *Mv = flexible ? Z + (int64_t)i * L : Z;
-      precon(V + (int64_t)i * L, Mv);                               // z = M^-1 V_i
+      precon(V + (int64_t)i * L, Mv, true);                         // z = M^-1 V_i
       elliptic_operator(el, Mv, w, true, s);                        // w = A z
       gmres_multi_dot(L, i + 1, L, wgt, V, w, part, yd, s);        // y_j = <V_j, w>
       gmres_gram_schmidt(L, i + 1, L, wgt, yd, V, w, part, yd + kGmresMaxVectors, s);
       NRS_HIP(hipMemcpyAsync(hy.data(), yd, (kGmresMaxVectors + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
       NRS_HIP(hipStreamSynchronize(s));
+      collect();
       const double nw = std::sqrt(hy[kGmresMaxVectors]);
       H[i + 1 + i * ldH] = nw;
       if (i < m - 1) axpby<double>(L, 1.0 / nw, w, 0.0, V + (int64_t)(i + 1) * L, s);
@@ -593,13 +615,10 @@ int elliptic_krylov(nrs_elliptic *el, double *r, double *x, double res0, double 
   const int perCheck = (none || jacobi) ? 4 : 1;
   for (;;) {
     for (int b = 0; b < perCheck; ++b) {
-      if (!none && !jacobi) {
-        auto p0 = std::chrono::steady_clock::now();
+      if (!none && !jacobi) {   // perCheck = 1 here: the events are read after the readback
+        if (timed) NRS_HIP(hipEventRecord(el->evPre0, s));
         elliptic_preconditioner(el, r, z, s);
-        if (timed) {  // per-phase timing only when stats are requested
-          NRS_HIP(hipStreamSynchronize(s));
-          precon_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
-        }
+        if (timed) NRS_HIP(hipEventRecord(el->evPre1, s));
       }
       pcg_precon_dots(R, L, w, el->invDiagA.get(), r, z, Ap, jacobi, flexible, none, scal, s);
       pcg_update_p(L, z, p, scal, s);
@@ -609,6 +628,11 @@ int elliptic_krylov(nrs_elliptic *el, double *r, double *x, double res0, double 
     }
     NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
     NRS_HIP(hipStreamSynchronize(s));
+    if (timed && !none && !jacobi) {
+      float ms = 0.0f;
+      NRS_HIP(hipEventElapsedTime(&ms, el->evPre0, el->evPre1));
+      precon_s += 1e-3 * ms;
+    }
     rdotr = el->scalHost[S_RDOTR];
     iter = (int)el->scalHost[S_ITER];
     if (std::isnan(rdotr)) return NRS_ERR_NAN;

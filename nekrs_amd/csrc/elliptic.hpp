@@ -62,6 +62,7 @@ struct nrs_elliptic {
   std::unique_ptr<nrs::Reducer> red;
   std::unique_ptr<nrs::MGSolver> mg;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evPre0 = nullptr, evPre1 = nullptr;   // around one preconditioner apply (timing)
 
   // Chunked operator: the masked handle's rows are grouped into kOpChunks
   // element chunks; Ax of chunk c+1 runs on the caller's stream while the
