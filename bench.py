@@ -429,7 +429,9 @@ def block_velocity(m, seed):
     """Three velocity components as ONE block solve (Nfields = 3: vectors
     [3][Nlocal], one gather-scatter of the three fields, inner products summed
     over the fields), flexible CG with the mixed p-MG applied to each field,
-    lambda1 = 10, to 1e-8."""
+    lambda1 = 10, to 1e-8; and the reference's default velocity solver, block
+    PCG with Jacobi, on a time step's velocity operator (mu = 0.01, rho bd0/dt =
+    1500) with the same right-hand side."""
     import numpy as np
     import torch
     import nekrs_amd
@@ -445,6 +447,19 @@ def block_velocity(m, seed):
     dofs = 3 * m.E * m.N ** 3
     out = {"iterations": st["iterations"], "seconds": st["seconds"], "converged": st["converged"],
            "dof_iterations_per_s": dofs * st["iterations"] / st["seconds"]}
+    del el
+    # the reference's default velocity solver, block PCG with Jacobi (parReader.cpp:996-998,1097),
+    # on a time step's velocity Helmholtz operator: mu = 0.01, rho * bd0 / dt = 1.5 / 1e-3
+    el = nekrs_amd.Elliptic(m, lambda0=0.01, lambda1=1500.0, poisson=False, nfields=3,
+                            solver="pcg", precon="jacobi", tol=1e-8, max_iter=2000)
+    for rep in range(2):
+        x = torch.zeros_like(rhs)
+        torch.cuda.synchronize()
+        st = el.solve(rhs.clone(), x)
+    out["jacobi_pcg"] = {"operator": "mu=0.01, rho*bd0/dt=1500",
+                         "iterations": st["iterations"], "seconds": st["seconds"],
+                         "converged": st["converged"],
+                         "dof_iterations_per_s": dofs * st["iterations"] / st["seconds"]}
     del el
     return out
 
@@ -483,8 +498,11 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
             (block_n, v["seconds_batched"], v["seconds_sequential"], v["iterations"]))
         out["velocity3_block"] = block_velocity(m, seed + 2)
         vb = out["velocity3_block"]
-        log("velocity block (3 fields, one operator, FCG + p-MG per field): %d it, %.3f s" %
-            (vb["iterations"], vb["seconds"]))
+        log("velocity block (3 fields, one operator, FCG + p-MG per field): %d it, %.3f s; "
+            "block Jacobi-PCG (the reference's default) on a time step's velocity operator: "
+            "%d it, %.3f s" %
+            (vb["iterations"], vb["seconds"], vb["jacobi_pcg"]["iterations"],
+             vb["jacobi_pcg"]["seconds"]))
     out["workload"] = ("%d Helmholtz problems (seed %d): N in {3,5,7,9}, E ~ U[512,8192] as "
                        "near-cubic boxes, lambda1 ~ U[0,10], Kershaw eps ~ U[0.05,0.3]; plus 3 "
                        "velocity components (lambda1=10) at %d^3 N=7 eps 0.3, as a batch and as "
