@@ -560,10 +560,27 @@ int nrs_re2_read(const char *path, double *vx, double *vy, double *vz, int *grou
  * edge; replaces nek's set_glo_num / parCon connectivity), then every node is
  * numbered by its vertex, edge (ordered by its end vertices), face (ordered
  * from its smallest vertex) or element interior.  Periodic faces are not
- * connected.  x, y, z, globalIds: host [E*(N+1)^3]. */
+ * connected here (nrs_mesh_from_vertices_periodic connects them).
+ * x, y, z, globalIds: host [E*(N+1)^3]. */
 int nrs_mesh_from_vertices(int N, int Nelements, const double *vx, const double *vy,
                            const double *vz, double tol, double *x, double *y, double *z,
                            long long *globalIds);
+
+/* Periodic face pairs of a .re2 file: for every 'P' boundary record (buf_to_bc,
+ * reader_re2.f:503-541: bc(1) = the connected element, bc(2) = its face),
+ * peer[(e*6 + f)*2 + 0/1] = the connected element (0-based) and face (solver
+ * order); -1 where face f of element e is not periodic.  peer: host [nelgt*6*2]. */
+int nrs_re2_periodic(const char *path, int *peer);
+/* nrs_mesh_from_vertices with periodic connectivity (nek's periodic vertex
+ * numbering in set_glo_num / parCon): the corners of each face pair in `peer`
+ * (as nrs_re2_periodic gives it) are one vertex, matched by position relative
+ * to the face centres, so the nodes of paired faces share global ids.  Fails
+ * (NRS_ERR_INVALID) when the identification joins two corners of one element
+ * or two different edges (fewer than three elements across a periodic
+ * direction). */
+int nrs_mesh_from_vertices_periodic(int N, int Nelements, const double *vx, const double *vy,
+                                    const double *vz, double tol, const int *peer, double *x, double *y,
+                                    double *z, long long *globalIds);
 
 #ifdef __cplusplus
 }

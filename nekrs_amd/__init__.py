@@ -132,30 +132,44 @@ def read_re2(path):
               group.ctypes.data, cbc.ctypes.data)
     codes = np.array([bytes(cbc[3 * n:3 * n + 3]).decode("ascii", "replace")
                       for n in range(E * 6)]).reshape(E, 6)
+    peer = np.zeros(E * 12, dtype=np.int32)
+    _lib.call("nrs_re2_periodic", path_b, peer.ctypes.data)
     return {"nelgt": E, "nelgv": nelgv.value, "ncurve": ncurve.value, "nbc": nbc.value,
-            "vx": vx, "vy": vy, "vz": vz, "group": group, "cbc": codes}
+            "vx": vx, "vy": vy, "vz": vz, "group": group, "cbc": codes,
+            "peer": peer.reshape(E, 6, 2)}
 
 
-def mesh_from_vertices(N, vx, vy, vz, tol=0.0):
+def mesh_from_vertices(N, vx, vy, vz, tol=0.0, peer=None):
     """(x, y, z, globalIds) of the order-N GLL nodes of a conforming hex mesh
-    from its corners [E*8] (lexicographic), nrs_mesh_from_vertices."""
+    from its corners [E*8] (lexicographic), nrs_mesh_from_vertices; with peer
+    ([E][6][2] periodic face pairs, -1 elsewhere) the paired faces are connected
+    (nrs_mesh_from_vertices_periodic)."""
     vx, vy, vz = (np.ascontiguousarray(a, dtype=np.float64) for a in (vx, vy, vz))
     E = vx.size // 8
     L = E * (N + 1) ** 3
     x, y, z = np.zeros(L), np.zeros(L), np.zeros(L)
     gid = np.zeros(L, dtype=np.int64)
-    _lib.call("nrs_mesh_from_vertices", N, E, vx.ctypes.data, vy.ctypes.data, vz.ctypes.data,
-              float(tol), x.ctypes.data, y.ctypes.data, z.ctypes.data, gid.ctypes.data)
+    if peer is None:
+        _lib.call("nrs_mesh_from_vertices", N, E, vx.ctypes.data, vy.ctypes.data, vz.ctypes.data,
+                  float(tol), x.ctypes.data, y.ctypes.data, z.ctypes.data, gid.ctypes.data)
+    else:
+        pr = np.ascontiguousarray(peer, dtype=np.int32).ravel()
+        assert pr.size == E * 12
+        _lib.call("nrs_mesh_from_vertices_periodic", N, E, vx.ctypes.data, vy.ctypes.data,
+                  vz.ctypes.data, float(tol), pr.ctypes.data, x.ctypes.data, y.ctypes.data,
+                  z.ctypes.data, gid.ctypes.data)
     return x, y, z, gid
 
 
 # .re2 boundary codes -> velocity boundary types (src/bdry/bcType.h)
-RE2_BC = {"W": 1, "v": 3, "V": 3, "O": 16, "o": 16, "ON": 15, "SYM": 7, "int": 2, "E": 0, "": 0}
+RE2_BC = {"W": 1, "v": 3, "V": 3, "O": 16, "o": 16, "ON": 15, "SYM": 7, "int": 2, "E": 0, "P": 0,
+          "": 0}
 
 
 class Re2Mesh:
     """A mesh read from a .re2 file at order N: coordinates (trilinear in the
-    corners, curved sides not applied), global ids from the corner topology,
+    corners, curved sides not applied), global ids from the corner topology
+    with periodic ('P') face pairs connected,
     EToB = 1 on faces whose code is in `dirichlet` (for the pressure solve the
     outflow faces), and bc_types for the pressure right-hand side."""
 
@@ -167,8 +181,10 @@ class Re2Mesh:
         self.Nlocal = self.E * self.Np
         self.cbc = d["cbc"]
         self.ncurve = d["ncurve"]
+        self.peer = d["peer"]
+        periodic = self.peer if np.any(self.peer >= 0) else None
         self.x, self.y, self.z, self.gid = mesh_from_vertices(N, d["vx"] * scale, d["vy"] * scale,
-                                                              d["vz"] * scale)
+                                                              d["vz"] * scale, peer=periodic)
         codes = np.char.strip(self.cbc.astype(str))
         self.EToB = np.isin(codes, list(dirichlet)).astype(np.int32).ravel()
         unknown = sorted(set(codes.ravel()) - set(RE2_BC))

@@ -682,6 +682,26 @@ int nrs_re2_read(const char *path, double *vx, double *vy, double *vz, int *grou
   NRS_API_END
 }
 
+int nrs_re2_periodic(const char *path, int *peer)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(path && peer, "re2: null argument");
+  const nrs::Re2Header h = nrs::re2_header(path);
+  std::vector<double> vx((size_t)h.nelgt * 8), vy(vx.size()), vz(vx.size());
+  nrs::re2_read(path, vx.data(), vy.data(), vz.data(), nullptr, nullptr, peer);
+  NRS_API_END
+}
+
+int nrs_mesh_from_vertices_periodic(int N, int Nelements, const double *vx, const double *vy,
+                                    const double *vz, double tol, const int *peer, double *x, double *y,
+                                    double *z, long long *globalIds)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(vx && vy && vz && peer && x && y && z && globalIds, "mesh: null argument");
+  nrs::mesh_from_vertices(N, Nelements, vx, vy, vz, tol, x, y, z, globalIds, peer);
+  NRS_API_END
+}
+
 int nrs_mesh_from_vertices(int N, int Nelements, const double *vx, const double *vy,
                            const double *vz, double tol, double *x, double *y, double *z,
                            long long *globalIds)

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <unordered_map>
 
 #include "mesh.hpp"
@@ -100,7 +101,7 @@ Re2Header re2_header(const char *path)
   return h;
 }
 
-void re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc)
+void re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc, int *peer)
 {
   const Re2Header h = re2_header(path);
   File fh(path);
@@ -121,18 +122,27 @@ void re2_read(const char *path, double *vx, double *vy, double *vz, int *group, 
   NRS_CHECK(std::fseek(fh.f, 8 + (long)h.ncurve * 8 * 8, SEEK_CUR) == 0, "re2: seek");
   double nb = 0.0;
   read_exact(fh.f, &nb, 8, "boundary count");
-  if (cbc) {
-    std::memset(cbc, ' ', (size_t)h.nelgt * 6 * 3);
+  if (cbc || peer) {
+    if (cbc) std::memset(cbc, ' ', (size_t)h.nelgt * 6 * 3);
+    if (peer) std::fill(peer, peer + (size_t)h.nelgt * 12, -1);
     std::vector<double> b(8);
     for (int r = 0; r < h.nbc; ++r) {
       read_exact(fh.f, b.data(), 64, "boundary record");
-      double eg = b[0], f = b[1];
+      double eg = b[0], f = b[1], c0 = b[2], c1 = b[3];
       char code[8];
       std::memcpy(code, &b[7], 8);
-      if (h.swap) { eg = swap8(eg); f = swap8(f); }
+      if (h.swap) { eg = swap8(eg); f = swap8(f); c0 = swap8(c0); c1 = swap8(c1); }
       const int e = (int)eg - 1, face = (int)f - 1;
       NRS_CHECK(e >= 0 && e < h.nelgt && face >= 0 && face < 6, "re2: boundary record out of range");
-      std::memcpy(cbc + ((size_t)e * 6 + kFaceMap[face]) * 3, code, 3);
+      const size_t sf = (size_t)e * 6 + kFaceMap[face];
+      if (cbc) std::memcpy(cbc + sf * 3, code, 3);
+      // periodic: bc(1) = the connected element, bc(2) = its face (buf_to_bc, reader_re2.f:503-541)
+      if (peer && code[0] == 'P' && code[1] == ' ') {
+        const int e2 = (int)c0 - 1, f2 = (int)c1 - 1;
+        NRS_CHECK(e2 >= 0 && e2 < h.nelgt && f2 >= 0 && f2 < 6, "re2: periodic record out of range");
+        peer[sf * 2] = e2;
+        peer[sf * 2 + 1] = kFaceMap[f2];
+      }
     }
   }
 }
@@ -190,12 +200,100 @@ std::vector<long long> vertex_ids(int E, const double *vx, const double *vy, con
   return ids;
 }
 
+namespace {
+
+// corners (lexicographic i + 2j + 4k) of the solver's faces 0..5
+const int kSolverFaceCorners[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {1, 3, 5, 7},
+                                      {2, 3, 6, 7}, {0, 2, 4, 6}, {4, 5, 6, 7}};
+
+long long find_root(std::vector<long long> &par, long long a)
+{
+  while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; }
+  return a;
+}
+
+// Vertex ids with the corners of periodic face pairs identified: each corner
+// of face (e, f) is matched with the corner of its peer face that lies closest
+// after the translation between the two face centres.  Refuses meshes whose
+// periodic identification makes two corners of one element, or two different
+// edges, the same (fewer than three elements across a periodic direction).
+void connect_periodic(int E, const double *vx, const double *vy, const double *vz, const int *peer,
+                      std::vector<long long> &vid)
+{
+  long long nv = 0;
+  for (long long v : vid) nv = std::max(nv, v);
+  std::vector<long long> par(nv + 1);
+  for (long long i = 0; i <= nv; ++i) par[i] = i;
+  auto P = [&](int64_t c) { return std::array<double, 3>{vx[c], vy[c], vz[c]}; };
+  for (int e = 0; e < E; ++e)
+    for (int f = 0; f < 6; ++f) {
+      const int e2 = peer[((size_t)e * 6 + f) * 2], f2 = peer[((size_t)e * 6 + f) * 2 + 1];
+      if (e2 < 0) continue;
+      NRS_CHECK(e2 < E && f2 >= 0 && f2 < 6, "mesh: periodic peer out of range");
+      std::array<double, 3> c1{0, 0, 0}, c2{0, 0, 0};
+      for (int q = 0; q < 4; ++q)
+        for (int d = 0; d < 3; ++d) {
+          c1[d] += 0.25 * P((int64_t)e * 8 + kSolverFaceCorners[f][q])[d];
+          c2[d] += 0.25 * P((int64_t)e2 * 8 + kSolverFaceCorners[f2][q])[d];
+        }
+      for (int q = 0; q < 4; ++q) {
+        const int64_t a = (int64_t)e * 8 + kSolverFaceCorners[f][q];
+        int64_t best = -1;
+        double bd = 1e300;
+        for (int q2 = 0; q2 < 4; ++q2) {
+          const int64_t b = (int64_t)e2 * 8 + kSolverFaceCorners[f2][q2];
+          double d2 = 0;
+          for (int d = 0; d < 3; ++d) {
+            const double t = (P(b)[d] - c2[d]) - (P(a)[d] - c1[d]);
+            d2 += t * t;
+          }
+          if (d2 < bd) { bd = d2; best = b; }
+        }
+        const long long ra = find_root(par, vid[a]), rb = find_root(par, vid[best]);
+        if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
+      }
+    }
+  // compact 1-based ids in order of first appearance of the roots
+  std::vector<long long> label(nv + 1, 0);
+  long long next = 0;
+  for (auto &v : vid) {
+    const long long r = find_root(par, v);
+    if (!label[r]) label[r] = ++next;
+    v = label[r];
+  }
+  const int edges[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3},
+                            {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  std::map<std::pair<long long, long long>, std::array<double, 3>> dir;
+  for (int e = 0; e < E; ++e) {
+    for (int a = 0; a < 8; ++a)
+      for (int b = a + 1; b < 8; ++b)
+        NRS_CHECK(vid[(size_t)e * 8 + a] != vid[(size_t)e * 8 + b],
+                  "mesh: periodic connection joins two corners of one element (fewer than three "
+                  "elements across a periodic direction)");
+    for (auto &ed : edges) {
+      int64_t a = (int64_t)e * 8 + ed[0], b = (int64_t)e * 8 + ed[1];
+      if (vid[a] > vid[b]) std::swap(a, b);
+      const std::array<double, 3> v{vx[b] - vx[a], vy[b] - vy[a], vz[b] - vz[a]};
+      auto it = dir.find({vid[a], vid[b]});
+      if (it == dir.end()) { dir[{vid[a], vid[b]}] = v; continue; }
+      const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      double d2 = 0;
+      for (int d = 0; d < 3; ++d) d2 += (it->second[d] - v[d]) * (it->second[d] - v[d]);
+      NRS_CHECK(std::sqrt(d2) <= 1e-6 * len, "mesh: periodic connection makes two different edges one "
+                                             "(fewer than three elements across a periodic direction)");
+    }
+  }
+}
+
+}  // namespace
+
 void mesh_from_vertices(int N, int E, const double *vx, const double *vy, const double *vz, double tol,
-                        double *x, double *y, double *z, long long *gids)
+                        double *x, double *y, double *z, long long *gids, const int *peer)
 {
   NRS_CHECK(N >= 1 && N <= 9, "mesh: N must be 1..9");
   NRS_CHECK(E > 0, "mesh: no elements");
-  const std::vector<long long> vid = vertex_ids(E, vx, vy, vz, tol);
+  std::vector<long long> vid = vertex_ids(E, vx, vy, vz, tol);
+  if (peer) connect_periodic(E, vx, vy, vz, peer, vid);
   const std::vector<long long> g = coarse_global_ids(1, E, vid.data(), N);
   std::vector<double> r, w;
   gll_nodes(N, r, w);

@@ -16,12 +16,14 @@ Re2Header re2_header(const char *path);
 // vx, vy, vz [nelgt][8] corners in lexicographic order (i + 2 j + 4 k); group
 // [nelgt] (may be null); cbc [nelgt][6][3] boundary codes of the first field in
 // the solver's face order (0 t-, 1 s-, 2 r+, 3 s+, 4 r-, 5 t+; blanks inside)
-void re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc);
+void re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc,
+              int *peer = nullptr);
 // 1-based ids of the corners [E][8]: corners closer than tol share an id
 // (tol <= 0: 1e-3 of the shortest element edge)
 std::vector<long long> vertex_ids(int E, const double *vx, const double *vy, const double *vz, double tol);
-// GLL node coordinates (trilinear map of the corners) and global ids at order N
+// GLL node coordinates (trilinear map of the corners) and global ids at order N;
+// peer [E*6*2] (or null): periodic face pairs whose corners are identified
 void mesh_from_vertices(int N, int E, const double *vx, const double *vy, const double *vz, double tol,
-                        double *x, double *y, double *z, long long *gids);
+                        double *x, double *y, double *z, long long *gids, const int *peer = nullptr);
 
 }  // namespace nrs

@@ -20,8 +20,10 @@ FILE_FACE_TO_SOLVER = [1, 2, 3, 4, 0, 5]      # meshNekReader.cpp:31
 FILE_FACE_CORNERS = [(0, 1, 4, 5), (1, 3, 5, 7), (2, 3, 6, 7), (0, 2, 4, 6), (0, 1, 2, 3), (4, 5, 6, 7)]
 
 
-def write(path, corners, codes=None, group=None, version="#v002", byteorder="<"):
-    """corners: [E][8][3] lexicographic; codes: {(e, file_face 0..5): "W  "}."""
+def write(path, corners, codes=None, group=None, version="#v002", byteorder="<", params=None):
+    """corners: [E][8][3] lexicographic; codes: {(e, file_face 0..5): "W  "};
+    params: {(e, file_face): (bc1, bc2)} - for "P  " the connected element
+    (1-based) and its file face (1..6), as buf_to_bc reads them."""
     corners = np.asarray(corners, dtype=np.float64)
     E = corners.shape[0]
     codes = codes or {}
@@ -36,7 +38,8 @@ def write(path, corners, codes=None, group=None, version="#v002", byteorder="<")
     buf += struct.pack(byteorder + "d", 0.0)            # no curved sides
     buf += struct.pack(byteorder + "d", float(len(codes)))
     for (e, f), c in sorted(codes.items()):
-        buf += struct.pack(byteorder + "7d", float(e + 1), float(f + 1), 0, 0, 0, 0, 0)
+        p1, p2 = (params or {}).get((e, f), (0.0, 0.0))
+        buf += struct.pack(byteorder + "7d", float(e + 1), float(f + 1), float(p1), float(p2), 0, 0, 0)
         buf += c.ljust(8).encode()[:8]
     with open(path, "wb") as fh:
         fh.write(bytes(buf))
