@@ -243,8 +243,11 @@ typedef struct {
                                   * of consecutive elements that meet i = N to
                                   * i = 0 inside the Ax kernel and leave those
                                   * nodes out of the operator's gather-scatter
-                                  * (same results bit for bit); with autotune on
-                                  * it is kept only where it times faster; 0: off */
+                                  * (same results bit for bit), in blocks of four
+                                  * elements, or of two where that times faster;
+                                  * with autotune on it is kept only where it
+                                  * times faster than no fusion; 2: blocks of two;
+                                  * 0: off */
   int preDegree[8];              /* per multigrid level (0: chebyDegree): Chebyshev
                                   * degree of the smoothing before restriction
                                   * (x = 0), DownLegChebyshevDegree               */

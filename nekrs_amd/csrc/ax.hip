@@ -255,29 +255,43 @@ void launch_ax_variant(int N, int mult, int E, const int *elist, const T *ggeo, 
   }
 }
 
-// Ax with the face fusion (N = 7, four elements per block); fx [E] relative to q
-template <typename T>
-void launch_ax_fused(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1, bool poisson,
-                     const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s)
+// Ax with the face fusion (N = 7, M elements per block); fx [E] relative to q
+template <typename T, int M>
+static void launch_ax_fused_m(int E, const T *ggeo, const T *D, const T *l0, const T *l1, bool poisson,
+                              const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s)
 {
-  NRS_CHECK(N == 7 && fx != nullptr, "Ax face fusion: order 7 only");
-  using S = AxShape<8, 4>;
+  using S = AxShape<8, M>;
+  static_assert(S::NE == M, "one element per wave at N = 7");
   const unsigned grid = (unsigned)((E + S::NE - 1) / S::NE);
   if (grid == 0) return;
   if (poisson)
-    ax_kernel<T, 8, 4, true, false, true><<<grid, S::THREADS, 0, s>>>(E, nullptr, ggeo, D, l0, l1, q, Aq,
+    ax_kernel<T, 8, M, true, false, true><<<grid, S::THREADS, 0, s>>>(E, nullptr, ggeo, D, l0, l1, q, Aq,
                                                                       maskBits, fx);
   else
-    ax_kernel<T, 8, 4, false, false, true><<<grid, S::THREADS, 0, s>>>(E, nullptr, ggeo, D, l0, l1, q, Aq,
+    ax_kernel<T, 8, M, false, false, true><<<grid, S::THREADS, 0, s>>>(E, nullptr, ggeo, D, l0, l1, q, Aq,
                                                                        maskBits, fx);
   NRS_LAUNCH_CHECK();
 }
+
+// nb: elements per block, the block the face-fusion flags were made for (fx[e] = 0 for e % nb = nb - 1)
+template <typename T>
+void launch_ax_fused(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1, bool poisson,
+                     const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s, int nb)
+{
+  NRS_CHECK(N == 7 && fx != nullptr, "Ax face fusion: order 7 only");
+  if (nb == 2)
+    launch_ax_fused_m<T, 2>(E, ggeo, D, l0, l1, poisson, q, Aq, maskBits, fx, s);
+  else if (nb == 4)
+    launch_ax_fused_m<T, 4>(E, ggeo, D, l0, l1, poisson, q, Aq, maskBits, fx, s);
+  else
+    throw Error(NRS_ERR_INVALID, "Ax face fusion: blocks of 2 or 4 elements");
+}
 template void launch_ax_fused<double>(int, int, const double *, const double *, const double *,
                                       const double *, bool, const double *, double *, const uint32_t *,
-                                      const uint8_t *, hipStream_t);
+                                      const uint8_t *, hipStream_t, int);
 template void launch_ax_fused<float>(int, int, const float *, const float *, const float *,
                                      const float *, bool, const float *, float *, const uint32_t *,
-                                     const uint8_t *, hipStream_t);
+                                     const uint8_t *, hipStream_t, int);
 
 // elements-per-block multiplier: the autotuner's choice for (N, type), else 1
 template <typename T>

@@ -90,7 +90,7 @@ std::vector<int> dirichlet_mask(int N, int E, const int *EToB, nrs_ogs *ogsFull,
   return ids;
 }
 
-std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids)
+std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, int nb)
 {
   std::vector<uint8_t> fx;
   if (N != 7 || E < 2) return fx;
@@ -98,7 +98,7 @@ std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids)
   fx.assign(E, 0);
   int64_t pairs = 0;
   for (int64_t e = 0; e + 1 < E; ++e) {
-    if (e % 4 == 3) continue;                       // partner in the next block
+    if (e % nb == nb - 1) continue;                 // partner in the next block
     bool ok = true;
     for (int k = 1; k < Nq - 1 && ok; ++k)
       for (int j = 1; j < Nq - 1 && ok; ++j) {
@@ -141,6 +141,8 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   // ellipticSetup.cpp:59-73: multigrid with non-Jacobi smoothers and its coarse solve
   // are Poisson-only in the reference; kept as in the reference.
   auto el = std::make_unique<nrs_elliptic>();
+  std::vector<long long> fusion2Ids;   // operator face fusion in blocks of two (timed at setup)
+  std::vector<uint8_t> fusion2;
   el->N = d->N; el->Nq = d->N + 1; el->Np = el->Nq * el->Nq * el->Nq; el->E = d->Nelements;
   el->Nf = d->Nfields <= 1 ? 1 : d->Nfields;
   NRS_CHECK(el->Nf == 1 || el->Nf == 3, "elliptic setup: Nfields must be 1 or 3");
@@ -189,11 +191,18 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     for (int id : mids) mg[id] = 0;
     el->ogs = ogs_setup(el->Nlocal, mg.data());
     if (o->operatorFusion && el->Nf == 1) {
-      const std::vector<uint8_t> fx = face_fusion(el->N, el->E, mg);
+      const int nb = o->operatorFusion == 2 ? 2 : 4;
+      std::vector<long long> m4(mg);
+      const std::vector<uint8_t> fx = face_fusion(el->N, el->E, m4, nb);
       if (!fx.empty()) {
-        el->ogsOp = ogs_setup(el->Nlocal, mg.data());
+        el->ogsOp = ogs_setup(el->Nlocal, m4.data());
         el->fx.upload(fx);
+        el->fusionBlock = nb;
         for (uint8_t f : fx) el->fusedPairs += f;
+        if (o->autotune && nb == 4) {   // the pairs-only form (blocks of two) is timed against it below
+          fusion2Ids = mg;
+          fusion2 = face_fusion(el->N, el->E, fusion2Ids, 2);
+        }
       }
     }
   }
@@ -233,7 +242,34 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
       const int cFus = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
                                                  el->lam0.get(), el->lam1.get(), el->poisson,
                                                  el->maskBits.get(), el->ogsOp, s, el->fx.get());
-      const double tFus = op_ms(el->ogsOp, el->fx.get(), cFus);
+      double tFus = op_ms(el->ogsOp, el->fx.get(), cFus);
+      int cBest = cFus;
+      if (!fusion2.empty()) {   // blocks of two elements: fewer pairs, a barrier over two waves
+        constexpr int F = nrs_elliptic::kOpChunks;
+        const int64_t epc = ((el->E + F - 1) / F + 3) / 4 * 4;
+        nrs_ogs *op2 = ogs_setup(el->Nlocal, fusion2Ids.data());
+        ogs_set_chunks(op2, epc * el->Np, F);
+        DevBuf<uint8_t> fx2;
+        fx2.upload(fusion2);
+        el->fusionBlock = 2;
+        const int c2 = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
+                                                 el->lam0.get(), el->lam1.get(), el->poisson,
+                                                 el->maskBits.get(), op2, s, fx2.get());
+        const double t2 = op_ms(op2, fx2.get(), c2);
+        if (t2 < tFus) {
+          delete el->ogsOp;
+          el->ogsOp = op2;
+          el->fx = std::move(fx2);
+          el->fusedPairs = 0;
+          for (uint8_t f : fusion2) el->fusedPairs += f;
+          tFus = t2;
+          cBest = c2;
+        } else {
+          delete op2;
+          el->fusionBlock = 4;
+        }
+        set_tuned(TUNE_OP, el->N, NRS_F64, el->E, cBest);
+      }
       if (!(tFus < tUnf)) {
         delete el->ogsOp;
         el->ogsOp = nullptr;
@@ -308,7 +344,7 @@ void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
     const uint32_t *mb = maskBits ? maskBits + a * NW : nullptr;
     if (fx)
       launch_ax_fused<T>(N, (int)n, ggeo + a * 7 * Np, D, l0, l1, poisson, q + a * Np, Aq + a * Np, mb,
-                         fx + a, s);
+                         fx + a, s, el->fusionBlock);
     else
       launch_ax<T>(N, (int)n, nullptr, ggeo + a * 7 * Np, D, l0, l1, poisson, false, q + a * Np,
                    Aq + a * Np, mb, s);

@@ -31,6 +31,7 @@ struct nrs_elliptic {
   // are summed inside Ax and left out of ogsOp, the operator's handle.
   nrs_ogs *ogsOp = nullptr;
   nrs::DevBuf<uint8_t> fx;
+  int fusionBlock = 4;               // elements per block of the fused Ax kernel (2 or 4)
   int64_t fusedPairs = 0;
   int64_t Nmasked = 0;
   nrs::DevBuf<int> maskIds;
@@ -132,7 +133,7 @@ int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
 // pairs (e, e+1), e % 4 != 3, that share face i = N / i = 0 node for node,
 // return their flags [E] and the ids with those face-interior nodes zeroed (the
 // operator's gather-scatter ids).  Empty flags when no pair qualifies.
-std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids);
+std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, int nb = 4);
 
 // Shared helpers for the multigrid levels.
 std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds);

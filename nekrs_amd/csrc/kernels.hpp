@@ -29,7 +29,7 @@ void launch_mask(int64_t Nm, const int *ids, T *v, hipStream_t s);
 // i = 0 interior nodes in the kernel (N = 7); blocks are 4 elements from q's start
 template <typename T>
 void launch_ax_fused(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1, bool poisson,
-                     const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s);
+                     const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s, int nb = 4);
 
 // mesh.hip
 void launch_geofacs(int N, int E, const double *D, const double *w, const double *x,

@@ -94,3 +94,25 @@ def test_autotune_keeps_fusion_only_when_faster(gpu, torch_mod):
     tuned.operator(q, a)
     off.operator(q, b)
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(8, 3, 2), (5, 5, 2)])
+def test_operator_fused_blocks_of_two_equals_unfused(gpu, torch_mod, shape):
+    """operatorFusion = 2: the fused Ax kernel with two elements per block (pairs
+    e, e+1 for even e) gives the unfused operator bit for bit at every chunk count."""
+    torch = torch_mod
+    m = gpu.BoxMesh(*shape, 7, "kershaw", 0.3)
+    two = gpu.Elliptic(m, precon="jacobi", autotune=0, operatorFusion=2)
+    four = gpu.Elliptic(m, precon="jacobi", autotune=0, operatorFusion=1)
+    off = gpu.Elliptic(m, precon="jacobi", autotune=0, operatorFusion=0)
+    print(shape, "fused pairs: blocks of two", two.fused_pairs, "of four", four.fused_pairs, flush=True)
+    assert 0 < two.fused_pairs < four.fused_pairs
+    q = torch.from_numpy(np.random.default_rng(2).uniform(-1, 1, m.Nlocal)).cuda()
+    for masked in (True, False):
+        a, b = torch.empty_like(q), torch.empty_like(q)
+        off.operator(q, b, masked=masked)
+        for chunks in (1, 4, 16):
+            gpu.set_variant("op", 7, "f64", m.E, chunks)
+            two.operator(q, a, masked=masked)
+            assert torch.equal(a, b), (masked, chunks)
+        gpu.set_variant("op", 7, "f64", m.E, 0)
