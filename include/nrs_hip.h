@@ -151,6 +151,40 @@ int nrs_ogs_inv_degree(nrs_ogs *ogs, double *invDegree, void *stream);
 int nrs_ogs_info(nrs_ogs *ogs, int64_t *Nrows, int64_t *Nmembers, int64_t *Nsingletons);
 
 /* ------------------------------------------------------------------------- */
+/* linAlg_t reductions and the PCG residual update                            */
+/* ------------------------------------------------------------------------- */
+/* The reductions of linAlg_t (src/linAlg/linAlg.hpp:156-215, kernels/linAlg/*):
+ *   NRS_RED_SUM     sum x                    (sum / sumMany)
+ *   NRS_RED_MAX     max x                    (max)
+ *   NRS_RED_MIN     min x                    (min)
+ *   NRS_RED_NORM2   sqrt(sum x^2)            (norm2 / norm2Many)
+ *   NRS_RED_NORM1   sum |x|                  (norm1 / norm1Many)
+ *   NRS_RED_AMAX    max |x|                  (amax / amaxMany)
+ *   NRS_RED_WNORM2  sqrt(sum_i w_i sum_f x^2)  (weightedNorm2Many)
+ *   NRS_RED_WDOT    sum_i w_i sum_f x y        (weightedInnerProdMany)
+ *   NRS_RED_DOT     sum x y                  (innerProd)
+ * over N entries of Nfields fields spaced fieldOffset apart (x, y device
+ * [Nfields][fieldOffset]; w device [N], read only by the weighted forms; y only
+ * by the inner products).  The result is written to the host double *result;
+ * the call synchronises `stream`.  N = 0 gives 0 (sums, norms, amax), -inf
+ * (max) or +inf (min).  The order of the additions is fixed (one partial per
+ * block of a grid that depends only on N, combined in a fixed tree), so a result
+ * is reproducible bit for bit. */
+enum nrs_reduction { NRS_RED_SUM = 0, NRS_RED_MAX = 1, NRS_RED_MIN = 2, NRS_RED_NORM2 = 3,
+                     NRS_RED_NORM1 = 4, NRS_RED_AMAX = 5, NRS_RED_WNORM2 = 6, NRS_RED_WDOT = 7,
+                     NRS_RED_DOT = 8 };
+int nrs_linalg_reduce(int op, int64_t N, int Nfields, int64_t fieldOffset, const double *w,
+                      const double *x, const double *y, double *result, void *stream);
+
+/* r -= alpha Ap on Nfields fields spaced `offset` apart and *rdotr (host) =
+ * sum_i invDegree_i sum_f r[i + f offset]^2 of the updated r (no square root).
+ * Replaces ellipticBlockUpdatePCG (kernels/elliptic/ellipticBlockUpdatePCG.c:27-50,
+ * called from PCG.cpp:67-83).  invDegree device [N]; Ap, r device
+ * [Nfields][offset].  Synchronises `stream`. */
+int nrs_block_update_pcg(int64_t N, int Nfields, int64_t offset, const double *invDegree,
+                         const double *Ap, double alpha, double *r, double *rdotr, void *stream);
+
+/* ------------------------------------------------------------------------- */
 /* Elliptic solver (elliptic_t + ellipticSolve)                               */
 /* ------------------------------------------------------------------------- */
 

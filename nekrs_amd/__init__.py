@@ -22,7 +22,8 @@ from ._lib import NrsError
 __all__ = ["NrsError", "init", "parse_mg_schedule", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
            "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh", "fdm_1d",
            "fdm_element_lengths", "Batch", "make_desc", "make_opts", "tuned_variant", "set_variant",
-           "PressureRhs", "BC", "read_re2", "mesh_from_vertices", "Re2Mesh"]
+           "PressureRhs", "BC", "read_re2", "mesh_from_vertices", "Re2Mesh", "linalg_reduce",
+           "block_update_pcg"]
 
 OPS = {"add": _lib.ADD, "mul": _lib.MUL, "min": _lib.MIN, "max": _lib.MAX}
 MAPS = {"none": 0, "sine": 1, "kershaw": 2}
@@ -286,6 +287,34 @@ class Ogs:
 # velocity boundary types of src/bdry/bcType.h (0: interior face)
 BC = {"interior": 0, "wall": 1, "interface": 2, "inlet": 3, "symx": 4, "symy": 5, "symz": 6,
       "sym": 7, "outflow": 16}
+
+
+REDUCTIONS = {"sum": 0, "max": 1, "min": 2, "norm2": 3, "norm1": 4, "amax": 5,
+              "weighted_norm2": 6, "weighted_inner_prod": 7, "inner_prod": 8}
+
+
+def linalg_reduce(op, x, y=None, w=None, N=None, nfields=1, field_offset=None, stream=None):
+    """linAlg_t reductions (sum, max, min, norm2, norm1, amax, weighted_norm2
+    = weightedNorm2Many, weighted_inner_prod = weightedInnerProdMany, inner_prod)
+    over N entries of `nfields` fields spaced `field_offset` apart; a float."""
+    if N is None:
+        N = x.numel() // nfields
+    off = N if field_offset is None else field_offset
+    out = C.c_double()
+    _lib.call("nrs_linalg_reduce", REDUCTIONS[op], N, nfields, off, _ptr(w), _ptr(x), _ptr(y),
+              C.byref(out), _stream(stream))
+    return out.value
+
+
+def block_update_pcg(inv_degree, Ap, alpha, r, N=None, nfields=1, offset=None, stream=None):
+    """ellipticBlockUpdatePCG: r -= alpha Ap in place; returns sum w r^2."""
+    if N is None:
+        N = inv_degree.numel()
+    off = N if offset is None else offset
+    out = C.c_double()
+    _lib.call("nrs_block_update_pcg", N, nfields, off, _ptr(inv_degree), _ptr(Ap), float(alpha),
+              _ptr(r), C.byref(out), _stream(stream))
+    return out.value
 
 
 class PressureRhs:

@@ -250,6 +250,41 @@ int nrs_ogs_info(nrs_ogs *ogs, int64_t *Nrows, int64_t *Nmembers, int64_t *Nsing
   NRS_API_END
 }
 
+int nrs_linalg_reduce(int op, int64_t N, int Nfields, int64_t fieldOffset, const double *w,
+                      const double *x, const double *y, double *result, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(result, "linalg reduce: null result");
+  NRS_CHECK(op >= NRS_RED_SUM && op <= NRS_RED_DOT, "linalg reduce: unknown reduction");
+  NRS_CHECK(N >= 0 && Nfields >= 1, "linalg reduce: N must be >= 0 and Nfields >= 1");
+  NRS_CHECK(Nfields == 1 || fieldOffset >= N, "linalg reduce: fieldOffset must be >= N");
+  NRS_CHECK(N == 0 || x, "linalg reduce: null x");
+  NRS_CHECK(N == 0 || !(op == NRS_RED_DOT || op == NRS_RED_WDOT) || y, "linalg reduce: null y");
+  NRS_CHECK(N == 0 || !(op == NRS_RED_WNORM2 || op == NRS_RED_WDOT) || w, "linalg reduce: null weight");
+  const hipStream_t s = as_stream(stream);
+  nrs::DevBuf<double> part((size_t)nrs::reduce_partials_needed(N) + 1);
+  nrs::linalg_reduce(op, N, Nfields, fieldOffset, w, x, y, part.get() + 1, part.get(), s);
+  NRS_HIP(hipMemcpyAsync(result, part.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  NRS_HIP(hipStreamSynchronize(s));
+  NRS_API_END
+}
+
+int nrs_block_update_pcg(int64_t N, int Nfields, int64_t offset, const double *invDegree,
+                         const double *Ap, double alpha, double *r, double *rdotr, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(rdotr, "update pcg: null rdotr");
+  NRS_CHECK(N >= 0 && Nfields >= 1, "update pcg: N must be >= 0 and Nfields >= 1");
+  NRS_CHECK(Nfields == 1 || offset >= N, "update pcg: offset must be >= N");
+  NRS_CHECK(N == 0 || (invDegree && Ap && r), "update pcg: null argument");
+  const hipStream_t s = as_stream(stream);
+  nrs::DevBuf<double> part((size_t)nrs::reduce_partials_needed(N) + 1);
+  nrs::block_update_pcg(N, Nfields, offset, invDegree, Ap, alpha, r, part.get() + 1, part.get(), s);
+  NRS_HIP(hipMemcpyAsync(rdotr, part.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  NRS_HIP(hipStreamSynchronize(s));
+  NRS_API_END
+}
+
 void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
 {
   if (!o) return;

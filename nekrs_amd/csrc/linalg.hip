@@ -280,7 +280,125 @@ template <typename A, typename B> __global__ void convert_kernel(int64_t n, cons
 constexpr int EB = 256;
 inline unsigned egrid(int64_t n) { return grid_for(n, EB, 8192); }
 
+// ------------------------------------------------------------------ linAlg_t reductions
+// One streaming kernel over N entries of Nfields fields spaced fieldOffset
+// apart, one value per thread accumulated in a fixed order, one partial per
+// block; a one-block kernel combines the partials in a fixed tree order.
+template <int OP> struct RedTraits {
+  static constexpr bool kMax = OP == NRS_RED_MAX || OP == NRS_RED_AMAX;
+  static constexpr bool kMin = OP == NRS_RED_MIN;
+  static __device__ __forceinline__ double identity()
+  {
+    return kMax ? (OP == NRS_RED_AMAX ? 0.0 : -HUGE_VAL) : (kMin ? HUGE_VAL : 0.0);
+  }
+  static __device__ __forceinline__ double combine(double a, double b)
+  {
+    return kMax ? fmax(a, b) : (kMin ? fmin(a, b) : a + b);
+  }
+};
+
+template <int OP>
+__device__ __forceinline__ double red_block(double v, double *s)
+{
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = RedTraits<OP>::combine(v, __shfl_down(v, off, 64));
+  if (lane == 0) s[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = s[0];
+#pragma unroll
+    for (int k = 1; k < RT / 64; ++k) v = RedTraits<OP>::combine(v, s[k]);
+  }
+  return v;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(RT)
+reduce_many_kernel(int64_t n, int nf, int64_t off, const double *__restrict__ w,
+                   const double *__restrict__ x, const double *__restrict__ y, double *partials)
+{
+  __shared__ double s[RT / 64];
+  double v = RedTraits<OP>::identity();
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    double t = 0.0;
+    for (int f = 0; f < nf; ++f) {
+      const double a = x[i + f * off];
+      double c;
+      switch (OP) {
+        case NRS_RED_SUM: case NRS_RED_MAX: case NRS_RED_MIN: c = a; break;
+        case NRS_RED_NORM2: case NRS_RED_WNORM2: c = a * a; break;
+        case NRS_RED_NORM1: case NRS_RED_AMAX: c = fabs(a); break;
+        default: c = a * y[i + f * off]; break;   // NRS_RED_DOT, NRS_RED_WDOT
+      }
+      t = f == 0 ? c : RedTraits<OP>::combine(t, c);
+    }
+    if (OP == NRS_RED_WNORM2 || OP == NRS_RED_WDOT) t *= w[i];
+    v = RedTraits<OP>::combine(v, t);
+  }
+  v = red_block<OP>(v, s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(RT) reduce_finish_kernel(int nb, const double *__restrict__ partials, double *out)
+{
+  __shared__ double s[RT / 64];
+  double v = RedTraits<OP>::identity();
+  for (int b = threadIdx.x; b < nb; b += RT) v = RedTraits<OP>::combine(v, partials[b]);
+  v = red_block<OP>(v, s);
+  if (threadIdx.x == 0) out[0] = (OP == NRS_RED_NORM2 || OP == NRS_RED_WNORM2) ? sqrt(v) : v;
+}
+
+// ellipticBlockUpdatePCG (kernels/elliptic/ellipticBlockUpdatePCG.c:27-50):
+// r -= alpha Ap on every field, partial sums of invDegree r^2.
+__global__ void __launch_bounds__(RT)
+block_update_pcg_kernel(int64_t n, int nf, int64_t off, const double *__restrict__ w,
+                        const double *__restrict__ Ap, double alpha, double *__restrict__ r,
+                        double *partials)
+{
+  __shared__ double s[RT / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    const double wi = w[i];
+    for (int f = 0; f < nf; ++f) {
+      const int64_t k = i + f * off;
+      const double rn = r[k] - alpha * Ap[k];
+      v += rn * rn * wi;
+      r[k] = rn;
+    }
+  }
+  v = red_block<NRS_RED_SUM>(v, s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
 }  // namespace
+
+int reduce_partials_needed(int64_t n) { return rgrid(n); }
+
+void linalg_reduce(int op, int64_t n, int nf, int64_t off, const double *w, const double *x,
+                   const double *y, double *partials, double *out, hipStream_t s)
+{
+  const int g = rgrid(n);
+#define NRS_RED_CASE(OP_)                                                                         case OP_:                                                                                         reduce_many_kernel<OP_><<<g, RT, 0, s>>>(n, nf, off, w, x, y, partials);                        reduce_finish_kernel<OP_><<<1, RT, 0, s>>>(g, partials, out);                                   break;
+  switch (op) {
+    NRS_RED_CASE(NRS_RED_SUM) NRS_RED_CASE(NRS_RED_MAX) NRS_RED_CASE(NRS_RED_MIN)
+    NRS_RED_CASE(NRS_RED_NORM2) NRS_RED_CASE(NRS_RED_NORM1) NRS_RED_CASE(NRS_RED_AMAX)
+    NRS_RED_CASE(NRS_RED_WNORM2) NRS_RED_CASE(NRS_RED_WDOT) NRS_RED_CASE(NRS_RED_DOT)
+    default: throw Error(NRS_ERR_INVALID, "linalg reduce: unknown reduction");
+  }
+#undef NRS_RED_CASE
+  NRS_LAUNCH_CHECK();
+}
+
+void block_update_pcg(int64_t n, int nf, int64_t off, const double *w, const double *Ap, double alpha,
+                      double *r, double *partials, double *out, hipStream_t s)
+{
+  const int g = rgrid(n);
+  block_update_pcg_kernel<<<g, RT, 0, s>>>(n, nf, off, w, Ap, alpha, r, partials);
+  reduce_finish_kernel<NRS_RED_SUM><<<1, RT, 0, s>>>(g, partials, out);
+  NRS_LAUNCH_CHECK();
+}
 
 // ------------------------------------------------------------------ host side
 

@@ -66,6 +66,15 @@ void pcg_alpha(Reducer &R, int64_t n, const double *w, const double *p, const do
 // r -= alpha Ap; x += alpha p; rdotr = sqrt(<r,r>_w * resNormFactor)
 void pcg_update_rx(Reducer &R, int64_t n, const double *w, const double *Ap, const double *p,
                    double *r, double *x, double resNormFactor, double *scal, hipStream_t s);
+// linAlg_t reductions over n entries of nf fields spaced off apart (op: nrs_reduction);
+// partials: device scratch of at least reduce_partials_needed(n) doubles; out: one
+// device double.  Deterministic: fixed per-thread order, fixed tree order.
+int reduce_partials_needed(int64_t n);
+void linalg_reduce(int op, int64_t n, int nf, int64_t off, const double *w, const double *x,
+                   const double *y, double *partials, double *out, hipStream_t s);
+// ellipticBlockUpdatePCG: r -= alpha Ap (nf fields), out = sum_i w_i sum_f r^2
+void block_update_pcg(int64_t n, int nf, int64_t off, const double *w, const double *Ap, double alpha,
+                      double *r, double *partials, double *out, hipStream_t s);
 // scal[slot] = sqrt(<a,a>_w * factor)
 void wnorm(Reducer &R, int64_t n, const double *w, const double *a, double factor, double *scal,
            int slot, hipStream_t s);

@@ -30,6 +30,8 @@ Precision: the setup is FP64, the operators are stored in the smoother type T
 whose inputs (S and each contracted work array) are rounded to that format with
 FP32 accumulation -- the mixed-precision variant the GPU path offers.
 """
+import os
+
 import numpy as np
 
 from . import BoxMesh, Ogs, DIRICHLET, gll
@@ -260,7 +262,26 @@ def pre_fdm(u, N):
 
 def fdm_box(w, S, invL, T=np.float32, fmt=None):
     """(Sz x Sy x Sx) invL (Sz x Sy x Sx)^T on each box (fusedFDM.c:121-175).
-    S: [3][E][nl][nl] as S[d][e][component][eigenvector]; invL: [E][k][j][i]."""
+    S: [3][E][nl][nl] as S[d][e][component][eigenvector]; invL: [E][k][j][i].
+    Large inputs are split into element chunks handled by a thread pool (numpy
+    releases the GIL in these loops); every element is computed alone, so the
+    result does not depend on the split."""
+    E = w.shape[0]
+    if E >= 1024:
+        import concurrent.futures as cf
+        nt = max(1, min(16, os.cpu_count() or 1))
+        cuts = np.linspace(0, E, nt + 1).astype(int)
+        out = np.empty(w.shape, dtype=T)
+
+        def part(a, b):
+            out[a:b] = _fdm_box(w[a:b], S[:, a:b], invL[a:b], T, fmt)
+        with cf.ThreadPoolExecutor(nt) as ex:
+            list(ex.map(lambda ab: part(*ab), zip(cuts[:-1], cuts[1:])))
+        return out
+    return _fdm_box(w, S, invL, T, fmt)
+
+
+def _fdm_box(w, S, invL, T, fmt):
     if fmt is None:
         Sx, Sy, Sz = S[0], S[1], S[2]
         t = np.einsum("elm,ekjl->ekjm", Sx, w)

@@ -30,7 +30,7 @@ def problems(gpu):
     rng = np.random.default_rng(0)
     out = []
     for i, (shape, N) in enumerate([((2, 2, 2), 3), ((3, 2, 2), 5), ((2, 3, 2), 7), ((3, 3, 2), 3),
-                                    ((2, 2, 3), 5), ((2, 2, 2), 7)]):
+                                    ((2, 2, 3), 5), ((2, 2, 2), 7), ((2, 2, 2), 9), ((3, 2, 2), 9)]):
         eps = float(rng.uniform(0.0, 0.3))
         m = gpu.BoxMesh(*shape, N, "kershaw", max(eps, 1e-3))
         lam = float(rng.uniform(0.0, 10.0))
@@ -89,7 +89,9 @@ def test_batch_equals_standalone(gpu, torch_mod):
                                                  (7, 8.0, "ras", "smooth", 0.25),
                                                  (5, 0.5, "jacobi", "smooth", 0.25),
                                                  (5, 4.0, "ras", "smooth_jacobi", 0.1),
-                                                 (7, 0.0, "asm", "smooth_jacobi", 0.3)])
+                                                 (7, 0.0, "asm", "smooth_jacobi", 0.3),
+                                                 (9, 2.0, "ras", "smooth", 0.2),
+                                                 (9, 6.0, "asm", "smooth_jacobi", 0.3)])
 def test_helmholtz_mg_smoothed_coarse_vs_oracle(gpu, torch_mod, orc, N, lam, S, coarse, eps):
     import oracle.mg as omg
     torch = torch_mod

@@ -54,7 +54,8 @@ def test_lambda_max_and_vcycle(gpu, torch_mod, orc, prec):
     assert rel(z.cpu().numpy(), zo) < tol
 
 
-@pytest.mark.parametrize("levels", [[7, 3, 1], [7, 5, 3, 1], [5, 3, 1], [3, 1]])
+@pytest.mark.parametrize("levels", [[7, 3, 1], [7, 5, 3, 1], [5, 3, 1], [3, 1], [9, 5, 3, 1],
+                                    [9, 3, 1]])
 def test_mg_fcg_solve_vs_oracle(gpu, torch_mod, orc, levels):
     import oracle.mg as omg
     torch = torch_mod

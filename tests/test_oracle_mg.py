@@ -187,3 +187,97 @@ def test_jacobi_smoothed_coarse_level(orc):
                            maxit=5000)
     assert info["iterations"] < ij["iterations"] // 4
     assert np.max(np.abs(x - xj)) <= 1e-7 * np.max(np.abs(xj))
+
+
+def _unique_keep(L):
+    keep = np.zeros(L.E * L.Np, bool)
+    seen = set()
+    masked = set(L.el.maskIds.tolist())
+    for i, g in enumerate(L.mesh.gid):
+        if i not in masked and g not in seen:
+            seen.add(g)
+            keep[i] = True
+    return keep
+
+
+def _fourth_kind_E(k, lam, rho):
+    """E_k(lam) = W_k(1 - 2 lam / rho) / (2k + 1), W_k(cos t) = sin((k + 1/2) t) / sin(t / 2),
+    E_0 = 1: the error propagator of the (unoptimised) 4th-kind Chebyshev
+    smoother of degree k (Lottes, arXiv:2202.08830, eq. for the 4th-kind
+    polynomials); evaluated through the three-term recurrence of W_k,
+    W_0 = 1, W_1 = 2x + 1, W_{j+1} = 2x W_j - W_{j-1}, which is valid for every real x."""
+    x = 1.0 - 2.0 * lam / rho
+    w0, w1 = np.ones_like(x), 2.0 * x + 1.0
+    if k == 0:
+        return w0
+    for _ in range(k - 1):
+        w0, w1 = w1, 2.0 * x * w1 - w0
+    return w1 / (2.0 * k + 1.0)
+
+
+@pytest.mark.parametrize("degree", [2, 3, 4])
+def test_fourth_kind_chebyshev_error_propagator(orc, monkeypatch, degree):
+    """The optimal 4th-kind smoother (smoothFourthKindChebyshev,
+    ellipticMultiGridLevel.cpp:212-270) pinned by its closed form, on the
+    eigenbasis of S A: with every beta = 1 the error propagator is E_k(lambda);
+    with a one-hot beta = e_j it is 1 - E_{j-1}(lambda) + E_j(lambda) (the j-th
+    update direction alone).  A shifted beta index, a wrong rC or dC, or a
+    wrong 4/(3 rho) start changes these polynomials and fails."""
+    import oracle.mg as omg
+    mo = orc.BoxMesh(2, 2, 2, 3, "kershaw", 0.3)
+    om = omg.MG(mo, [3, 1], T=np.float64, degree=degree, cheby="opt4", coarse_tol=1e-14,
+                coarse_maxit=1000)
+    L = om.levels[0]
+    keep = _unique_keep(L)
+    A = _dense(lambda v: om.op(L, v), mo.gid, keep)
+    S = np.diag(L.invDiagT[keep])
+    col = {g: c for c, g in enumerate(mo.gid[keep])}
+    idx = np.array([col.get(g, -1) for g in mo.gid])
+
+    def propagator():
+        def smooth_unique(b_u):
+            b = np.where(idx >= 0, b_u[np.maximum(idx, 0)], 0.0)
+            b[L.el.maskIds] = 0
+            x = np.zeros(mo.Nlocal)
+            om.smooth(L, b, x, True)
+            return x[keep]
+        n = keep.sum()
+        return np.eye(n) - np.array([smooth_unique(c) for c in A.T]).T
+
+    lam, V = np.linalg.eig(S @ A)
+    lam = lam.real
+    rho = L.lmax
+    assert lam.max() < rho and lam.min() > 0
+    cases = [(np.ones(degree), _fourth_kind_E(degree, lam, rho))]
+    for j in range(1, degree + 1):
+        beta = np.zeros(degree)
+        beta[j - 1] = 1.0
+        cases.append((beta, 1.0 - _fourth_kind_E(j - 1, lam, rho) + _fourth_kind_E(j, lam, rho)))
+    for beta, expect in cases:
+        monkeypatch.setitem(omg.OPT4_BETAS, degree, list(beta))
+        E = propagator()
+        got = np.diag(np.linalg.solve(V, E @ V)).real
+        assert np.allclose(got, expect, atol=1e-9), (beta, np.max(np.abs(got - expect)))
+        # and E is diagonal in that basis (a polynomial in S A)
+        off = np.linalg.solve(V, E @ V)
+        assert np.max(np.abs(off - np.diag(np.diag(off)))) < 1e-8
+    monkeypatch.undo()
+    # the optimised betas: the propagator stays a polynomial in S A with |E| < 1 on (0, rho]
+    E = propagator()
+    got = np.diag(np.linalg.solve(V, E @ V)).real
+    assert np.max(np.abs(got)) < 1.0
+    expect = np.zeros_like(lam)
+    b = omg.OPT4_BETAS[degree]
+    for j in range(1, degree + 1):
+        expect += b[j - 1] * (_fourth_kind_E(j - 1, lam, rho) - _fourth_kind_E(j, lam, rho))
+    assert np.allclose(got, 1.0 - expect, atol=1e-9)
+
+
+def test_fourth_kind_polynomials_closed_form():
+    """The recurrence used above against the trigonometric definition."""
+    t = np.linspace(0.1, 3.0, 17)
+    rho = 2.0
+    lam = 0.5 * rho * (1.0 - np.cos(t))          # 1 - 2 lam / rho = cos t
+    for k in range(0, 7):
+        w = np.sin((k + 0.5) * t) / np.sin(0.5 * t)
+        assert np.allclose(_fourth_kind_E(k, lam, rho), w / (2 * k + 1), atol=1e-13)
