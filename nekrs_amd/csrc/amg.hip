@@ -130,13 +130,28 @@ double rho_dinv_a(const CsrHost &A, const std::vector<double> &dinv, const AmgPa
   return lam;
 }
 
-// dense inverse of an SPD matrix by Cholesky (row-major n x n)
-std::vector<double> dense_inverse(const CsrHost &A)
+// dense inverse of an SPD matrix by Cholesky (row-major n x n).  B given: A is
+// symmetric positive semidefinite with null space span{B}; with b = B / |B|,
+// A + a b b^T (a = the largest diagonal entry) is SPD, its inverse maps b to
+// b / a and agrees with A^+ on the range of A, so P (A + a b b^T)^-1 P with
+// P = I - b b^T is the pseudo-inverse A^+.
+std::vector<double> dense_inverse(const CsrHost &A, const std::vector<double> *B)
 {
   const int n = A.n;
   std::vector<double> M((size_t)n * n, 0.0), L((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i)
     for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) M[(size_t)i * n + A.ci[p]] += A.v[p];
+  std::vector<double> b;   // unit null vector
+  if (B) {
+    double nb = 0.0, shift = 0.0;
+    for (double v : *B) nb += v * v;
+    nb = std::sqrt(nb);
+    b.resize(n);
+    for (int i = 0; i < n; ++i) b[i] = (*B)[i] / nb;
+    for (int i = 0; i < n; ++i) shift = std::max(shift, M[(size_t)i * n + i]);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) M[(size_t)i * n + j] += shift * b[i] * b[j];
+  }
   for (int j = 0; j < n; ++j) {
     double d = M[(size_t)j * n + j];
     for (int k = 0; k < j; ++k) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
@@ -162,6 +177,18 @@ std::vector<double> dense_inverse(const CsrHost &A)
       double s = y[i];
       for (int k = i + 1; k < n; ++k) s -= L[(size_t)k * n + i] * inv[(size_t)k * n + c];
       inv[(size_t)i * n + c] = s / L[(size_t)i * n + i];
+    }
+  }
+  if (B) {   // inv <- P inv P, P = I - b b^T (rows, then columns projected)
+    for (int i = 0; i < n; ++i) {
+      double m = 0.0;
+      for (int j = 0; j < n; ++j) m += inv[(size_t)i * n + j] * b[j];
+      for (int j = 0; j < n; ++j) inv[(size_t)i * n + j] -= m * b[j];
+    }
+    for (int j = 0; j < n; ++j) {
+      double m = 0.0;
+      for (int i = 0; i < n; ++i) m += b[i] * inv[(size_t)i * n + j];
+      for (int i = 0; i < n; ++i) inv[(size_t)i * n + j] -= b[i] * m;
     }
   }
   return inv;
@@ -283,6 +310,10 @@ std::vector<AmgHostLevel> amg_build(const CsrHost &A0, const AmgParams &p)
   NRS_CHECK(A0.n == A0.m && A0.n > 0, "amg: square nonempty matrix required");
   std::vector<AmgHostLevel> out;
   CsrHost A = A0;
+  // null vector carried down the levels (singular problems): T B_c = B, so the
+  // Galerkin matrices keep B as null vector; without a null space B = 1 on
+  // every level (unit aggregate indicators)
+  std::vector<double> B(A0.n, 1.0);
   for (;;) {
     AmgHostLevel h;
     h.A = A;
@@ -294,7 +325,7 @@ std::vector<AmgHostLevel> amg_build(const CsrHost &A0, const AmgParams &p)
       h.dinv[i] = 1.0 / d;
     }
     if (n <= p.coarsest || (int)out.size() + 1 == p.maxLevels) {
-      h.inv = dense_inverse(A);
+      h.inv = dense_inverse(A, p.nullSpace ? &B : nullptr);
       out.push_back(std::move(h));
       break;
     }
@@ -337,14 +368,15 @@ std::vector<AmgHostLevel> amg_build(const CsrHost &A0, const AmgParams &p)
     }
     if (na >= n) {   // nothing to coarsen: solve this level directly
       h.omega = 0.0;
-      h.inv = dense_inverse(A);
+      h.inv = dense_inverse(A, p.nullSpace ? &B : nullptr);
       NRS_CHECK(n <= 8192, "amg: level cannot be coarsened and is too large to invert");
       out.push_back(std::move(h));
       break;
     }
     // tentative prolongation T (unit columns), then P = T - (omega D^-1) (A T)
-    std::vector<int> size(na, 0);
-    for (int i = 0; i < n; ++i) size[agg[i]]++;
+    std::vector<double> norm(na, 0.0);   // |B on each aggregate| (B = 1: sqrt of its size)
+    for (int i = 0; i < n; ++i) norm[agg[i]] += B[i] * B[i];
+    for (auto &v : norm) v = std::sqrt(v);
     CsrHost T;
     T.n = n;
     T.m = na;
@@ -354,7 +386,7 @@ std::vector<AmgHostLevel> amg_build(const CsrHost &A0, const AmgParams &p)
     for (int i = 0; i < n; ++i) {
       T.rp[i] = i;
       T.ci[i] = agg[i];
-      T.v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
+      T.v[i] = B[i] / norm[agg[i]];
     }
     T.rp[n] = n;
     const CsrHost AT = spgemm(A, T);
@@ -390,6 +422,8 @@ std::vector<AmgHostLevel> amg_build(const CsrHost &A0, const AmgParams &p)
     h.R = transpose(P);
     A = spgemm(spgemm(h.R, A), P);
     out.push_back(std::move(h));
+    if (p.nullSpace) B = norm;
+    else B.assign(na, 1.0);
   }
   return out;
 }

@@ -28,6 +28,9 @@ struct AmgParams {
   int sweeps = 2;                // damped Jacobi sweeps before and after the coarse correction
   int powerSteps = 20;           // for rho(D^-1 A)
   uint64_t seed = 7;
+  // singular Poisson matrix with the constant vector as null space (all-Neumann):
+  // the dense level stores the pseudo-inverse P (A + a 1 1^T)^-1 P, P = I - 1 1^T / n
+  bool nullSpace = false;
 };
 
 // Host side of the hierarchy (kept for tests and the device upload).
@@ -35,7 +38,7 @@ struct AmgHostLevel {
   CsrHost A, P, R;               // P: n x nc, R = P^T
   std::vector<double> dinv;
   double omega = 0.0;            // 4 / (3 rho(D^-1 A))
-  std::vector<double> inv;       // dense inverse (last level only), row-major
+  std::vector<double> inv;       // dense (pseudo-)inverse (last level only), row-major
 };
 std::vector<AmgHostLevel> amg_build(const CsrHost &A, const AmgParams &p);
 

@@ -151,6 +151,17 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   el->Nlocal = (int64_t)el->E * el->Np;
   NRS_CHECK(el->Nlocal < (int64_t)INT32_MAX, "E*Np must fit in int32 (dlong)");
   el->poisson = d->poisson != 0;
+  // allNeumann: Poisson and no face of a boundary type other than NEUMANN (2)
+  // (ellipticSetup.cpp:187-206)
+  if (el->poisson) {
+    bool all = true;
+    for (int64_t f = 0; f < (int64_t)el->E * 6 && all; ++f)
+      if (d->EToB[f] > 0 && d->EToB[f] != 2) all = false;
+    el->allNeumann = all;
+  }
+  if (el->allNeumann && el->Nf > 1)   // ellipticZeroMean.cpp:36-38 aborts for block solvers
+    throw Error(NRS_ERR_UNSUPPORTED, "elliptic setup: the null space of an all-Neumann block solve "
+                                     "is not supported (as in the reference)");
   el->lambda0 = d->lambda0;
   el->lambda1 = d->poisson ? 0.0 : d->lambda1;
   el->opts = *o;
@@ -467,6 +478,11 @@ void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool maske
                            s, true, el->ogsOp ? el->fx.get() : nullptr);
 }
 
+void zero_mean(nrs_elliptic *el, double *q, hipStream_t s)
+{
+  remove_mean(*el->red, el->Nlocal, nullptr, (double)el->Nlocal, q, el->scal.get(), S_TMP1, s);
+}
+
 void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s)
 {
   switch (el->opts.preconditioner) {
@@ -482,6 +498,7 @@ void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipSt
       NRS_HIP(hipMemcpyAsync(z, r, el->Nf * el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
       break;
   }
+  if (el->allNeumann) zero_mean(el, z, s);   // ellipticPreconditioner.cpp:82-83
 }
 
 // Restarted (flexible) PGMRES (src/solvers/elliptic/linearSolver/PGMRES.cpp:111-340):
@@ -625,7 +642,8 @@ int elliptic_krylov(nrs_elliptic *el, double *r, double *x, double res0, double 
   const auto &o = el->opts;
   const bool flexible = o.solver == NRS_SOLVER_FCG;
   const bool none = o.preconditioner == NRS_PRECON_NONE;
-  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI;
+  // Jacobi fused into the dot kernel, unless the mean of z must be removed first
+  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI && !el->allNeumann;
   const double resNormFactor = 1.0 / el->volume;
   NRS_CHECK(maxIter >= 1, "maxIter must be >= 1");
   NRS_CHECK(o.preconditioner != NRS_PRECON_USER || el->userPrecon,
@@ -648,13 +666,13 @@ int elliptic_krylov(nrs_elliptic *el, double *r, double *x, double res0, double 
   // between readbacks.  Every update kernel is a no-op once the device-side
   // flag says the iteration has stopped (converged or maxIter), so x, r and
   // the count are those of checking after each iteration.
-  const int perCheck = (none || jacobi) ? 4 : 1;
+  const int perCheck = (none || o.preconditioner == NRS_PRECON_JACOBI) ? 4 : 1;
   for (;;) {
     for (int b = 0; b < perCheck; ++b) {
-      if (!none && !jacobi) {   // perCheck = 1 here: the events are read after the readback
-        if (timed) NRS_HIP(hipEventRecord(el->evPre0, s));
+      if (!none && !jacobi) {   // timed only with perCheck = 1: the events are read after the readback
+        if (timed && perCheck == 1) NRS_HIP(hipEventRecord(el->evPre0, s));
         elliptic_preconditioner(el, r, z, s);
-        if (timed) NRS_HIP(hipEventRecord(el->evPre1, s));
+        if (timed && perCheck == 1) NRS_HIP(hipEventRecord(el->evPre1, s));
       }
       pcg_precon_dots(R, L, w, el->invDiagA.get(), r, z, Ap, jacobi, flexible, none, scal, s);
       pcg_update_p(L, z, p, scal, s);
@@ -664,7 +682,7 @@ int elliptic_krylov(nrs_elliptic *el, double *r, double *x, double res0, double 
     }
     NRS_HIP(hipMemcpyAsync(el->scalHost, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
     NRS_HIP(hipStreamSynchronize(s));
-    if (timed && !none && !jacobi) {
+    if (timed && !none && !jacobi && perCheck == 1) {
       float ms = 0.0f;
       NRS_HIP(hipEventElapsedTime(&ms, el->evPre0, el->evPre1));
       precon_s += 1e-3 * ms;
@@ -696,6 +714,7 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   // r = rhs - A x0 (element apply only), mask, gs   (ellipticSolve.cpp:91-103)
   element_operator(el, x, el->Ap.get(), s);
   axpby<double>(L, -1.0, el->Ap.get(), 1.0, r, s);
+  if (el->allNeumann) zero_mean(el, r, s);
   mask_fields(el, r, s);
   ogs_gather_scatter<double>(el->ogs, r, el->Nf, el->Nlocal, NRS_ADD, s);
   // x0 = x, x = 0   (:105-106)
@@ -722,6 +741,7 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
   auto finish = [&](int status) {
     if (project && status == NRS_OK) projection_post(el, x, s);
     axpby<double>(L, 1.0, el->x0.get(), 1.0, x, s);
+    if (el->allNeumann) zero_mean(el, x, s);   // ellipticSolve.cpp:188-189
   };
   auto fill_stats = [&](int iter, double rdotr, double precon_s) {
     if (!st) return;
@@ -766,6 +786,7 @@ void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t 
   launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
                     el->lam1.get(), el->poisson, false, x, el->Ap.get(), nullptr, s);
   axpby<double>(L, -1.0, el->Ap.get(), 1.0, rhs, s);
+  if (el->allNeumann) zero_mean(el, rhs, s);
   launch_mask<double>(el->Nmasked, el->maskIds.get(), rhs, s);
   ogs_gather_scatter<double>(el->ogs, rhs, 1, L, NRS_ADD, s);
   axpby<double>(L, 1.0, x, 0.0, el->x0.get(), s);
@@ -781,7 +802,7 @@ void elliptic_iterate(nrs_elliptic *el, double *r, double *x, hipStream_t s)
   const double *w = el->ogs->invDegree.get();
   const auto &o = el->opts;
   const bool none = o.preconditioner == NRS_PRECON_NONE;
-  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI;
+  const bool jacobi = o.preconditioner == NRS_PRECON_JACOBI && !el->allNeumann;
   double *z = none ? r : el->zv.get();
   if (!none && !jacobi) elliptic_preconditioner(el, r, z, s);
   pcg_precon_dots(*el->red, L, w, el->invDiagA.get(), r, z, el->Ap.get(), jacobi,
@@ -795,6 +816,7 @@ void elliptic_iterate(nrs_elliptic *el, double *r, double *x, hipStream_t s)
 void elliptic_solve_end(nrs_elliptic *el, double *x, hipStream_t s)
 {
   axpby<double>(el->Nlocal, 1.0, el->x0.get(), 1.0, x, s);
+  if (el->allNeumann) zero_mean(el, x, s);
 }
 
 }  // namespace nrs

@@ -16,6 +16,10 @@ struct nrs_elliptic {
   int Nf = 1;                        // fields (3: block solve, vectors [3][Nlocal])
   int64_t Nlocal = 0;                // per field
   bool poisson = true;
+  // a Poisson problem without Dirichlet faces (elliptic_t::allNeumann,
+  // ellipticSetup.cpp:187-206): the mean is removed from the residual, the
+  // preconditioned residual and the solution (ellipticZeroMean)
+  bool allNeumann = false;
   double lambda0 = 1.0, lambda1 = 0.0;
   double volume = 0.0;
   nrs_elliptic_opts opts{};
@@ -134,6 +138,10 @@ int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
 // return their flags [E] and the ids with those face-interior nodes zeroed (the
 // operator's gather-scatter ids).  Empty flags when no pair qualifies.
 std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, int nb = 4);
+
+// ellipticZeroMean (src/solvers/elliptic/ellipticZeroMean.cpp:31-45): q -= the
+// mean of its E-vector entries (sum over E*Np, divided by E*Np); enqueued work only.
+void zero_mean(nrs_elliptic *el, double *q, hipStream_t s);
 
 // Shared helpers for the multigrid levels.
 std::vector<uint32_t> mask_bits(int Np, int E, const std::vector<int> &maskIds);

@@ -329,11 +329,12 @@ reduce_many_kernel(int64_t n, int nf, int64_t off, const double *__restrict__ w,
         case NRS_RED_SUM: case NRS_RED_MAX: case NRS_RED_MIN: c = a; break;
         case NRS_RED_NORM2: case NRS_RED_WNORM2: c = a * a; break;
         case NRS_RED_NORM1: case NRS_RED_AMAX: c = fabs(a); break;
+        case RED_WSUM: c = a; break;
         default: c = a * y[i + f * off]; break;   // NRS_RED_DOT, NRS_RED_WDOT
       }
       t = f == 0 ? c : RedTraits<OP>::combine(t, c);
     }
-    if (OP == NRS_RED_WNORM2 || OP == NRS_RED_WDOT) t *= w[i];
+    if (OP == NRS_RED_WNORM2 || OP == NRS_RED_WDOT || OP == RED_WSUM) t *= w[i];
     v = RedTraits<OP>::combine(v, t);
   }
   v = red_block<OP>(v, s);
@@ -348,6 +349,15 @@ __global__ void __launch_bounds__(RT) reduce_finish_kernel(int nb, const double 
   for (int b = threadIdx.x; b < nb; b += RT) v = RedTraits<OP>::combine(v, partials[b]);
   v = red_block<OP>(v, s);
   if (threadIdx.x == 0) out[0] = (OP == NRS_RED_NORM2 || OP == NRS_RED_WNORM2) ? sqrt(v) : v;
+}
+
+// y += factor * scal[slot] (the scalar read on the device)
+__global__ void add_scalar_kernel(int64_t n, double factor, const double *__restrict__ scal, int slot,
+                                  double *__restrict__ y)
+{
+  const double a = factor * scal[slot];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a;
 }
 
 // ellipticBlockUpdatePCG (kernels/elliptic/ellipticBlockUpdatePCG.c:27-50):
@@ -385,10 +395,26 @@ void linalg_reduce(int op, int64_t n, int nf, int64_t off, const double *w, cons
     NRS_RED_CASE(NRS_RED_SUM) NRS_RED_CASE(NRS_RED_MAX) NRS_RED_CASE(NRS_RED_MIN)
     NRS_RED_CASE(NRS_RED_NORM2) NRS_RED_CASE(NRS_RED_NORM1) NRS_RED_CASE(NRS_RED_AMAX)
     NRS_RED_CASE(NRS_RED_WNORM2) NRS_RED_CASE(NRS_RED_WDOT) NRS_RED_CASE(NRS_RED_DOT)
+    NRS_RED_CASE(RED_WSUM)
     default: throw Error(NRS_ERR_INVALID, "linalg reduce: unknown reduction");
   }
 #undef NRS_RED_CASE
   NRS_LAUNCH_CHECK();
+}
+
+void add_device_scalar(int64_t n, double factor, const double *scal, int slot, double *y, hipStream_t s)
+{
+  if (n <= 0) return;
+  add_scalar_kernel<<<egrid(n), EB, 0, s>>>(n, factor, scal, slot, y);
+  NRS_LAUNCH_CHECK();
+}
+
+void remove_mean(Reducer &R, int64_t n, const double *w, double wsum, double *q, double *scal, int slot,
+                 hipStream_t s)
+{
+  if (n <= 0) return;
+  linalg_reduce(w ? RED_WSUM : NRS_RED_SUM, n, 1, n, w, q, nullptr, R.partials.get(), scal + slot, s);
+  add_device_scalar(n, -1.0 / wsum, scal, slot, q, s);
 }
 
 void block_update_pcg(int64_t n, int nf, int64_t off, const double *w, const double *Ap, double alpha,

@@ -70,11 +70,18 @@ void pcg_update_rx(Reducer &R, int64_t n, const double *w, const double *Ap, con
 // partials: device scratch of at least reduce_partials_needed(n) doubles; out: one
 // device double.  Deterministic: fixed per-thread order, fixed tree order.
 int reduce_partials_needed(int64_t n);
+constexpr int RED_WSUM = 16;   // internal: sum_i w_i x_i (not part of the C ABI)
 void linalg_reduce(int op, int64_t n, int nf, int64_t off, const double *w, const double *x,
                    const double *y, double *partials, double *out, hipStream_t s);
 // ellipticBlockUpdatePCG: r -= alpha Ap (nf fields), out = sum_i w_i sum_f r^2
 void block_update_pcg(int64_t n, int nf, int64_t off, const double *w, const double *Ap, double alpha,
                       double *r, double *partials, double *out, hipStream_t s);
+// y += factor * scal[slot]
+void add_device_scalar(int64_t n, double factor, const double *scal, int slot, double *y, hipStream_t s);
+// q -= (sum_i w_i q_i) / wsum (w = null: plain sum), the sum kept in scal[slot];
+// enqueued work only (no host round trip)
+void remove_mean(Reducer &R, int64_t n, const double *w, double wsum, double *q, double *scal, int slot,
+                 hipStream_t s);
 // scal[slot] = sqrt(<a,a>_w * factor)
 void wnorm(Reducer &R, int64_t n, const double *w, const double *a, double factor, double *scal,
            int slot, hipStream_t s);

@@ -304,6 +304,13 @@ struct CoarsePCG {
   // smoothed-aggregation V-cycle; E-vector <-> rows through rowOf / first.
   bool amgMode = false;
   bool deviceOnly = false;   // AMG with tol <= 0: exactly maxit iterations, no host round trip
+  // all-Neumann problem: the coarse matrix is singular with the constant
+  // vector as null space; the right-hand side is made consistent (its mean over
+  // the unique nodes removed) and the AMG's dense level is pseudo-inverted
+  // (the reference tells BoomerAMG the same, coarseLevel.cpp:113-146 ->
+  // hypreWrapper.cpp:143-147)
+  bool nullSpace = false;
+  double uniqueNodes = 0.0;  // sum of invDegree = unique unmasked nodes (E-vector mode)
   int n = 0;
   Amg amg;
   DevBuf<int> rowOf, first, arp, aci;
@@ -369,7 +376,7 @@ struct CoarsePCG {
     if (amgMode) { solve_amg(rhs, xout, s); return; }
     const double *w = ogs->invDegree.get();
     k_cast<T, double>(L, rhs, r.get(), s);
-    if (std::is_same<T, double>::value && (const void *)rhs == (const void *)r.get()) {}
+    if (nullSpace) remove_mean(*red, L, w, uniqueNodes, r.get(), scal.get(), S_TMP1, s);
     fill<double>(L, 0.0, x.get(), s);
     fill<double>(L, 0.0, p.get(), s);
     wnorm(*red, L, w, r.get(), 1.0, scal.get(), S_TMP0, s);
@@ -398,6 +405,7 @@ struct CoarsePCG {
       // device (capturable): the reference's coarse solve is likewise a fixed
       // BoomerAMG cycle count (coarseLevel.cpp:80-93)
       rows_gather<T>(n, first.get(), rhs, ru.get(), s);
+      if (nullSpace) remove_mean(*red, n, nullptr, (double)n, ru.get(), scal.get(), S_TMP1, s);
       fill<double>(n, 0.0, xu.get(), s);
       fill<double>(n, 0.0, pu.get(), s);
       wnorm(*red, n, ones.get(), ru.get(), 1.0, scal.get(), S_RDOTR, s);
@@ -416,6 +424,7 @@ struct CoarsePCG {
       return;
     }
     rows_gather<T>(n, first.get(), rhs, ru.get(), s);
+    if (nullSpace) remove_mean(*red, n, nullptr, (double)n, ru.get(), scal.get(), S_TMP1, s);
     fill<double>(n, 0.0, xu.get(), s);
     fill<double>(n, 0.0, pu.get(), s);
     wnorm(*red, n, ones.get(), ru.get(), 1.0, scal.get(), S_TMP0, s);
@@ -515,7 +524,9 @@ struct CoarsePCG {
     arp.upload(A.rp); aci.upload(A.ci); av.upload(A.v);
     ones.upload(std::vector<double>(n, 1.0));
     ru.alloc(n); xu.alloc(n); zu.alloc(n); pu.alloc(n); Apu.alloc(n); tmp.alloc(n);
-    amg.setup(A, AmgParams{});
+    AmgParams ap;
+    ap.nullSpace = nullSpace;
+    amg.setup(A, ap);
     amgMode = true;
     deviceOnly = !(tol > 0.0);
     NRS_CHECK(!deviceOnly || (maxit >= 1 && maxit <= 64),
@@ -892,6 +903,11 @@ struct MGImpl final : MGSolver {
       NRS_HIP(hipMemset(c.scal.get(), 0, S_COUNT * sizeof(double)));
       NRS_HIP(hipHostMalloc((void **)&c.scalHost, S_COUNT * sizeof(double), hipHostMallocDefault));
       c.red = std::make_unique<Reducer>();
+      c.nullSpace = el->allNeumann;
+      if (c.nullSpace) {
+        const auto wh = l.ogs->invDegree.download();
+        for (double v : wh) c.uniqueNodes += v;
+      }
       if (el->opts.coarseSolver == NRS_COARSE_AMG) c.setup_amg(mg, s);
     }
     NRS_HIP(hipStreamSynchronize(s));

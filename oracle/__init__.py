@@ -63,7 +63,7 @@ class _OrcElliptic(C.Structure):
                 ("ogs", C.POINTER(_OrcOgs)), ("Nmasked", C.c_int64),
                 ("maskIds", C.POINTER(C.c_int64)),
                 ("lambda0", C.c_double), ("lambda1", C.c_double),
-                ("poisson", C.c_int), ("volume", C.c_double)]
+                ("poisson", C.c_int), ("volume", C.c_double), ("allNeumann", C.c_int)]
 
 
 _lib = None
@@ -145,11 +145,21 @@ def kershaw_map(eps, x, y, z):
 
 
 # ---------------------------------------------------------------- mesh
-class BoxMesh:
-    """Structured box [0,1]^3 (lexicographic elements, GLL nodes)."""
+NEUMANN = 2
 
-    def __init__(self, nx, ny, nz, N, map="none", eps=0.0):
+
+class BoxMesh:
+    """Structured box [0,1]^3 (lexicographic elements, GLL nodes).
+
+    bc: "dirichlet" (DIRICHLET = 1 on every boundary face) or "neumann"
+    (NEUMANN = 2, a closed box); periodic: the axes ("x", "y", "z") whose
+    opposite faces are connected (the last node layer takes the ids of the
+    first, those faces carry no boundary type)."""
+
+    def __init__(self, nx, ny, nz, N, map="none", eps=0.0, bc="dirichlet", periodic=""):
+        assert bc in ("dirichlet", "neumann")
         self.nx, self.ny, self.nz, self.N = nx, ny, nz, N
+        self.bc, self.periodic = bc, "".join(sorted(periodic))
         self.Nq = N + 1
         self.Np = self.Nq ** 3
         self.E = nx * ny * nz
@@ -159,6 +169,20 @@ class BoxMesh:
         self.EToB = np.zeros(self.E * 6, dtype=np.int32)
         lib().orc_box_mesh(nx, ny, nz, N, MAP[map], eps, self.x, self.y, self.z, self.gid,
                            self.EToB.ctypes.data)
+        if bc == "neumann":
+            self.EToB[self.EToB == DIRICHLET] = NEUMANN
+        if self.periodic:
+            gx, gy = nx * N + 1, ny * N + 1
+            g = self.gid - 1
+            G = [g % gx, (g // gx) % gy, g // (gx * gy)]
+            for a, (ax, n) in enumerate(zip("xyz", (nx, ny, nz))):
+                if ax in self.periodic:
+                    G[a] = G[a] % (n * N)
+            self.gid = 1 + G[0] + gx * (G[1] + gy * G[2])
+            eb = self.EToB.reshape(-1, 6)
+            for ax, faces in (("x", (2, 4)), ("y", (1, 3)), ("z", (0, 5))):
+                if ax in self.periodic:
+                    eb[:, list(faces)] = 0
         self.r, self.w = gll(N)
         self.D = dmatrix(N, self.r)
         self.ggeo = np.zeros(self.E * 7 * self.Np)
@@ -288,8 +312,14 @@ class Elliptic:
     Mirrors the state ellipticSolveSetup builds: masked ogs, maskIds,
     invDegree (from the masked ogs), constant lambda0/lambda1."""
 
-    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True):
+    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, zero_mean=None):
+        """all_neumann: Poisson with no face of a boundary type other than
+        NEUMANN (ellipticSetup.cpp:187-206); zero_mean (default: all_neumann)
+        makes solve() remove the mean of r, z and x (ellipticZeroMean)."""
         self.mesh = mesh
+        eb = np.asarray(mesh.EToB)
+        self.all_neumann = bool(poisson) and not np.any((eb > 0) & (eb != NEUMANN))
+        self.zero_mean = self.all_neumann if zero_mean is None else bool(zero_mean)
         self.ogs_full = Ogs(mesh.gid)
         self.maskIds = mask_ids(mesh, self.ogs_full)
         self.ogs = Ogs(masked_gids(mesh.gid, self.maskIds))
@@ -301,7 +331,8 @@ class Elliptic:
                                self._D.ctypes.data_as(C.POINTER(C.c_double)), self.ogs._p,
                                len(self.maskIds),
                                self.maskIds.ctypes.data_as(C.POINTER(C.c_int64)),
-                               lambda0, lambda1, int(poisson), self.volume)
+                               lambda0, lambda1, int(poisson), self.volume,
+                               int(self.zero_mean))
 
     def operator(self, q):
         Aq = np.zeros(self.mesh.Nlocal)

@@ -18,7 +18,11 @@ step by step as the library builds it:
                 strong neighbours
   prolongation  P = (I - omega D^-1 A) T, T = aggregate indicator with unit
                 columns, omega = 4 / (3 rho(D^-1 A)), rho from 20 power steps
-                from a fixed splitmix64 vector
+                from a fixed splitmix64 vector; for a singular (all-Neumann)
+                matrix the columns are the null vector B restricted to the
+                aggregate and normalised, B_coarse = the aggregates' norms
+                (standard smoothed aggregation), and the last level is the
+                pseudo-inverse
   coarse matrix P^T A P; levels until n <= 256 or 12 levels; the last is
                 inverted densely (Cholesky)
   V-cycle       pre: 2 sweeps damped Jacobi (omega_s = 4 / (3 rho)), first
@@ -113,10 +117,13 @@ def aggregate(S):
     return agg, na
 
 
-def tentative(agg, na):
+def tentative(agg, na, B=None):
+    """T[i, agg[i]] = B_i / ||B on agg[i]|| (B = 1: unit aggregate indicators) and
+    the coarse null vector T^T B (the aggregates' norms), so that T B_c = B."""
     n = agg.size
-    size = np.bincount(agg, minlength=na).astype(np.float64)
-    return sp.csr_matrix((1.0 / np.sqrt(size[agg]), (np.arange(n), agg)), shape=(n, na))
+    B = np.ones(n) if B is None else B
+    norm = np.sqrt(np.bincount(agg, weights=B * B, minlength=na))
+    return sp.csr_matrix((B / norm[agg], (np.arange(n), agg)), shape=(n, na)), norm
 
 
 def rho_dinv_a(A, seed=7):
@@ -136,23 +143,33 @@ def rho_dinv_a(A, seed=7):
 class AMG:
     """Smoothed-aggregation hierarchy of an SPD CSR matrix."""
 
-    def __init__(self, A, coarsest=COARSEST):
+    def __init__(self, A, coarsest=COARSEST, null_space=False):
+        """null_space: A is singular with the constant vector as null space (an
+        all-Neumann problem): the last level is the Moore-Penrose pseudo-inverse."""
         self.levels = []
+        # with a null space the constant fine vector is carried down as B
+        # (T B_c = B, so every Galerkin matrix keeps B as its null vector) and the
+        # dense level is the pseudo-inverse (the null direction cut off);
+        # without one B stays 1 on every level (unit aggregate indicators)
+        dense_inv = ((lambda M: np.linalg.pinv(M, rcond=1e-10, hermitian=True)) if null_space
+                     else np.linalg.inv)
+        B = np.ones(A.shape[0])
         A = sp.csr_matrix(A)
         while True:
-            lv = {"A": A, "Dinv": 1.0 / A.diagonal()}
+            lv = {"A": A, "Dinv": 1.0 / A.diagonal(), "B": B}
             self.levels.append(lv)
             n = A.shape[0]
             if n <= coarsest or len(self.levels) == MAX_LEVELS:
-                lv["inv"] = np.linalg.inv(A.toarray())
+                lv["inv"] = dense_inv(A.toarray())
                 break
             rho = rho_dinv_a(A)
             lv["omega"] = 4.0 / (3.0 * rho)
             agg, na = aggregate(strength(A))
             if na >= n:                       # no coarsening possible: solve here
-                lv["inv"] = np.linalg.inv(A.toarray())
+                lv["inv"] = dense_inv(A.toarray())
                 break
-            T = tentative(agg, na)
+            T, Bc = tentative(agg, na, B if null_space else None)
+            B = Bc if null_space else np.ones(na)
             P = (T - lv["omega"] * sp.diags(lv["Dinv"]) @ (A @ T)).tocsr()
             lv["P"] = P
             lv["R"] = P.T.tocsr()

@@ -375,7 +375,8 @@ class Schwarz:
         self.S = S.astype(T)
         # extended mesh: ids from an order N+2 box of the same elements (or the
         # caller's order N+2 mesh with gid and EToB, for reoriented elements)
-        me = BoxMesh(m.nx, m.ny, m.nz, m.N + 2) if mesh_ext is None else mesh_ext
+        me = (BoxMesh(m.nx, m.ny, m.nz, m.N + 2, bc=getattr(m, "bc", "dirichlet"),
+                      periodic=getattr(m, "periodic", "")) if mesh_ext is None else mesh_ext)
         mids = extended_mask(me)
         g = me.gid.copy()
         g[mids] = 0

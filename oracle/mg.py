@@ -187,6 +187,12 @@ class MG:
         self.coarse = coarse
         self.coarse_tol, self.coarse_maxit = coarse_tol, coarse_maxit
         self.levels = []
+        # all-Neumann (ellipticSetup.cpp:187-206): the coarse problem is singular;
+        # its right-hand side is made consistent and the AMG's dense level is the
+        # pseudo-inverse (the reference passes the flag to BoomerAMG,
+        # coarseLevel.cpp:113-146)
+        eb = np.asarray(mesh.EToB)
+        self.null_space = bool(poisson) and not np.any((eb > 0) & (eb != 2))
         x, y, z = mesh.x, mesh.y, mesh.z
         Nprev = mesh.N
         for li, N in enumerate(orders):
@@ -197,13 +203,15 @@ class MG:
                 x = interp_coords(Nprev, mesh.E, x, N)
                 y = interp_coords(Nprev, mesh.E, y, N)
                 z = interp_coords(Nprev, mesh.E, z, N)
-            m = BoxMesh(mesh.nx, mesh.ny, mesh.nz, N)          # structured ids and BCs at order N
+            # structured ids and BCs at order N (the fine mesh's boundary types and periodicity)
+            m = BoxMesh(mesh.nx, mesh.ny, mesh.nz, N, bc=getattr(mesh, "bc", "dirichlet"),
+                        periodic=getattr(mesh, "periodic", ""))
             m.x, m.y, m.z = x, y, z
             _, wq = gll(N)
             m.ggeo = np.zeros(m.E * 7 * m.Np)
             lib().orc_geofacs(N, m.E, m.D.ravel().copy(), wq, x, y, z, m.ggeo, None)
             L.mesh = m
-            L.el = Elliptic(m, lambda0, lambda1, poisson)       # masked ogs, mask, invDegree
+            L.el = Elliptic(m, lambda0, lambda1, poisson, zero_mean=False)   # masked ogs, mask, invDegree
             L.ggeoT = m.ggeo.astype(T)
             L.DT = m.D.astype(T)
             L.invDegreeT = L.el.invDegree.astype(T)
@@ -248,7 +256,7 @@ class MG:
                     seen[r] = True
                     cl.amg_first[r] = i
             cl.amg_A = _amg.assemble(cl, lambda u: self.op_local64(cl, u), cl.amg_rows)
-            cl.amg = _amg.AMG(cl.amg_A)
+            cl.amg = _amg.AMG(cl.amg_A, null_space=self.null_space)
 
     # ---- level operators
     def op_local64(self, L, q):
@@ -382,6 +390,8 @@ class MG:
         mode; scalars as in PCG.cpp:85-203)."""
         A, H = L.amg_A, L.amg
         b = rhs.astype(np.float64)[L.amg_first]
+        if self.null_space:
+            b = b - np.sum(b) / b.size
         x = np.zeros_like(b)
         r = b.copy()
         r0 = np.sqrt(r @ r)
@@ -413,6 +423,8 @@ class MG:
         r = rhs.astype(np.float64).copy()
         x = np.zeros_like(r)
         w = el.invDegree
+        if self.null_space:         # mean over the unique nodes removed
+            r = r - np.sum(w * r) / np.sum(w)
         r0 = np.sqrt(np.sum(w * r * r) / el.volume)
         if not r0 > 0:
             return x.astype(self.T), 0
@@ -463,10 +475,21 @@ def fcg_solve(el, rhs, precon, flexible=True, tol=1e-8, maxit=500, x0=None):
     from . import ax
     Ax0 = ax(m.N, m.ggeo, m.D, x, el.lambda0, el.lambda1, el.poisson)
     r = rhs - Ax0
+    if el.zero_mean:                                   # ellipticSolve.cpp:101
+        r = zero_mean(r)
     r[el.maskIds] = 0
     r = el.ogs.gs(r, "add")
     xs, st = pcg(el, r, precon, flexible, tol, maxit)
-    return xs + x, st
+    xs = xs + x
+    if el.zero_mean:                                   # ellipticSolve.cpp:188-189
+        xs = zero_mean(xs)
+    return xs, st
+
+
+def zero_mean(q):
+    """ellipticZeroMean (ellipticZeroMean.cpp:31-45): q - sum(q) / size, the sum
+    over every local node."""
+    return q - np.sum(q) / q.size
 
 
 def pcg(el, r, precon, flexible=True, tol=1e-8, maxit=500):
@@ -486,6 +509,8 @@ def pcg(el, r, precon, flexible=True, tol=1e-8, maxit=500):
     while True:
         it += 1
         z = precon(r)
+        if el.zero_mean:                               # ellipticPreconditioner.cpp:82-83
+            z = zero_mean(z)
         rdotz2 = rdotz1
         rdotz1 = np.sum(w * r * z)
         beta = 0.0
@@ -512,16 +537,22 @@ def gmres_solve(el, rhs, precon, flexible=True, restart=15, tol=1e-8, maxit=500,
     the same operator on each field, inner products summed over the fields."""
     L = el.mesh.Nlocal
     mask = np.isin(np.arange(L), el.maskIds)
+    if el.zero_mean:
+        rhs = zero_mean(np.asarray(rhs, dtype=np.float64))
     r = np.concatenate([el.ogs.gs(np.where(mask, 0.0, rhs[f * L:(f + 1) * L]), "add")
                         for f in range(nfields)])
-    return gmres(el, r, precon, flexible, restart, tol, maxit, nfields)
+    x, st = gmres(el, r, precon, flexible, restart, tol, maxit, nfields)
+    if el.zero_mean:
+        x = zero_mean(x)
+    return x, st
 
 
 def gmres(el, r, precon, flexible=True, restart=15, tol=1e-8, maxit=500, nfields=1):
     """pgmres from x = 0 on the assembled, masked residual r (all fields)."""
     L = el.mesh.Nlocal
     A = lambda v: np.concatenate([el.operator(v[f * L:(f + 1) * L]) for f in range(nfields)])
-    M = lambda v: np.concatenate([precon(v[f * L:(f + 1) * L]) for f in range(nfields)])
+    zm = zero_mean if el.zero_mean else (lambda q: q)
+    M = lambda v: np.concatenate([zm(precon(v[f * L:(f + 1) * L])) for f in range(nfields)])
     w = np.tile(el.invDegree, nfields)
     rnf = 1.0 / el.volume
     dot = lambda a, b: float(np.sum(w * a * b))

@@ -714,6 +714,14 @@ void orc_operator(const orc_elliptic *el, const double *q, double *Aq)
   orc_gs_f64(el->ogs, Aq, 1, el->ogs->N, ORC_OP_ADD);
 }
 
+void orc_zero_mean(int64_t n, double *q)
+{
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += q[i];
+  s /= (double)n;
+  for (int64_t i = 0; i < n; ++i) q[i] -= s;
+}
+
 /* src/solvers/elliptic/linearSolver/PCG.cpp:33-83 (update) and 85-203 (pcg).
  * precon 0 = NONE (z = r), 1 = JACOBI (z = invDiag .* r).
  * reshist (optional, maxit+1 entries) receives rdotr after each iteration. */
@@ -733,6 +741,7 @@ int orc_pcg(const orc_elliptic *el, int precon, const double *invDiag, int flexi
     const double rdotz2 = rdotz1;
     if (precon == 1) {
       for (int64_t n = 0; n < L; ++n) z[n] = invDiag[n] * r[n];
+      if (el->allNeumann) orc_zero_mean(L, z);      /* ellipticPreconditioner.cpp:82-83 */
       rdotz1 = orc_wdot(L, w, r, z);
     } else {
       /* PCG.cpp:101,128-130: with no preconditioner z aliases r and rdotz1 is
@@ -763,7 +772,7 @@ int orc_pcg(const orc_elliptic *el, int precon, const double *invDiag, int flexi
   return iter;
 }
 
-/* src/solvers/elliptic/ellipticSolve.cpp:32-190 (no projection, not all-Neumann):
+/* src/solvers/elliptic/ellipticSolve.cpp:32-190 (no projection):
  * r = rhs - A x0 (element Ax only), mask, gs; x0 saved, x = 0; res0 = ||r||_w
  * scaled by 1/volume; tol *= res0 if relative; pcg; x += x0. */
 int orc_solve(const orc_elliptic *el, int precon, const double *invDiag, int flexible,
@@ -776,6 +785,7 @@ int orc_solve(const orc_elliptic *el, int precon, const double *invDiag, int fle
   double *x0 = (double *)malloc(sizeof(double) * L);
   orc_ax_f64(el->N, el->E, NULL, el->ggeo, el->D, &el->lambda0, &el->lambda1, el->poisson, 0, x, Ax);
   for (int64_t n = 0; n < L; ++n) r[n] = rhs[n] - Ax[n];
+  if (el->allNeumann) orc_zero_mean(L, r);          /* ellipticSolve.cpp:101 */
   orc_mask_f64(el->Nmasked, el->maskIds, r);
   orc_gs_f64(el->ogs, r, 1, L, ORC_OP_ADD);
   memcpy(x0, x, sizeof(double) * L);
@@ -785,6 +795,7 @@ int orc_solve(const orc_elliptic *el, int precon, const double *invDiag, int fle
   double rn = r0;
   const int it = orc_pcg(el, precon, invDiag, flexible, r, x, tol, maxit, &rn, reshist);
   for (int64_t n = 0; n < L; ++n) x[n] += x0[n];
+  if (el->allNeumann) orc_zero_mean(L, x);          /* ellipticSolve.cpp:188-189 */
   if (res0) *res0 = r0;
   if (res) *res = rn;
   free(r); free(Ax); free(x0);

@@ -95,7 +95,12 @@ typedef struct {
   double lambda0, lambda1;     /* constant coefficients                          */
   int poisson;
   double volume;               /* resNormFactor = 1/volume                       */
+  int allNeumann;              /* remove the mean of r, z and x (ellipticZeroMean) */
 } orc_elliptic;
+
+/* ellipticZeroMean (src/solvers/elliptic/ellipticZeroMean.cpp:31-45):
+ * q -= sum(q) / n over the n local nodes */
+void orc_zero_mean(int64_t n, double *q);
 
 void orc_operator(const orc_elliptic *el, const double *q, double *Aq);
 /* precon: 0 = NONE, 1 = JACOBI (invDiag given) */
