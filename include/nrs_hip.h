@@ -200,6 +200,17 @@ typedef struct {
                                   * fields [3][E*Np] with one operator (the
                                   * velocity solve, elliptic_t::Nfields = 3,
                                   * elliptic.h:77; Jacobi or no preconditioner) */
+  /* ELLIPTIC COEFF FIELD = TRUE (ellipticOperator.cpp:44, elliptic_t::o_lambda0 /
+   * o_lambda1, elliptic.h:168-169): per-node coefficients, device [E*Np] FP64,
+   * owned by the caller and read at every operator apply, so the caller may
+   * change their values between solves (nekRS points them into
+   * nrs->o_ellipticCoeff, setup.cpp:784-785).  One field serves every field
+   * of a block solve (elliptic_t::loffset = 0).  NULL: the constants above.
+   * lambda1Field is required for a Helmholtz problem with lambda0Field and
+   * ignored for Poisson.  Preconditioners NONE, JACOBI and USER (multigrid
+   * levels take constant coefficients: NRS_ERR_UNSUPPORTED). */
+  const double *lambda0Field;
+  const double *lambda1Field;
 } nrs_elliptic_desc;
 
 /* SOLVER = PCG, PCG+FLEXIBLE, PGMRES, PGMRES+FLEXIBLE (PGMRES.cpp:111-340) */
@@ -288,6 +299,11 @@ typedef struct {
   int postDegree[8];             /* ... after prolongation, UpLegChebyshevDegree
                                   * (ellipticMultiGridLevel.cpp:146,214); the
                                   * coarsest level, when smoothed, keeps degree 3 */
+  int precoCoeffField;           /* ELLIPTIC PRECO COEFF FIELD (parReader.cpp:999):
+                                  * with coefficient fields and JACOBI, every solve
+                                  * first rebuilds the Jacobi diagonal from the
+                                  * fields' current values (ellipticSolve.cpp:79-88,
+                                  * ellipticUpdateJacobi.cpp:91-115); default 1 */
 } nrs_elliptic_opts;
 
 typedef struct {
@@ -313,11 +329,12 @@ typedef struct {
  * the smoothing after prolongation the down leg's (the reference's assignment,
  * kept).  defaultDegree fills entries of orders above the coarsest that give
  * none; N = the mesh order, which the finest entry must equal.  A degree given
- * for the coarsest order is not used (the coarsest level, when smoothed, keeps
- * degree 3; the reference's lookup of the absent up-leg entry would read 0).
+ * for the coarsest order is refused (the coarsest level, when smoothed, keeps
+ * degree 3; the reference's level setup would take it as that level's degree
+ * after prolongation and read the absent up-leg entry as 0 before restriction).
  * Errors (NRS_ERR_INVALID, the reasons in nrs_last_error): unknown token,
  * missing order or degree, different down- and up-leg orders, an order < 1,
- * finest order != N, more than 8 levels.  As in the reference, a token is an
+ * finest order != N, more than 8 levels, a degree on the coarsest order.  As in the reference, a token is an
  * order when it contains 'p' (so "sweeps=2" reads as order 2).  Host only. */
 int nrs_parse_mg_schedule(const char *schedule, int N, int defaultDegree, nrs_elliptic_opts *opts);
 
@@ -344,6 +361,21 @@ int nrs_elliptic_fusion_info(nrs_elliptic *el, int64_t *fusedPairs);
 int nrs_elliptic_buffer(nrs_elliptic *el, int which, void **ptr, int64_t *count);
 /* Copy `count` entries of buffer `which` into the caller's device array dst. */
 int nrs_elliptic_copy_buffer(nrs_elliptic *el, int which, void *dst, int64_t count, void *stream);
+
+/* Point the handle's coefficients at new fields (the assignment of
+ * elliptic->o_lambda0 / o_lambda1, setup.cpp:784-785): lambda0 / lambda1 device
+ * [E*Np] FP64, kept by reference as in nrs_elliptic_desc; both NULL: back to
+ * the descriptor's constants.  Rebuilds the Jacobi diagonal from the new values
+ * (ellipticUpdateJacobi). */
+int nrs_elliptic_set_coefficients(nrs_elliptic *el, const double *lambda0, const double *lambda1,
+                                  void *stream);
+
+/* ellipticUpdateJacobi(elliptic) (src/solvers/elliptic/ellipticUpdateJacobi.cpp:
+ * 32-89,91-115): invDiagA = 1 / gs(diag(A)) from the coefficients' current
+ * values, enqueued on `stream` (a no-op without the JACOBI preconditioner).
+ * nrs_elliptic_solve calls it first when opts.precoCoeffField is set and the
+ * handle has coefficient fields (ellipticSolve.cpp:79-88). */
+int nrs_elliptic_update_jacobi(nrs_elliptic *el, void *stream);
 
 /* Aq = gs(mask(Ax(q))) in FP64 on device E-vectors.
  * Replaces ellipticOperator(elliptic, o_q, o_Aq, "double", masked)

@@ -425,11 +425,22 @@ COARSE = {"pcg": 0, "smooth": 1, "smooth_jacobi": 2, "amg": 3, "user": 4}
 INITIAL_GUESS = {"previous": 0, "projection": 1, "projection-aconj": 2}
 
 
-def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1):
-    """nrs_elliptic_desc of a mesh (host arrays stay owned by the mesh)."""
+def make_desc(mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1, lambda0_field=None,
+              lambda1_field=None):
+    """nrs_elliptic_desc of a mesh (host arrays stay owned by the mesh);
+    lambda0_field / lambda1_field: per-node FP64 device tensors [E*Np] (ELLIPTIC
+    COEFF FIELD), referenced, not copied: the caller keeps them alive."""
     return _lib.EllipticDesc(mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
                              mesh.z.ctypes.data, mesh.gid.ctypes.data, mesh.EToB.ctypes.data,
-                             int(poisson), float(lambda0), float(lambda1), int(nfields))
+                             int(poisson), float(lambda0), float(lambda1), int(nfields),
+                             _field_ptr(lambda0_field), _field_ptr(lambda1_field))
+
+
+def _field_ptr(t):
+    if t is None:
+        return None
+    assert t.is_cuda and t.dtype == _torch().float64 and t.is_contiguous(), "coefficient field: FP64 device"
+    return t.data_ptr()
 
 
 def parse_mg_schedule(schedule, N, default_degree=3, opts=None):
@@ -565,12 +576,16 @@ class Batch:
 class Elliptic:
     """elliptic_t for one scalar field (ellipticSolveSetup / ellipticSolve)."""
 
-    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1, **kw):
+    def __init__(self, mesh, lambda0=1.0, lambda1=0.0, poisson=True, nfields=1, lambda0_field=None,
+                 lambda1_field=None, **kw):
         """nfields=3: block solve of three fields (vectors [3][Nlocal]), Jacobi or
-        no preconditioner; Nlocal stays the per-field length."""
+        no preconditioner; Nlocal stays the per-field length.  lambda0_field /
+        lambda1_field: per-node coefficients (FP64 device tensors [E*Np], kept
+        by reference: changing their values changes the operator)."""
         self.mesh = mesh
         self.nfields = nfields
-        d = make_desc(mesh, lambda0, lambda1, poisson, nfields)
+        self._fields = (lambda0_field, lambda1_field)
+        d = make_desc(mesh, lambda0, lambda1, poisson, nfields, lambda0_field, lambda1_field)
         if kw.get("schedule") is not None:
             kw.setdefault("order", mesh.N)
         o = make_opts(**kw)
@@ -628,6 +643,17 @@ class Elliptic:
     @property
     def mask_ids(self):
         return self.buffer(3)
+
+    def set_coefficients(self, lambda0_field=None, lambda1_field=None, stream=None):
+        """Point the operator at new coefficient fields (None, None: the
+        constants) and rebuild the Jacobi diagonal."""
+        _lib.call("nrs_elliptic_set_coefficients", self._h, _field_ptr(lambda0_field),
+                  _field_ptr(lambda1_field), _stream(stream))
+        self._fields = (lambda0_field, lambda1_field)
+
+    def update_jacobi(self, stream=None):
+        """ellipticUpdateJacobi: the Jacobi diagonal from the coefficients' current values."""
+        _lib.call("nrs_elliptic_update_jacobi", self._h, _stream(stream))
 
     def operator(self, q, Aq, masked=True, stream=None):
         _lib.call("nrs_elliptic_operator", self._h, _ptr(q), _ptr(Aq), int(masked), _stream(stream))

@@ -27,7 +27,7 @@ class EllipticDesc(C.Structure):
                 ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
                 ("globalIds", C.c_void_p), ("EToB", C.c_void_p),
                 ("poisson", C.c_int), ("lambda0", C.c_double), ("lambda1", C.c_double),
-                ("Nfields", C.c_int)]
+                ("Nfields", C.c_int), ("lambda0Field", C.c_void_p), ("lambda1Field", C.c_void_p)]
 
 
 class EllipticOpts(C.Structure):
@@ -39,7 +39,8 @@ class EllipticOpts(C.Structure):
                 ("mgPrecision", C.c_int), ("fdmPrecision", C.c_int), ("seed", C.c_ulonglong),
                 ("useGraphs", C.c_int), ("autotune", C.c_int), ("gmresRestart", C.c_int),
                 ("initialGuess", C.c_int), ("projVectors", C.c_int), ("projStart", C.c_int),
-                ("operatorFusion", C.c_int), ("preDegree", C.c_int * 8), ("postDegree", C.c_int * 8)]
+                ("operatorFusion", C.c_int), ("preDegree", C.c_int * 8), ("postDegree", C.c_int * 8),
+                ("precoCoeffField", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -81,6 +82,8 @@ SIGNATURES = {
     "nrs_elliptic_fusion_info": ([_vp, C.POINTER(_i64)], _i),
     "nrs_elliptic_buffer": ([_vp, _i, C.POINTER(_vp), C.POINTER(_i64)], _i),
     "nrs_elliptic_copy_buffer": ([_vp, _i, _vp, _i64, _vp], _i),
+    "nrs_elliptic_set_coefficients": ([_vp, _vp, _vp, _vp], _i),
+    "nrs_elliptic_update_jacobi": ([_vp, _vp], _i),
     "nrs_elliptic_operator": ([_vp, _vp, _vp, _i, _vp], _i),
     "nrs_elliptic_preconditioner": ([_vp, _vp, _vp, _vp], _i),
     "nrs_elliptic_solve": ([_vp, _vp, _vp, C.POINTER(SolveStats), _vp], _i),

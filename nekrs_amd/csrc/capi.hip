@@ -312,6 +312,7 @@ void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
   o->projVectors = 8;
   o->projStart = 5;
   o->operatorFusion = 1;
+  o->precoCoeffField = 1;
 }
 
 int nrs_elliptic_setup(const nrs_elliptic_desc *desc, const nrs_elliptic_opts *opts, nrs_elliptic **out)
@@ -371,6 +372,23 @@ int nrs_elliptic_copy_buffer(nrs_elliptic *el, int which, void *dst, int64_t cou
   const size_t esz = (which == 3) ? sizeof(int) : sizeof(double);
   if (count)
     NRS_HIP(hipMemcpyAsync(dst, p, count * esz, hipMemcpyDeviceToDevice, as_stream(stream)));
+  NRS_API_END
+}
+
+int nrs_elliptic_set_coefficients(nrs_elliptic *el, const double *lambda0, const double *lambda1,
+                                  void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el, "set coefficients: null handle");
+  nrs::elliptic_set_coefficients(el, lambda0, lambda1, as_stream(stream));
+  NRS_API_END
+}
+
+int nrs_elliptic_update_jacobi(nrs_elliptic *el, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(el, "update jacobi: null handle");
+  nrs::elliptic_update_jacobi(el, as_stream(stream));
   NRS_API_END
 }
 

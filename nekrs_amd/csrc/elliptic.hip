@@ -90,7 +90,8 @@ std::vector<int> dirichlet_mask(int N, int E, const int *EToB, nrs_ogs *ogsFull,
   return ids;
 }
 
-std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, int nb)
+std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, const std::vector<double> &invDegree,
+                                 int nb)
 {
   std::vector<uint8_t> fx;
   if (N != 7 || E < 2) return fx;
@@ -102,8 +103,10 @@ std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, int 
     bool ok = true;
     for (int k = 1; k < Nq - 1 && ok; ++k)
       for (int j = 1; j < Nq - 1 && ok; ++j) {
-        const long long a = ids[e * Np + k * 64 + j * 8 + 7], b = ids[(e + 1) * Np + k * 64 + j * 8];
-        ok = a != 0 && a == b;
+        const int64_t ia = e * Np + k * 64 + j * 8 + 7, ib = (e + 1) * Np + k * 64 + j * 8;
+        // the pair must be the node's only two copies: a third copy elsewhere would lose
+        // its contribution once the pair leaves the operator's gather-scatter
+        ok = ids[ia] != 0 && ids[ia] == ids[ib] && invDegree[ia] == 0.5;
       }
     if (!ok) continue;
     fx[e] = 1;
@@ -164,6 +167,15 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
                                      "is not supported (as in the reference)");
   el->lambda0 = d->lambda0;
   el->lambda1 = d->poisson ? 0.0 : d->lambda1;
+  if (d->lambda0Field || d->lambda1Field) {
+    NRS_CHECK(d->lambda0Field, "elliptic setup: lambda1Field without lambda0Field");
+    NRS_CHECK(el->poisson || d->lambda1Field, "elliptic setup: a Helmholtz coefficient field needs lambda1Field");
+    if (o->preconditioner == NRS_PRECON_MULTIGRID)
+      throw Error(NRS_ERR_UNSUPPORTED, "elliptic setup: coefficient fields with the multigrid preconditioner "
+                                       "(its levels take constant coefficients)");
+    el->lf0 = d->lambda0Field;
+    el->lf1 = el->poisson ? nullptr : d->lambda1Field;
+  }
   el->opts = *o;
   hipStream_t s = nullptr;
 
@@ -201,10 +213,10 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     std::vector<long long> mg(d->globalIds, d->globalIds + el->Nlocal);
     for (int id : mids) mg[id] = 0;
     el->ogs = ogs_setup(el->Nlocal, mg.data());
-    if (o->operatorFusion && el->Nf == 1) {
+    if (o->operatorFusion && el->Nf == 1 && !el->lfield()) {
       const int nb = o->operatorFusion == 2 ? 2 : 4;
       std::vector<long long> m4(mg);
-      const std::vector<uint8_t> fx = face_fusion(el->N, el->E, m4, nb);
+      const std::vector<uint8_t> fx = face_fusion(el->N, el->E, m4, el->ogs->invDegreeHost, nb);
       if (!fx.empty()) {
         el->ogsOp = ogs_setup(el->Nlocal, m4.data());
         el->fx.upload(fx);
@@ -212,7 +224,7 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
         for (uint8_t f : fx) el->fusedPairs += f;
         if (o->autotune && nb == 4) {   // the pairs-only form (blocks of two) is timed against it below
           fusion2Ids = mg;
-          fusion2 = face_fusion(el->N, el->E, fusion2Ids, 2);
+          fusion2 = face_fusion(el->N, el->E, fusion2Ids, el->ogs->invDegreeHost, 2);
         }
       }
     }
@@ -258,26 +270,34 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
       if (!fusion2.empty()) {   // blocks of two elements: fewer pairs, a barrier over two waves
         constexpr int F = nrs_elliptic::kOpChunks;
         const int64_t epc = ((el->E + F - 1) / F + 3) / 4 * 4;
-        nrs_ogs *op2 = ogs_setup(el->Nlocal, fusion2Ids.data());
-        ogs_set_chunks(op2, epc * el->Np, F);
+        std::unique_ptr<nrs_ogs> op2(ogs_setup(el->Nlocal, fusion2Ids.data()));
+        ogs_set_chunks(op2.get(), epc * el->Np, F);
         DevBuf<uint8_t> fx2;
         fx2.upload(fusion2);
-        el->fusionBlock = 2;
-        const int c2 = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
-                                                 el->lam0.get(), el->lam1.get(), el->poisson,
-                                                 el->maskBits.get(), op2, s, fx2.get());
-        const double t2 = op_ms(op2, fx2.get(), c2);
+        double t2 = 0;
+        int c2 = 0;
+        {
+          // the kernels read the block size from the handle: two while timing, four after
+          // (also when the timing throws)
+          struct Block2 {
+            nrs_elliptic *el;
+            explicit Block2(nrs_elliptic *e) : el(e) { el->fusionBlock = 2; }
+            ~Block2() { el->fusionBlock = 4; }
+          } guard(el.get());
+          c2 = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
+                                         el->lam0.get(), el->lam1.get(), el->poisson,
+                                         el->maskBits.get(), op2.get(), s, fx2.get());
+          t2 = op_ms(op2.get(), fx2.get(), c2);
+        }
         if (t2 < tFus) {
           delete el->ogsOp;
-          el->ogsOp = op2;
+          el->ogsOp = op2.release();
           el->fx = std::move(fx2);
+          el->fusionBlock = 2;
           el->fusedPairs = 0;
           for (uint8_t f : fusion2) el->fusedPairs += f;
           tFus = t2;
           cBest = c2;
-        } else {
-          delete op2;
-          el->fusionBlock = 4;
         }
         set_tuned(TUNE_OP, el->N, NRS_F64, el->E, cBest);
       }
@@ -325,16 +345,9 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   NRS_HIP(hipEventCreate(&el->evPre0));
   NRS_HIP(hipEventCreate(&el->evPre1));
 
-  // Jacobi: ellipticUpdateJacobi (src/solvers/elliptic/ellipticUpdateJacobi.cpp:62-85)
   if (o->preconditioner == NRS_PRECON_JACOBI) {
     el->invDiagA.alloc(Lt);
-    launch_build_diag<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                              el->lam1.get(), el->poisson, false, el->invDiagA.get(), s);
-    ogs_gather_scatter<double>(el->ogs, el->invDiagA.get(), 1, el->Nlocal, NRS_ADD, s);
-    reciprocal<double>(el->Nlocal, el->invDiagA.get(), s);
-    for (int f = 1; f < el->Nf; ++f)   // the same operator for every field
-      NRS_HIP(hipMemcpyAsync(el->invDiagA.get() + f * el->Nlocal, el->invDiagA.get(),
-                             el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
+    elliptic_update_jacobi(el.get(), s);
   }
   if (o->preconditioner == NRS_PRECON_MULTIGRID) {
     el->mg = mg_create(el.get(), d, s);
@@ -343,19 +356,55 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   return el.release();
 }
 
+// ellipticUpdateJacobi (src/solvers/elliptic/ellipticUpdateJacobi.cpp:62-85): the
+// element diagonal with the current coefficients, assembled, inverted; every
+// field of a block solve has the same operator (loffset = 0)
+void elliptic_update_jacobi(nrs_elliptic *el, hipStream_t s)
+{
+  if (el->opts.preconditioner != NRS_PRECON_JACOBI) return;
+  launch_build_diag<double>(el->N, el->E, el->ggeo.get(), el->D.get(), el->l0ptr(), el->l1ptr(),
+                            el->poisson, el->lfield(), el->invDiagA.get(), s);
+  ogs_gather_scatter<double>(el->ogs, el->invDiagA.get(), 1, el->Nlocal, NRS_ADD, s);
+  reciprocal<double>(el->Nlocal, el->invDiagA.get(), s);
+  for (int f = 1; f < el->Nf; ++f)
+    NRS_HIP(hipMemcpyAsync(el->invDiagA.get() + f * el->Nlocal, el->invDiagA.get(),
+                           el->Nlocal * sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+
+void elliptic_set_coefficients(nrs_elliptic *el, const double *l0, const double *l1, hipStream_t s)
+{
+  if (!l0 && !l1) {
+    el->lf0 = el->lf1 = nullptr;
+  } else {
+    NRS_CHECK(l0, "set coefficients: lambda1 without lambda0");
+    NRS_CHECK(el->poisson || l1, "set coefficients: a Helmholtz coefficient field needs lambda1");
+    if (el->opts.preconditioner == NRS_PRECON_MULTIGRID)
+      throw Error(NRS_ERR_UNSUPPORTED, "set coefficients: coefficient fields with the multigrid "
+                                       "preconditioner (its levels take constant coefficients)");
+    el->lf0 = l0;
+    el->lf1 = el->poisson ? nullptr : l1;
+  }
+  elliptic_update_jacobi(el, s);
+}
+
 int operator_chunks(int N, int dtype, int64_t E) { return tuned(TUNE_OP, N, dtype, E, 1); }
 
 template <typename T>
 void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
-                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record, const uint8_t *fx)
+                      const T *q, T *Aq, int nChunks, hipStream_t s, bool record, const uint8_t *fx,
+                      bool lfield)
 {
+  NRS_CHECK(!(fx && lfield), "operator: the face fusion takes constant coefficients");
   auto ax = [&](int64_t a, int64_t n) {
     const int Np = (N + 1) * (N + 1) * (N + 1), NW = (Np + 31) / 32;
     const uint32_t *mb = maskBits ? maskBits + a * NW : nullptr;
     if (fx)
       launch_ax_fused<T>(N, (int)n, ggeo + a * 7 * Np, D, l0, l1, poisson, q + a * Np, Aq + a * Np, mb,
                          fx + a, s, el->fusionBlock);
+    else if (lfield)
+      launch_ax<T>(N, (int)n, nullptr, ggeo + a * 7 * Np, D, l0 + a * Np, poisson ? l1 : l1 + a * Np,
+                   poisson, true, q + a * Np, Aq + a * Np, mb, s);
     else
       launch_ax<T>(N, (int)n, nullptr, ggeo + a * 7 * Np, D, l0, l1, poisson, false, q + a * Np,
                    Aq + a * Np, mb, s);
@@ -399,11 +448,11 @@ void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
 template void chunked_operator<double>(nrs_elliptic *, int, int, const double *, const double *,
                                        const double *, const double *, bool, const uint32_t *,
                                        const nrs_ogs *, const double *, double *, int, hipStream_t, bool,
-                                       const uint8_t *);
+                                       const uint8_t *, bool);
 template void chunked_operator<float>(nrs_elliptic *, int, int, const float *, const float *,
                                       const float *, const float *, bool, const uint32_t *,
                                       const nrs_ogs *, const float *, float *, int, hipStream_t, bool,
-                                      const uint8_t *);
+                                      const uint8_t *, bool);
 
 template <typename T>
 int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
@@ -448,11 +497,11 @@ template int autotune_operator<float>(nrs_elliptic *, int, int, const float *, c
 static void element_operator(nrs_elliptic *el, const double *q, double *Aq, hipStream_t s)
 {
   if (el->Nf > 1)
-    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                    el->lam1.get(), q, Aq, s, false);
+    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->l0ptr(),
+                    el->l1ptr(), q, Aq, s, el->lfield());
   else
-    launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                      el->lam1.get(), el->poisson, false, q, Aq, nullptr, s);
+    launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->l0ptr(),
+                      el->l1ptr(), el->poisson, el->lfield(), q, Aq, nullptr, s);
 }
 
 static void mask_fields(nrs_elliptic *el, double *v, hipStream_t s)
@@ -466,16 +515,18 @@ void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool maske
     // three fields: the block apply (ellipticBlockPartialAxCoeffHex3D), the
     // mask of each field, one gather-scatter of the three vectors
     // (oogs::startFinish with k = 3)
-    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                    el->lam1.get(), q, Aq, s, false);
+    launch_block_ax(el->N, el->E, el->Nlocal, 0, el->ggeo.get(), el->D.get(), el->l0ptr(),
+                    el->l1ptr(), q, Aq, s, el->lfield());
     if (masked) mask_fields(el, Aq, s);
     ogs_gather_scatter<double>(el->ogs, Aq, el->Nf, el->Nlocal, NRS_ADD, s);
     return;
   }
-  chunked_operator<double>(el, el->N, el->E, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                           el->lam1.get(), el->poisson, masked ? el->maskBits.get() : nullptr,
-                           el->ogsOp ? el->ogsOp : el->ogs, q, Aq, operator_chunks(el->N, NRS_F64, el->E),
-                           s, true, el->ogsOp ? el->fx.get() : nullptr);
+  // coefficient fields take the unfused kernel and the full masked handle
+  const bool fused = el->ogsOp && !el->lfield();
+  chunked_operator<double>(el, el->N, el->E, el->ggeo.get(), el->D.get(), el->l0ptr(),
+                           el->l1ptr(), el->poisson, masked ? el->maskBits.get() : nullptr,
+                           fused ? el->ogsOp : el->ogs, q, Aq, operator_chunks(el->N, NRS_F64, el->E),
+                           s, true, fused ? el->fx.get() : nullptr, el->lfield());
 }
 
 void zero_mean(nrs_elliptic *el, double *q, hipStream_t s)
@@ -710,6 +761,9 @@ int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st
             "multigrid: coarse solver USER without a coarse solve");
   auto t0 = std::chrono::steady_clock::now();
   NRS_HIP(hipEventRecord(el->ev0, s));
+  // ELLIPTIC PRECO COEFF FIELD: the Jacobi diagonal from the fields' current values
+  // (ellipticSolve.cpp:79-88)
+  if (el->lfield() && o.precoCoeffField) elliptic_update_jacobi(el, s);
 
   // r = rhs - A x0 (element apply only), mask, gs   (ellipticSolve.cpp:91-103)
   element_operator(el, x, el->Ap.get(), s);
@@ -783,8 +837,9 @@ void elliptic_solve_begin(nrs_elliptic *el, double *rhs, double *x, hipStream_t 
   const int64_t L = el->Nlocal;
   NRS_CHECK(!(el->mg && !el->mg->device_only()),
             "device-driven solve: the multigrid coarse level must be smoothed, not solved");
-  launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->lam0.get(),
-                    el->lam1.get(), el->poisson, false, x, el->Ap.get(), nullptr, s);
+  if (el->lfield() && el->opts.precoCoeffField) elliptic_update_jacobi(el, s);
+  launch_ax<double>(el->N, el->E, nullptr, el->ggeo.get(), el->D.get(), el->l0ptr(),
+                    el->l1ptr(), el->poisson, el->lfield(), x, el->Ap.get(), nullptr, s);
   axpby<double>(L, -1.0, el->Ap.get(), 1.0, rhs, s);
   if (el->allNeumann) zero_mean(el, rhs, s);
   launch_mask<double>(el->Nmasked, el->maskIds.get(), rhs, s);

@@ -21,6 +21,12 @@ struct nrs_elliptic {
   // preconditioned residual and the solution (ellipticZeroMean)
   bool allNeumann = false;
   double lambda0 = 1.0, lambda1 = 0.0;
+  // ELLIPTIC COEFF FIELD: per-node coefficients [Nlocal] owned by the caller
+  // (elliptic_t::o_lambda0 / o_lambda1); null: the constants above
+  const double *lf0 = nullptr, *lf1 = nullptr;
+  bool lfield() const { return lf0 != nullptr; }
+  const double *l0ptr() const { return lf0 ? lf0 : lam0.get(); }
+  const double *l1ptr() const { return lf0 ? (lf1 ? lf1 : lam1.get()) : lam1.get(); }
   double volume = 0.0;
   nrs_elliptic_opts opts{};
 
@@ -95,6 +101,10 @@ namespace nrs {
 nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts *o);
 void elliptic_operator(nrs_elliptic *el, const double *q, double *Aq, bool masked, hipStream_t s);
 void elliptic_preconditioner(nrs_elliptic *el, const double *r, double *z, hipStream_t s);
+// ellipticUpdateJacobi: invDiagA from the current coefficients (JACOBI only)
+void elliptic_update_jacobi(nrs_elliptic *el, hipStream_t s);
+// repoint the coefficient fields (both null: the constants) and update the Jacobi diagonal
+void elliptic_set_coefficients(nrs_elliptic *el, const double *l0, const double *l1, hipStream_t s);
 int elliptic_solve(nrs_elliptic *el, double *rhs, double *x, nrs_solve_stats *st, hipStream_t s);
 // pcg / pgmres on the masked, assembled residual r, x += increment; res0 the
 // normalised initial norm, tol absolute; returns NRS_OK or NRS_ERR_NAN
@@ -121,11 +131,12 @@ void batch_set_mode(nrs_batch *b, int mode);
 // = 1 or an unchunked handle runs Ax then gs.  Same results in every form.
 // record: add the Ax / gs spans to the handle's device timers when enabled.
 // fx: the face-fusion flags (launch_ax_fused; ogs then is the reduced handle) or null.
+// lfield: l0 / l1 are per-node fields [E*Np] (fx must then be null).
 template <typename T>
 void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
                       const T *q, T *Aq, int nChunks, hipStream_t s, bool record = false,
-                      const uint8_t *fx = nullptr);
+                      const uint8_t *fx = nullptr, bool lfield = false);
 // the operator's chunk count for (N, type, E): tuned, else 1
 int operator_chunks(int N, int dtype, int64_t E);
 // time nChunks = 1, 2, 4, 8, 16 on this operator, record the fastest
@@ -134,10 +145,13 @@ int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
                       hipStream_t s, const uint8_t *fx = nullptr);
 // Face fusion of an order-7 operator: from the masked ids [E*512] find the
-// pairs (e, e+1), e % 4 != 3, that share face i = N / i = 0 node for node,
-// return their flags [E] and the ids with those face-interior nodes zeroed (the
-// operator's gather-scatter ids).  Empty flags when no pair qualifies.
-std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, int nb = 4);
+// pairs (e, e+1), e % nb != nb - 1, that share face i = N / i = 0 node for node
+// and hold the only two copies of each of those nodes (invDegree = 1/2, from
+// the handle of the same ids), return their flags [E] and the ids with those
+// face-interior nodes zeroed (the operator's gather-scatter ids).  Empty flags
+// when no pair qualifies.
+std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, const std::vector<double> &invDegree,
+                                 int nb = 4);
 
 // ellipticZeroMean (src/solvers/elliptic/ellipticZeroMean.cpp:31-45): q -= the
 // mean of its E-vector entries (sum over E*Np, divided by E*Np); enqueued work only.

@@ -116,6 +116,14 @@ void parse_mg_schedule(const std::string &schedule, int N, int defaultDegree, nr
   else if (downOrders.front() != N)
     err += "finest degree " + std::to_string(downOrders.front()) + " does not match polynomial degree " +
            std::to_string(N) + "!\n";
+  // A degree on the coarsest order is refused: the coarsest level is solved or
+  // smoothed by its own settings.  (The reference's level setup would take it as
+  // the coarsest level's degree after prolongation and, through std::map's
+  // operator[], set the one before restriction to 0.)
+  for (const auto &kv : sched)
+    if (kv.first.first == minOrder && kv.second != INVALID)
+      err += "degree " + std::to_string(kv.second) + " given for the coarsest order " +
+             std::to_string(minOrder) + ", which keeps its own smoothing!\n";
   for (const auto &kv : sched)
     if (kv.first.first != minOrder && kv.second != INVALID && (kv.second < 1 || kv.second > 16))
       err += "degree " + std::to_string(kv.second) + " of order " + std::to_string(kv.first.first) +

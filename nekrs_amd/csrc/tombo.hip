@@ -583,7 +583,7 @@ nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const dou
   t->ogs = ogs_setup(t->Nlocal, gids);
   if (flags & NRS_TOMBO_FUSION) {
     std::vector<long long> ids(gids, gids + t->Nlocal);
-    const std::vector<uint8_t> fxv = face_fusion(N, E, ids);
+    const std::vector<uint8_t> fxv = face_fusion(N, E, ids, t->ogs->invDegreeHost);
     if (!fxv.empty()) {
       t->ogsOp = ogs_setup(t->Nlocal, ids.data());
       t->fx.upload(fxv);

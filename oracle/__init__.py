@@ -360,6 +360,33 @@ class Elliptic:
                    "history": hist[: abs(it)].copy()}
 
 
+class FieldElliptic(Elliptic):
+    """The same state with per-node coefficients (ELLIPTIC COEFF FIELD =
+    TRUE): ellipticOperator with o_lambda0 / o_lambda1 fields
+    (ellipticOperator.cpp:77-99: Ax with the field kernel, mask, gather-scatter)
+    and the Jacobi diagonal of ellipticUpdateJacobi (ellipticUpdateJacobi.cpp:
+    62-85: element diagonal with the fields, gather-scatter, reciprocal).
+    Built from ax() and build_diag(), which take fields; the solve is
+    oracle.mg.fcg_solve (pcg) with this operator."""
+
+    def __init__(self, mesh, lambda0_field, lambda1_field=None, poisson=True):
+        super().__init__(mesh, 1.0, 0.0, poisson)
+        self.lambda0 = np.ascontiguousarray(lambda0_field, dtype=np.float64)
+        self.lambda1 = (np.ascontiguousarray(lambda1_field, dtype=np.float64) if lambda1_field is not None
+                        else np.zeros(mesh.Nlocal))
+
+    def operator(self, q):
+        m = self.mesh
+        Aq = ax(m.N, m.ggeo, m.D, q, self.lambda0, self.lambda1, self.poisson)
+        Aq[self.maskIds] = 0.0
+        return self.ogs.gs(Aq, "add")
+
+    def inv_diag(self):
+        m = self.mesh
+        d = build_diag(m.N, m.ggeo, m.D, self.lambda0, self.lambda1, self.poisson)
+        return 1.0 / self.ogs.gs(d, "add")
+
+
 # ---------------------------------------------------------------- reference build
 _ref = None
 

@@ -116,3 +116,29 @@ def test_operator_fused_blocks_of_two_equals_unfused(gpu, torch_mod, shape):
             two.operator(q, a, masked=masked)
             assert torch.equal(a, b), (masked, chunks)
         gpu.set_variant("op", 7, "f64", m.E, 0)
+
+
+def test_third_copy_keeps_pair_unfused(gpu, torch_mod):
+    """A face-interior id of a fusable pair that also appears a third time
+    elsewhere (caller-supplied global ids may do that) keeps that pair out of
+    the fusion: the fused operator still equals the unfused one bit for bit,
+    and one fewer pair is fused than on the plain box."""
+    torch = torch_mod
+    m = gpu.BoxMesh(8, 2, 2, 7)
+    plain = gpu.Elliptic(m, precon="jacobi", operatorFusion=1, autotune=0)
+    # element 1's face i = N against element 2's face i = 0 (a fused pair of the
+    # first block of four): give node (i=7, j=3, k=3) of element 1 a third copy in
+    # the interior of element 6 (an interior node is otherwise a singleton)
+    a = 1 * 512 + 3 * 64 + 3 * 8 + 7
+    other = 6 * 512 + 4 * 64 + 4 * 8 + 4
+    m.gid[other] = m.gid[a]
+    on, off = pair(gpu, m, precon="jacobi")
+    print("fused pairs: plain %d, third copy %d" % (plain.fused_pairs, on.fused_pairs), flush=True)
+    assert on.fused_pairs == plain.fused_pairs - 1
+    q = torch.from_numpy(np.random.default_rng(5).uniform(-1, 1, m.Nlocal)).cuda()
+    x, y = torch.empty_like(q), torch.empty_like(q)
+    on.operator(q, x)
+    off.operator(q, y)
+    assert torch.equal(x, y)
+    # the third copy took part in the sum
+    assert x[a].item() == x[other].item()

@@ -120,6 +120,28 @@ def test_build_diag_brute_force(orc):
     assert np.allclose(d, diag, rtol=1e-13)
 
 
+@pytest.mark.parametrize("poisson", [True, False])
+def test_build_diag_fields_brute_force(orc, poisson):
+    """Per-node coefficients (ELLIPTIC COEFF FIELD): the diagonal equals
+    e_n . A e_n with the field operator; the field form with constant fields
+    equals the constant form."""
+    N = 3
+    m = orc.BoxMesh(2, 1, 1, N, "kershaw", 0.3)
+    rng = np.random.default_rng(7)
+    l0 = rng.uniform(0.5, 2.0, m.Nlocal)
+    l1 = rng.uniform(0.0, 3.0, m.Nlocal)
+    d = orc.build_diag(N, m.ggeo, m.D, l0, l1, poisson=poisson)
+    diag = np.zeros(m.Nlocal)
+    for n in range(m.Nlocal):
+        e = np.zeros(m.Nlocal)
+        e[n] = 1.0
+        diag[n] = orc.ax(N, m.ggeo, m.D, e, l0, l1, poisson=poisson)[n]
+    assert np.allclose(d, diag, rtol=1e-13)
+    c0, c1 = np.full(m.Nlocal, 1.7), np.full(m.Nlocal, 0.4)
+    assert np.allclose(orc.build_diag(N, m.ggeo, m.D, c0, c1, poisson=poisson),
+                       orc.build_diag(N, m.ggeo, m.D, 1.7, 0.4, poisson=poisson), rtol=1e-14)
+
+
 def test_ogs_rows_and_brute_force(orc):
     rng = np.random.default_rng(3)
     ids = rng.integers(0, 30, 200).astype(np.int64)
@@ -260,3 +282,15 @@ def test_manufactured_solution_converges(orc):
         errs.append(np.max(np.abs(x - u)))
     assert errs[0] > errs[1] > errs[2]
     assert errs[2] < 1e-5
+
+
+def test_field_elliptic_constant_fields_equal_constant_operator(orc):
+    """The oracle's field operator (ax with fields, mask, gather-scatter) with
+    constant fields equals its constant-coefficient operator (orc_operator)."""
+    m = orc.BoxMesh(3, 2, 2, 4, "kershaw", 0.3)
+    q = np.random.default_rng(3).uniform(-1, 1, m.Nlocal)
+    ref = orc.Elliptic(m, 1.7, 0.4, False)
+    fe = orc.FieldElliptic(m, np.full(m.Nlocal, 1.7), np.full(m.Nlocal, 0.4), poisson=False)
+    a, b = fe.operator(q), ref.operator(q)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    assert np.allclose(fe.inv_diag(), ref.inv_diag(), rtol=1e-14)

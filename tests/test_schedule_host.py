@@ -36,8 +36,8 @@ KNOWN = [
     # asymmetric: before restriction the up leg's degree, after prolongation the down leg's
     ("p=7+degree=2, p=3+degree=1, p=1, p=3+degree=4, p=7+degree=5", 7, 3,
      ([7, 3, 1], [5, 4, 0], [2, 1, 0])),
-    # entries without a degree take the default; a degree on the coarsest order is not used
-    ("p=7, p=5, p=3+degree=4, p=1+degree=6, p=3, p=5+degree=1, p=7", 7, 3,
+    # entries without a degree take the default
+    ("p=7, p=5, p=3+degree=4, p=1, p=3, p=5+degree=1, p=7", 7, 3,
      ([7, 5, 3, 1], [3, 1, 3, 0], [3, 3, 4, 0])),
     # white space anywhere, empty entries dropped
     ("  p = 7 + degree = 2 ,, p=1 ,p=7+degree=2  ", 7, 3, ([7, 1], [2, 0], [2, 0])),
@@ -67,6 +67,32 @@ ERRORS = [
     ("p=7+sweeps=2, p=1, p=7", 7, "down leg and up leg orders must be identical"),
     ("", 7, "no levels"),
 ]
+
+
+def coarse_degree(s):
+    """True when an entry of the coarsest order carries a degree token."""
+    ent = [[t for t in e.replace(" ", "").split("+") if t] for e in s.replace(" ", "").split(",") if e]
+    orders = [omg._stoi(t) for e in ent for t in e if "p" in t]
+    if not orders:
+        return False
+    m = min(orders)
+    return any(any("p" in t and omg._stoi(t) == m for t in e) and any("degree" in t for t in e)
+               for e in ent)
+
+
+def test_coarsest_degree_refused(lib):
+    """A degree on the coarsest order is refused by the library.  The reference's
+    parser accepts it and its level setup (ellipticMultiGridLevelSetup.cpp:162-167)
+    would make it the coarsest level's degree after prolongation while std::map's
+    operator[] sets the one before restriction to 0; the library keeps the coarsest
+    level's own smoothing and says so instead of silently ignoring the entry."""
+    s = "p=7+degree=2, p=3+degree=2, p=1+degree=5, p=3+degree=2, p=7+degree=2"
+    got, err = parse(lib, s, 7)
+    assert got is None and "coarsest order 1" in err, err
+    lv, pre, post, errs = omg.parse_mg_schedule(s, 7, 3)     # the reference's parser accepts it
+    assert errs == [] and lv == [7, 3, 1]
+    assert coarse_degree(s) and not coarse_degree(s.replace("p=1+degree=5", "p=1"))
+    assert parse(lib, s.replace("p=1+degree=5", "p=1"), 7)[0] == ([7, 3, 1], [2, 2, 0], [2, 2, 0])
 
 
 @pytest.mark.parametrize("case", ERRORS, ids=[str(i) for i in range(len(ERRORS))])
@@ -106,6 +132,9 @@ def test_random_schedules_agree_with_oracle(lib):
         d = int(rng.integers(1, 9))
         got, err = parse(lib, s, N, d)
         lv, pre, post, errs = omg.parse_mg_schedule(s, N, d)
+        if not errs and coarse_degree(s):
+            assert got is None and "coarsest order" in err, (s, err)
+            continue
         assert (got is None) == bool(errs), (s, N, err, errs)
         if got is not None:
             valid += 1
