@@ -142,6 +142,20 @@ int nrs_ogs_free(nrs_ogs *ogs);
 int nrs_ogs_gather_scatter(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int dtype, int op,
                            void *stream);
 
+/* The split form of the same call, the overlap seam of oogs::start /
+ * oogs::finish (3rd_party/gslib/ogs/ogs.hpp:282-283, src/oogs.cpp:682-821) as
+ * ellipticOperator uses it (ellipticOperator.cpp:129-157): start, then the
+ * caller's work on v, then finish with the same arguments.  The reference's
+ * start packs and posts the rows shared with other ranks; on one device every
+ * row is local, so start only checks and records its arguments and does not
+ * read v (the caller may still write any entry of v), and finish performs the
+ * whole gather-scatter on `stream` -- the same result, bit for bit, as
+ * nrs_ogs_gather_scatter of v as it is at finish.  A finish without a start,
+ * a second start before finish, or a finish whose arguments differ from its
+ * start's is NRS_ERR_INVALID. */
+int nrs_ogs_start(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int dtype, int op, void *stream);
+int nrs_ogs_finish(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int dtype, int op, void *stream);
+
 /* 1/multiplicity per node (1 at excluded nodes), FP64 device [N]
  * (ogs->o_invDegree, ogsSetup.cpp:364-392). */
 int nrs_ogs_inv_degree(nrs_ogs *ogs, double *invDegree, void *stream);

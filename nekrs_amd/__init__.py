@@ -271,6 +271,16 @@ class Ogs:
         _lib.call("nrs_ogs_gather_scatter", self._h, _ptr(v), nvec, stride, _dtype_code(v),
                   OPS[op], _stream(stream))
 
+    def start(self, v, op="add", nvec=1, stride=None, stream=None):
+        """oogs::start: the first half of the split gather-scatter (v is not read)."""
+        stride = self.N if stride is None else stride
+        _lib.call("nrs_ogs_start", self._h, _ptr(v), nvec, stride, _dtype_code(v), OPS[op], _stream(stream))
+
+    def finish(self, v, op="add", nvec=1, stride=None, stream=None):
+        """oogs::finish: the gather-scatter of v as it is now (same arguments as start)."""
+        stride = self.N if stride is None else stride
+        _lib.call("nrs_ogs_finish", self._h, _ptr(v), nvec, stride, _dtype_code(v), OPS[op], _stream(stream))
+
     def inv_degree(self, out=None, stream=None):
         torch = _torch()
         if out is None:

@@ -71,6 +71,8 @@ SIGNATURES = {
     "nrs_ogs_setup": ([_i64, _vp, C.POINTER(_vp)], _i),
     "nrs_ogs_free": ([_vp], _i),
     "nrs_ogs_gather_scatter": ([_vp, _vp, _i, _i64, _i, _i, _vp], _i),
+    "nrs_ogs_start": ([_vp, _vp, _i, _i64, _i, _i, _vp], _i),
+    "nrs_ogs_finish": ([_vp, _vp, _i, _i64, _i, _i, _vp], _i),
     "nrs_ogs_inv_degree": ([_vp, _vp, _vp], _i),
     "nrs_ogs_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)], _i),
     "nrs_linalg_reduce": ([_i, _i64, _i, _i64, _vp, _vp, _vp, C.POINTER(_d), _vp], _i),

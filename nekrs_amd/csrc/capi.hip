@@ -230,6 +230,34 @@ int nrs_ogs_gather_scatter(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int 
   NRS_API_END
 }
 
+int nrs_ogs_start(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int dtype, int op, void *stream)
+{
+  NRS_API_BEGIN
+  (void)stream;
+  NRS_CHECK(ogs && (v || ogs->N == 0), "gather-scatter start: null argument");
+  NRS_CHECK(op >= NRS_ADD && op <= NRS_MAX, "gather-scatter start: unknown op");
+  NRS_CHECK(dtype == NRS_F64 || dtype == NRS_F32 || dtype == NRS_I32, "gather-scatter start: unknown dtype");
+  NRS_CHECK(nvec >= 1 && (nvec == 1 || stride >= ogs->N), "gather-scatter start: bad nvec / stride");
+  NRS_CHECK(!ogs->pending, "gather-scatter start: the previous start has no finish");
+  // one device holds every row: no halo to pack or exchange, v is not read here
+  ogs->pend = {v, nvec, stride, dtype, op};
+  ogs->pending = true;
+  NRS_API_END
+}
+
+int nrs_ogs_finish(nrs_ogs *ogs, void *v, int nvec, int64_t stride, int dtype, int op, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(ogs, "gather-scatter finish: null handle");
+  NRS_CHECK(ogs->pending, "gather-scatter finish: no matching start");
+  const auto &p = ogs->pend;
+  NRS_CHECK(p.v == v && p.nvec == nvec && p.stride == stride && p.dtype == dtype && p.op == op,
+            "gather-scatter finish: arguments differ from the start");
+  ogs->pending = false;
+  return nrs_ogs_gather_scatter(ogs, v, nvec, stride, dtype, op, stream);
+  NRS_API_END
+}
+
 int nrs_ogs_inv_degree(nrs_ogs *ogs, double *invDegree, void *stream)
 {
   NRS_API_BEGIN

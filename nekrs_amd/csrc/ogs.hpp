@@ -27,6 +27,10 @@ struct nrs_ogs {
   int64_t nodesPerChunk = 0;
   std::vector<int64_t> cOff[4];
   std::vector<int> h2, h4, h8, hxo, hxi;   // host copies of the buckets
+  // the call between nrs_ogs_start and nrs_ogs_finish (oogs::start / finish)
+  struct Pending { const void *v; int nvec; int64_t stride; int dtype, op; };
+  bool pending = false;
+  Pending pend{};
 };
 
 namespace nrs {

@@ -229,6 +229,43 @@ def test_gather_scatter_many_vectors(gpu, torch_mod, orc):
     assert np.array_equal(host(d)[n:stride], v[n:stride])   # padding untouched
 
 
+def test_gather_scatter_start_finish(gpu, torch_mod, orc):
+    """oogs::start / finish as ellipticOperator uses them: start, then the
+    caller keeps writing v (the second half of the elements' values, as the Ax of
+    the local elements does), then finish: equal bit for bit to the one-call
+    gather-scatter of the final v, for one and for three vectors; a finish
+    without a start, a second start and a finish with other arguments are refused."""
+    torch = torch_mod
+    ids = orc.BoxMesh(4, 3, 2, 5, "kershaw", 0.3).gid
+    n = ids.size
+    og = gpu.Ogs(ids)
+    rng = np.random.default_rng(21)
+    for nvec, stride in ((1, n), (3, n + 64)):
+        a = rng.uniform(-1, 1, nvec * stride)
+        b = rng.uniform(-1, 1, nvec * stride)
+        half = np.zeros(nvec * stride, dtype=bool)
+        for f in range(nvec):
+            half[f * stride + n // 2: f * stride + n] = True
+        final = np.where(half, b, a)
+        ref = dev(torch, final)
+        og.gather_scatter(ref, "add", nvec=nvec, stride=stride)
+        v = dev(torch, a)
+        og.start(v, "add", nvec=nvec, stride=stride)
+        v[torch.from_numpy(half).cuda()] = dev(torch, b[half])    # the caller's work between the halves
+        og.finish(v, "add", nvec=nvec, stride=stride)
+        torch.cuda.synchronize()
+        assert torch.equal(v, ref), nvec
+    v = dev(torch, rng.uniform(-1, 1, n))
+    with pytest.raises(gpu.NrsError):
+        og.finish(v)                                   # no start
+    og.start(v)
+    with pytest.raises(gpu.NrsError):
+        og.start(v)                                    # start twice
+    with pytest.raises(gpu.NrsError):
+        og.finish(v, "min")                            # other arguments
+    og.finish(v)
+
+
 def test_inv_degree_and_info(gpu, torch_mod, orc):
     ids = orc.BoxMesh(4, 3, 2, 7, "kershaw", 0.3).gid
     og_o, og_g = orc.Ogs(ids), gpu.Ogs(ids)
