@@ -106,7 +106,13 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(nx, N, seconds, seed):
-    """The oracle's Ax+mask+gs (OpenMP) on the same mesh, bounded sample."""
+    """The operator (Ax + mask + gather-scatter, FP64) on the host cores, on the
+    same mesh, over a bounded sample.  Where oracle/_ref/libnrsref.so was built
+    (from the reference's own sources, by oracle/build_ref.sh), the timed path is
+    the reference's: its serial CPU kernel ellipticPartialAxCoeffHex3D.c (OpenMP
+    over elements) and its vendored gslib gs (serial); the oracle's OpenMP
+    operator is timed beside it ("port").  Otherwise the port alone."""
+    import ctypes as C
     import numpy as np
     import oracle
     oracle.build()
@@ -115,18 +121,59 @@ def cpu_baseline(nx, N, seconds, seed):
     el = oracle.Elliptic(m)
     q = np.random.default_rng(seed).uniform(-1, 1, m.Nlocal)
     log("cpu baseline setup %.1f s" % (time.perf_counter() - t0))
-    el.operator(q)                          # warm-up
-    reps, t = 0, 0.0
-    while t < seconds or reps < 2:
-        a = time.perf_counter()
-        el.operator(q)
-        t += time.perf_counter() - a
-        reps += 1
-    gdofs = m.E * N ** 3 * reps / t / 1e9
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    return {"value": gdofs, "unit": "GDOF/s", "cores": cores, "kind": "port",
+
+    def timed(fn, budget):
+        fn()                                # warm-up
+        reps, t = 0, 0.0
+        while t < budget or reps < 2:
+            a = time.perf_counter()
+            fn()
+            t += time.perf_counter() - a
+            reps += 1
+        return reps, t
+
+    ref = oracle.ref_lib()
+    port_budget = seconds if ref is None else seconds / 3
+    reps, t = timed(lambda: el.operator(q), port_budget)
+    port = {"value": m.E * N ** 3 * reps / t / 1e9, "unit": "GDOF/s", "cores": cores, "kind": "port",
             "sample": "%d operator applications (Ax+mask+gs, FP64) on the same %d^3 N=%d "
                       "Kershaw mesh, %.1f s" % (reps, nx, N, t)}
+    if ref is None:
+        return port
+    Nq = N + 1
+    ax = getattr(ref, "ref_ax_poisson_double_nq%d" % Nq)
+    dp = lambda a: a.ctypes.data_as(C.c_void_p)
+    E, L = m.E, m.Nlocal
+    elist = np.arange(E, dtype=np.int32)
+    D = np.ascontiguousarray(m.D, dtype=np.float64)
+    S = np.ascontiguousarray(D.T)
+    l0, l1 = np.ones(1), np.zeros(1)
+    gid = oracle.masked_gids(m.gid, el.maskIds)
+    ref.ref_gslib_setup.restype = C.c_void_p
+    h = C.c_void_p(ref.ref_gslib_setup(dp(gid), C.c_uint(L)))
+    Aq = np.zeros(L)
+    mask = el.maskIds
+    cE, cL = C.c_int(E), C.c_int(L)          # the kernel takes its sizes by reference (dlong &)
+
+    def apply():
+        ax(C.byref(cE), C.byref(cL), C.byref(cL), dp(elist), dp(m.ggeo), dp(D), dp(S), dp(l0), dp(l1),
+           dp(q), dp(Aq))
+        Aq[mask] = 0.0
+        ref.ref_gslib_apply(h, dp(Aq), 0, 0)
+    try:
+        apply()
+        check = el.operator(q)
+        err = float(np.max(np.abs(Aq - check)) / np.max(np.abs(check)))
+        reps, t = timed(apply, seconds * 2 / 3)
+    finally:
+        ref.ref_gslib_free(h)
+    return {"value": E * N ** 3 * reps / t / 1e9, "unit": "GDOF/s", "cores": cores, "kind": "reference",
+            "sample": "%d operator applications (the reference's serial CPU kernel "
+                      "ellipticPartialAxCoeffHex3D.c, OpenMP over elements, built from its sources; "
+                      "mask; its gslib gs, serial) on the same %d^3 N=%d Kershaw mesh, %.1f s; "
+                      "rel. difference to the oracle's operator %.1e" % (reps, nx, N, t, err),
+            "port": port}
 
 
 def solve_cases(n, N, seed=0):

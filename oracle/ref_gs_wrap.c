@@ -33,3 +33,38 @@ int ref_gslib_gs(const long long *ids, unsigned n, void *v, int dom, int op)
   free(id);
   return 0;
 }
+
+/* The same with a handle kept across calls, for timing the reference's
+ * gather-scatter as a CPU baseline: setup once, apply many times. */
+struct ref_gs_handle {
+  struct comm c;
+  struct gs_data *g;
+};
+
+void *ref_gslib_setup(const long long *ids, unsigned n)
+{
+  struct ref_gs_handle *h = (struct ref_gs_handle *)malloc(sizeof(*h));
+  slong *id = (slong *)malloc(sizeof(slong) * (n ? n : 1));
+  for (unsigned i = 0; i < n; ++i) id[i] = (slong)ids[i];
+  comm_init(&h->c, 0);
+  h->g = gs_setup(id, n, &h->c, 0, gs_pairwise, 0);
+  free(id);
+  return h;
+}
+
+int ref_gslib_apply(void *handle, void *v, int dom, int op)
+{
+  struct ref_gs_handle *h = (struct ref_gs_handle *)handle;
+  const gs_dom d = dom == 0 ? gs_double : (dom == 1 ? gs_float : gs_int);
+  const gs_op o = op == 0 ? gs_add : (op == 1 ? gs_mul : (op == 2 ? gs_min : gs_max));
+  gs(v, d, o, 0, h->g, 0);
+  return 0;
+}
+
+void ref_gslib_free(void *handle)
+{
+  struct ref_gs_handle *h = (struct ref_gs_handle *)handle;
+  gs_free(h->g);
+  comm_free(&h->c);
+  free(h);
+}
