@@ -591,14 +591,18 @@ typedef struct nrs_tombo nrs_tombo;
 enum nrs_tombo_flags {
   NRS_TOMBO_FUSION = 1,          /* N = 7: faces shared by consecutive elements summed in the
                                   * curl kernels (bit-identical), gather-scatters without them */
-  NRS_TOMBO_AUTOTUNE = 2         /* time the load schedules, keep the fusion only if faster  */
+  NRS_TOMBO_AUTOTUNE = 2,        /* time the load schedules, keep the fusion only if faster  */
+  NRS_TOMBO_STRESS = 4           /* VELOCITY STRESSFORMULATION = TRUE (tombo.cpp:57-66): the
+                                  * second curl less JW (2 sym(grad Ue) grad(mue)/mue
+                                  * - 2/3 div grad(mue)/mue) (pressureStressHex3D.okl)        */
 };
 int nrs_tombo_setup(int N, int Nelements, const double *x, const double *y, const double *z,
                     const long long *globalIds, const int *bcType, int flags, nrs_tombo **out);
 int nrs_tombo_free(nrs_tombo *h);
 
 /* rhs = the pressure Poisson right-hand side of tombo::pressureSolve
- * (src/navierStokes/tombo.cpp:11-121, VELOCITY STRESSFORMULATION off):
+ * (src/navierStokes/tombo.cpp:11-121; with NRS_TOMBO_STRESS the stress term is
+ * taken from JW curl W first, tombo.cpp:57-66):
  *   W = gs(JW curl Ue) / gs(JW);  rhsU = BF - mue irho (JW curl W - 4/3 JW grad div)
  *   (rhsU = BF when viscContribution = 0, i.e. nBDF = 1);  rhsU = gs(rhsU) / gs(JW);
  *   rhs = wdiv(rhsU) + g0idt JW div - WSJ n.(g0idt U) on V/INT faces - WSJ n.rhsU on SYM faces.

@@ -333,7 +333,9 @@ class PressureRhs:
     (BC ids; default: the mesh's own bc_types when it has them (Re2Mesh), else every
     boundary face an inlet, 0 inside)."""
 
-    def __init__(self, mesh, bc_types=None, fusion=True, autotune=True):
+    def __init__(self, mesh, bc_types=None, fusion=True, autotune=True, stress=False):
+        """stress: VELOCITY STRESSFORMULATION = TRUE (the stress term of
+        pressureStressHex3D taken from the second curl)."""
         self.mesh = mesh
         self.N, self.E = mesh.N, mesh.E
         self.Nlocal = mesh.E * (mesh.N + 1) ** 3
@@ -344,7 +346,7 @@ class PressureRhs:
         self.bc_types = np.ascontiguousarray(bc_types, dtype=np.int32).reshape(-1)
         assert self.bc_types.size == 6 * mesh.E
         h = C.c_void_p()
-        flags = (1 if fusion else 0) | (2 if autotune else 0)
+        flags = (1 if fusion else 0) | (2 if autotune else 0) | (4 if stress else 0)
         _lib.call("nrs_tombo_setup", mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
                   mesh.z.ctypes.data, mesh.gid.ctypes.data, self.bc_types.ctypes.data, flags,
                   C.byref(h))

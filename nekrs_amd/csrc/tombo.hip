@@ -115,7 +115,7 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
             int64_t offOut, const double *__restrict__ invLMM, const double *__restrict__ U,
             const double *__restrict__ div, const double *__restrict__ BF, int64_t offBF,
             const double *__restrict__ mue, const double *__restrict__ irho, double *__restrict__ out,
-            const uint8_t *__restrict__ fx = nullptr)
+            const uint8_t *__restrict__ fx = nullptr, const double *__restrict__ S = nullptr)
 {
   constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
   constexpr int NF = RHS ? 4 : 3;                 // u, v, w (+ div)
@@ -203,6 +203,11 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
       const double wx = rx * dr[2] + sx * ds[2] + tx * dt[2];
       const double wy = ry * dr[2] + sy * ds[2] + ty * dt[2];
       double c0 = (wy - vz) * JW, c1 = (uz - wx) * JW, c2 = (vx - uy) * JW;
+      if (RHS && S) {           // VELOCITY STRESSFORMULATION: pressureStressHex3D's OUT -= JW (...)
+        c0 -= S[id];
+        c1 -= S[id + offOut];
+        c2 -= S[id + 2 * offOut];
+      }
       if (RHS) {
         constexpr int P = NF - 1;
         const double gx = JW * (rx * dr[P] + sx * ds[P] + tx * dt[P]);
@@ -244,6 +249,75 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
           out[id + c * offOut] = s_x[lo][1][c][j][k] + s_x[lo + 1][0][c][j][k];
       }
     }
+  }
+}
+
+// The stress-formulation term (pressureStressHex3D.okl): with g = grad(mue) / mue,
+//   S = JW (2 sym(grad Ue) g - 2/3 div g),
+// written to S [3][Nlocal]; the right-hand-side kernel subtracts it from the
+// second curl.  Thread (i,j) per element with k-pencils of u, v, w, mue in
+// registers; the r/s derivatives through one LDS plane per field.
+template <int NQ>
+__global__ void __launch_bounds__(NQ * NQ)
+stress_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ Dg, int64_t off,
+              const double *__restrict__ U, const double *__restrict__ mue, const double *__restrict__ div,
+              double *__restrict__ S)
+{
+  constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
+  __shared__ double s_D[NQ][NQ], s_f[4][NQ][NQ];
+  const int e = blockIdx.x;
+  const int i = threadIdx.x % NQ, j = threadIdx.x / NQ;
+  s_D[j][i] = Dg[j * NQ + i];
+  const int64_t base = (int64_t)e * NP + j * NQ + i;
+  double pf[4][NQ];
+#pragma unroll
+  for (int l = 0; l < NQ; ++l) {
+    const int64_t id = base + l * NFP;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pf[c][l] = U[id + c * off];
+    pf[3][l] = mue[id];
+  }
+  for (int k = 0; k < NQ; ++k) {
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_f[c][j][i] = pf[c][k];
+    __syncthreads();
+    double dr[4], ds[4], dt[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { dr[c] = 0.0; ds[c] = 0.0; dt[c] = 0.0; }
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) {
+      const double Dr = s_D[i][n], Ds = s_D[j][n], Dt = s_D[k][n];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        dr[c] += Dr * s_f[c][j][n];
+        ds[c] += Ds * s_f[c][n][i];
+        dt[c] += Dt * pf[c][n];
+      }
+    }
+    const int64_t id = base + k * NFP;
+    const double *G = vgeo + (int64_t)e * NV * NP + k * NFP + j * NQ + i;
+    const double rx = G[0 * NP], ry = G[1 * NP], rz = G[2 * NP];
+    const double sx = G[3 * NP], sy = G[4 * NP], sz = G[5 * NP];
+    const double tx = G[6 * NP], ty = G[7 * NP], tz = G[8 * NP];
+    const double JW = G[9 * NP];
+    double gx[4], gy[4], gz[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      gx[c] = rx * dr[c] + sx * ds[c] + tx * dt[c];
+      gy[c] = ry * dr[c] + sy * ds[c] + ty * dt[c];
+      gz[c] = rz * dr[c] + sz * ds[c] + tz * dt[c];
+    }
+    const double im = 1.0 / pf[3][k];
+    const double mx = gx[3] * im, my = gy[3] * im, mz = gz[3] * im;
+    const double w1 = mx * 2 * gx[0] + my * (gy[0] + gx[1]) + mz * (gz[0] + gx[2]);
+    const double w2 = mx * (gy[0] + gx[1]) + my * 2 * gy[1] + mz * (gz[1] + gy[2]);
+    const double w3 = mx * (gz[0] + gx[2]) + my * (gz[1] + gy[2]) + mz * 2 * gz[2];
+    const double d = div[id];
+    const double scale = -2. / 3;
+    S[id] = JW * (w1 + scale * d * mx);
+    S[id + NP * (int64_t)E] = JW * (w2 + scale * d * my);
+    S[id + 2 * NP * (int64_t)E] = JW * (w3 + scale * d * mz);
   }
 }
 
@@ -415,7 +489,7 @@ template <int NQ> struct CurlL {
 template <int NQ> struct RhsL {
   static void run(const nrs_tombo *t, bool pf, const double *W, const double *div, const double *BF,
                   int64_t offBF, const double *mue, const double *irho, double *out, hipStream_t s,
-                  const uint8_t *fx = nullptr)
+                  const uint8_t *fx = nullptr, const double *S = nullptr)
   {
     const double *G = t->vgeo.get(), *D = t->D.get(), *il = t->invLMM.get();
     const int64_t L = t->Nlocal;
@@ -424,18 +498,27 @@ template <int NQ> struct RhsL {
         const dim3 g((t->E + 3) / 4), b(256);
         if (pf)
           curl_kernel<NQ, true, true, true, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue,
-                                                                  irho, out, fx);
+                                                                  irho, out, fx, S);
         else
           curl_kernel<NQ, true, true, false, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue,
-                                                                   irho, out, fx);
+                                                                   irho, out, fx, S);
         return;
       }
     }
     const dim3 g(t->E), b(NQ * NQ);
     if (pf)
-      curl_kernel<NQ, true, true, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out);
+      curl_kernel<NQ, true, true, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out,
+                                                        nullptr, S);
     else
-      curl_kernel<NQ, true, true, false><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out);
+      curl_kernel<NQ, true, true, false><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out,
+                                                         nullptr, S);
+  }
+};
+template <int NQ> struct StressL {
+  static void run(const nrs_tombo *t, int64_t off, const double *U, const double *mue, const double *div,
+                  double *S, hipStream_t s)
+  {
+    stress_kernel<NQ><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), off, U, mue, div, S);
   }
 };
 template <int NQ> struct WdivL {
@@ -598,6 +681,7 @@ nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const dou
   NRS_LAUNCH_CHECK();
   t->W.alloc(3 * (size_t)t->Nlocal);
   t->F.alloc(3 * (size_t)t->Nlocal);
+  if (flags & NRS_TOMBO_STRESS) t->S.alloc(3 * (size_t)t->Nlocal);
   NRS_HIP(hipStreamSynchronize(s));
   if (flags & NRS_TOMBO_AUTOTUNE) autotune(t.get(), s);
   return t.release();
@@ -623,7 +707,9 @@ void tombo_pressure_rhs(nrs_tombo *t, int64_t off, const double *Ue, const doubl
     const bool pf1 = tuned(TUNE_TOMBO, t->N, 1, t->E, 1) == 2;
     by_order<CurlL>(t->N, t, off, L, false, pf0, Ue, W, s, t->fx.get());   // W = JW curl(Ue)
     ogs_gather_scatter<double>(gsh, W, 3, L, NRS_ADD, s);
-    by_order<RhsL>(t->N, t, pf1, W, div, BF, off, mue, irho, F, s, t->fx.get());   // F = rhsU
+    double *S = t->S.get();                                             // stress term (tombo.cpp:57-66)
+    if (S) by_order<StressL>(t->N, t, off, Ue, mue, div, S, s);
+    by_order<RhsL>(t->N, t, pf1, W, div, BF, off, mue, irho, F, s, t->fx.get(), S);   // F = rhsU
   } else {
     for (int c = 0; c < 3; ++c)                                         // F = BF
       NRS_HIP(hipMemcpyAsync(F + c * L, BF + c * off, L * sizeof(double), hipMemcpyDeviceToDevice, s));

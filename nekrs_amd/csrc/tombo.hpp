@@ -21,6 +21,7 @@ struct nrs_tombo {
   nrs::DevBuf<int> bElem, bType;     // [Nb] element, [Nb][6] face boundary type
   nrs::DevBuf<double> bGeo;          // [Nb][6][Nq*Nq][4]: nx ny nz WSJ
   nrs::DevBuf<double> W, F;          // [3][Nlocal] work: curl, then the assembled rhsU
+  nrs::DevBuf<double> S;             // [3][Nlocal] stress term (NRS_TOMBO_STRESS), else empty
   ~nrs_tombo();
 };
 

@@ -1,14 +1,15 @@
 """Oracle for the pressure right-hand side of the TOMBO splitting.
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).  A numpy restatement of
-tombo::pressureSolve's right-hand side (src/navierStokes/tombo.cpp:11-121,
-VELOCITY STRESSFORMULATION off) and of the kernels it calls:
+tombo::pressureSolve's right-hand side (src/navierStokes/tombo.cpp:11-121)
+and of the kernels it calls:
 
   vgeo        kernels/mesh/geometricFactorsHex3D.okl:27-143 (rx..tz, J, JW)
   sgeo        kernels/mesh/surfaceGeometricFactorsHex3D.okl (n, sJ, WSJ)
   invLMM      src/mesh/meshComputeInvLMM.cpp (1 / gs(JW))
   curl        kernels/nrs/curlHex3D.okl (scaleJW = 1)
   gradient    kernels/nrs/gradientVolumeHex3D.okl
+  stress      kernels/nrs/pressureStressHex3D.okl (VELOCITY STRESSFORMULATION)
   rhsU        kernels/nrs/pressureRhsHex3D.okl
   wdiv        kernels/nrs/wDivergenceVolumeHex3D.okl
   addQtl      kernels/nrs/pressureAddQtl.okl
@@ -121,6 +122,34 @@ class Geometry:
         gz = self.JW * (self.rz * pr + self.sz * ps + self.tz * pt)
         return np.concatenate([gx.ravel(), gy.ravel(), gz.ravel()])
 
+    def stress(self, U, mue, div, off=None):
+        """The stress-formulation term JW (w + (-2/3) div grad(mue) / mue) that
+        pressureStressHex3D.okl subtracts from the second curl, with
+        w = 2 sym(grad U) . grad(mue) / mue; returns [3][Nlocal]."""
+        L = self.E * self.Nq ** 3
+        sh = (self.E, self.Nq, self.Nq, self.Nq)
+        u, v, w = self._fields(U, off or L)
+        m = mue.reshape(sh)
+
+        def grad(f):
+            fr, fs, ft = _d(self.D, f)
+            return (self.rx * fr + self.sx * fs + self.tx * ft,
+                    self.ry * fr + self.sy * fs + self.ty * ft,
+                    self.rz * fr + self.sz * fs + self.tz * ft)
+        dudx, dudy, dudz = grad(u)
+        dvdx, dvdy, dvdz = grad(v)
+        dwdx, dwdy, dwdz = grad(w)
+        mx, my, mz = grad(m)
+        inv = 1.0 / m
+        mx, my, mz = mx * inv, my * inv, mz * inv
+        w1 = mx * 2 * dudx + my * (dudy + dvdx) + mz * (dudz + dwdx)
+        w2 = mx * (dudy + dvdx) + my * 2 * dvdy + mz * (dvdz + dwdy)
+        w3 = mx * (dudz + dwdx) + my * (dvdz + dwdy) + mz * 2 * dwdz
+        d = div.reshape(sh)
+        sc = -2.0 / 3.0
+        return np.concatenate([(self.JW * (w1 + sc * d * mx)).ravel(), (self.JW * (w2 + sc * d * my)).ravel(),
+                               (self.JW * (w3 + sc * d * mz)).ravel()])
+
     def wdiv(self, U, off=None):
         """Weak divergence sum_d D_d^T (JW (d_d . U)) (wDivergenceVolumeHex3D)."""
         L = self.E * self.Nq ** 3
@@ -148,8 +177,10 @@ class Geometry:
             out[sel] += -WSJ * flux
         return out.ravel()
 
-    def pressure_rhs(self, bc, Ue, U, BF, mue, irho, div, g0idt, visc=True, off=None):
-        """tombo::pressureSolve's rhs (tombo.cpp:11-121)."""
+    def pressure_rhs(self, bc, Ue, U, BF, mue, irho, div, g0idt, visc=True, off=None, stress=False):
+        """tombo::pressureSolve's rhs (tombo.cpp:11-121); stress: VELOCITY
+        STRESSFORMULATION = TRUE (tombo.cpp:57-66), the second curl less the
+        stress term."""
         L = self.E * self.Nq ** 3
         off = off or L
         rep = lambda a: np.tile(a, 3)
@@ -157,6 +188,8 @@ class Geometry:
             W = self.curl(Ue, off)
             W = self.ogs.gs(W, "add", nvec=3, stride=L) * rep(self.invLMM)
             NC = self.curl(W)
+            if stress:
+                NC = NC - self.stress(Ue, mue, div, off)
             gD = self.gradient(div)
             nu = rep(mue * irho)
             B = np.concatenate([BF[c * off:c * off + L] for c in range(3)])

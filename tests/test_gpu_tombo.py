@@ -86,6 +86,36 @@ def test_pressure_rhs_vs_oracle(gpu, torch_mod, orc, N, visc, seed):
     assert rel(got, ref) <= 1e-11
 
 
+@pytest.mark.parametrize("N,fusion", [(7, True), (7, False), (5, True)])
+def test_pressure_rhs_stress_vs_oracle(gpu, torch_mod, orc, N, fusion):
+    """VELOCITY STRESSFORMULATION = TRUE: the stress term of pressureStressHex3D
+    taken from the second curl (oracle pinned by the closed form of
+    test_oracle_tombo.py::test_stress_term_closed_form), with a varying viscosity;
+    at N = 7 also through the face-fused curl kernels."""
+    import oracle.tombo as ot
+    torch = torch_mod
+    m, mo = meshes(gpu, orc, (4, 3, 3), N)
+    bc = bc_mix(m, 5)
+    t = gpu.PressureRhs(m, bc, fusion=fusion, autotune=False, stress=True)
+    g = ot.Geometry(N, m.x, m.y, m.z, m.gid)
+    L = m.x.size
+    off = L + 64
+    rng = np.random.default_rng(7)
+    Ue, U, BF = (rng.uniform(-1, 1, 3 * off) for _ in range(3))
+    mue = 0.01 * (1.0 + 0.4 * np.sin(2 * m.x) * np.cos(m.y) + 0.2 * m.z)
+    irho = rng.uniform(0.5, 1.5, L)
+    div = rng.uniform(-0.1, 0.1, L)
+    g0idt = 1.5 / 0.01
+    ref = g.pressure_rhs(bc, Ue, U, BF, mue, irho, div, g0idt, True, off, stress=True)
+    plain = g.pressure_rhs(bc, Ue, U, BF, mue, irho, div, g0idt, True, off, stress=False)
+    dv = lambda a: torch.from_numpy(a).cuda()
+    got = t.compute(dv(Ue), dv(U), dv(BF), dv(mue), dv(irho), dv(div), g0idt, True,
+                    field_offset=off).cpu().numpy()
+    print("stress N", N, "fusion", fusion, "rel", rel(got, ref), "term size", rel(ref, plain), flush=True)
+    assert rel(ref, plain) > 1e-8                  # the term is not negligible here
+    assert rel(got, ref) <= 1e-11
+
+
 def test_pressure_rhs_closed_form_affine(gpu, torch_mod, orc):
     """Walls everywhere: sum_i x_i rhs_i = sum JW (x + nu (2 + 4/3 a_x)) + g0idt sum x JW div
     for Ue = (y^2, z^2, x^2), BF = JW (x, 1, z), div = c + a.x (the oracle's closed form)."""

@@ -130,3 +130,35 @@ def test_pressure_rhs_closed_form(orc, visc):
     assert abs(np.sum(y * rhs) - expect_y) <= 1e-10 * abs(expect_y)
     # phi = 1: the weak divergence sums to zero
     assert abs(np.sum(rhs) - g0idt * np.sum(JW * div)) <= 1e-10 * abs(g0idt * np.sum(JW * div))
+
+
+def test_stress_term_closed_form(orc):
+    """The stress-formulation term (pressureStressHex3D.okl) of quadratic
+    velocity and viscosity fields on the affine mesh, where the nodal
+    derivatives are exact: JW (2 sym(grad U) grad(mu)/mu - 2/3 div grad(mu)/mu)
+    against the closed form; a constant viscosity gives zero."""
+    m, x, y, z, g = geom(orc)
+    L = x.size
+    U = np.concatenate([x * y + z, y * z - 0.5 * x * x, z * x + y * y])
+    mu = 1.5 + 0.2 * x - 0.1 * y * y + 0.05 * x * z
+    div = 0.3 + 0.7 * x - 0.2 * z                          # any field: the term is linear in it
+    S = g.stress(U, mu, div).reshape(3, L)
+    grad_u = np.stack([np.stack([y, x, np.ones(L)]),            # d(u)/d(x, y, z)
+                       np.stack([-x, z, y]),                     # d(v)/d(...)
+                       np.stack([z, 2 * y, x])])                 # d(w)/d(...)
+    gm = np.stack([0.2 + 0.05 * z, -0.2 * y, 0.05 * x]) / mu
+    sym = grad_u + grad_u.transpose(1, 0, 2)
+    w = np.einsum("abn,bn->an", sym, gm)
+    ref = g.JW.ravel()[None, :] * (w - (2.0 / 3.0) * div[None, :] * gm)
+    assert np.max(np.abs(S - ref)) <= 1e-10 * np.max(np.abs(ref))
+    assert np.max(np.abs(g.stress(U, np.full(L, 2.0), div))) <= 1e-12
+    # in the right-hand side it is taken from the second curl
+    rng = np.random.default_rng(0)
+    bc = np.zeros(6 * g.E, dtype=np.int32)
+    BF = rng.uniform(-1, 1, 3 * L)
+    irho = np.full(L, 0.7)
+    a = g.pressure_rhs(bc, U, U, BF, mu, irho, div, 2.0, visc=True, stress=True)
+    b = g.pressure_rhs(bc, U, U, BF, mu, irho, div, 2.0, visc=True, stress=False)
+    nu = np.tile(mu * irho, 3)
+    dU = g.ogs.gs(nu * S.ravel(), "add", nvec=3, stride=L) * np.tile(g.invLMM, 3)
+    assert np.max(np.abs((a - b) - g.wdiv(dU))) <= 1e-10 * np.max(np.abs(a))
