@@ -167,7 +167,7 @@ int nrs_ogs_info(nrs_ogs *ogs, int64_t *Nrows, int64_t *Nmembers, int64_t *Nsing
 /* ------------------------------------------------------------------------- */
 /* linAlg_t reductions and the PCG residual update                            */
 /* ------------------------------------------------------------------------- */
-/* The reductions of linAlg_t (src/linAlg/linAlg.hpp:156-215, kernels/linAlg/*):
+/* The reductions of linAlg_t (src/linAlg/linAlg.hpp:156-215, the kernels under kernels/linAlg):
  *   NRS_RED_SUM     sum x                    (sum / sumMany)
  *   NRS_RED_MAX     max x                    (max)
  *   NRS_RED_MIN     min x                    (min)
@@ -189,6 +189,20 @@ enum nrs_reduction { NRS_RED_SUM = 0, NRS_RED_MAX = 1, NRS_RED_MIN = 2, NRS_RED_
                      NRS_RED_DOT = 8 };
 int nrs_linalg_reduce(int op, int64_t N, int Nfields, int64_t fieldOffset, const double *w,
                       const double *x, const double *y, double *result, void *stream);
+
+/* nrs_linalg_reduce without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops that must not call hipMalloc, harnesses that forbid it).  Same
+ * arguments, result and error behaviour as nrs_linalg_reduce, bit for bit; the
+ * partial sums live in a fixed device array of 1056 doubles (8.25 KiB) that
+ * every GPU the library runs on keeps until the process ends (its address is
+ * looked up in nrs_init, or on the first call on a device).  Every N fits
+ * (the partials grid is capped at 1024 blocks); a grid beyond the array is
+ * refused with NRS_ERR_INVALID.  All calls on one GPU share the array: calls
+ * that overlap in time on one GPU (other streams or threads) race.  Like
+ * nrs_linalg_reduce it synchronises `stream`, so it cannot be captured. */
+int nrs_linalg_reduce_noalloc(int op, int64_t N, int Nfields, int64_t fieldOffset, const double *w,
+                              const double *x, const double *y, double *result, void *stream);
 
 /* r -= alpha Ap on Nfields fields spaced `offset` apart and *rdotr (host) =
  * sum_i invDegree_i sum_f r[i + f offset]^2 of the updated r (no square root).

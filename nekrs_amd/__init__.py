@@ -303,16 +303,19 @@ REDUCTIONS = {"sum": 0, "max": 1, "min": 2, "norm2": 3, "norm1": 4, "amax": 5,
               "weighted_norm2": 6, "weighted_inner_prod": 7, "inner_prod": 8}
 
 
-def linalg_reduce(op, x, y=None, w=None, N=None, nfields=1, field_offset=None, stream=None):
+def linalg_reduce(op, x, y=None, w=None, N=None, nfields=1, field_offset=None, stream=None,
+                  noalloc=False):
     """linAlg_t reductions (sum, max, min, norm2, norm1, amax, weighted_norm2
     = weightedNorm2Many, weighted_inner_prod = weightedInnerProdMany, inner_prod)
-    over N entries of `nfields` fields spaced `field_offset` apart; a float."""
+    over N entries of `nfields` fields spaced `field_offset` apart; a float.
+    noalloc=True calls the allocation-free entry point (partials in the
+    library's fixed device array)."""
     if N is None:
         N = x.numel() // nfields
     off = N if field_offset is None else field_offset
     out = C.c_double()
-    _lib.call("nrs_linalg_reduce", REDUCTIONS[op], N, nfields, off, _ptr(w), _ptr(x), _ptr(y),
-              C.byref(out), _stream(stream))
+    _lib.call("nrs_linalg_reduce_noalloc" if noalloc else "nrs_linalg_reduce", REDUCTIONS[op], N,
+              nfields, off, _ptr(w), _ptr(x), _ptr(y), C.byref(out), _stream(stream))
     return out.value
 
 

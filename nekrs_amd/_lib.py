@@ -76,6 +76,7 @@ SIGNATURES = {
     "nrs_ogs_inv_degree": ([_vp, _vp, _vp], _i),
     "nrs_ogs_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)], _i),
     "nrs_linalg_reduce": ([_i, _i64, _i, _i64, _vp, _vp, _vp, C.POINTER(_d), _vp], _i),
+    "nrs_linalg_reduce_noalloc": ([_i, _i64, _i, _i64, _vp, _vp, _vp, C.POINTER(_d), _vp], _i),
     "nrs_block_update_pcg": ([_i64, _i, _i64, _vp, _vp, _d, _vp, C.POINTER(_d), _vp], _i),
     "nrs_elliptic_default_opts": ([C.POINTER(EllipticOpts)], None),
     "nrs_elliptic_setup": ([C.POINTER(EllipticDesc), C.POINTER(EllipticOpts), C.POINTER(_vp)], _i),

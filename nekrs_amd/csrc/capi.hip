@@ -3,6 +3,7 @@
 // status codes; nothing here aborts the process.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "batch.hpp"
@@ -20,6 +21,36 @@ namespace nrs {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &m) { g_last_error = m; }
 }  // namespace nrs
+
+// Scratch of the allocation-free reduction (nrs_linalg_reduce_noalloc): the
+// per-block partials and the result slot, in a file-scope device array that
+// each GPU holds for the life of the process.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in symbol): the
+// device address is looked up once per device with hipGetSymbolAddress, with
+// that device current, and kept in g_scratch_ptr.
+namespace {
+constexpr int kScratchPartials = nrs::Reducer::kBlocks;           // the reduction grid's cap
+constexpr int kScratchDoubles = kScratchPartials + 32;            // + the result, 256-B aligned
+__device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Looks up (once) the address of `scratch` on `device`, which must be current.
+double *scratch_setup(int device)
+{
+  if (device < 0 || device >= kMaxScratchDevices)
+    throw nrs::Error(NRS_ERR_UNSUPPORTED, "scratch: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    NRS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw nrs::Error(NRS_ERR_HIP, "scratch: hipGetSymbolAddress returned null");
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 #define NRS_API_BEGIN try {
 #define NRS_API_END                                   \
@@ -53,6 +84,7 @@ int nrs_init(int device)
   NRS_HIP(hipGetDeviceProperties(&prop, device));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     throw nrs::Error(NRS_ERR_UNSUPPORTED, std::string("library built for gfx950, device is ") + prop.gcnArchName);
+  scratch_setup(device);
   NRS_API_END
 }
 
@@ -293,6 +325,36 @@ int nrs_linalg_reduce(int op, int64_t N, int Nfields, int64_t fieldOffset, const
   nrs::DevBuf<double> part((size_t)nrs::reduce_partials_needed(N) + 1);
   nrs::linalg_reduce(op, N, Nfields, fieldOffset, w, x, y, part.get() + 1, part.get(), s);
   NRS_HIP(hipMemcpyAsync(result, part.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  NRS_HIP(hipStreamSynchronize(s));
+  NRS_API_END
+}
+
+int nrs_linalg_reduce_noalloc(int op, int64_t N, int Nfields, int64_t fieldOffset, const double *w,
+                              const double *x, const double *y, double *result, void *stream)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(result, "linalg reduce (no alloc): null result");
+  NRS_CHECK(op >= NRS_RED_SUM && op <= NRS_RED_DOT, "linalg reduce (no alloc): unknown reduction");
+  NRS_CHECK(N >= 0 && Nfields >= 1, "linalg reduce (no alloc): N must be >= 0 and Nfields >= 1");
+  NRS_CHECK(Nfields == 1 || fieldOffset >= N, "linalg reduce (no alloc): fieldOffset must be >= N");
+  NRS_CHECK(N == 0 || x, "linalg reduce (no alloc): null x");
+  NRS_CHECK(N == 0 || !(op == NRS_RED_DOT || op == NRS_RED_WDOT) || y,
+            "linalg reduce (no alloc): null y");
+  NRS_CHECK(N == 0 || !(op == NRS_RED_WNORM2 || op == NRS_RED_WDOT) || w,
+            "linalg reduce (no alloc): null weight");
+  const int partials = nrs::reduce_partials_needed(N);
+  if (partials > kScratchPartials)
+    throw nrs::Error(NRS_ERR_INVALID, "linalg reduce (no alloc): " + std::to_string(partials) +
+                                          " partial sums exceed the scratch of " +
+                                          std::to_string(kScratchPartials));
+  int device = 0;
+  NRS_HIP(hipGetDevice(&device));
+  double *const sp = scratch_setup(device);      // the device address, never `scratch` itself
+  const hipStream_t s = as_stream(stream);
+  // every partial slot the finish kernel reads is written by the streaming kernel
+  // of the same call, so nothing is cleared first
+  nrs::linalg_reduce(op, N, Nfields, fieldOffset, w, x, y, sp, sp + kScratchPartials, s);
+  NRS_HIP(hipMemcpyAsync(result, sp + kScratchPartials, sizeof(double), hipMemcpyDeviceToHost, s));
   NRS_HIP(hipStreamSynchronize(s));
   NRS_API_END
 }
