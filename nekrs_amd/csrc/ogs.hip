@@ -6,10 +6,13 @@
 // excluded nodes).  Unlike the reference, singleton rows are not stored at all
 // and multi-member rows are bucketed by length (2, 4, 8, other) so that each
 // thread reads its member indices with one aligned vector load and runs a
-// fixed-trip loop.  The reduction order inside a row is the reference's
+// fixed-trip loop.  On the device the rows are grouped into tiles by their last
+// member (one block per tile, XCD-contiguous), which keeps the face lines a
+// tile's rows share in L2.  The reduction order inside a row is the reference's
 // (gatherScatterMany.okl:58-86: identity op m0 op m1 ...), so FP64 results match
 // the reference kernel bit for bit.
 #include <algorithm>
+#include <memory>
 #include <numeric>
 
 #include "ogs.hpp"
@@ -87,62 +90,110 @@ __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__r
   }
 }
 
-// One launch covers all buckets: blocks [0,b2) pairs, [b2,b4) quads, [b4,b8) octs,
-// [b8,end) CSR rows; the bucket choice is uniform per block.
-template <typename T, int OP>
+// One block per tile; the tile's pairs, quads, octs and CSR rows run one after
+// the other, so the lines they share (a face line holds face, edge and vertex
+// nodes) stay in one XCD's L2 between them.  Blocks are dealt round-robin to
+// the 8 XCDs, so block b runs tile (b mod 8) * per + b / 8: each XCD walks a
+// contiguous run of tiles and the lines that neighbouring tiles share stay in
+// its L2 as well.  WEIGHTED: v <- gs_add(w .* v) (nvec = 1).
+template <typename T, int OP, bool WEIGHTED>
 __global__ void __launch_bounds__(kGsBlock)
-gs_kernel(int n2, int n4, int n8, int nx, int b2, int b4, int b8, const int *__restrict__ pairs,
-          const int *__restrict__ quads, const int *__restrict__ octs, const int *__restrict__ xoff,
-          const int *__restrict__ xids, T *__restrict__ v, int nvec, int64_t stride)
+gs_kernel(int t0, int nt, int per, const int *__restrict__ toff, int tstride,
+          const int *__restrict__ pairs, const int *__restrict__ quads, const int *__restrict__ octs,
+          const int *__restrict__ xoff, const int *__restrict__ xids, const T *__restrict__ w,
+          T *__restrict__ v, int nvec, int64_t stride)
 {
-  const int b = blockIdx.x;
-  if (b < b2) {
-    const int r = b * kGsBlock + threadIdx.x;
-    if (r < n2) gs_fixed_row<T, OP, 2>(pairs + 2 * (int64_t)r, v, nvec, stride);
-  } else if (b < b4) {
-    const int r = (b - b2) * kGsBlock + threadIdx.x;
-    if (r < n4) gs_fixed_row<T, OP, 4>(quads + 4 * (int64_t)r, v, nvec, stride);
-  } else if (b < b8) {
-    const int r = (b - b4) * kGsBlock + threadIdx.x;
-    if (r < n8) gs_fixed_row<T, OP, 8>(octs + 8 * (int64_t)r, v, nvec, stride);
-  } else {
-    const int r = (b - b8) * kGsBlock + threadIdx.x;
-    if (r >= nx) return;
-    const int s = xoff[r], e = xoff[r + 1];
-    for (int k = 0; k < nvec; ++k) {
-      T *vk = v + (int64_t)k * stride;
-      T g = Red<T, OP>::id();
-      for (int m = s; m < e; ++m) g = Red<T, OP>::apply(g, vk[xids[m]]);
-      for (int m = s; m < e; ++m) vk[xids[m]] = g;
+  const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (t >= nt) return;
+  const int *tb = toff + t0 + t;
+  const int s2 = tb[0], e2 = tb[1];
+  const int s4 = tb[tstride], e4 = tb[tstride + 1];
+  const int s8 = tb[2 * tstride], e8 = tb[2 * tstride + 1];
+  const int sx = tb[3 * tstride], ex = tb[3 * tstride + 1];
+  for (int r = s2 + (int)threadIdx.x; r < e2; r += kGsBlock)
+    gs_fixed_row<T, OP, 2, WEIGHTED>(pairs + 2 * (int64_t)r, v, nvec, stride);
+  for (int r = s4 + (int)threadIdx.x; r < e4; r += kGsBlock)
+    gs_fixed_row<T, OP, 4, WEIGHTED>(quads + 4 * (int64_t)r, v, nvec, stride);
+  for (int r = s8 + (int)threadIdx.x; r < e8; r += kGsBlock)
+    gs_fixed_row<T, OP, 8, WEIGHTED>(octs + 8 * (int64_t)r, v, nvec, stride);
+  for (int r = sx + (int)threadIdx.x; r < ex; r += kGsBlock) {
+    const int s0 = xoff[r], e0 = xoff[r + 1];
+    if constexpr (WEIGHTED) {
+      T g = T(0);
+      for (int m = s0; m < e0; ++m) g = g + v[xids[m]] * w[xids[m]];
+      for (int m = s0; m < e0; ++m) v[xids[m]] = g;
+    } else {
+      for (int k = 0; k < nvec; ++k) {
+        T *vk = v + (int64_t)k * stride;
+        T g = Red<T, OP>::id();
+        for (int m = s0; m < e0; ++m) g = Red<T, OP>::apply(g, vk[xids[m]]);
+        for (int m = s0; m < e0; ++m) vk[xids[m]] = g;
+      }
     }
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kGsBlock)
-gs_weighted_kernel(int n2, int n4, int n8, int nx, int b2, int b4, int b8, const int *__restrict__ pairs,
-                   const int *__restrict__ quads, const int *__restrict__ octs,
-                   const int *__restrict__ xoff, const int *__restrict__ xids,
-                   const T *__restrict__ w, T *__restrict__ v)
+// Launches tiles [t0, t1) of the handle.
+template <typename T, int OP, bool WEIGHTED>
+void launch_gs(const nrs_ogs *o, int t0, int t1, const T *w, T *v, int nvec, int64_t stride,
+               hipStream_t s)
 {
-  const int b = blockIdx.x;
-  if (b < b2) {
-    const int r = b * kGsBlock + threadIdx.x;
-    if (r < n2) gs_fixed_row<T, NRS_ADD, 2, true>(pairs + 2 * (int64_t)r, v, 1, 0);
-  } else if (b < b4) {
-    const int r = (b - b2) * kGsBlock + threadIdx.x;
-    if (r < n4) gs_fixed_row<T, NRS_ADD, 4, true>(quads + 4 * (int64_t)r, v, 1, 0);
-  } else if (b < b8) {
-    const int r = (b - b4) * kGsBlock + threadIdx.x;
-    if (r < n8) gs_fixed_row<T, NRS_ADD, 8, true>(octs + 8 * (int64_t)r, v, 1, 0);
-  } else {
-    const int r = (b - b8) * kGsBlock + threadIdx.x;
-    if (r >= nx) return;
-    const int s0 = xoff[r], e0 = xoff[r + 1];
-    T g = T(0);
-    for (int m = s0; m < e0; ++m) g = g + v[xids[m]] * w[xids[m]];
-    for (int m = s0; m < e0; ++m) v[xids[m]] = g;
+  const int nt = t1 - t0;
+  if (nt <= 0) return;
+  const int per = (nt + 7) / 8;
+  gs_kernel<T, OP, WEIGHTED><<<8 * per, kGsBlock, 0, s>>>(
+      t0, nt, per, o->tOff.get(), o->nTiles + 1, o->pairs.get(), o->quads.get(), o->octs.get(),
+      o->xoff.get(), o->xids.get(), w, v, nvec, stride);
+  NRS_LAUNCH_CHECK();
+}
+
+// nodes per tile: a few hundred rows per block (N = 7: four elements, ~500 rows)
+constexpr int64_t kTileNodes = 2048;
+
+// Device layout of the buckets grouped by tile (last member / npt); tiles
+// never straddle a chunk when npt divides the chunk size.
+void ogs_layout(nrs_ogs *o, int64_t npt)
+{
+  const int64_t nt64 = std::max<int64_t>(1, (o->N + npt - 1) / npt);
+  NRS_CHECK(nt64 < (int64_t)INT32_MAX / 4, "ogs: too many tiles");
+  const int nt = (int)nt64;
+  std::vector<int> toff(4 * (size_t)(nt + 1), 0);
+  auto tile_of = [&](int last) { return (int)std::min<int64_t>(last / npt, nt - 1); };
+  auto group = [&](const std::vector<int> &h, int W, int b, DevBuf<int> &dev) {
+    const int64_t n = (int64_t)h.size() / W;
+    int *off = toff.data() + (size_t)b * (nt + 1);
+    for (int64_t r = 0; r < n; ++r) off[tile_of(h[r * W + W - 1]) + 1]++;
+    for (int t = 0; t < nt; ++t) off[t + 1] += off[t];
+    std::vector<int> out(h.size());
+    std::vector<int> pos(off, off + nt);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t d = pos[tile_of(h[r * W + W - 1])]++;
+      std::copy(h.begin() + r * W, h.begin() + (r + 1) * W, out.begin() + d * W);
+    }
+    dev.upload(out);
+  };
+  group(o->h2, 2, 0, o->pairs);
+  group(o->h4, 4, 1, o->quads);
+  group(o->h8, 8, 2, o->octs);
+  {   // CSR rows
+    const int64_t n = (int64_t)o->hxo.size() - 1;
+    int *off = toff.data() + (size_t)3 * (nt + 1);
+    for (int64_t r = 0; r < n; ++r) off[tile_of(o->hxi[o->hxo[r + 1] - 1]) + 1]++;
+    for (int t = 0; t < nt; ++t) off[t + 1] += off[t];
+    std::vector<int> pos(off, off + nt);
+    std::vector<int64_t> dst(n);
+    for (int64_t r = 0; r < n; ++r) dst[r] = pos[tile_of(o->hxi[o->hxo[r + 1] - 1])]++;
+    std::vector<int> len(n), xo(n + 1, 0), xi(o->hxi.size());
+    for (int64_t r = 0; r < n; ++r) len[dst[r]] = o->hxo[r + 1] - o->hxo[r];
+    for (int64_t r = 0; r < n; ++r) xo[r + 1] = xo[r] + len[r];
+    for (int64_t r = 0; r < n; ++r)
+      std::copy(o->hxi.begin() + o->hxo[r], o->hxi.begin() + o->hxo[r + 1], xi.begin() + xo[dst[r]]);
+    o->xoff.upload(xo);
+    o->xids.upload(xi);
   }
+  o->tOff.upload(toff);
+  o->nTiles = nt;
+  o->nodesPerTile = npt;
 }
 
 }  // namespace
@@ -152,14 +203,7 @@ void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t
 {
   NRS_CHECK(o != nullptr, "ogs: null handle");
   if (o->Nrows == 0) return;
-  const int b2 = (int)((o->n2 + kGsBlock - 1) / kGsBlock);
-  const int b4 = b2 + (int)((o->n4 + kGsBlock - 1) / kGsBlock);
-  const int b8 = b4 + (int)((o->n8 + kGsBlock - 1) / kGsBlock);
-  const int bt = b8 + (int)((o->nx + kGsBlock - 1) / kGsBlock);
-  gs_weighted_kernel<T><<<bt, kGsBlock, 0, s>>>((int)o->n2, (int)o->n4, (int)o->n8, (int)o->nx, b2, b4,
-                                               b8, o->pairs.get(), o->quads.get(), o->octs.get(),
-                                               o->xoff.get(), o->xids.get(), w, v);
-  NRS_LAUNCH_CHECK();
+  launch_gs<T, NRS_ADD, true>(o, 0, o->nTiles, w, v, 1, 0, s);
 }
 template void ogs_gather_scatter_weighted<double>(const nrs_ogs *, double *, const double *, hipStream_t);
 template void ogs_gather_scatter_weighted<float>(const nrs_ogs *, float *, const float *, hipStream_t);
@@ -171,23 +215,13 @@ void ogs_gather_scatter(const nrs_ogs *o, T *v, int nvec, int64_t stride, int op
   NRS_CHECK(nvec >= 1, "ogs: nvec must be >= 1");
   NRS_CHECK(nvec == 1 || stride >= o->N, "ogs: stride smaller than the vector length");
   if (o->Nrows == 0) return;
-  const int b2 = (int)((o->n2 + kGsBlock - 1) / kGsBlock);
-  const int b4 = b2 + (int)((o->n4 + kGsBlock - 1) / kGsBlock);
-  const int b8 = b4 + (int)((o->n8 + kGsBlock - 1) / kGsBlock);
-  const int bt = b8 + (int)((o->nx + kGsBlock - 1) / kGsBlock);
-#define GS_LAUNCH(OPV)                                                                             \
-  gs_kernel<T, OPV><<<bt, kGsBlock, 0, s>>>((int)o->n2, (int)o->n4, (int)o->n8, (int)o->nx, b2, b4, \
-                                            b8, o->pairs.get(), o->quads.get(), o->octs.get(),     \
-                                            o->xoff.get(), o->xids.get(), v, nvec, stride)
   switch (op) {
-    case NRS_ADD: GS_LAUNCH(NRS_ADD); break;
-    case NRS_MUL: GS_LAUNCH(NRS_MUL); break;
-    case NRS_MIN: GS_LAUNCH(NRS_MIN); break;
-    case NRS_MAX: GS_LAUNCH(NRS_MAX); break;
+    case NRS_ADD: launch_gs<T, NRS_ADD, false>(o, 0, o->nTiles, nullptr, v, nvec, stride, s); break;
+    case NRS_MUL: launch_gs<T, NRS_MUL, false>(o, 0, o->nTiles, nullptr, v, nvec, stride, s); break;
+    case NRS_MIN: launch_gs<T, NRS_MIN, false>(o, 0, o->nTiles, nullptr, v, nvec, stride, s); break;
+    case NRS_MAX: launch_gs<T, NRS_MAX, false>(o, 0, o->nTiles, nullptr, v, nvec, stride, s); break;
     default: throw Error(NRS_ERR_INVALID, "ogs: unknown op");
   }
-#undef GS_LAUNCH
-  NRS_LAUNCH_CHECK();
 }
 template void ogs_gather_scatter<double>(const nrs_ogs *, double *, int, int64_t, int, hipStream_t);
 template void ogs_gather_scatter<float>(const nrs_ogs *, float *, int, int64_t, int, hipStream_t);
@@ -198,18 +232,8 @@ void ogs_gather_scatter_chunks(const nrs_ogs *o, T *v, int c0, int c1, hipStream
 {
   NRS_CHECK(o != nullptr, "ogs: null handle");
   NRS_CHECK(o->nChunks > 0 && c0 >= 0 && c0 <= c1 && c1 <= o->nChunks, "ogs: bad chunk range");
-  const int64_t r2 = o->cOff[0][c0], r4 = o->cOff[1][c0], r8 = o->cOff[2][c0], rx = o->cOff[3][c0];
-  const int n2 = (int)(o->cOff[0][c1] - r2), n4 = (int)(o->cOff[1][c1] - r4);
-  const int n8 = (int)(o->cOff[2][c1] - r8), nx = (int)(o->cOff[3][c1] - rx);
-  const int b2 = (n2 + kGsBlock - 1) / kGsBlock;
-  const int b4 = b2 + (n4 + kGsBlock - 1) / kGsBlock;
-  const int b8 = b4 + (n8 + kGsBlock - 1) / kGsBlock;
-  const int bt = b8 + (nx + kGsBlock - 1) / kGsBlock;
-  if (bt == 0) return;
-  gs_kernel<T, NRS_ADD><<<bt, kGsBlock, 0, s>>>(n2, n4, n8, nx, b2, b4, b8, o->pairs.get() + 2 * r2,
-                                                o->quads.get() + 4 * r4, o->octs.get() + 8 * r8,
-                                                o->xoff.get() + rx, o->xids.get(), v, 1, 0);
-  NRS_LAUNCH_CHECK();
+  if (o->Nrows == 0) return;
+  launch_gs<T, NRS_ADD, false>(o, o->cTile[c0], o->cTile[c1], nullptr, v, 1, 0, s);
 }
 template void ogs_gather_scatter_chunks<double>(const nrs_ogs *, double *, int, int, hipStream_t);
 template void ogs_gather_scatter_chunks<float>(const nrs_ogs *, float *, int, int, hipStream_t);
@@ -217,46 +241,17 @@ template void ogs_gather_scatter_chunks<float>(const nrs_ogs *, float *, int, in
 void ogs_set_chunks(nrs_ogs *o, int64_t nodesPerChunk, int nChunks)
 {
   NRS_CHECK(o != nullptr && nodesPerChunk >= 1 && nChunks >= 1, "ogs: bad chunking");
-  auto chunk_of = [&](int last) { return (int)std::min<int64_t>(last / nodesPerChunk, nChunks - 1); };
-  // fixed-width buckets: stable sort of rows by chunk
-  auto regroup = [&](std::vector<int> &h, int W, std::vector<int64_t> &off) {
-    const int64_t n = (int64_t)h.size() / W;
-    off.assign(nChunks + 1, 0);
-    for (int64_t r = 0; r < n; ++r) off[chunk_of(h[r * W + W - 1]) + 1]++;
-    for (int c = 0; c < nChunks; ++c) off[c + 1] += off[c];
-    std::vector<int> out(h.size());
-    std::vector<int64_t> pos(off.begin(), off.end() - 1);
-    for (int64_t r = 0; r < n; ++r) {
-      const int64_t d = pos[chunk_of(h[r * W + W - 1])]++;
-      std::copy(h.begin() + r * W, h.begin() + (r + 1) * W, out.begin() + d * W);
-    }
-    h.swap(out);
-  };
-  regroup(o->h2, 2, o->cOff[0]);
-  regroup(o->h4, 4, o->cOff[1]);
-  regroup(o->h8, 8, o->cOff[2]);
-  {   // CSR rows
-    const int64_t n = (int64_t)o->hxo.size() - 1;
-    auto &off = o->cOff[3];
-    off.assign(nChunks + 1, 0);
-    for (int64_t r = 0; r < n; ++r) off[chunk_of(o->hxi[o->hxo[r + 1] - 1]) + 1]++;
-    for (int c = 0; c < nChunks; ++c) off[c + 1] += off[c];
-    std::vector<int64_t> pos(off.begin(), off.end() - 1);
-    std::vector<int64_t> dst(n);
-    for (int64_t r = 0; r < n; ++r) dst[r] = pos[chunk_of(o->hxi[o->hxo[r + 1] - 1])]++;
-    std::vector<int> len(n), xo(n + 1, 0), xi(o->hxi.size());
-    for (int64_t r = 0; r < n; ++r) len[dst[r]] = o->hxo[r + 1] - o->hxo[r];
-    for (int64_t r = 0; r < n; ++r) xo[r + 1] = xo[r] + len[r];
-    for (int64_t r = 0; r < n; ++r)
-      std::copy(o->hxi.begin() + o->hxo[r], o->hxi.begin() + o->hxo[r + 1], xi.begin() + xo[dst[r]]);
-    o->hxo.swap(xo);
-    o->hxi.swap(xi);
-  }
-  o->pairs.upload(o->h2);
-  o->quads.upload(o->h4);
-  o->octs.upload(o->h8);
-  o->xoff.upload(o->hxo);
-  o->xids.upload(o->hxi);
+  // tiles of npt = nodesPerChunk / k nodes, the largest such divisor not above the default
+  int64_t k = (nodesPerChunk + kTileNodes - 1) / kTileNodes;
+  while (nodesPerChunk % k) ++k;
+  const int64_t npt = nodesPerChunk / k;
+  ogs_layout(o, npt);
+  // chunk c = min(last / nodesPerChunk, nChunks - 1) covers tiles [c k, (c + 1) k), the
+  // last chunk every tile after it
+  o->cTile.assign(nChunks + 1, 0);
+  for (int c = 0; c <= nChunks; ++c)
+    o->cTile[c] = (int)std::min<int64_t>((int64_t)c * k, o->nTiles);
+  o->cTile[nChunks] = o->nTiles;
   o->nChunks = nChunks;
   o->nodesPerChunk = nodesPerChunk;
 }
@@ -265,7 +260,7 @@ nrs_ogs *ogs_setup(int64_t N, const long long *ids)
 {
   NRS_CHECK(N >= 0 && N < (int64_t)INT32_MAX, "ogs: N must fit in int32 (dlong)");
   NRS_CHECK(N == 0 || ids != nullptr, "ogs: null id array");
-  auto *o = new nrs_ogs();
+  std::unique_ptr<nrs_ogs> o(new nrs_ogs());
   o->N = N;
   // group members by id: counting sort when ids are dense, comparison sort otherwise
   long long maxId = 0, minId = 0;
@@ -342,14 +337,10 @@ nrs_ogs *ogs_setup(int64_t N, const long long *ids)
   o->n4 = (int64_t)h4.size() / 4;
   o->n8 = (int64_t)h8.size() / 8;
   o->nx = (int64_t)hxo.size() - 1;
-  o->pairs.upload(h2);
-  o->quads.upload(h4);
-  o->octs.upload(h8);
-  o->xoff.upload(hxo);
-  o->xids.upload(hxi);
-  o->invDegree.upload(o->invDegreeHost);
   o->h2.swap(h2); o->h4.swap(h4); o->h8.swap(h8); o->hxo.swap(hxo); o->hxi.swap(hxi);
-  return o;
+  ogs_layout(o.get(), kTileNodes);
+  o->invDegree.upload(o->invDegreeHost);
+  return o.release();
 }
 
 }  // namespace nrs

@@ -18,15 +18,22 @@ struct nrs_ogs {
   nrs::DevBuf<int> xids;
   nrs::DevBuf<double> invDegree;  // [N]
   std::vector<double> invDegreeHost;
-  // Chunks (optional, nrs::ogs_set_chunks): inside each bucket the rows are
-  // grouped by the chunk of their last member (local index / nodesPerChunk),
-  // keeping their order inside a group; chunk c's rows of bucket b are
-  // [cOff[b][c], cOff[b][c+1]).  A row of chunk c only touches nodes below
-  // the end of chunk c.
+  // Tiles: on the device the rows of each bucket are grouped by the tile of
+  // their last member (local index / nodesPerTile), first-appearance order
+  // inside a tile; tile t's rows of bucket b are [tOff[b][t], tOff[b][t+1]).
+  // One block runs one tile, all four buckets, so the lines that a tile's
+  // pairs, quads and octs share are fetched once into one XCD's L2.
+  int nTiles = 0;
+  int64_t nodesPerTile = 0;
+  nrs::DevBuf<int> tOff;      // [4][nTiles + 1]
+  // Chunks (optional, nrs::ogs_set_chunks): chunk c (last member / nodesPerChunk,
+  // at most nChunks - 1) is the tile range [cTile[c], cTile[c+1]).  A row of
+  // chunk c only touches nodes below the end of chunk c.
   int nChunks = 0;
   int64_t nodesPerChunk = 0;
-  std::vector<int64_t> cOff[4];
-  std::vector<int> h2, h4, h8, hxo, hxi;   // host copies of the buckets
+  std::vector<int> cTile;
+  // host copies of the buckets in first-appearance order
+  std::vector<int> h2, h4, h8, hxo, hxi;
   // the call between nrs_ogs_start and nrs_ogs_finish (oogs::start / finish)
   struct Pending { const void *v; int nvec; int64_t stride; int dtype, op; };
   bool pending = false;
@@ -43,8 +50,9 @@ nrs_ogs *ogs_setup(int64_t N, const long long *ids);
 // are (their weight is 1).
 template <typename T>
 void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s);
-// Group rows by chunk (chunk = last member / nodesPerChunk, at most nChunks - 1).
-// Reorders rows inside buckets only: every gather-scatter result is unchanged.
+// Group rows by chunk (chunk = last member / nodesPerChunk, at most nChunks - 1),
+// re-tiling so that tiles nest in chunks.  Reorders rows inside buckets only:
+// every gather-scatter result is unchanged.
 void ogs_set_chunks(nrs_ogs *o, int64_t nodesPerChunk, int nChunks);
 // v <- gs_add(v) on the rows of chunks [c0, c1) only (one vector).  After the
 // nodes of those chunks are final, this completes their rows; rows of later
