@@ -124,6 +124,47 @@ std::vector<uint8_t> face_fusion(int N, int E, std::vector<long long> &ids, cons
   return fx;
 }
 
+// Elements per operator chunk: kOpChunks chunks, each a multiple of four elements
+// so that chunks hold whole blocks of the fused Ax kernel.
+int64_t op_chunk_elements(int64_t E)
+{
+  constexpr int F = nrs_elliptic::kOpChunks;
+  return ((E + F - 1) / F + 3) / 4 * 4;
+}
+
+namespace {
+// A form of the operator face fusion: blocks of nb elements (nb waves).
+struct FusionForm { int nb; };
+struct FusedOperator {
+  std::unique_ptr<nrs_ogs> ogs;     // the operator's handle without the fused face nodes
+  DevBuf<uint8_t> fx;
+  int64_t pairs = 0;
+};
+
+// The handle and flags of a fusion form; false when no pair of elements qualifies.
+bool make_fused(nrs_elliptic *el, const std::vector<long long> &maskedIds, FusionForm f, FusedOperator &c)
+{
+  std::vector<long long> ids(maskedIds);
+  const std::vector<uint8_t> fx = face_fusion(el->N, el->E, ids, el->ogs->invDegreeHost, f.nb);
+  if (fx.empty()) return false;
+  c.ogs.reset(ogs_setup(el->Nlocal, ids.data()));
+  ogs_set_chunks(c.ogs.get(), op_chunk_elements(el->E) * el->Np, nrs_elliptic::kOpChunks);
+  c.fx.upload(fx);
+  c.pairs = 0;
+  for (uint8_t v : fx) c.pairs += v;
+  return true;
+}
+
+void keep_fused(nrs_elliptic *el, FusionForm f, FusedOperator &c)
+{
+  delete el->ogsOp;
+  el->ogsOp = c.ogs.release();
+  el->fx = std::move(c.fx);
+  el->fusedPairs = c.pairs;
+  el->fusionBlock = f.nb;
+}
+}  // namespace
+
 nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts *o)
 {
   NRS_CHECK(d && o, "elliptic setup: null descriptor/options");
@@ -144,8 +185,6 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   // ellipticSetup.cpp:59-73: multigrid with non-Jacobi smoothers and its coarse solve
   // are Poisson-only in the reference; kept as in the reference.
   auto el = std::make_unique<nrs_elliptic>();
-  std::vector<long long> fusion2Ids;   // operator face fusion in blocks of two (timed at setup)
-  std::vector<uint8_t> fusion2;
   el->N = d->N; el->Nq = d->N + 1; el->Np = el->Nq * el->Nq * el->Nq; el->E = d->Nelements;
   el->Nf = d->Nfields <= 1 ? 1 : d->Nfields;
   NRS_CHECK(el->Nf == 1 || el->Nf == 3, "elliptic setup: Nfields must be 1 or 3");
@@ -209,34 +248,25 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
   el->Nmasked = (int64_t)mids.size();
   el->maskIds.upload(mids);
   el->maskBits.upload(mask_bits(el->Np, el->E, mids));
+  std::vector<long long> mg(d->globalIds, d->globalIds + el->Nlocal);   // Dirichlet ids zeroed
+  std::vector<FusionForm> fusionForms;
   {
-    std::vector<long long> mg(d->globalIds, d->globalIds + el->Nlocal);
     for (int id : mids) mg[id] = 0;
     el->ogs = ogs_setup(el->Nlocal, mg.data());
-    if (o->operatorFusion && el->Nf == 1 && !el->lfield()) {
-      const int nb = o->operatorFusion == 2 ? 2 : 4;
-      std::vector<long long> m4(mg);
-      const std::vector<uint8_t> fx = face_fusion(el->N, el->E, m4, el->ogs->invDegreeHost, nb);
-      if (!fx.empty()) {
-        el->ogsOp = ogs_setup(el->Nlocal, m4.data());
-        el->fx.upload(fx);
-        el->fusionBlock = nb;
-        for (uint8_t f : fx) el->fusedPairs += f;
-        if (o->autotune && nb == 4) {   // the pairs-only form (blocks of two) is timed against it below
-          fusion2Ids = mg;
-          fusion2 = face_fusion(el->N, el->E, fusion2Ids, el->ogs->invDegreeHost, 2);
-        }
-      }
+    // operator face fusion forms to try (elements per block): with autotune on,
+    // blocks of four and of two are timed against no fusion
+    if (o->operatorFusion && el->Nf == 1 && !el->lfield() && el->N == 7) {
+      if (o->operatorFusion == 2) fusionForms = {{2}};
+      else if (o->autotune) fusionForms = {{4}, {2}};
+      else fusionForms = {{4}};
     }
   }
 
   // operator chunks: rows of the operator's handle grouped by element chunk
-  // (chunks of a multiple of four elements, the fused kernel's block)
+  // (chunks of a multiple of four elements, whole blocks of the fused kernel)
   {
     constexpr int F = nrs_elliptic::kOpChunks;
-    const int64_t epc = ((el->E + F - 1) / F + 3) / 4 * 4;
-    ogs_set_chunks(el->ogs, epc * el->Np, F);
-    if (el->ogsOp) ogs_set_chunks(el->ogsOp, epc * el->Np, F);
+    ogs_set_chunks(el->ogs, op_chunk_elements(el->E) * el->Np, F);
     NRS_HIP(hipStreamCreateWithFlags(&el->aux, hipStreamNonBlocking));
     for (auto &e : el->chunkEv) NRS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     NRS_HIP(hipEventCreateWithFlags(&el->auxDone, hipEventDisableTiming));
@@ -260,55 +290,33 @@ nrs_elliptic *elliptic_setup(const nrs_elliptic_desc *d, const nrs_elliptic_opts
     const int cUnf = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
                                                el->lam0.get(), el->lam1.get(), el->poisson,
                                                el->maskBits.get(), el->ogs, s);
-    if (el->ogsOp) {
-      const double tUnf = op_ms(el->ogs, nullptr, cUnf);
-      const int cFus = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
-                                                 el->lam0.get(), el->lam1.get(), el->poisson,
-                                                 el->maskBits.get(), el->ogsOp, s, el->fx.get());
-      double tFus = op_ms(el->ogsOp, el->fx.get(), cFus);
-      int cBest = cFus;
-      if (!fusion2.empty()) {   // blocks of two elements: fewer pairs, a barrier over two waves
-        constexpr int F = nrs_elliptic::kOpChunks;
-        const int64_t epc = ((el->E + F - 1) / F + 3) / 4 * 4;
-        std::unique_ptr<nrs_ogs> op2(ogs_setup(el->Nlocal, fusion2Ids.data()));
-        ogs_set_chunks(op2.get(), epc * el->Np, F);
-        DevBuf<uint8_t> fx2;
-        fx2.upload(fusion2);
-        double t2 = 0;
-        int c2 = 0;
-        {
-          // the kernels read the block size from the handle: two while timing, four after
-          // (also when the timing throws)
-          struct Block2 {
-            nrs_elliptic *el;
-            explicit Block2(nrs_elliptic *e) : el(e) { el->fusionBlock = 2; }
-            ~Block2() { el->fusionBlock = 4; }
-          } guard(el.get());
-          c2 = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
-                                         el->lam0.get(), el->lam1.get(), el->poisson,
-                                         el->maskBits.get(), op2.get(), s, fx2.get());
-          t2 = op_ms(op2.get(), fx2.get(), c2);
-        }
-        if (t2 < tFus) {
-          delete el->ogsOp;
-          el->ogsOp = op2.release();
-          el->fx = std::move(fx2);
-          el->fusionBlock = 2;
-          el->fusedPairs = 0;
-          for (uint8_t f : fusion2) el->fusedPairs += f;
-          tFus = t2;
-          cBest = c2;
-        }
-        set_tuned(TUNE_OP, el->N, NRS_F64, el->E, cBest);
-      }
-      if (!(tFus < tUnf)) {
-        delete el->ogsOp;
-        el->ogsOp = nullptr;
-        el->fx.release();
-        el->fusedPairs = 0;
-        set_tuned(TUNE_OP, el->N, NRS_F64, el->E, cUnf);
+    double tBest = fusionForms.empty() ? 0.0 : op_ms(el->ogs, nullptr, cUnf);
+    int cBest = cUnf;
+    for (const FusionForm &f : fusionForms) {
+      FusedOperator c;
+      if (!make_fused(el.get(), mg, f, c)) continue;
+      // the kernels read the block size from the handle while this form is timed
+      // (restored on exit, also when the timing throws)
+      struct Form {
+        nrs_elliptic *el; int nb;
+        Form(nrs_elliptic *e, FusionForm f) : el(e), nb(e->fusionBlock) { el->fusionBlock = f.nb; }
+        ~Form() { el->fusionBlock = nb; }
+      } guard(el.get(), f);
+      const int cf = autotune_operator<double>(el.get(), el->N, el->E, el->ggeo.get(), el->D.get(),
+                                               el->lam0.get(), el->lam1.get(), el->poisson,
+                                               el->maskBits.get(), c.ogs.get(), s, c.fx.get());
+      const double tf = op_ms(c.ogs.get(), c.fx.get(), cf);
+      if (tf < tBest) {
+        tBest = tf;
+        cBest = cf;
+        keep_fused(el.get(), f, c);
+        guard.nb = f.nb;
       }
     }
+    set_tuned(TUNE_OP, el->N, NRS_F64, el->E, cBest);
+  } else if (!fusionForms.empty()) {
+    FusedOperator c;
+    if (make_fused(el.get(), mg, fusionForms[0], c)) keep_fused(el.get(), fusionForms[0], c);
   }
 
   // workspace
@@ -425,7 +433,7 @@ void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
     NRS_CHECK(F % nChunks == 0, "operator: chunk count must divide 16");
     const int Np = (N + 1) * (N + 1) * (N + 1);
     const int64_t epc = ogs->nodesPerChunk / Np;
-    NRS_CHECK(!fx || epc % 4 == 0, "operator: fused chunks must hold whole blocks of four elements");
+    NRS_CHECK(!fx || epc % el->fusionBlock == 0, "operator: fused chunks must hold whole blocks of elements");
     const int per = F / nChunks;
     for (int c = 0; c < nChunks; ++c) {
       const int64_t a = std::min<int64_t>(E, (int64_t)c * per * epc);

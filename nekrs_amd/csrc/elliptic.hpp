@@ -37,8 +37,9 @@ struct nrs_elliptic {
   nrs_ogs *ogsFull = nullptr;        // unmasked (mesh->ogs)
   nrs_ogs *ogs = nullptr;            // masked   (elliptic->ogs)
   // Operator face fusion (N = 7): fx[e] = 1 when elements e, e+1 (same block
-  // of four) share face i = N / i = 0 node for node; their face-interior nodes
-  // are summed inside Ax and left out of ogsOp, the operator's handle.
+  // of fusionBlock elements) share face i = N / i = 0 node for node; their
+  // face-interior nodes are summed inside Ax and left out of ogsOp, the
+  // operator's handle.
   nrs_ogs *ogsOp = nullptr;
   nrs::DevBuf<uint8_t> fx;
   int fusionBlock = 4;               // elements per block of the fused Ax kernel (2 or 4)
