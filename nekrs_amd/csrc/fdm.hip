@@ -655,14 +655,24 @@ __device__ __forceinline__ float wave_max(float v)
 
 template <int NQ> struct MfmaCfg {
   static constexpr int NL = NQ + 2;
-  // plane stride (halves): 16 x 16 + 8, more if the staged results of an
-  // element (ASM: NP + 6 NQ^2 floats) would not fit one box buffer
-  static constexpr int PSMIN = (2 * (NQ * NQ * NQ + 6 * NQ * NQ) + NL - 1) / NL;
-  static constexpr int PS = (PSMIN <= 264) ? 264 : (PSMIN + 7) / 8 * 8;
+  // Row stride RW = 20 halves (10 dwords): the rows an operand read takes
+  // (lane c reads row c) start at 10 c mod 32 dwords, distinct even banks.
+  // Plane stride PS: NL rows (the MFMA's rows NL..15 are zero, not stored), at
+  // least the staged results of an element (ASM: NP + 6 NQ^2 floats) over NL
+  // planes, and 2 mod 4 dwords, so the transposed result store (lane c writes
+  // plane c) starts at distinct even banks too.
+  static constexpr int RW = 20;
+  static constexpr int PSMIN0 = (2 * (NQ * NQ * NQ + 6 * NQ * NQ) + NL - 1) / NL;
+  static constexpr int PSMIN = (NL * RW > PSMIN0) ? NL * RW : PSMIN0;
+  static constexpr int PSD0 = (PSMIN + 1) / 2;
+  static constexpr int PSD = PSD0 + ((2 - PSD0 % 4) + 4) % 4;
+  static constexpr int PS = 2 * PSD;
   static constexpr int BUF = NL * PS;
   static constexpr int SST = (3 * NL * NL + 7) / 8 * 8;   // operator halves per element, 16-byte multiple
+  static constexpr int TAIL = SST;
   static constexpr int WAVES = 4;
   static_assert(NL <= 16, "one 16-wide MFMA tile per axis");
+  static_assert(PSD % 4 == 2, "plane stride 2 mod 4 dwords");
 };
 
 // Each wavefront walks its elements in a grid-stride loop.  The global loads
@@ -678,7 +688,7 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
     float *__restrict__ out, float *__restrict__ gout)
 {
   using C = MfmaCfg<NQ>;
-  constexpr int NL = C::NL, PS = C::PS, BUF = C::BUF, NP = NQ * NQ * NQ, NQ2 = NQ * NQ;
+  constexpr int NL = C::NL, PS = C::PS, RW = C::RW, BUF = C::BUF, NP = NQ * NQ * NQ, NQ2 = NQ * NQ;
   constexpr int SST = C::SST;                  // halves of operators per element (16-byte rows)
   constexpr int SQ4 = SST / 8;                 // uint4 per element's operators
   constexpr bool SCALE = (FMT == NRS_FDM_F16);
@@ -687,13 +697,13 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
   constexpr int RS = (SQ4 + 63) / 64;          // operator uint4 per lane
   constexpr int OUTF = ASM ? NP + 6 * NQ2 : NP;   // staged floats per element
   static_assert(2 * OUTF <= BUF, "staged results fit one box buffer");
-  __shared__ __align__(16) uint16_t lds[C::WAVES][2 * BUF + SST];
+  __shared__ __align__(16) uint16_t lds[C::WAVES][2 * BUF + C::TAIL];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   uint16_t *X = lds[wv], *Y = lds[wv] + BUF, *SB = lds[wv] + 2 * BUF;
   float *stage = reinterpret_cast<float *>(Y);   // results of the previous element
   {
     uint4 *p = reinterpret_cast<uint4 *>(lds[wv]);
-    for (int n = lane; n < (2 * BUF + SST) * 2 / 16; n += 64) p[n] = make_uint4(0u, 0u, 0u, 0u);
+    for (int n = lane; n < (2 * BUF + C::TAIL) * 2 / 16; n += 64) p[n] = make_uint4(0u, 0u, 0u, 0u);
   }
   wave_lds_sync();
   const int stride = gridDim.x * C::WAVES;
@@ -753,11 +763,6 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
     if constexpr (ASM) {
       for (int n = lane; n < 6 * NQ2; n += 64) gout[(int64_t)ep * 6 * NQ2 + n] = stage[NP + n];
     }
-    // the padding rows NL..15 of every plane are read as zero by the contractions
-    for (int n = lane; n < NL * (16 - NL) * 2; n += 64) {
-      const int pl = n / ((16 - NL) * 2), rr = NL + (n / 2) % (16 - NL), hh = n & 1;
-      *reinterpret_cast<uint4 *>(Y + pl * PS + rr * 16 + hh * 8) = make_uint4(0u, 0u, 0u, 0u);
-    }
   };
   load_a(e);
   load_b();
@@ -807,7 +812,7 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
     for (int t = 0; t < RI; ++t) {
       const int n = lane + 64 * t;
       if (n < NQ2) {
-        uint16_t *row = X + (n / NQ) * PS + (n % NQ) * 16;
+        uint16_t *row = X + (n / NQ) * PS + (n % NQ) * RW;
 #pragma unroll
         for (int i = 0; i + 1 < NQ; i += 2)
           *reinterpret_cast<uint32_t *>(row + i) = pack2<FMT>(vi[t][i] * sc, vi[t][i + 1] * sc);
@@ -823,12 +828,12 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
           const int uu = n % NQ, vv = n / NQ;
           int pos;
           switch (f) {   // (plane, row, col) of the ghost in storage order
-            case 0: pos = NQ * PS + vv * 16 + uu; break;
-            case 5: pos = (NQ + 1) * PS + vv * 16 + uu; break;
-            case 1: pos = vv * PS + NQ * 16 + uu; break;
-            case 3: pos = vv * PS + (NQ + 1) * 16 + uu; break;
-            case 4: pos = vv * PS + uu * 16 + NQ; break;
-            default: pos = vv * PS + uu * 16 + NQ + 1; break;
+            case 0: pos = NQ * PS + vv * RW + uu; break;
+            case 5: pos = (NQ + 1) * PS + vv * RW + uu; break;
+            case 1: pos = vv * PS + NQ * RW + uu; break;
+            case 3: pos = vv * PS + (NQ + 1) * RW + uu; break;
+            case 4: pos = vv * PS + uu * RW + NQ; break;
+            default: pos = vv * PS + uu * RW + NQ + 1; break;
           }
           X[pos] = (uint16_t)(pack2<FMT>(vg[f][t] * sc, 0.f) & 0xFFFFu);
         }
@@ -836,8 +841,8 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
     for (int n = lane; n < 12 * NL; n += 64) {   // box edges (two ghost indices) are zero
       const int line = n / NL, sidx = n % NL;
       const int c0 = NQ + (line & 1), c1 = NQ + ((line >> 1) & 1), dir = line >> 2;
-      const int pos = (dir == 0) ? c1 * PS + c0 * 16 + sidx
-                                 : (dir == 1) ? c1 * PS + sidx * 16 + c0 : sidx * PS + c1 * 16 + c0;
+      const int pos = (dir == 0) ? c1 * PS + c0 * RW + sidx
+                                 : (dir == 1) ? c1 * PS + sidx * RW + c0 : sidx * PS + c1 * RW + c0;
       X[pos] = 0;
     }
 #pragma unroll
@@ -871,7 +876,13 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
     auto contract = [&](const uint16_t *in, uint2 bfrag) {
 #pragma unroll
       for (int a = 0; a < NL; ++a)
-        D[a] = mfma16<FMT>(*reinterpret_cast<const uint2 *>(in + a * PS + c * 16 + 4 * g), bfrag);
+      {
+        // rows NL..15 of the operand are zero (never read): their outputs land in
+        // the padding columns of the next operand, which meet the zero rows of B
+        uint2 av = make_uint2(0u, 0u);
+        if (c < NL) av = *reinterpret_cast<const uint2 *>(in + a * PS + c * RW + 4 * g);
+        D[a] = mfma16<FMT>(av, bfrag);
+      }
     };
     auto rescale = [&]() {   // FP16: bring the phase result back to [2^14, 2^15)
       if constexpr (SCALE) {
@@ -895,7 +906,7 @@ __global__ __launch_bounds__(256) void fdm_mfma_kernel(
       if (c < NL) {
 #pragma unroll
         for (int a = 0; a < NL; ++a)
-          *reinterpret_cast<uint2 *>(outb + c * PS + a * 16 + 4 * g) = pack4<FMT>(D[a]);
+          *reinterpret_cast<uint2 *>(outb + c * PS + a * RW + 4 * g) = pack4<FMT>(D[a]);
       }
       wave_lds_sync();
     };
