@@ -653,8 +653,25 @@ int nrs_re2_info(const char *path, int *nelgt, int *nelgv, int *ncurve, int *nbc
  * group: host [nelgt] or NULL; cbc: host [nelgt*6*3] three-character codes of
  * the first field ("W  ", "v  ", "O  ", "E  ", ...; blanks where the file has
  * none) in the solver's face order (meshNekReader.cpp:31). Curved sides are
- * not applied (nrs_re2_info reports their count). */
+ * read by nrs_re2_curves. */
 int nrs_re2_read(const char *path, double *vx, double *vy, double *vz, int *group, char *cbc);
+/* The curved-side records of a .re2 file, in file order (readp_re2_curve /
+ * buf_to_curve, 3rd_party/nek5000/core/reader_re2.f:159-284,473-498): elem
+ * 0-based element, side the file's edge (1..12) or face number, params the
+ * record's curve(1:5) (for a midside node 'm': its x, y, z), type its
+ * ccurve character.  Host arrays of ncurve (nrs_re2_info), params [ncurve*5]. */
+int nrs_re2_curves(const char *path, int *elem, int *side, double *params, char *type);
+/* Applies midside-node curved edges ('m') to GLL coordinates made by
+ * nrs_mesh_from_vertices: each element with an 'm' edge becomes the
+ * triquadratic Gordon-Hall map through its corners and edge midpoints
+ * (Nek5000's xyzquad, 3rd_party/nek5000/core/genxyz.f:1334-1410, with
+ * gh_face_extend, navier5.f:2501-2590; linquad keeps order 1 trilinear),
+ * overwriting its x, y, z [E*(N+1)^3].  Records with a blank type are
+ * skipped; circle arcs ('C'), spheres ('s') and 'e' surfaces give
+ * NRS_ERR_UNSUPPORTED.  Global ids are unchanged (the topology is). */
+int nrs_mesh_midside(int N, int Nelements, const double *vx, const double *vy, const double *vz,
+                     int ncurve, const int *elem, const int *side, const double *params,
+                     const char *type, double *x, double *y, double *z);
 /* GLL node coordinates (trilinear in the corners) and 1-based global ids of
  * an order-N conforming hexahedral mesh from its corners [E*8] (lexicographic):
  * corners closer than tol are one vertex (tol <= 0: 1e-3 of the shortest

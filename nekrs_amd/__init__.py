@@ -20,6 +20,7 @@ from . import _lib
 from ._lib import NrsError
 
 __all__ = ["NrsError", "init", "parse_mg_schedule", "box_mesh", "gll", "ax", "block_ax", "build_diagonal", "mask",
+           "re2_curves", "mesh_midside",
            "coarsen", "prolongate", "geometric_factors", "Ogs", "Elliptic", "BoxMesh", "fdm_1d",
            "fdm_element_lengths", "Batch", "make_desc", "make_opts", "tuned_variant", "set_variant",
            "PressureRhs", "BC", "read_re2", "mesh_from_vertices", "Re2Mesh", "linalg_reduce",
@@ -162,6 +163,39 @@ def mesh_from_vertices(N, vx, vy, vz, tol=0.0, peer=None):
     return x, y, z, gid
 
 
+def re2_curves(path):
+    """Curved-side records of a .re2 file (nrs_re2_curves): elem (0-based), side
+    (1..12), params [n][5], type (one character per record)."""
+    path_b = str(path).encode()
+    ncurve = C.c_int()
+    _lib.call("nrs_re2_info", path_b, None, None, C.byref(ncurve), None)
+    n = ncurve.value
+    elem, side = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    params = np.zeros(n * 5)
+    typ = np.zeros(max(n, 1), dtype=np.uint8)
+    if n:
+        _lib.call("nrs_re2_curves", path_b, elem.ctypes.data, side.ctypes.data, params.ctypes.data,
+                  typ.ctypes.data)
+    return {"elem": elem, "side": side, "params": params.reshape(n, 5),
+            "type": bytes(typ[:n]).decode("ascii", "replace")}
+
+
+def mesh_midside(N, vx, vy, vz, curves, x, y, z):
+    """Applies midside-node ('m') curved edges to x, y, z in place (nrs_mesh_midside)."""
+    vx, vy, vz = (np.ascontiguousarray(a, dtype=np.float64) for a in (vx, vy, vz))
+    E = vx.size // 8
+    n = len(curves["elem"])
+    elem = np.ascontiguousarray(curves["elem"], dtype=np.int32)
+    side = np.ascontiguousarray(curves["side"], dtype=np.int32)
+    params = np.ascontiguousarray(curves["params"], dtype=np.float64).ravel()
+    typ = np.frombuffer(curves["type"].encode("ascii").ljust(max(n, 1)), dtype=np.uint8).copy()
+    for a in (x, y, z):
+        assert a.dtype == np.float64 and a.flags.c_contiguous and a.size == E * (N + 1) ** 3
+    _lib.call("nrs_mesh_midside", N, E, vx.ctypes.data, vy.ctypes.data, vz.ctypes.data, n,
+              elem.ctypes.data, side.ctypes.data, params.ctypes.data, typ.ctypes.data,
+              x.ctypes.data, y.ctypes.data, z.ctypes.data)
+
+
 # .re2 boundary codes -> velocity boundary types (src/bdry/bcType.h)
 RE2_BC = {"W": 1, "v": 3, "V": 3, "O": 16, "o": 16, "ON": 15, "SYM": 7, "int": 2, "E": 0, "P": 0,
           "": 0}
@@ -169,8 +203,8 @@ RE2_BC = {"W": 1, "v": 3, "V": 3, "O": 16, "o": 16, "ON": 15, "SYM": 7, "int": 2
 
 class Re2Mesh:
     """A mesh read from a .re2 file at order N: coordinates (trilinear in the
-    corners, curved sides not applied), global ids from the corner topology
-    with periodic ('P') face pairs connected,
+    corners, elements with midside-node curved edges triquadratic), global ids
+    from the corner topology with periodic ('P') face pairs connected,
     EToB = 1 on faces whose code is in `dirichlet` (for the pressure solve the
     outflow faces), and bc_types for the pressure right-hand side."""
 
@@ -186,6 +220,10 @@ class Re2Mesh:
         periodic = self.peer if np.any(self.peer >= 0) else None
         self.x, self.y, self.z, self.gid = mesh_from_vertices(N, d["vx"] * scale, d["vy"] * scale,
                                                               d["vz"] * scale, peer=periodic)
+        self.curves = re2_curves(path) if self.ncurve else None
+        if self.curves is not None:
+            cv = dict(self.curves, params=self.curves["params"] * scale)
+            mesh_midside(N, d["vx"] * scale, d["vy"] * scale, d["vz"] * scale, cv, self.x, self.y, self.z)
         codes = np.char.strip(self.cbc.astype(str))
         self.EToB = np.isin(codes, list(dirichlet)).astype(np.int32).ravel()
         unknown = sorted(set(codes.ravel()) - set(RE2_BC))

@@ -94,6 +94,8 @@ SIGNATURES = {
     "nrs_elliptic_mg_level": ([_vp, _i, _i, _vp, _vp, _vp, _i, _vp], _i),
     "nrs_parse_mg_schedule": ([C.c_char_p, _i, _i, C.POINTER(EllipticOpts)], _i),
     "nrs_elliptic_set_user_coarse_solver": ([_vp, USER_COARSE_FN, _vp], _i),
+    "nrs_re2_curves": ([C.c_char_p, _vp, _vp, _vp, _vp], _i),
+    "nrs_mesh_midside": ([_i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "nrs_re2_periodic": ([C.c_char_p, _vp], _i),
     "nrs_mesh_from_vertices_periodic": ([_i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp], _i),
     "nrs_elliptic_krylov": ([_vp, _vp, _vp, _d, _i, C.POINTER(_d), C.POINTER(_i), _vp], _i),

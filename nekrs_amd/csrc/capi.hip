@@ -825,6 +825,26 @@ int nrs_re2_read(const char *path, double *vx, double *vy, double *vz, int *grou
   NRS_API_END
 }
 
+int nrs_re2_curves(const char *path, int *elem, int *side, double *params, char *type)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(path && elem && side && params && type, "re2 curves: null argument");
+  nrs::re2_curves(path, elem, side, params, type);
+  NRS_API_END
+}
+
+int nrs_mesh_midside(int N, int Nelements, const double *vx, const double *vy, const double *vz,
+                     int ncurve, const int *elem, const int *side, const double *params,
+                     const char *type, double *x, double *y, double *z)
+{
+  NRS_API_BEGIN
+  NRS_CHECK(Nelements > 0 && ncurve >= 0, "midside: bad sizes");
+  NRS_CHECK(vx && vy && vz && x && y && z, "midside: null array");
+  NRS_CHECK(ncurve == 0 || (elem && side && params && type), "midside: null curve array");
+  nrs::midside_map(N, Nelements, vx, vy, vz, ncurve, elem, side, params, type, x, y, z);
+  NRS_API_END
+}
+
 int nrs_re2_periodic(const char *path, int *peer)
 {
   NRS_API_BEGIN

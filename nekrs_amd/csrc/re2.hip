@@ -319,4 +319,123 @@ void mesh_from_vertices(int N, int E, const double *vx, const double *vy, const 
         }
 }
 
+// Curved-side records (readp_re2_curve / buf_to_curve, reader_re2.f:159-284,
+// 473-498): [element, side, curve[5], type] in 8-byte words, the type a
+// character in the last word.
+void re2_curves(const char *path, int *elem, int *side, double *params, char *type)
+{
+  const Re2Header h = re2_header(path);
+  File fh(path);
+  NRS_CHECK(fh.f != nullptr, std::string("re2: cannot open ") + path);
+  NRS_CHECK(std::fseek(fh.f, 84 + (long)h.nelgt * 25 * 8 + 8, SEEK_SET) == 0, "re2: seek");
+  double b[8];
+  for (int r = 0; r < h.ncurve; ++r) {
+    read_exact(fh.f, b, 64, "curve record");
+    char t[8];
+    std::memcpy(t, &b[7], 8);
+    if (h.swap) for (int k = 0; k < 7; ++k) b[k] = swap8(b[k]);
+    const int e = (int)b[0] - 1, sd = (int)b[1];
+    NRS_CHECK(e >= 0 && e < h.nelgt && sd >= 1 && sd <= 12, "re2: curve record out of range");
+    elem[r] = e;
+    side[r] = sd;
+    for (int k = 0; k < 5; ++k) params[(int64_t)r * 5 + k] = b[2 + k];
+    type[r] = t[0];
+  }
+}
+
+// Midside-node ('m') curved edges: the element becomes the triquadratic
+// Gordon-Hall map through its corners and edge midpoints, as Nek5000's
+// xyzquad (3rd_party/nek5000/core/genxyz.f:1334-1410, called from linquad
+// :1235-1267): the trilinear map sampled on the 3 x 3 x 3 points r, s, t in
+// {-1, 0, 1}; each 'm' edge replaces its midpoint; vertex + edge transfinite
+// interpolation (gh_face_extend_3d, gh_type = 2, navier5.f:2501-2590) fills
+// the face and interior points; quadratic Lagrange interpolation onto the GLL
+// nodes.  Other curve types (circle arcs 'C', spheres 's', 'e') are refused.
+void midside_map(int N, int E, const double *vx, const double *vy, const double *vz, int ncurve,
+                 const int *elem, const int *side, const double *params, const char *type, double *x,
+                 double *y, double *z)
+{
+  NRS_CHECK(N >= 1 && N <= 9, "midside: N must be 1..9");
+  // 3 x 3 x 3 index (i + 3 j + 9 k) of the midpoint of Nek's edges 1..12 (xyzquad's eindx)
+  static const int kEdgeMid[12] = {1, 5, 7, 3, 19, 23, 25, 21, 9, 11, 17, 15};
+  std::vector<std::vector<int>> curved(E);
+  for (int r = 0; r < ncurve; ++r) {
+    NRS_CHECK(elem[r] >= 0 && elem[r] < E && side[r] >= 1 && side[r] <= 12,
+              "midside: curve record out of range");
+    const char t = type[r];
+    if (t == ' ' || t == '\0') continue;
+    if (t != 'm')
+      throw Error(NRS_ERR_UNSUPPORTED, std::string("midside: curved side type '") + t +
+                                           "' (only midside nodes 'm' are applied)");
+    curved[elem[r]].push_back(r);
+  }
+  if (N == 1) return;            // linquad: order 1 keeps the trilinear map
+  std::vector<double> gz, gw;
+  gll_nodes(N, gz, gw);
+  const int Nq = N + 1, Np = Nq * Nq * Nq;
+  // quadratic Lagrange weights through -1, 0, 1 at the GLL nodes
+  std::vector<std::array<double, 3>> L(Nq);
+  for (int i = 0; i < Nq; ++i) {
+    const double t = gz[i];
+    L[i] = {0.5 * t * (t - 1.0), (1.0 - t) * (1.0 + t), 0.5 * t * (t + 1.0)};
+  }
+  const double zg[3] = {-1.0, 0.0, 1.0};
+  auto hat = [&](int end, int i) {   // linear blend of end 0 / 2 at point i
+    return end == 0 ? 0.5 * (1.0 - zg[i]) : 0.5 * (1.0 + zg[i]);
+  };
+  for (int e = 0; e < E; ++e) {
+    if (curved[e].empty()) continue;
+    for (int d = 0; d < 3; ++d) {
+      const double *vc = (d == 0 ? vx : d == 1 ? vy : vz) + (int64_t)e * 8;
+      double q[27];
+      for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+          for (int i = 0; i < 3; ++i) {
+            double s = 0.0;
+            for (int c = 0; c < 8; ++c)
+              s += hat(c & 1 ? 2 : 0, i) * hat(c & 2 ? 2 : 0, j) * hat(c & 4 ? 2 : 0, k) * vc[c];
+            q[i + 3 * j + 9 * k] = s;
+          }
+      for (int r : curved[e]) q[kEdgeMid[side[r] - 1]] = params[(int64_t)r * 5 + d];
+      // vertex interpolant v, then v + the 12 edge corrections
+      double v[27] = {0}, g[27];
+      for (int kk = 0; kk < 3; kk += 2)
+        for (int jj = 0; jj < 3; jj += 2)
+          for (int ii = 0; ii < 3; ii += 2)
+            for (int n = 0; n < 27; ++n)
+              v[n] += hat(ii, n % 3) * hat(jj, (n / 3) % 3) * hat(kk, n / 9) * q[ii + 3 * jj + 9 * kk];
+      for (int n = 0; n < 27; ++n) g[n] = 0.0;
+      for (int n = 0; n < 27; ++n) {
+        const int i = n % 3, j = (n / 3) % 3, k = n / 9;
+        for (int kk = 0; kk < 3; kk += 2)
+          for (int jj = 0; jj < 3; jj += 2) {   // x-edges
+            const int m = i + 3 * jj + 9 * kk;
+            g[n] += hat(jj, j) * hat(kk, k) * (q[m] - v[m]);
+          }
+        for (int kk = 0; kk < 3; kk += 2)
+          for (int ii = 0; ii < 3; ii += 2) {   // y-edges
+            const int m = ii + 3 * j + 9 * kk;
+            g[n] += hat(ii, i) * hat(kk, k) * (q[m] - v[m]);
+          }
+        for (int jj = 0; jj < 3; jj += 2)
+          for (int ii = 0; ii < 3; ii += 2) {   // z-edges
+            const int m = ii + 3 * jj + 9 * k;
+            g[n] += hat(ii, i) * hat(jj, j) * (q[m] - v[m]);
+          }
+      }
+      for (int n = 0; n < 27; ++n) g[n] += v[n];
+      double *out = (d == 0 ? x : d == 1 ? y : z) + (int64_t)e * Np;
+      for (int k = 0; k < Nq; ++k)
+        for (int j = 0; j < Nq; ++j)
+          for (int i = 0; i < Nq; ++i) {
+            double s = 0.0;
+            for (int c = 0; c < 3; ++c)
+              for (int b2 = 0; b2 < 3; ++b2)
+                for (int a = 0; a < 3; ++a) s += L[i][a] * L[j][b2] * L[k][c] * g[a + 3 * b2 + 9 * c];
+            out[i + Nq * (j + Nq * k)] = s;
+          }
+    }
+  }
+}
+
 }  // namespace nrs

@@ -26,4 +26,14 @@ std::vector<long long> vertex_ids(int E, const double *vx, const double *vy, con
 void mesh_from_vertices(int N, int E, const double *vx, const double *vy, const double *vz, double tol,
                         double *x, double *y, double *z, long long *gids, const int *peer = nullptr);
 
+// Curved-side records in file order: elem 0-based, side 1..12 (Nek's edge or
+// face number), params [ncurve][5], type one character per record
+void re2_curves(const char *path, int *elem, int *side, double *params, char *type);
+// Overwrites x, y, z [E][(N+1)^3] of the elements with midside-node ('m')
+// curved edges by their triquadratic Gordon-Hall map (Nek5000's xyzquad);
+// other curve types throw NRS_ERR_UNSUPPORTED
+void midside_map(int N, int E, const double *vx, const double *vy, const double *vz, int ncurve,
+                 const int *elem, const int *side, const double *params, const char *type, double *x,
+                 double *y, double *z);
+
 }  // namespace nrs

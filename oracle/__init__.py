@@ -195,6 +195,30 @@ class BoxMesh:
         return self.E * self.Np
 
 
+class ArrayMesh:
+    """A mesh given by its GLL coordinates, global ids and boundary flags (for
+    meshes read from files, e.g. with curved elements): the same fields and
+    geometric factors as BoxMesh."""
+
+    def __init__(self, N, x, y, z, gid, EToB):
+        self.N, self.Nq = N, N + 1
+        self.Np = self.Nq ** 3
+        self.x, self.y, self.z = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in (x, y, z))
+        self.E = self.x.size // self.Np
+        self.gid = np.ascontiguousarray(gid, dtype=np.int64).copy()
+        self.EToB = np.ascontiguousarray(EToB, dtype=np.int32).copy()
+        self.r, self.w = gll(N)
+        self.D = dmatrix(N, self.r)
+        self.ggeo = np.zeros(self.E * 7 * self.Np)
+        self.J = np.zeros(self.E * self.Np)
+        lib().orc_geofacs(N, self.E, self.D.ravel().copy(), self.w, self.x, self.y, self.z,
+                          self.ggeo, self.J.ctypes.data)
+
+    @property
+    def Nlocal(self):
+        return self.E * self.Np
+
+
 # ---------------------------------------------------------------- ogs
 class Ogs:
     """Local gather-scatter handle (restates ogsSetup's local rows)."""
