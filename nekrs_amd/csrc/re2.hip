@@ -13,6 +13,7 @@
 // (nek's set_glo_num from coordinates), then every GLL node is numbered by the
 // vertex, edge, face or interior it belongs to (the same topology rule as the
 // multigrid levels' ids).
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -87,17 +88,26 @@ Re2Header re2_header(const char *path)
     NRS_CHECK(std::fabs(test - 6.54321f) < 1e-5f, "re2: bad byte-order word");
     h.swap = true;
   }
+  // the counts must fit the file: no record or array is sized beyond what it holds
+  NRS_CHECK(std::fseek(fh.f, 0, SEEK_END) == 0, "re2: seek");
+  const long long bytes = std::ftell(fh.f);
+  NRS_CHECK(bytes >= 84 + (long long)h.nelgt * 200 + 16, "re2: file shorter than its element count");
   // skip the elements, read the curve and boundary counts
   NRS_CHECK(std::fseek(fh.f, 84 + (long)h.nelgt * 25 * 8, SEEK_SET) == 0, "re2: seek");
   double c = 0.0;
   read_exact(fh.f, &c, 8, "curve count");
-  h.ncurve = (int)(h.swap ? swap8(c) : c);
-  NRS_CHECK(h.ncurve >= 0, "re2: bad curve count");
+  c = h.swap ? swap8(c) : c;
+  NRS_CHECK(c >= 0.0 && c <= (double)(bytes - 84 - (long long)h.nelgt * 200 - 16) / 64.0,
+            "re2: bad curve count");
+  h.ncurve = (int)c;
   NRS_CHECK(std::fseek(fh.f, (long)h.ncurve * 8 * 8, SEEK_CUR) == 0, "re2: seek");
   double nb = 0.0;
   read_exact(fh.f, &nb, 8, "boundary count");
-  h.nbc = (int)(h.swap ? swap8(nb) : nb);
-  NRS_CHECK(h.nbc >= 0, "re2: bad boundary count");
+  nb = h.swap ? swap8(nb) : nb;
+  NRS_CHECK(nb >= 0.0 && nb <= (double)(bytes - 84 - (long long)h.nelgt * 200 - 16 -
+                                         (long long)h.ncurve * 64) / 64.0,
+            "re2: bad boundary count");
+  h.nbc = (int)nb;
   return h;
 }
 
@@ -166,10 +176,18 @@ std::vector<long long> vertex_ids(int E, const double *vx, const double *vy, con
   }
   // spatial hash with cells of size 4 tol; search the 27 neighbouring cells
   const double cell = 4.0 * tol;
+  double vmax = 0.0;
+  for (int64_t n = 0; n < (int64_t)E * 8; ++n) {
+    NRS_CHECK(std::isfinite(vx[n]) && std::isfinite(vy[n]) && std::isfinite(vz[n]),
+              "mesh: corner coordinates must be finite");
+    vmax = std::max(vmax, std::max(std::fabs(vx[n]), std::max(std::fabs(vy[n]), std::fabs(vz[n]))));
+  }
+  NRS_CHECK(vmax / cell < 1e15, "mesh: tolerance too small for the coordinate range");
   struct KeyHash {
     size_t operator()(const std::array<long long, 3> &k) const
     {
-      return (size_t)(k[0] * 73856093LL ^ k[1] * 19349663LL ^ k[2] * 83492791LL);
+      return (size_t)((uint64_t)k[0] * 73856093ULL ^ (uint64_t)k[1] * 19349663ULL ^
+                      (uint64_t)k[2] * 83492791ULL);
     }
   };
   std::unordered_map<std::array<long long, 3>, std::vector<int>, KeyHash> grid;
