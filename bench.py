@@ -176,8 +176,9 @@ def cpu_baseline(nx, N, seconds, seed):
             "port": port}
 
 
-def solve_cases(n, N, seed=0):
-    """The FP64 and mixed-precision p-MG solves on the n^3 Kershaw mesh."""
+def solve_cases(n, N, seed=0, only=None):
+    """The FP64 and mixed-precision p-MG solves on the n^3 Kershaw mesh (`only`:
+    the names of the cases to run, all by default)."""
     import numpy as np
     import torch
     import nekrs_amd
@@ -193,6 +194,8 @@ def solve_cases(n, N, seed=0):
              # mixed preconditioner
              ("mixed_fgmres", dict(smoother="opt4chebyshev+ras", cheby_degree=4, mgPrecision=0,
                                    fdmPrecision=1, solver="fgmres", **amg))]
+    if only:
+        cases = [c for c in cases if c[0] in only]
     out, xs = {}, {}
     for name, kw in cases:
         t0 = time.perf_counter()
@@ -234,11 +237,12 @@ def solve_cases(n, N, seed=0):
         log("%s: %d it, %.3f s (setup %.1f s), true rel res %.2e" %
             (name, st["iterations"], st["seconds"], t_setup, res_true / st["res0"]))
         del el
-    a, b_ = xs["fp64_chebyshev_jacobi"], xs["mixed_opt4chebyshev_ras"]
-    out["mixed_opt4chebyshev_ras"]["err_vs_fp64_rel"] = float(np.max(np.abs(b_ - a)) /
-                                                              np.max(np.abs(a)))
-    out["speedup_mixed_vs_fp64"] = (out["fp64_chebyshev_jacobi"]["seconds"] /
-                                    out["mixed_opt4chebyshev_ras"]["seconds"])
+    if "fp64_chebyshev_jacobi" in xs and "mixed_opt4chebyshev_ras" in xs:
+        a, b_ = xs["fp64_chebyshev_jacobi"], xs["mixed_opt4chebyshev_ras"]
+        out["mixed_opt4chebyshev_ras"]["err_vs_fp64_rel"] = float(np.max(np.abs(b_ - a)) /
+                                                                  np.max(np.abs(a)))
+        out["speedup_mixed_vs_fp64"] = (out["fp64_chebyshev_jacobi"]["seconds"] /
+                                        out["mixed_opt4chebyshev_ras"]["seconds"])
     out["workload"] = ("%d^3=%d hex elements N=%d Kershaw eps=0.3, p-MG %d-3-1 with an FP64 coarse "
                        "solve (2 AMG-preconditioned PCG iterations), FCG to 1e-8 relative, rhs "
                        "sine + 1e-3 noise (seed %d)" % (n, n ** 3, N, N, seed))
@@ -398,8 +402,11 @@ def pressure_rhs_case(mesh, N, steps, warmup, seed=0):
                         "fields U(-1,1) seed %d" % (mesh.E, N, seed)}
 
 
+# the mixed-precision p-MG of the 48^3 mixed solve, with its FP64 coarse solve (two AMG-preconditioned
+# PCG iterations on the assembled order-1 matrix, run on the device so each problem replays as a graph)
 BATCH_KW = dict(solver="fcg", precon="multigrid", smoother="opt4chebyshev+asm", cheby_degree=3,
-                mgPrecision=0, fdmPrecision=1, coarse="smooth_jacobi", tol=1e-8, max_iter=500)
+                mgPrecision=0, fdmPrecision=1, coarse="amg", coarse_tol=0.0, coarse_max_iter=2,
+                tol=1e-8, max_iter=500)
 
 
 def box_dims(E):
@@ -521,7 +528,7 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
         N = int(rng.choice([3, 5, 7, 9]))
         Et = int(rng.integers(512, 8193))
         lam = float(rng.uniform(0.0, 10.0))
-        eps = float(rng.uniform(0.05, 0.3))
+        eps = float(rng.uniform(0.0, 0.3))
         nx, ny, nz = box_dims(Et)
         probs.append((nekrs_amd.BoxMesh(nx, ny, nz, N, "kershaw", eps),
                       dict(BATCH_KW, lambda0=1.0, lambda1=lam, poisson=False)))
@@ -551,13 +558,57 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
             (vb["iterations"], vb["seconds"], vb["jacobi_pcg"]["iterations"],
              vb["jacobi_pcg"]["seconds"]))
     out["workload"] = ("%d Helmholtz problems (seed %d): N in {3,5,7,9}, E ~ U[512,8192] as "
-                       "near-cubic boxes, lambda1 ~ U[0,10], Kershaw eps ~ U[0.05,0.3]; plus 3 "
+                       "near-cubic boxes, lambda1 ~ U[0,10], Kershaw eps ~ U[0,0.3]; plus 3 "
                        "velocity components (lambda1=10) at %d^3 N=7 eps 0.3, as a batch and as "
                        "one 3-field block solve; FP32 p-MG, opt. 4th-kind Chebyshev + ASM with "
-                       "BF16 FDM, Chebyshev-Jacobi smoothed coarse level (the block solve: AMG "
-                       "coarse solve, p-MG applied per field), FCG to 1e-8 relative" %
+                       "BF16 FDM, FP64 coarse solve (2 AMG-preconditioned PCG iterations on the "
+                       "order-1 matrix; the block solve: p-MG applied per field), FCG to 1e-8 "
+                       "relative" %
                        (count, seed, block_n))
     return out
+
+
+def profile_summary(db, top=40):
+    """Markdown summary of a rocprofv3 kernel trace (its rocpd SQLite output):
+    per roctx range pushed by this script (nrs:solve:..., nrs:kershaw:...) --
+    or over the whole run when there is none -- the kernels grouped by name and
+    grid size (so the launches of one kernel on different multigrid levels
+    stay apart): calls, total and mean duration."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    kern = list(con.execute("select name, grid_x, workgroup_x, start, end from kernels"))
+    regs = []
+    try:   # roctx ranges: the message is in the region's extdata ({"message": ...})
+        for name, a, b, extra in con.execute("select name, start, end, extdata from regions"):
+            try:
+                msg = json.loads(extra or "{}").get("message", name)
+            except ValueError:
+                msg = name
+            if str(msg).startswith("nrs:"):
+                regs.append((msg, a, b))
+    except sqlite3.Error:
+        regs = []
+    spans = [(n, a, b) for n, a, b in regs] or [("whole run", min(k[3] for k in kern),
+                                                 max(k[4] for k in kern))]
+    lines = []
+    for name, a, b in spans:
+        groups = {}
+        for kn, gx, wx, s0, s1 in kern:
+            if s0 >= a and s1 <= b:
+                g = groups.setdefault((kn, gx, wx), [0, 0.0])
+                g[0] += 1
+                g[1] += (s1 - s0) * 1e-3
+        tot = sum(v[1] for v in groups.values())
+        lines.append("### %s: %.3f ms of kernels, %.3f ms wall" % (name, tot * 1e-3, (b - a) * 1e-6))
+        lines.append("")
+        lines.append("| kernel | grid | wg | calls | total ms | mean us | % |")
+        lines.append("|---|---|---|---|---|---|---|")
+        for (kn, gx, wx), (n, t) in sorted(groups.items(), key=lambda kv: -kv[1][1])[:top]:
+            short = kn if len(kn) <= 90 else kn[:87] + "..."
+            lines.append("| `%s` | %d | %d | %d | %.3f | %.1f | %.1f |" %
+                         (short, gx, wx, n, t * 1e-3, t / n, 100.0 * t / max(tot, 1e-30)))
+        lines.append("")
+    print("\n".join(lines))
 
 
 def main():
@@ -571,6 +622,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solve-n", type=int, default=48, help="elements per direction of the solves")
     ap.add_argument("--no-solves", action="store_true")
+    ap.add_argument("--solve-cases", default="",
+                    help="comma-separated names of the p-MG solves to run (default: all)")
     ap.add_argument("--no-pressure-rhs", action="store_true")
     ap.add_argument("--no-kershaw", action="store_true",
                     help="skip the reference's Kershaw BPS5/BP5 benchmarks (20^3, N=7)")
@@ -580,7 +633,13 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 512 MiB (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters), then exit")
+    ap.add_argument("--profile-summary", default="",
+                    help="print a per-range, per-kernel summary of a rocprofv3 rocpd database "
+                         "(.db) and exit")
     args = ap.parse_args()
+    if args.profile_summary:
+        profile_summary(args.profile_summary)
+        return
     if args.copy_calibration:
         import torch
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
@@ -620,11 +679,13 @@ def main():
 
     el.profile(True)
     torch.cuda.synchronize()
+    ranges().push("nrs:operator")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         el.operator(q, Aq)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    ranges().pop()
     prof = el.profile_read()
     el.profile(False)
     ms_step = dt / args.steps * 1e3
@@ -662,11 +723,13 @@ def main():
         el0.operator(q, Aq)
     el0.profile(True)
     torch.cuda.synchronize()
+    ranges().push("nrs:operator_unfused")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         el0.operator(q, Aq)
     torch.cuda.synchronize()
     ms0 = (time.perf_counter() - t0) / args.steps * 1e3
+    ranges().pop()
     p0 = el0.profile_read()
     el0.profile(False)
     del el0
@@ -685,7 +748,7 @@ def main():
     solves = None
     if not args.no_solves:
         log("p-MG solves at %d^3 ..." % args.solve_n)
-        solves = solve_cases(args.solve_n, N)
+        solves = solve_cases(args.solve_n, N, only=[c for c in args.solve_cases.split(",") if c])
 
     batch = None
     if not args.no_batch:
