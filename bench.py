@@ -568,6 +568,32 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
     return out
 
 
+def counter_summary(d, last=0):
+    """Mean of each counter per (kernel, grid size) over a rocprofv3 --pmc run
+    written with --output-format csv (its *counter_collection.csv files); with
+    last > 0 only the last `last` dispatches of each (kernel, grid) count (the
+    timed loop, after the setup's tuning launches)."""
+    import csv
+    import glob
+    rows = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                k = (row["Kernel_Name"], int(row["Grid_Size"]), row["Counter_Name"])
+                rows.setdefault(k, []).append((int(row["Dispatch_Id"]), float(row["Counter_Value"])))
+    acc = {}
+    for k, v in rows.items():
+        v.sort()
+        if last > 0:
+            v = v[-last:]
+        acc[k] = (sum(x for _, x in v), len(v))
+    print("| kernel | grid | counter | dispatches | mean per dispatch |")
+    print("|---|---|---|---|---|")
+    for (kn, g, cn), (v, n) in sorted(acc.items()):
+        short = kn if len(kn) <= 90 else kn[:87] + "..."
+        print("| `%s` | %d | %s | %d | %.6g |" % (short, g, cn, n, v / n))
+
+
 def profile_summary(db, top=40):
     """Markdown summary of a rocprofv3 kernel trace (its rocpd SQLite output):
     per roctx range pushed by this script (nrs:solve:..., nrs:kershaw:...) --
@@ -621,6 +647,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solve-n", type=int, default=48, help="elements per direction of the solves")
+    ap.add_argument("--no-unfused", action="store_true",
+                    help="skip the comparison loop of the operator without face fusion")
+    ap.add_argument("--operator-fusion", type=int, default=-1,
+                    help="operatorFusion of the timed operator (default: the library's, timed at setup)")
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--solve-cases", default="",
                     help="comma-separated names of the p-MG solves to run (default: all)")
@@ -635,10 +665,17 @@ def main():
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters), then exit")
     ap.add_argument("--profile-summary", default="",
                     help="print a per-range, per-kernel summary of a rocprofv3 rocpd database "
-                         "(.db) and exit")
+                         "(.db), or the per-kernel counter means of a --pmc csv output "
+                         "directory, and exit")
+    ap.add_argument("--last", type=int, default=0,
+                    help="with --profile-summary of counters: only the last N dispatches of "
+                         "each kernel and grid")
     args = ap.parse_args()
     if args.profile_summary:
-        profile_summary(args.profile_summary)
+        if os.path.isdir(args.profile_summary):
+            counter_summary(args.profile_summary, args.last)
+        else:
+            profile_summary(args.profile_summary)
         return
     if args.copy_calibration:
         import torch
@@ -662,7 +699,8 @@ def main():
     n, N = args.n, args.order
     t0 = time.perf_counter()
     mesh = nekrs_amd.BoxMesh(n, n, n, N, "kershaw", 0.3)
-    el = nekrs_amd.Elliptic(mesh, precon="none")
+    el = (nekrs_amd.Elliptic(mesh, precon="none") if args.operator_fusion < 0 else
+          nekrs_amd.Elliptic(mesh, precon="none", operatorFusion=args.operator_fusion))
     log("setup %.1f s: E=%d N=%d Nlocal=%d Nmasked=%d" %
         (time.perf_counter() - t0, mesh.E, N, el.Nlocal, el.Nmasked))
     # gather-scatter algorithmic bytes: rows with more than one member of the
@@ -717,27 +755,28 @@ def main():
     Nlocal = el.Nlocal
     fused_pairs = el.fused_pairs
     del el
-    # the same loop without the operator face fusion, on the same box, for comparison
-    el0 = nekrs_amd.Elliptic(mesh, precon="none", operatorFusion=0)
-    for _ in range(args.warmup):
-        el0.operator(q, Aq)
-    el0.profile(True)
-    torch.cuda.synchronize()
-    ranges().push("nrs:operator_unfused")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        el0.operator(q, Aq)
-    torch.cuda.synchronize()
-    ms0 = (time.perf_counter() - t0) / args.steps * 1e3
-    ranges().pop()
-    p0 = el0.profile_read()
-    el0.profile(False)
-    del el0
-    fusion = {"fused_pairs": fused_pairs, "fused_ms_per_step": ms_step, "unfused_ms_per_step": ms0,
-              "unfused_ax_ms": p0["ax"][0] / max(p0["ax"][1], 1),
-              "unfused_gs_ms": p0["gs"][0] / max(p0["gs"][1], 1)}
-    log("without the face fusion: step %.4f ms (%.2f GDOF/s); Ax %.4f ms, gs %.4f ms" %
-        (ms0, dofs / (ms0 * 1e-3) / 1e9, fusion["unfused_ax_ms"], fusion["unfused_gs_ms"]))
+    fusion = {"fused_pairs": fused_pairs, "fused_ms_per_step": ms_step}
+    if not args.no_unfused:
+        # the same loop without the operator face fusion, on the same box, for comparison
+        el0 = nekrs_amd.Elliptic(mesh, precon="none", operatorFusion=0)
+        for _ in range(args.warmup):
+            el0.operator(q, Aq)
+        el0.profile(True)
+        torch.cuda.synchronize()
+        ranges().push("nrs:operator_unfused")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            el0.operator(q, Aq)
+        torch.cuda.synchronize()
+        ms0 = (time.perf_counter() - t0) / args.steps * 1e3
+        ranges().pop()
+        p0 = el0.profile_read()
+        el0.profile(False)
+        del el0
+        fusion.update({"unfused_ms_per_step": ms0, "unfused_ax_ms": p0["ax"][0] / max(p0["ax"][1], 1),
+                       "unfused_gs_ms": p0["gs"][0] / max(p0["gs"][1], 1)})
+        log("without the face fusion: step %.4f ms (%.2f GDOF/s); Ax %.4f ms, gs %.4f ms" %
+            (ms0, dofs / (ms0 * 1e-3) / 1e9, fusion["unfused_ax_ms"], fusion["unfused_gs_ms"]))
     prhs = None
     if not args.no_pressure_rhs:
         prhs = pressure_rhs_case(mesh, N, 20, 3)
