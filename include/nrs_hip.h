@@ -332,6 +332,14 @@ typedef struct {
                                   * first rebuilds the Jacobi diagonal from the
                                   * fields' current values (ellipticSolve.cpp:79-88,
                                   * ellipticUpdateJacobi.cpp:91-115); default 1 */
+  int mgGeometry;                /* 1 (default): when every element is trilinear
+                                  * (its nodes the trilinear map of its corners),
+                                  * the multigrid levels' operators form their
+                                  * geometric factors from the corners in the
+                                  * kernel instead of streaming stored ones (FP32
+                                  * at every order, FP64 up to order 7); 0: the
+                                  * stored factors, as the reference.  The outer
+                                  * operator always uses the stored factors. */
 } nrs_elliptic_opts;
 
 typedef struct {

@@ -40,7 +40,7 @@ class EllipticOpts(C.Structure):
                 ("useGraphs", C.c_int), ("autotune", C.c_int), ("gmresRestart", C.c_int),
                 ("initialGuess", C.c_int), ("projVectors", C.c_int), ("projStart", C.c_int),
                 ("operatorFusion", C.c_int), ("preDegree", C.c_int * 8), ("postDegree", C.c_int * 8),
-                ("precoCoeffField", C.c_int)]
+                ("precoCoeffField", C.c_int), ("mgGeometry", C.c_int)]
 
 
 class SolveStats(C.Structure):

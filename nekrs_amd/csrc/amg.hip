@@ -223,15 +223,22 @@ __global__ void jacobi0_kernel(int n, const double *__restrict__ b, const double
   if (i < n) x[i] = omega * dinv[i] * b[i];
 }
 
-// x = inv b with the symmetric inverse read by columns (coalesced across rows)
-__global__ void dense_kernel(int n, const double *__restrict__ inv, const double *__restrict__ b,
-                             double *__restrict__ x)
+// x = inv b, one wave per row: the lanes read the row of the symmetric inverse
+// (coalesced), each sums every 64th product, then a fixed butterfly adds the
+// lanes' partial sums (deterministic; rows spread over the whole GPU instead of
+// one thread walking a row)
+__global__ void __launch_bounds__(256) dense_kernel(int n, const double *__restrict__ inv,
+                                                    const double *__restrict__ b, double *__restrict__ x)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= n) return;
+  const double *row = inv + (size_t)i * n;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) s += inv[(size_t)j * n + i] * b[j];
-  x[i] = s;
+  for (int j = lane; j < n; j += 64) s += row[j] * b[j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) x[i] = s;
 }
 
 __global__ void copy_kernel(int n, const double *__restrict__ a, double *__restrict__ b)
@@ -464,7 +471,7 @@ static double *vcycle(std::vector<AmgDevLevel> &lv, int sweeps, size_t k, const 
 {
   AmgDevLevel &l = lv[k];
   if (l.dense) {
-    dense_kernel<<<blocks(l.n, 64), 64, 0, s>>>(l.n, l.inv.get(), b, xa);
+    dense_kernel<<<(unsigned)((l.n + 3) / 4), 256, 0, s>>>(l.n, l.inv.get(), b, xa);
     NRS_LAUNCH_CHECK();
     return xa;
   }

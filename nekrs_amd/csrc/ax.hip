@@ -65,12 +65,20 @@ template <int NQ, int MULT = 1> struct AxShape {
 // face i = N / i = 0 node for node; the interior nodes of that face are summed
 // here through LDS (both copies get v_e + v_e+1, the gather-scatter's pair sum,
 // bit for bit) and the caller's gather-scatter handle leaves them out.
-template <typename T, int NQ, int MULT, bool POISSON, bool LFIELD, bool FUSEX = false>
+//
+// TRI: trilinear elements.  The geometric factors are not read but formed at
+// each node from the element's eight corners (corners [E][3][8], x y z, corner
+// a + 2b + 4c at (r, s, t) = (+-1, +-1, +-1)) and the GLL nodes and weights
+// (zw [2][NQ]), with geometricFactorsHex3D's formulas (J from the derivatives
+// of the map, the inverse by cofactors, JW = J w_i w_j w_k); for a trilinear
+// map the derivatives D x of the reference equal these up to rounding.
+template <typename T, int NQ, int MULT, bool POISSON, bool LFIELD, bool FUSEX = false, bool TRI = false>
 __global__ void __launch_bounds__((AxShape<NQ, MULT>::THREADS), AX_WAVES_PER_EU)
 ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
           const T *__restrict__ Dg, const T *__restrict__ lam0, const T *__restrict__ lam1,
           const T *__restrict__ q, T *__restrict__ Aq, const uint32_t *__restrict__ maskBits,
-          const uint8_t *__restrict__ fx = nullptr)
+          const uint8_t *__restrict__ fx = nullptr, const T *__restrict__ corners = nullptr,
+          const T *__restrict__ zw = nullptr)
 {
   constexpr int NE = AxShape<NQ, MULT>::NE;
   constexpr int NP = NQ * NQ * NQ;
@@ -115,9 +123,30 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
   const T l1c = (LFIELD || POISSON) ? T(0) : lam1[0];
 
   // geometric factors of plane 0; later planes are prefetched one plane ahead
+  // (TRI: the corner differences that the derivatives of the map take at this
+  // lane's (r, s), formed once per element)
   T g[7];
+  T cDrB[3], cDrT[3], cDsB[3], cDsT[3], cDt[3], wij = T(0);
+  if constexpr (TRI) {
+    const T r = zw[i], sj = zw[j];
+    const T hr0 = T(0.5) * (T(1) - r), hr1 = T(0.5) * (T(1) + r);
+    const T hs0 = T(0.5) * (T(1) - sj), hs1 = T(0.5) * (T(1) + sj);
+    wij = zw[NQ + i] * zw[NQ + j];
+    const T *P0 = corners + e * 24;
 #pragma unroll
-  for (int c = 0; c < 7; ++c) g[c] = (active && (c < 6 || !POISSON)) ? stream_load(G + c * NP + lid) : T(0);
+    for (int c = 0; c < 3; ++c) {
+      const T *P = P0 + 8 * c;
+      const T p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3], p4 = P[4], p5 = P[5], p6 = P[6], p7 = P[7];
+      cDrB[c] = T(0.5) * (hs0 * (p1 - p0) + hs1 * (p3 - p2));
+      cDrT[c] = T(0.5) * (hs0 * (p5 - p4) + hs1 * (p7 - p6));
+      cDsB[c] = T(0.5) * (hr0 * (p2 - p0) + hr1 * (p3 - p1));
+      cDsT[c] = T(0.5) * (hr0 * (p6 - p4) + hr1 * (p7 - p5));
+      cDt[c] = T(0.5) * (hs0 * (hr0 * (p4 - p0) + hr1 * (p5 - p1)) + hs1 * (hr0 * (p6 - p2) + hr1 * (p7 - p3)));
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) g[c] = (active && (c < 6 || !POISSON)) ? stream_load(G + c * NP + lid) : T(0);
+  }
 
 #pragma unroll 1
   for (int k = 0; k < NQ; ++k) {
@@ -126,10 +155,33 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
 #pragma unroll
     for (int m = 1; m < NQ; ++m) qk = (k == m) ? r_q[m] : qk;
     T gn[7];
-    const bool more = active && (k + 1 < NQ);
+    if constexpr (TRI) {
+      const T t = zw[k];
+      const T ht0 = T(0.5) * (T(1) - t), ht1 = T(0.5) * (T(1) + t);
+      const T xr = ht0 * cDrB[0] + ht1 * cDrT[0], yr = ht0 * cDrB[1] + ht1 * cDrT[1],
+              zr = ht0 * cDrB[2] + ht1 * cDrT[2];
+      const T xs = ht0 * cDsB[0] + ht1 * cDsT[0], ys = ht0 * cDsB[1] + ht1 * cDsT[1],
+              zs = ht0 * cDsB[2] + ht1 * cDsT[2];
+      const T xt = cDt[0], yt = cDt[1], zt = cDt[2];
+      const T J = xr * (ys * zt - zs * yt) - yr * (xs * zt - zs * xt) + zr * (xs * yt - ys * xt);
+      const T Jinv = T(1) / J;
+      const T JW = J * wij * zw[NQ + k];
+      const T rx = (ys * zt - zs * yt) * Jinv, ry = -(xs * zt - zs * xt) * Jinv, rz = (xs * yt - ys * xt) * Jinv;
+      const T sx = -(yr * zt - zr * yt) * Jinv, sy = (xr * zt - zr * xt) * Jinv, sz = -(xr * yt - yr * xt) * Jinv;
+      const T tx = (yr * zs - zr * ys) * Jinv, ty = -(xr * zs - zr * xs) * Jinv, tz = (xr * ys - yr * xs) * Jinv;
+      g[kG00] = JW * (rx * rx + ry * ry + rz * rz);
+      g[kG01] = JW * (rx * sx + ry * sy + rz * sz);
+      g[kG02] = JW * (rx * tx + ry * ty + rz * tz);
+      g[kG11] = JW * (sx * sx + sy * sy + sz * sz);
+      g[kG12] = JW * (sx * tx + sy * ty + sz * tz);
+      g[kG22] = JW * (tx * tx + ty * ty + tz * tz);
+      g[kGWJ] = JW;
+    } else {
+      const bool more = active && (k + 1 < NQ);
 #pragma unroll
-    for (int c = 0; c < 7; ++c)
-      gn[c] = (more && (c < 6 || !POISSON)) ? stream_load(G + c * NP + n + NQ * NQ + lid) : T(0);
+      for (int c = 0; c < 7; ++c)
+        gn[c] = (more && (c < 6 || !POISSON)) ? stream_load(G + c * NP + n + NQ * NQ + lid) : T(0);
+    }
     T l0 = l0c, l1 = l1c;
     if (LFIELD && active) {
       l0 = lam0[e * NP + n + lid];
@@ -169,8 +221,10 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
     for (int m = 0; m < NQ; ++m) r_Aq[m] += Dg[k * NQ + m] * Gqt;
 #pragma unroll
     for (int m = 0; m < NQ; ++m) r_Aq[m] += (k == m) ? ab : T(0);
+    if constexpr (!TRI) {
 #pragma unroll
-    for (int c = 0; c < 7; ++c) g[c] = gn[c];
+      for (int c = 0; c < 7; ++c) g[c] = gn[c];
+    }
   }
 
   if constexpr (FUSEX) {
@@ -254,6 +308,55 @@ void launch_ax_variant(int N, int mult, int E, const int *elist, const T *ggeo, 
     default: throw Error(NRS_ERR_UNSUPPORTED, "Ax: order N must be 1..9");
   }
 }
+
+// Ax with the geometric factors formed from the element corners (trilinear
+// elements); corners [E][24] relative to q, zw [2][N+1]
+template <typename T, int NQ>
+static void launch_ax_trilin_nq(int mult, int E, const T *corners, const T *zw, const T *D, const T *l0,
+                                const T *l1, bool poisson, const T *q, T *Aq, const uint32_t *maskBits,
+                                hipStream_t s)
+{
+#define NRS_AXT(M_)                                                                                   \
+  {                                                                                                  \
+    using S = AxShape<NQ, M_>;                                                                       \
+    const unsigned grid = (unsigned)((E + S::NE - 1) / S::NE);                                       \
+    if (grid == 0) return;                                                                           \
+    if (poisson)                                                                                     \
+      ax_kernel<T, NQ, M_, true, false, false, true><<<grid, S::THREADS, 0, s>>>(                     \
+          E, nullptr, nullptr, D, l0, l1, q, Aq, maskBits, nullptr, corners, zw);                    \
+    else                                                                                             \
+      ax_kernel<T, NQ, M_, false, false, false, true><<<grid, S::THREADS, 0, s>>>(                    \
+          E, nullptr, nullptr, D, l0, l1, q, Aq, maskBits, nullptr, corners, zw);                    \
+  }
+  switch (mult) {
+    case 2: NRS_AXT(2) break;
+    case 4: NRS_AXT(4) break;
+    default: NRS_AXT(1) break;
+  }
+#undef NRS_AXT
+  NRS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_ax_trilin(int N, int E, const T *corners, const T *zw, const T *D, const T *l0, const T *l1,
+                      bool poisson, const T *q, T *Aq, const uint32_t *maskBits, hipStream_t s)
+{
+  const int mult = tuned(TUNE_AX, N, dtype_of<T>::value, E, 1);
+  switch (N + 1) {
+#define NRS_AXT_CASE(NQ_)                                                                          \
+  case NQ_: launch_ax_trilin_nq<T, NQ_>(mult, E, corners, zw, D, l0, l1, poisson, q, Aq, maskBits, s); break;
+    NRS_AXT_CASE(2) NRS_AXT_CASE(3) NRS_AXT_CASE(4) NRS_AXT_CASE(5) NRS_AXT_CASE(6) NRS_AXT_CASE(7)
+    NRS_AXT_CASE(8) NRS_AXT_CASE(9) NRS_AXT_CASE(10)
+#undef NRS_AXT_CASE
+    default: throw Error(NRS_ERR_UNSUPPORTED, "Ax: order N must be 1..9");
+  }
+}
+template void launch_ax_trilin<double>(int, int, const double *, const double *, const double *,
+                                       const double *, const double *, bool, const double *, double *,
+                                       const uint32_t *, hipStream_t);
+template void launch_ax_trilin<float>(int, int, const float *, const float *, const float *, const float *,
+                                      const float *, bool, const float *, float *, const uint32_t *,
+                                      hipStream_t);
 
 // Ax with the face fusion (N = 7, M elements per block); fx [E] relative to q
 template <typename T, int M>

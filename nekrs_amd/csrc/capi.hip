@@ -402,6 +402,7 @@ void nrs_elliptic_default_opts(nrs_elliptic_opts *o)
   o->projVectors = 8;
   o->projStart = 5;
   o->operatorFusion = 1;
+  o->mgGeometry = 1;
   o->precoCoeffField = 1;
 }
 

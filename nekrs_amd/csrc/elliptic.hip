@@ -401,13 +401,16 @@ template <typename T>
 void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
                       const T *q, T *Aq, int nChunks, hipStream_t s, bool record, const uint8_t *fx,
-                      bool lfield)
+                      bool lfield, const T *corners, const T *zw)
 {
   NRS_CHECK(!(fx && lfield), "operator: the face fusion takes constant coefficients");
+  NRS_CHECK(!(corners && (fx || lfield)), "operator: corner geometry takes constant coefficients, no fusion");
   auto ax = [&](int64_t a, int64_t n) {
     const int Np = (N + 1) * (N + 1) * (N + 1), NW = (Np + 31) / 32;
     const uint32_t *mb = maskBits ? maskBits + a * NW : nullptr;
-    if (fx)
+    if (corners)
+      launch_ax_trilin<T>(N, (int)n, corners + a * 24, zw, D, l0, l1, poisson, q + a * Np, Aq + a * Np, mb, s);
+    else if (fx)
       launch_ax_fused<T>(N, (int)n, ggeo + a * 7 * Np, D, l0, l1, poisson, q + a * Np, Aq + a * Np, mb,
                          fx + a, s, el->fusionBlock);
     else if (lfield)
@@ -456,11 +459,11 @@ void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D,
 template void chunked_operator<double>(nrs_elliptic *, int, int, const double *, const double *,
                                        const double *, const double *, bool, const uint32_t *,
                                        const nrs_ogs *, const double *, double *, int, hipStream_t, bool,
-                                       const uint8_t *, bool);
+                                       const uint8_t *, bool, const double *, const double *);
 template void chunked_operator<float>(nrs_elliptic *, int, int, const float *, const float *,
                                       const float *, const float *, bool, const uint32_t *,
                                       const nrs_ogs *, const float *, float *, int, hipStream_t, bool,
-                                      const uint8_t *, bool);
+                                      const uint8_t *, bool, const float *, const float *);
 
 template <typename T>
 int autotune_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,

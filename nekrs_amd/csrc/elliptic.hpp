@@ -137,7 +137,8 @@ template <typename T>
 void chunked_operator(nrs_elliptic *el, int N, int E, const T *ggeo, const T *D, const T *l0,
                       const T *l1, bool poisson, const uint32_t *maskBits, const nrs_ogs *ogs,
                       const T *q, T *Aq, int nChunks, hipStream_t s, bool record = false,
-                      const uint8_t *fx = nullptr, bool lfield = false);
+                      const uint8_t *fx = nullptr, bool lfield = false, const T *corners = nullptr,
+                      const T *zw = nullptr);
 // the operator's chunk count for (N, type, E): tuned, else 1
 int operator_chunks(int N, int dtype, int64_t E);
 // time nChunks = 1, 2, 4, 8, 16 on this operator, record the fastest

@@ -31,6 +31,13 @@ template <typename T>
 void launch_ax_fused(int N, int E, const T *ggeo, const T *D, const T *l0, const T *l1, bool poisson,
                      const T *q, T *Aq, const uint32_t *maskBits, const uint8_t *fx, hipStream_t s, int nb = 4);
 
+// Ax of trilinear elements with the geometric factors formed in the kernel
+// from the element corners [E][3][8] and the GLL nodes and weights zw [2][N+1]
+// (the tuned elements-per-block multiplier of launch_ax)
+template <typename T>
+void launch_ax_trilin(int N, int E, const T *corners, const T *zw, const T *D, const T *l0, const T *l1,
+                      bool poisson, const T *q, T *Aq, const uint32_t *maskBits, hipStream_t s);
+
 // mesh.hip
 void launch_geofacs(int N, int E, const double *D, const double *w, const double *x,
                     const double *y, const double *z, double *ggeo, hipStream_t s);

@@ -1,4 +1,5 @@
 // mesh.hip -- setup geometry: GLL basis, structured box meshes, geometric factors.
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -250,6 +251,46 @@ void launch_geofacs(int N, int E, const double *D, const double *w, const double
   }
 #undef GF
   NRS_LAUNCH_CHECK();
+}
+
+bool trilinear_corners(int N, int E, const double *x, const double *y, const double *z,
+                       std::vector<double> &corners)
+{
+  if (N < 1 || E <= 0) return false;
+  std::vector<double> r, w;
+  gll_nodes(N, r, w);
+  const int Nq = N + 1, Np = Nq * Nq * Nq;
+  corners.assign((size_t)E * 24, 0.0);
+  const double *X[3] = {x, y, z};
+  for (int64_t e = 0; e < E; ++e) {
+    double *C = corners.data() + e * 24;
+    for (int d = 0; d < 3; ++d)
+      for (int c = 0; c < 8; ++c) {
+        const int i = (c & 1) * N, j = ((c >> 1) & 1) * N, k = ((c >> 2) & 1) * N;
+        C[8 * d + c] = X[d][e * Np + i + Nq * (j + Nq * k)];
+      }
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = hi[d] = C[8 * d];
+      for (int c = 1; c < 8; ++c) { lo[d] = std::min(lo[d], C[8 * d + c]); hi[d] = std::max(hi[d], C[8 * d + c]); }
+    }
+    const double size = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(size > 0.0)) return false;
+    const double tol = 1e-12 * size;
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const double a[2] = {0.5 * (1 - r[i]), 0.5 * (1 + r[i])};
+          const double b[2] = {0.5 * (1 - r[j]), 0.5 * (1 + r[j])};
+          const double c2[2] = {0.5 * (1 - r[k]), 0.5 * (1 + r[k])};
+          for (int d = 0; d < 3; ++d) {
+            double v = 0.0;
+            for (int c = 0; c < 8; ++c) v += a[c & 1] * b[(c >> 1) & 1] * c2[(c >> 2) & 1] * C[8 * d + c];
+            if (std::fabs(v - X[d][e * Np + i + Nq * (j + Nq * k)]) > tol) return false;
+          }
+        }
+  }
+  return true;
 }
 
 }  // namespace nrs

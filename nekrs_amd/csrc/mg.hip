@@ -266,6 +266,8 @@ struct Level {
   DevBuf<T> invDegree;      // T copy of the masked ogs invDegree (o_invDegree of the level)
   DevBuf<T> invDiagA;
   DevBuf<T> R;              // [Nq][NqFiner] transfer from the finer level
+  DevBuf<T> zw;             // GLL nodes and weights [2][Nq] (corner-geometry operator)
+  const T *corners = nullptr;   // element corners [E][3][8] when the operator forms its geometry
   DevBuf<T> rhsOwn, xOwn, res;
   T *rhs = nullptr, *x = nullptr;
   double rho = 0, lmin = 0, lmax = 0;
@@ -540,6 +542,7 @@ struct MGImpl final : MGSolver {
   std::vector<std::unique_ptr<Level<T>>> levels;
   DevBuf<T> sRes, sAd, sD, sTmp;  // shared smoother work vectors (finest size)
   DevBuf<T> rP, zP;               // level-0 FP32 copies of r and z
+  DevBuf<T> cornersT;             // element corners [E][3][8] of a trilinear mesh (else empty)
   std::unique_ptr<CoarsePCG> coarse;
   enum Cheby { NO_CHEBY, CHEBY1, CHEBY_OPT4 };
   int cheby = CHEBY1;
@@ -569,7 +572,7 @@ struct MGImpl final : MGSolver {
   {
     const int k = (l.ogs->nChunks == nrs_elliptic::kOpChunks) ? operator_chunks(l.N, dtype_of<T>::value, l.E) : 1;
     chunked_operator<T>(el, l.N, l.E, l.ggeo, l.D, l.lam0.get(), l.lam1.get(), el->poisson, l.maskBits,
-                        l.ogs, q, Aq, k, s);
+                        l.ogs, q, Aq, k, s, false, nullptr, false, l.corners, l.corners ? l.zw.get() : nullptr);
   }
 
   // pMGLevel::smoother (ellipticMultiGridLevel.cpp:104-111); out != in
@@ -966,6 +969,15 @@ struct MGImpl final : MGSolver {
 
     std::vector<double> xf(d->x, d->x + e->Nlocal), yf(d->y, d->y + e->Nlocal), zf(d->z, d->z + e->Nlocal);
     std::vector<long long> gf(d->globalIds, d->globalIds + e->Nlocal);
+    // trilinear elements: the level operators form their geometric factors from
+    // the corners (the same map on every level: coarse coordinates interpolate it)
+    {
+      std::vector<double> cc;
+      if (o.mgGeometry != 0 && trilinear_corners(e->N, e->E, d->x, d->y, d->z, cc)) {
+        std::vector<T> ct(cc.begin(), cc.end());
+        cornersT.upload(ct);
+      }
+    }
     // Schwarz: face links of the mesh and element lengths of the finest level
     // (pMGLevel::build passes the finest elliptic to compute_element_lengths)
     std::vector<FaceLink> links;
@@ -1017,6 +1029,15 @@ struct MGImpl final : MGSolver {
         for (int i = 0; i < NqC; ++i)
           for (int j = 0; j < NqF; ++j) R[i * NqF + j] = (T)I[j * NqC + i];
         l.R.upload(R);
+      }
+      // corner geometry: FP32 at every order, FP64 up to order 7 (beyond, its registers spill)
+      if (cornersT.size() && (!std::is_same<T, double>::value || l.N <= 7)) {
+        std::vector<double> zl, wl;
+        gll_nodes(l.N, zl, wl);
+        std::vector<T> zwh(2 * (size_t)l.Nq);
+        for (int q = 0; q < l.Nq; ++q) { zwh[q] = (T)zl[q]; zwh[l.Nq + q] = (T)wl[q]; }
+        l.zw.upload(zwh);
+        l.corners = cornersT.get();
       }
       l.invDegree.alloc(l.L);
       k_cast<double, T>(l.L, l.ogs->invDegree.get(), l.invDegree.get(), s);
