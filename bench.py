@@ -335,7 +335,8 @@ def pressure_rhs_case(mesh, N, steps, warmup, seed=0):
     surface terms.  Algorithmic bytes (FP64): curl 128 B/node (Ue 24, rx..tz and
     JW 80, W 24), fused rhsU 184 B/node (W 24, invLMM 8, div 8, geometry 80, BF
     24, mue 8, irho 8, rhsU 24), weak divergence 128 B/node (rhsU 24, invLMM 8,
-    geometry 80, div 8, rhs 8); each 3-field gather-scatter 52 B per member of
+    geometry 80, div 8, rhs 8) -- on trilinear elements each kernel's 80 B/node
+    of geometry is 192 B/element of corners instead; each 3-field gather-scatter 52 B per member of
     a shared row (3 x 16 B values + 4 B index) + 4 B per row; the surface
     kernel 16 B per node of each element with a flux face (rhs read and
     written) + 32 B geometry and 24 B field (+8 B invLMM on symmetry faces) per
@@ -355,18 +356,25 @@ def pressure_rhs_case(mesh, N, steps, warmup, seed=0):
     div = dv(rng.uniform(-0.1, 0.1, L))
     rhs = torch.empty(L, dtype=torch.float64, device="cuda")
     g0idt = 1.5 / 1e-3
-    for _ in range(warmup):
-        pr.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
-    torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ranges().push("nrs:pressure_rhs")
-    a.record()
-    for _ in range(steps):
-        pr.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
-    b.record()
-    torch.cuda.synchronize()
-    ranges().pop()
-    ms = a.elapsed_time(b) / steps
+
+    def timed(h, rng_name=None):
+        for _ in range(warmup):
+            h.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
+        torch.cuda.synchronize()
+        if rng_name:
+            ranges().push(rng_name)
+        a.record()
+        for _ in range(steps):
+            h.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
+        b.record()
+        torch.cuda.synchronize()
+        if rng_name:
+            ranges().pop()
+        return a.elapsed_time(b) / steps
+
+    ms = timed(pr, "nrs:pressure_rhs")
+    corner = pr.corner_geometry
     info = nekrs_amd.Ogs(mesh.gid).info()
     gs3 = 52.0 * info["members"] + 4.0 * (info["rows"] + 1)
     Np, Nfp = (N + 1) ** 3, (N + 1) ** 2
@@ -374,27 +382,29 @@ def pressure_rhs_case(mesh, N, steps, warmup, seed=0):
     flux = np.isin(e2b, [2, 3, 4, 5, 6, 7])
     sym = np.isin(e2b, [4, 5, 6, 7])
     surf = (16.0 * Np * np.sum(flux.any(1)) + 56.0 * Nfp * flux.sum() + 8.0 * Nfp * sym.sum())
-    alg = 440.0 * L + 2 * gs3 + surf
+    # each of the three volume kernels reads rx..tz, JW (80 B/node) or, on trilinear
+    # elements, the 8 corners (192 B/element) it forms them from
+    geo = 3 * (192.0 * mesh.E if corner else 80.0 * L)
+    alg = 200.0 * L + geo + 2 * gs3 + surf
     gbs = alg / (ms * 1e-3) / 1e9
-    log("pressure rhs (%d elements N=%d, channel): %.4f ms per call, %.2f GB algorithmic, "
-        "%.0f GB/s (%.1f %% of HBM peak)" % (mesh.E, N, ms, alg / 1e9, gbs, 100 * gbs / PEAK_HBM_GBS))
+    log("pressure rhs (%d elements N=%d, channel, %s geometry): %.4f ms per call, %.2f GB algorithmic, "
+        "%.0f GB/s (%.1f %% of HBM peak)" % (mesh.E, N, "corner" if corner else "stored", ms, alg / 1e9,
+                                             gbs, 100 * gbs / PEAK_HBM_GBS))
     variants = {k: nekrs_amd.tuned_variant("tombo", N, k, mesh.E) for k in ("curl", "rhs")}
     fused = pr.fused_pairs
-    # the same calls without the face fusion, on the same box
+    # the same calls without the face fusion, and streaming the stored factors, on the same box
     pr0 = nekrs_amd.PressureRhs(mesh, bc, fusion=False)
-    for _ in range(warmup):
-        pr0.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
-    a.record()
-    for _ in range(steps):
-        pr0.compute(Ue, U, BF, mue, irho, div, g0idt, True, rhs=rhs)
-    b.record()
-    torch.cuda.synchronize()
-    ms0 = a.elapsed_time(b) / steps
+    ms0 = timed(pr0)
     del pr0
-    log("pressure rhs: %d face pairs fused; without the fusion %.4f ms per call" % (fused, ms0))
+    prs = nekrs_amd.PressureRhs(mesh, bc, stored_geometry=True)
+    ms_stored = timed(prs)
+    del prs
+    log("pressure rhs: %d face pairs fused; without the fusion %.4f ms per call; streaming the stored "
+        "factors %.4f ms per call" % (fused, ms0, ms_stored))
     return {"ms_per_call": ms, "setup_seconds": setup, "algorithmic_bytes": alg,
+            "geometry": "corner" if corner else "stored",
             "load_schedule_variants": variants, "fused_face_pairs": fused,
-            "unfused_ms_per_call": ms0,
+            "unfused_ms_per_call": ms0, "stored_geometry_ms_per_call": ms_stored,
             "achieved_GBps": gbs, "roofline_frac": gbs / PEAK_HBM_GBS, "steps": steps,
             "flux_elements": int(flux.any(1).sum()),
             "workload": "tombo::pressureSolve right-hand side (viscous term on), %d hex elements "

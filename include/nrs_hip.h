@@ -614,9 +614,12 @@ enum nrs_tombo_flags {
   NRS_TOMBO_FUSION = 1,          /* N = 7: faces shared by consecutive elements summed in the
                                   * curl kernels (bit-identical), gather-scatters without them */
   NRS_TOMBO_AUTOTUNE = 2,        /* time the load schedules, keep the fusion only if faster  */
-  NRS_TOMBO_STRESS = 4           /* VELOCITY STRESSFORMULATION = TRUE (tombo.cpp:57-66): the
+  NRS_TOMBO_STRESS = 4,          /* VELOCITY STRESSFORMULATION = TRUE (tombo.cpp:57-66): the
                                   * second curl less JW (2 sym(grad Ue) grad(mue)/mue
                                   * - 2/3 div grad(mue)/mue) (pressureStressHex3D.okl)        */
+  NRS_TOMBO_STORED_GEOMETRY = 8  /* read the stored rx..tz, JW even where every element is
+                                  * trilinear (by default such meshes have the volume kernels
+                                  * form them from the element corners)                     */
 };
 int nrs_tombo_setup(int N, int Nelements, const double *x, const double *y, const double *z,
                     const long long *globalIds, const int *bcType, int flags, nrs_tombo **out);
@@ -643,7 +646,8 @@ int nrs_tombo_info(nrs_tombo *h, int64_t *fusedPairs, int64_t *fluxElements);
 int nrs_tombo_curl(nrs_tombo *h, int64_t fieldOffset, const double *U, double *W, void *stream);
 
 /* Device buffers owned by the handle: 0 vgeo (f64 [E][10][Np]: rx ry rz sx sy
- * sz tx ty tz JW), 1 invLMM (f64 [E*Np]). */
+ * sz tx ty tz JW), 1 invLMM (f64 [E*Np]), 2 the element corners the volume
+ * kernels form their factors from (f64 [E][3][8]; count 0 when they stream vgeo). */
 int nrs_tombo_buffer(nrs_tombo *h, int which, void **ptr, int64_t *count);
 /* Copy `count` entries of buffer `which` into the caller's device array dst. */
 int nrs_tombo_copy_buffer(nrs_tombo *h, int which, double *dst, int64_t count, void *stream);

@@ -374,9 +374,11 @@ class PressureRhs:
     (BC ids; default: the mesh's own bc_types when it has them (Re2Mesh), else every
     boundary face an inlet, 0 inside)."""
 
-    def __init__(self, mesh, bc_types=None, fusion=True, autotune=True, stress=False):
+    def __init__(self, mesh, bc_types=None, fusion=True, autotune=True, stress=False,
+                 stored_geometry=False):
         """stress: VELOCITY STRESSFORMULATION = TRUE (the stress term of
-        pressureStressHex3D taken from the second curl)."""
+        pressureStressHex3D taken from the second curl); stored_geometry: read
+        the stored geometric factors even on a mesh of trilinear elements."""
         self.mesh = mesh
         self.N, self.E = mesh.N, mesh.E
         self.Nlocal = mesh.E * (mesh.N + 1) ** 3
@@ -387,7 +389,8 @@ class PressureRhs:
         self.bc_types = np.ascontiguousarray(bc_types, dtype=np.int32).reshape(-1)
         assert self.bc_types.size == 6 * mesh.E
         h = C.c_void_p()
-        flags = (1 if fusion else 0) | (2 if autotune else 0) | (4 if stress else 0)
+        flags = ((1 if fusion else 0) | (2 if autotune else 0) | (4 if stress else 0) |
+                 (8 if stored_geometry else 0))
         _lib.call("nrs_tombo_setup", mesh.N, mesh.E, mesh.x.ctypes.data, mesh.y.ctypes.data,
                   mesh.z.ctypes.data, mesh.gid.ctypes.data, self.bc_types.ctypes.data, flags,
                   C.byref(h))
@@ -425,6 +428,14 @@ class PressureRhs:
     @property
     def inv_lmm(self):
         return self._copy(1)
+
+    @property
+    def corner_geometry(self):
+        """True when the volume kernels form rx..tz, JW from the element corners
+        (trilinear elements, stored_geometry False) instead of streaming vgeo."""
+        p, n = C.c_void_p(), C.c_int64()
+        _lib.call("nrs_tombo_buffer", self._h, 2, C.byref(p), C.byref(n))
+        return n.value > 0
 
     def curl(self, U, W=None, field_offset=None, stream=None):
         torch = _torch()

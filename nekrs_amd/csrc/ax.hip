@@ -12,6 +12,7 @@
 // Small orders pack several elements per block so that a block is >= 64 lanes.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "trilin.hpp"
 #include "tune.hpp"
 
 namespace nrs {
@@ -67,11 +68,8 @@ template <int NQ, int MULT = 1> struct AxShape {
 // bit for bit) and the caller's gather-scatter handle leaves them out.
 //
 // TRI: trilinear elements.  The geometric factors are not read but formed at
-// each node from the element's eight corners (corners [E][3][8], x y z, corner
-// a + 2b + 4c at (r, s, t) = (+-1, +-1, +-1)) and the GLL nodes and weights
-// (zw [2][NQ]), with geometricFactorsHex3D's formulas (J from the derivatives
-// of the map, the inverse by cofactors, JW = J w_i w_j w_k); for a trilinear
-// map the derivatives D x of the reference equal these up to rounding.
+// each node from the element's eight corners (corners [E][3][8]) and the GLL
+// nodes and weights (zw [2][NQ]) by TriGeo (trilin.hpp).
 template <typename T, int NQ, int MULT, bool POISSON, bool LFIELD, bool FUSEX = false, bool TRI = false>
 __global__ void __launch_bounds__((AxShape<NQ, MULT>::THREADS), AX_WAVES_PER_EU)
 ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
@@ -126,23 +124,9 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
   // (TRI: the corner differences that the derivatives of the map take at this
   // lane's (r, s), formed once per element)
   T g[7];
-  T cDrB[3], cDrT[3], cDsB[3], cDsT[3], cDt[3], wij = T(0);
+  TriGeo<T> tri;
   if constexpr (TRI) {
-    const T r = zw[i], sj = zw[j];
-    const T hr0 = T(0.5) * (T(1) - r), hr1 = T(0.5) * (T(1) + r);
-    const T hs0 = T(0.5) * (T(1) - sj), hs1 = T(0.5) * (T(1) + sj);
-    wij = zw[NQ + i] * zw[NQ + j];
-    const T *P0 = corners + e * 24;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const T *P = P0 + 8 * c;
-      const T p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3], p4 = P[4], p5 = P[5], p6 = P[6], p7 = P[7];
-      cDrB[c] = T(0.5) * (hs0 * (p1 - p0) + hs1 * (p3 - p2));
-      cDrT[c] = T(0.5) * (hs0 * (p5 - p4) + hs1 * (p7 - p6));
-      cDsB[c] = T(0.5) * (hr0 * (p2 - p0) + hr1 * (p3 - p1));
-      cDsT[c] = T(0.5) * (hr0 * (p6 - p4) + hr1 * (p7 - p5));
-      cDt[c] = T(0.5) * (hs0 * (hr0 * (p4 - p0) + hr1 * (p5 - p1)) + hs1 * (hr0 * (p6 - p2) + hr1 * (p7 - p3)));
-    }
+    tri.init(corners + e * 24, zw[i], zw[j], zw[NQ + i], zw[NQ + j]);
   } else {
 #pragma unroll
     for (int c = 0; c < 7; ++c) g[c] = (active && (c < 6 || !POISSON)) ? stream_load(G + c * NP + lid) : T(0);
@@ -156,19 +140,10 @@ ax_kernel(int E, const int *__restrict__ elist, const T *__restrict__ ggeo,
     for (int m = 1; m < NQ; ++m) qk = (k == m) ? r_q[m] : qk;
     T gn[7];
     if constexpr (TRI) {
-      const T t = zw[k];
-      const T ht0 = T(0.5) * (T(1) - t), ht1 = T(0.5) * (T(1) + t);
-      const T xr = ht0 * cDrB[0] + ht1 * cDrT[0], yr = ht0 * cDrB[1] + ht1 * cDrT[1],
-              zr = ht0 * cDrB[2] + ht1 * cDrT[2];
-      const T xs = ht0 * cDsB[0] + ht1 * cDsT[0], ys = ht0 * cDsB[1] + ht1 * cDsT[1],
-              zs = ht0 * cDsB[2] + ht1 * cDsT[2];
-      const T xt = cDt[0], yt = cDt[1], zt = cDt[2];
-      const T J = xr * (ys * zt - zs * yt) - yr * (xs * zt - zs * xt) + zr * (xs * yt - ys * xt);
-      const T Jinv = T(1) / J;
-      const T JW = J * wij * zw[NQ + k];
-      const T rx = (ys * zt - zs * yt) * Jinv, ry = -(xs * zt - zs * xt) * Jinv, rz = (xs * yt - ys * xt) * Jinv;
-      const T sx = -(yr * zt - zr * yt) * Jinv, sy = (xr * zt - zr * xt) * Jinv, sz = -(xr * yt - yr * xt) * Jinv;
-      const T tx = (yr * zs - zr * ys) * Jinv, ty = -(xr * zs - zr * xs) * Jinv, tz = (xr * ys - yr * xs) * Jinv;
+      T v[10];
+      tri.at(zw[k], zw[NQ + k], v);
+      const T rx = v[0], ry = v[1], rz = v[2], sx = v[3], sy = v[4], sz = v[5], tx = v[6], ty = v[7], tz = v[8];
+      const T JW = v[9];
       g[kG00] = JW * (rx * rx + ry * ry + rz * rz);
       g[kG01] = JW * (rx * sx + ry * sy + rz * sz);
       g[kG02] = JW * (rx * tx + ry * ty + rz * tz);

@@ -789,7 +789,8 @@ int nrs_tombo_buffer(nrs_tombo *h, int which, void **ptr, int64_t *count)
   switch (which) {
     case 0: *ptr = h->vgeo.get(); *count = (int64_t)h->vgeo.n; break;
     case 1: *ptr = h->invLMM.get(); *count = (int64_t)h->invLMM.n; break;
-    default: throw nrs::Error(NRS_ERR_INVALID, "tombo buffer: which must be 0 or 1");
+    case 2: *ptr = h->corners.get(); *count = (int64_t)h->corners.n; break;
+    default: throw nrs::Error(NRS_ERR_INVALID, "tombo buffer: which must be 0, 1 or 2");
   }
   NRS_API_END
 }

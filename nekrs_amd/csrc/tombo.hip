@@ -18,6 +18,7 @@
 #include "elliptic.hpp"
 #include "mesh.hpp"
 #include "tombo.hpp"
+#include "trilin.hpp"
 #include "tune.hpp"
 
 nrs_tombo::~nrs_tombo()
@@ -109,13 +110,14 @@ vgeo_kernel(int E, const double *__restrict__ Dg, const double *__restrict__ wg,
 // by consecutive elements with fx[e] = 1 get the pair sum here (the lower
 // element's value first, as the gather-scatter adds them), so the following
 // gather-scatter leaves them out.
-template <int NQ, bool PRESCALE, bool RHS, bool PF, bool FX = false>
+template <int NQ, bool PRESCALE, bool RHS, bool PF, bool FX = false, bool TRI = false>
 __global__ void __launch_bounds__(NQ * NQ * (FX ? 4 : 1))
 curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ Dg, int64_t offIn,
             int64_t offOut, const double *__restrict__ invLMM, const double *__restrict__ U,
             const double *__restrict__ div, const double *__restrict__ BF, int64_t offBF,
             const double *__restrict__ mue, const double *__restrict__ irho, double *__restrict__ out,
-            const uint8_t *__restrict__ fx = nullptr, const double *__restrict__ S = nullptr)
+            const uint8_t *__restrict__ fx = nullptr, const double *__restrict__ S = nullptr,
+            const double *__restrict__ corners = nullptr, const double *__restrict__ zw = nullptr)
 {
   constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
   constexpr int NF = RHS ? 4 : 3;                 // u, v, w (+ div)
@@ -153,12 +155,22 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
       for (int c = 0; c < 3; ++c) pf[c][l] = PRESCALE ? sc * U[id + c * offIn] : U[id + c * offIn];
       if (RHS) pf[NF - 1][l] = div[id];
     }
-    // per-plane inputs: geometry (and BF, mue, irho), at use or one plane ahead
+    // per-plane inputs: geometry (and BF, mue, irho), at use or one plane ahead;
+    // TRI: the geometry formed from the element's corners
     constexpr int NX = RHS ? NV + 5 : NV;
-    const double *Ge = vgeo + e * NV * NP + j * NQ + i;
+    const double *Ge = TRI ? nullptr : vgeo + e * NV * NP + j * NQ + i;
+    TriGeo<double> tri;
+    if constexpr (TRI) tri.init(corners + e * 24, zw[i], zw[j], zw[NQ + i], zw[NQ + j]);
     auto load_plane = [&](int k, double (&g)[NX]) {
+      if constexpr (TRI) {
+        double v[NV];
+        tri.at(zw[k], zw[NQ + k], v);
 #pragma unroll
-      for (int f = 0; f < NV; ++f) g[f] = __builtin_nontemporal_load(Ge + f * NP + k * NFP);
+        for (int f = 0; f < NV; ++f) g[f] = v[f];
+      } else {
+#pragma unroll
+        for (int f = 0; f < NV; ++f) g[f] = __builtin_nontemporal_load(Ge + f * NP + k * NFP);
+      }
       if (RHS) {
         const int64_t id = base + k * NFP;
         g[NV + 0] = BF[id];
@@ -257,11 +269,11 @@ curl_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
 // written to S [3][Nlocal]; the right-hand-side kernel subtracts it from the
 // second curl.  Thread (i,j) per element with k-pencils of u, v, w, mue in
 // registers; the r/s derivatives through one LDS plane per field.
-template <int NQ>
+template <int NQ, bool TRI>
 __global__ void __launch_bounds__(NQ * NQ)
 stress_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ Dg, int64_t off,
               const double *__restrict__ U, const double *__restrict__ mue, const double *__restrict__ div,
-              double *__restrict__ S)
+              double *__restrict__ S, const double *__restrict__ corners, const double *__restrict__ zw)
 {
   constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
   __shared__ double s_D[NQ][NQ], s_f[4][NQ][NQ];
@@ -269,6 +281,8 @@ stress_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__
   const int i = threadIdx.x % NQ, j = threadIdx.x / NQ;
   s_D[j][i] = Dg[j * NQ + i];
   const int64_t base = (int64_t)e * NP + j * NQ + i;
+  TriGeo<double> tri;
+  if constexpr (TRI) tri.init(corners + (int64_t)e * 24, zw[i], zw[j], zw[NQ + i], zw[NQ + j]);
   double pf[4][NQ];
 #pragma unroll
   for (int l = 0; l < NQ; ++l) {
@@ -296,11 +310,18 @@ stress_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__
       }
     }
     const int64_t id = base + k * NFP;
-    const double *G = vgeo + (int64_t)e * NV * NP + k * NFP + j * NQ + i;
-    const double rx = G[0 * NP], ry = G[1 * NP], rz = G[2 * NP];
-    const double sx = G[3 * NP], sy = G[4 * NP], sz = G[5 * NP];
-    const double tx = G[6 * NP], ty = G[7 * NP], tz = G[8 * NP];
-    const double JW = G[9 * NP];
+    double v[NV];
+    if constexpr (TRI) {
+      tri.at(zw[k], zw[NQ + k], v);
+    } else {
+      const double *G = vgeo + (int64_t)e * NV * NP + k * NFP + j * NQ + i;
+#pragma unroll
+      for (int f = 0; f < NV; ++f) v[f] = G[f * NP];
+    }
+    const double rx = v[0], ry = v[1], rz = v[2];
+    const double sx = v[3], sy = v[4], sz = v[5];
+    const double tx = v[6], ty = v[7], tz = v[8];
+    const double JW = v[9];
     double gx[4], gy[4], gz[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -322,11 +343,12 @@ stress_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__
 }
 
 // rhs = wdiv(invLMM F) + lambda JW div   (wDivergenceVolumeHex3D.okl + pressureAddQtl.okl)
-template <int NQ>
+template <int NQ, bool TRI>
 __global__ void __launch_bounds__(NQ * NQ)
 wdiv_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ Dg, int64_t off,
             const double *__restrict__ invLMM, const double *__restrict__ F,
-            const double *__restrict__ div, double lambda, double *__restrict__ rhs)
+            const double *__restrict__ div, double lambda, double *__restrict__ rhs,
+            const double *__restrict__ corners, const double *__restrict__ zw)
 {
   constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
   __shared__ double s_D[NQ][NQ], s_u[NQ][NQ], s_v[NQ][NQ], s_w[NQ][NQ];
@@ -337,18 +359,27 @@ wdiv_kernel(int E, const double *__restrict__ vgeo, const double *__restrict__ D
 #pragma unroll
   for (int k = 0; k < NQ; ++k) r_div[k] = 0.0;
   const int64_t base = (int64_t)e * NP + j * NQ + i;
+  TriGeo<double> tri;
+  if constexpr (TRI) tri.init(corners + (int64_t)e * 24, zw[i], zw[j], zw[NQ + i], zw[NQ + j]);
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
     const int64_t id = base + k * NFP;
-    const double *G = vgeo + (int64_t)e * NV * NP + k * NFP + j * NQ + i;
+    double v[NV];
+    if constexpr (TRI) {
+      tri.at(zw[k], zw[NQ + k], v);
+    } else {
+      const double *G = vgeo + (int64_t)e * NV * NP + k * NFP + j * NQ + i;
+#pragma unroll
+      for (int f = 0; f < NV; ++f) v[f] = G[f * NP];
+    }
     const double sc = invLMM[id];
     const double Un = sc * F[id], Vn = sc * F[id + off], Wn = sc * F[id + 2 * off];
-    const double JW = G[9 * NP];
+    const double JW = v[9];
     jw[k] = JW;
     __syncthreads();
-    s_u[j][i] = JW * (G[0 * NP] * Un + G[1 * NP] * Vn + G[2 * NP] * Wn);
-    s_v[j][i] = JW * (G[3 * NP] * Un + G[4 * NP] * Vn + G[5 * NP] * Wn);
-    s_w[j][i] = JW * (G[6 * NP] * Un + G[7 * NP] * Vn + G[8 * NP] * Wn);
+    s_u[j][i] = JW * (v[0] * Un + v[1] * Vn + v[2] * Wn);
+    s_v[j][i] = JW * (v[3] * Un + v[4] * Vn + v[5] * Wn);
+    s_w[j][i] = JW * (v[6] * Un + v[7] * Vn + v[8] * Wn);
     __syncthreads();
 #pragma unroll
     for (int n = 0; n < NQ; ++n) {
@@ -466,12 +497,20 @@ template <int NQ> struct CurlL {
   static void run(const nrs_tombo *t, int64_t offIn, int64_t offOut, bool prescale, bool pf,
                   const double *U, double *W, hipStream_t s, const uint8_t *fx = nullptr)
   {
+    if (t->tri) go<true>(t, offIn, offOut, prescale, pf, U, W, s, fx);
+    else go<false>(t, offIn, offOut, prescale, pf, U, W, s, fx);
+  }
+  template <bool TRI>
+  static void go(const nrs_tombo *t, int64_t offIn, int64_t offOut, bool prescale, bool pf,
+                 const double *U, double *W, hipStream_t s, const uint8_t *fx)
+  {
     const double *G = t->vgeo.get(), *D = t->D.get(), *il = t->invLMM.get();
+    const double *C = t->corners.get(), *zw = t->zw.get();
     if constexpr (NQ == 8) {
       if (fx) {
         const dim3 g((t->E + 3) / 4), b(256);
-#define CURLF(P, F) curl_kernel<NQ, P, false, F, true><<<g, b, 0, s>>>(t->E, G, D, offIn, offOut, P ? il : nullptr, \
-                                                                       U, nullptr, nullptr, 0, nullptr, nullptr, W, fx)
+#define CURLF(P, F) curl_kernel<NQ, P, false, F, true, TRI><<<g, b, 0, s>>>(t->E, G, D, offIn, offOut, \
+    P ? il : nullptr, U, nullptr, nullptr, 0, nullptr, nullptr, W, fx, nullptr, C, zw)
         if (prescale) { if (pf) CURLF(true, true); else CURLF(true, false); }
         else { if (pf) CURLF(false, true); else CURLF(false, false); }
 #undef CURLF
@@ -479,8 +518,8 @@ template <int NQ> struct CurlL {
       }
     }
     const dim3 g(t->E), b(NQ * NQ);
-#define CURL(P, F) curl_kernel<NQ, P, false, F><<<g, b, 0, s>>>(t->E, G, D, offIn, offOut, P ? il : nullptr, \
-                                                                U, nullptr, nullptr, 0, nullptr, nullptr, W)
+#define CURL(P, F) curl_kernel<NQ, P, false, F, false, TRI><<<g, b, 0, s>>>(t->E, G, D, offIn, offOut, \
+    P ? il : nullptr, U, nullptr, nullptr, 0, nullptr, nullptr, W, nullptr, nullptr, C, zw)
     if (prescale) { if (pf) CURL(true, true); else CURL(true, false); }
     else { if (pf) CURL(false, true); else CURL(false, false); }
 #undef CURL
@@ -491,42 +530,61 @@ template <int NQ> struct RhsL {
                   int64_t offBF, const double *mue, const double *irho, double *out, hipStream_t s,
                   const uint8_t *fx = nullptr, const double *S = nullptr)
   {
+    if (t->tri) go<true>(t, pf, W, div, BF, offBF, mue, irho, out, s, fx, S);
+    else go<false>(t, pf, W, div, BF, offBF, mue, irho, out, s, fx, S);
+  }
+  template <bool TRI>
+  static void go(const nrs_tombo *t, bool pf, const double *W, const double *div, const double *BF,
+                 int64_t offBF, const double *mue, const double *irho, double *out, hipStream_t s,
+                 const uint8_t *fx, const double *S)
+  {
     const double *G = t->vgeo.get(), *D = t->D.get(), *il = t->invLMM.get();
+    const double *C = t->corners.get(), *zw = t->zw.get();
     const int64_t L = t->Nlocal;
     if constexpr (NQ == 8) {
       if (fx) {
         const dim3 g((t->E + 3) / 4), b(256);
         if (pf)
-          curl_kernel<NQ, true, true, true, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue,
-                                                                  irho, out, fx, S);
+          curl_kernel<NQ, true, true, true, true, TRI><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF,
+                                                                       mue, irho, out, fx, S, C, zw);
         else
-          curl_kernel<NQ, true, true, false, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue,
-                                                                   irho, out, fx, S);
+          curl_kernel<NQ, true, true, false, true, TRI><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF,
+                                                                        mue, irho, out, fx, S, C, zw);
         return;
       }
     }
     const dim3 g(t->E), b(NQ * NQ);
     if (pf)
-      curl_kernel<NQ, true, true, true><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out,
-                                                        nullptr, S);
+      curl_kernel<NQ, true, true, true, false, TRI><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue,
+                                                                    irho, out, nullptr, S, C, zw);
     else
-      curl_kernel<NQ, true, true, false><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF, mue, irho, out,
-                                                         nullptr, S);
+      curl_kernel<NQ, true, true, false, false, TRI><<<g, b, 0, s>>>(t->E, G, D, L, L, il, W, div, BF, offBF,
+                                                                     mue, irho, out, nullptr, S, C, zw);
   }
 };
 template <int NQ> struct StressL {
   static void run(const nrs_tombo *t, int64_t off, const double *U, const double *mue, const double *div,
                   double *S, hipStream_t s)
   {
-    stress_kernel<NQ><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), off, U, mue, div, S);
+    if (t->tri)
+      stress_kernel<NQ, true><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), off, U, mue, div, S,
+                                                       t->corners.get(), t->zw.get());
+    else
+      stress_kernel<NQ, false><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), off, U, mue, div, S,
+                                                        nullptr, nullptr);
   }
 };
 template <int NQ> struct WdivL {
   static void run(const nrs_tombo *t, const double *F, const double *div, double lambda,
                   double *rhs, hipStream_t s)
   {
-    wdiv_kernel<NQ><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
-                                              t->invLMM.get(), F, div, lambda, rhs);
+    if (t->tri)
+      wdiv_kernel<NQ, true><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
+                                                     t->invLMM.get(), F, div, lambda, rhs, t->corners.get(),
+                                                     t->zw.get());
+    else
+      wdiv_kernel<NQ, false><<<t->E, NQ * NQ, 0, s>>>(t->E, t->vgeo.get(), t->D.get(), t->Nlocal,
+                                                      t->invLMM.get(), F, div, lambda, rhs, nullptr, nullptr);
   }
 };
 template <int NQ> struct SurfL {
@@ -661,6 +719,17 @@ nrs_tombo *tombo_setup(int N, int E, const double *x, const double *y, const dou
                     t->vgeo.get(), t->bGeo.get(), s);
     NRS_LAUNCH_CHECK();
     NRS_HIP(hipStreamSynchronize(s));
+  }
+  // trilinear elements: the volume kernels form their factors from the corners
+  {
+    std::vector<double> cc;
+    if (!(flags & NRS_TOMBO_STORED_GEOMETRY) && trilinear_corners(N, E, x, y, z, cc)) {
+      std::vector<double> zwh(2 * (size_t)t->Nq);
+      for (int q = 0; q < t->Nq; ++q) { zwh[q] = z1[q]; zwh[t->Nq + q] = w1[q]; }
+      t->corners.upload(cc);
+      t->zw.upload(zwh);
+      t->tri = true;
+    }
   }
   // invLMM = 1 / gs(JW)   (meshComputeInvLMM.cpp)
   t->ogs = ogs_setup(t->Nlocal, gids);

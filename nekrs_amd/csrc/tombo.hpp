@@ -22,6 +22,10 @@ struct nrs_tombo {
   nrs::DevBuf<double> bGeo;          // [Nb][6][Nq*Nq][4]: nx ny nz WSJ
   nrs::DevBuf<double> W, F;          // [3][Nlocal] work: curl, then the assembled rhsU
   nrs::DevBuf<double> S;             // [3][Nlocal] stress term (NRS_TOMBO_STRESS), else empty
+  // trilinear elements: the volume kernels form rx..tz, JW from the corners
+  // [E][3][8] and the GLL nodes and weights zw [2][Nq] instead of reading vgeo
+  bool tri = false;
+  nrs::DevBuf<double> corners, zw;
   ~nrs_tombo();
 };
 

@@ -199,3 +199,55 @@ def test_face_fusion_bitwise(gpu, torch_mod, orc, visc):
     assert np.array_equal(a, b)
     g = ot.Geometry(N, m.x, m.y, m.z, m.gid)
     assert rel(a, g.pressure_rhs(bc, Ue, U, BF, mue, irho, div, 80.0, visc, off_)) <= 1e-11
+
+
+def _rhs_inputs(torch, L, off, seed):
+    rng = np.random.default_rng(seed)
+    dv = lambda a: torch.from_numpy(a).cuda()
+    Ue, U, BF = (dv(rng.uniform(-1, 1, 3 * off)) for _ in range(3))
+    return (Ue, U, BF, dv(rng.uniform(0.5, 1.5, L)), dv(rng.uniform(0.5, 1.5, L)),
+            dv(rng.uniform(-0.1, 0.1, L)))
+
+
+@pytest.mark.parametrize("N", [1, 3, 5, 7, 9])
+def test_corner_geometry_matches_stored(gpu, torch_mod, orc, N):
+    """On trilinear elements (the Kershaw box) the volume kernels form rx..tz, JW
+    from the element corners; the right-hand side, the curl and the stress term
+    agree with the kernels that stream the stored factors to 1e-13 relative (the
+    formed factors differ from the stored ones by rounding only)."""
+    torch = torch_mod
+    m, mo = meshes(gpu, orc, (4, 3, 3), N)
+    bc = bc_mix(m, N)
+    L = m.x.size
+    off = L + 32
+    for stress in (False, True):
+        on = gpu.PressureRhs(m, bc, stress=stress)
+        off_h = gpu.PressureRhs(m, bc, stress=stress, stored_geometry=True)
+        assert on.corner_geometry and not off_h.corner_geometry
+        args = _rhs_inputs(torch, L, off, N)
+        for visc in ((True, False) if not stress else (True,)):
+            a = on.compute(*args, 90.0, visc, field_offset=off).cpu().numpy()
+            b = off_h.compute(*args, 90.0, visc, field_offset=off).cpu().numpy()
+            print("N", N, "stress", stress, "visc", visc, "rel", rel(a, b), flush=True)
+            assert rel(a, b) <= 1e-13
+        wa = on.curl(args[1], field_offset=off).cpu().numpy()
+        wb = off_h.curl(args[1], field_offset=off).cpu().numpy()
+        idx = np.concatenate([np.arange(c * off, c * off + L) for c in range(3)])
+        assert rel(wa[idx], wb[idx]) <= 1e-13
+
+
+def test_corner_geometry_off_on_curved_mesh(gpu, torch_mod, orc):
+    """Sine-mapped nodes are not trilinear: the kernels stream the stored
+    factors whatever the flag says, bit for bit."""
+    torch = torch_mod
+    N = 5
+    m = gpu.BoxMesh(3, 3, 2, N, "sine", 0.0)
+    bc = bc_mix(m, 3)
+    a_h = gpu.PressureRhs(m, bc)
+    b_h = gpu.PressureRhs(m, bc, stored_geometry=True)
+    assert not a_h.corner_geometry
+    L = m.x.size
+    args = _rhs_inputs(torch, L, L, 4)
+    a = a_h.compute(*args, 70.0, True).cpu().numpy()
+    b = b_h.compute(*args, 70.0, True).cpu().numpy()
+    assert np.array_equal(a, b)
