@@ -16,7 +16,9 @@ times 100 steps after 10 warm-up steps, inputs resident in HBM.
   roofline   dominant kernel = FP64 Ax: 64 algorithmic bytes per local node
              (q 8 + Aq 8 + 6 geometric factors 48) / average launch time
              measured with HIP events around every Ax launch in the timed loop
-  cpu_baseline  the oracle's OpenMP CPU operator on the same mesh
+  cpu_baseline  the operator on the host cores on the same mesh: the reference's
+             own serial CPU kernel and gslib where oracle/_ref is built, the
+             oracle's OpenMP port beside it
   solves     the second half of the metric: time to a 1e-8 relative residual of
              the 48^3 N=7 Kershaw pressure-type Poisson solve (manufactured sine
              solution plus seeded 1e-3 Gaussian noise in the forcing), flexible
@@ -30,7 +32,9 @@ times 100 steps after 10 warm-up steps, inputs resident in HBM.
              then timed over one solve from x = 0 (setup excluded).
   pressure_rhs  the step before the solve: tombo::pressureSolve's right-hand
              side on the same 32^3 mesh (channel boundary faces), ms per call and
-             algorithmic GB/s
+             algorithmic GB/s, beside the same call streaming stored factors
+  plain_poisson  the reference's CPU-runnable case: 8^3 N=7 sine-deformed cube,
+             Jacobi PCG to 1e-8, FP64, and the oracle's solve on the host beside it
   kershaw    the reference's own published benchmarks (examples/kershaw): BPS5,
              the pressure solve with the reference's defaults (PGMRES+FLEXIBLE,
              RAS + opt. 4th-kind Chebyshev degree 3, FP32 multigrid) on the 20^3
@@ -173,7 +177,7 @@ def cpu_baseline(nx, N, seconds, seed):
                       "ellipticPartialAxCoeffHex3D.c, OpenMP over elements, built from its sources; "
                       "mask; its gslib gs, serial) on the same %d^3 N=%d Kershaw mesh, %.1f s; "
                       "rel. difference to the oracle's operator %.1e" % (reps, nx, N, t, err),
-            "port": port}
+            "port": port, "port_over_reference": port["value"] / (E * N ** 3 * reps / t / 1e9)}
 
 
 def solve_cases(n, N, seed=0, only=None):
@@ -246,6 +250,49 @@ def solve_cases(n, N, seed=0, only=None):
     out["workload"] = ("%d^3=%d hex elements N=%d Kershaw eps=0.3, p-MG %d-3-1 with an FP64 coarse "
                        "solve (2 AMG-preconditioned PCG iterations), FCG to 1e-8 relative, rhs "
                        "sine + 1e-3 noise (seed %d)" % (n, n ** 3, N, N, seed))
+    return out
+
+
+def plain_poisson_case(seed=0, cpu=False):
+    """The reference's CPU-runnable case: 8^3 = 512 hex elements at N=7 on the unit
+    cube, nodes moved by 0.05 sin(pi x) sin(pi y) sin(pi z), manufactured solution
+    sin(pi x) sin(pi y) sin(pi z) with Dirichlet boundaries, Jacobi PCG to 1e-8
+    relative, FP64 throughout; with cpu, the oracle's Jacobi PCG on the host cores
+    beside it (same mesh and right-hand side)."""
+    import numpy as np
+    import torch
+    import nekrs_amd
+    mesh = nekrs_amd.BoxMesh(8, 8, 8, 7, "sine", 0.0)
+    el = nekrs_amd.Elliptic(mesh, precon="jacobi", solver="pcg", tol=1e-8, max_iter=1000)
+    Np = 8 ** 3
+    u = np.sin(np.pi * mesh.x) * np.sin(np.pi * mesh.y) * np.sin(np.pi * mesh.z)
+    gw = el.ggeo.cpu().numpy().reshape(-1, 7, Np)[:, 6].ravel()
+    rhs = torch.from_numpy(3 * np.pi ** 2 * u * gw).cuda()
+    st = None
+    for rep in range(3):
+        x = torch.zeros_like(rhs)
+        torch.cuda.synchronize()
+        st = el.solve(rhs.clone(), x)
+    err = float(np.max(np.abs(x.cpu().numpy() - u)))
+    out = {"iterations": st["iterations"], "seconds": st["seconds"], "converged": st["converged"],
+           "err_vs_exact_max": err,
+           "workload": "8^3=512 hex elements N=7, sine-deformed unit cube, manufactured sine "
+                       "solution, Dirichlet, Jacobi PCG to 1e-8 relative, FP64"}
+    log("plain Poisson 8^3 N=7 Jacobi PCG: %d it, %.4f s, max error vs exact %.2e" %
+        (st["iterations"], st["seconds"], err))
+    if cpu:
+        import oracle
+        oracle.build()
+        mo = oracle.BoxMesh(8, 8, 8, 7, "sine", 0.0)
+        eo = oracle.Elliptic(mo)
+        b = 3 * np.pi ** 2 * u * mo.ggeo.reshape(-1, 7, Np)[:, 6].ravel()
+        eo.solve(b, tol=1e-8, maxit=1000)                 # warm-up
+        t0 = time.perf_counter()
+        _, info = eo.solve(b, tol=1e-8, maxit=1000)
+        out["cpu_port"] = {"seconds": time.perf_counter() - t0, "iterations": info["iterations"],
+                           "cores": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)}
+        log("plain Poisson on the host (oracle, %d cores): %d it, %.3f s" %
+            (out["cpu_port"]["cores"], info["iterations"], out["cpu_port"]["seconds"]))
     return out
 
 
@@ -665,6 +712,8 @@ def main():
     ap.add_argument("--solve-cases", default="",
                     help="comma-separated names of the p-MG solves to run (default: all)")
     ap.add_argument("--no-pressure-rhs", action="store_true")
+    ap.add_argument("--no-plain-poisson", action="store_true",
+                    help="skip the 8^3 sine-deformed Jacobi-PCG solve")
     ap.add_argument("--no-kershaw", action="store_true",
                     help="skip the reference's Kershaw BPS5/BP5 benchmarks (20^3, N=7)")
     ap.add_argument("--batch-problems", type=int, default=64)
@@ -790,6 +839,9 @@ def main():
     prhs = None
     if not args.no_pressure_rhs:
         prhs = pressure_rhs_case(mesh, N, 20, 3)
+    plain = None
+    if not args.no_plain_poisson:
+        plain = plain_poisson_case(cpu=not args.no_cpu_baseline)
     kershaw = None
     if not args.no_kershaw:
         log("Kershaw BPS5 / BP5 at 20^3 ...")
@@ -839,6 +891,7 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles()},
         "cpu_baseline": cpu,
         "pressure_rhs": prhs,
+        "plain_poisson": plain,
         "solves": solves,
         "kershaw": kershaw,
         "batch": batch,

@@ -58,7 +58,11 @@ template <typename T, int OP> struct Red {
 
 constexpr int kGsBlock = 256;
 
-template <typename T, int OP, int W, bool WEIGHTED = false>
+// NV > 0: the vector count known at compile time; every vector's members are
+// loaded before any is stored (the vectors are stride >= N apart, so a row's
+// stores never touch another vector's members), which keeps NV x W loads in
+// flight instead of W.  NV = 0: a loop over nvec.
+template <typename T, int OP, int W, bool WEIGHTED, int NV>
 __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__restrict__ v,
                                              int nvec, int64_t stride)
 {
@@ -73,20 +77,42 @@ __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__r
       m[4 * c] = a.x; m[4 * c + 1] = a.y; m[4 * c + 2] = a.z; m[4 * c + 3] = a.w;
     }
   }
-  for (int k = 0; k < nvec; ++k) {
-    T *vk = v + (int64_t)k * stride;
-    T val[W];
+  if constexpr (NV > 0) {
+    T val[NV][W];
 #pragma unroll
-    for (int c = 0; c < W; ++c) val[c] = vk[m[c]];
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int c = 0; c < W; ++c) val[k][c] = v[(int64_t)k * stride + m[c]];
     if constexpr (WEIGHTED) {   // invDegree of a member of a W-row is 1/W, exact for W = 2, 4, 8
 #pragma unroll
-      for (int c = 0; c < W; ++c) val[c] = val[c] * (T(1) / T(W));
+      for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int c = 0; c < W; ++c) val[k][c] = val[k][c] * (T(1) / T(W));
     }
-    T g = Red<T, OP>::id();
 #pragma unroll
-    for (int c = 0; c < W; ++c) g = Red<T, OP>::apply(g, val[c]);
+    for (int k = 0; k < NV; ++k) {
+      T g = Red<T, OP>::id();
 #pragma unroll
-    for (int c = 0; c < W; ++c) vk[m[c]] = g;
+      for (int c = 0; c < W; ++c) g = Red<T, OP>::apply(g, val[k][c]);
+#pragma unroll
+      for (int c = 0; c < W; ++c) v[(int64_t)k * stride + m[c]] = g;
+    }
+  } else {
+    for (int k = 0; k < nvec; ++k) {
+      T *vk = v + (int64_t)k * stride;
+      T val[W];
+#pragma unroll
+      for (int c = 0; c < W; ++c) val[c] = vk[m[c]];
+      if constexpr (WEIGHTED) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) val[c] = val[c] * (T(1) / T(W));
+      }
+      T g = Red<T, OP>::id();
+#pragma unroll
+      for (int c = 0; c < W; ++c) g = Red<T, OP>::apply(g, val[c]);
+#pragma unroll
+      for (int c = 0; c < W; ++c) vk[m[c]] = g;
+    }
   }
 }
 
@@ -95,8 +121,9 @@ __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__r
 // nodes) stay in one XCD's L2 between them.  Blocks are dealt round-robin to
 // the 8 XCDs, so block b runs tile (b mod 8) * per + b / 8: each XCD walks a
 // contiguous run of tiles and the lines that neighbouring tiles share stay in
-// its L2 as well.  WEIGHTED: v <- gs_add(w .* v) (nvec = 1).
-template <typename T, int OP, bool WEIGHTED>
+// its L2 as well.  WEIGHTED: v <- gs_add(w .* v) (nvec = 1, NV = 1).  NV: the
+// vector count when fixed at compile time (1 or 3), 0 for any nvec.
+template <typename T, int OP, bool WEIGHTED, int NV>
 __global__ void __launch_bounds__(kGsBlock)
 gs_kernel(int t0, int nt, int per, const int *__restrict__ toff, int tstride,
           const int *__restrict__ pairs, const int *__restrict__ quads, const int *__restrict__ octs,
@@ -111,11 +138,11 @@ gs_kernel(int t0, int nt, int per, const int *__restrict__ toff, int tstride,
   const int s8 = tb[2 * tstride], e8 = tb[2 * tstride + 1];
   const int sx = tb[3 * tstride], ex = tb[3 * tstride + 1];
   for (int r = s2 + (int)threadIdx.x; r < e2; r += kGsBlock)
-    gs_fixed_row<T, OP, 2, WEIGHTED>(pairs + 2 * (int64_t)r, v, nvec, stride);
+    gs_fixed_row<T, OP, 2, WEIGHTED, NV>(pairs + 2 * (int64_t)r, v, nvec, stride);
   for (int r = s4 + (int)threadIdx.x; r < e4; r += kGsBlock)
-    gs_fixed_row<T, OP, 4, WEIGHTED>(quads + 4 * (int64_t)r, v, nvec, stride);
+    gs_fixed_row<T, OP, 4, WEIGHTED, NV>(quads + 4 * (int64_t)r, v, nvec, stride);
   for (int r = s8 + (int)threadIdx.x; r < e8; r += kGsBlock)
-    gs_fixed_row<T, OP, 8, WEIGHTED>(octs + 8 * (int64_t)r, v, nvec, stride);
+    gs_fixed_row<T, OP, 8, WEIGHTED, NV>(octs + 8 * (int64_t)r, v, nvec, stride);
   for (int r = sx + (int)threadIdx.x; r < ex; r += kGsBlock) {
     const int s0 = xoff[r], e0 = xoff[r + 1];
     if constexpr (WEIGHTED) {
@@ -141,9 +168,14 @@ void launch_gs(const nrs_ogs *o, int t0, int t1, const T *w, T *v, int nvec, int
   const int nt = t1 - t0;
   if (nt <= 0) return;
   const int per = (nt + 7) / 8;
-  gs_kernel<T, OP, WEIGHTED><<<8 * per, kGsBlock, 0, s>>>(
-      t0, nt, per, o->tOff.get(), o->nTiles + 1, o->pairs.get(), o->quads.get(), o->octs.get(),
-      o->xoff.get(), o->xids.get(), w, v, nvec, stride);
+  auto go = [&](auto kernel) {
+    kernel<<<8 * per, kGsBlock, 0, s>>>(t0, nt, per, o->tOff.get(), o->nTiles + 1, o->pairs.get(),
+                                        o->quads.get(), o->octs.get(), o->xoff.get(), o->xids.get(), w, v,
+                                        nvec, stride);
+  };
+  if (nvec == 1) go(gs_kernel<T, OP, WEIGHTED, 1>);
+  else if (nvec == 3 && !WEIGHTED) go(gs_kernel<T, OP, WEIGHTED, 3>);
+  else go(gs_kernel<T, OP, WEIGHTED, 0>);
   NRS_LAUNCH_CHECK();
 }
 

@@ -213,20 +213,25 @@ def test_gather_scatter_vs_gslib_golden(gpu, torch_mod, golden):
                     assert rel(host(v), ref) < (1e-13 if T == np.float64 else 1e-6)
 
 
-def test_gather_scatter_many_vectors(gpu, torch_mod, orc):
+@pytest.mark.parametrize("nvec,op", [(3, "add"), (3, "min"), (3, "mul"), (2, "add"), (4, "max")])
+def test_gather_scatter_many_vectors(gpu, torch_mod, orc, nvec, op):
+    """Several vectors stride apart in one call (gatherScatterMany): three take
+    the kernel form that loads every vector's members before storing, other
+    counts the loop over vectors; both bit for bit the oracle, padding untouched."""
     torch = torch_mod
     ids = orc.BoxMesh(3, 2, 2, 5, "kershaw", 0.3).gid
     n = ids.size
     stride = n + 100
     rng = np.random.default_rng(2)
-    v = rng.uniform(-1, 1, 3 * stride)
+    v = rng.uniform(0.5, 1.5, nvec * stride) if op == "mul" else rng.uniform(-1, 1, nvec * stride)
     og_o, og_g = orc.Ogs(ids), gpu.Ogs(ids)
-    ref = og_o.gs(v, "add", nvec=3, stride=stride)
+    ref = og_o.gs(v, op, nvec=nvec, stride=stride)
     d = dev(torch, v)
-    og_g.gather_scatter(d, "add", nvec=3, stride=stride)
+    og_g.gather_scatter(d, op, nvec=nvec, stride=stride)
     torch.cuda.synchronize()
     assert np.array_equal(host(d), ref)
-    assert np.array_equal(host(d)[n:stride], v[n:stride])   # padding untouched
+    for f in range(nvec):                                    # padding untouched
+        assert np.array_equal(host(d)[f * stride + n:(f + 1) * stride], v[f * stride + n:(f + 1) * stride])
 
 
 def test_gather_scatter_start_finish(gpu, torch_mod, orc):
