@@ -118,7 +118,8 @@ __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__r
 
 // One block per tile; the tile's pairs, quads, octs and CSR rows run one after
 // the other, so the lines they share (a face line holds face, edge and vertex
-// nodes) stay in one XCD's L2 between them.  Blocks are dealt round-robin to
+// nodes) stay in one XCD's L2 between them (dealing all of a tile's rows to the
+// threads as one list measured 3-6 % slower).  Blocks are dealt round-robin to
 // the 8 XCDs, so block b runs tile (b mod 8) * per + b / 8: each XCD walks a
 // contiguous run of tiles and the lines that neighbouring tiles share stay in
 // its L2 as well.  WEIGHTED: v <- gs_add(w .* v) (nvec = 1, NV = 1).  NV: the
@@ -179,8 +180,9 @@ void launch_gs(const nrs_ogs *o, int t0, int t1, const T *w, T *v, int nvec, int
   NRS_LAUNCH_CHECK();
 }
 
-// nodes per tile: a few hundred rows per block (N = 7: four elements, ~500 rows)
-constexpr int64_t kTileNodes = 2048;
+// nodes per tile: a few hundred rows per block (N = 7: two elements, ~250 rows;
+// 512 / 1024 / 2048 / 4096 timed at 32^3 and 48^3, 1024 the fastest by 5-8 %)
+constexpr int64_t kTileNodes = 1024;
 
 // Device layout of the buckets grouped by tile (last member / npt); tiles
 // never straddle a chunk when npt divides the chunk size.
