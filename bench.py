@@ -708,6 +708,10 @@ def main():
                     help="skip the comparison loop of the operator without face fusion")
     ap.add_argument("--operator-fusion", type=int, default=-1,
                     help="operatorFusion of the timed operator (default: the library's, timed at setup)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="set up the timed operator with the autotuner off (face fusion in blocks of "
+                         "four, default launch variants): for counter passes, where the profiler's "
+                         "per-dispatch serialisation would skew the setup's timings")
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--solve-cases", default="",
                     help="comma-separated names of the p-MG solves to run (default: all)")
@@ -758,8 +762,10 @@ def main():
     n, N = args.n, args.order
     t0 = time.perf_counter()
     mesh = nekrs_amd.BoxMesh(n, n, n, N, "kershaw", 0.3)
-    el = (nekrs_amd.Elliptic(mesh, precon="none") if args.operator_fusion < 0 else
-          nekrs_amd.Elliptic(mesh, precon="none", operatorFusion=args.operator_fusion))
+    okw = {} if args.operator_fusion < 0 else {"operatorFusion": args.operator_fusion}
+    if args.no_autotune:
+        okw["autotune"] = 0
+    el = nekrs_amd.Elliptic(mesh, precon="none", **okw)
     log("setup %.1f s: E=%d N=%d Nlocal=%d Nmasked=%d" %
         (time.perf_counter() - t0, mesh.E, N, el.Nlocal, el.Nmasked))
     # gather-scatter algorithmic bytes: rows with more than one member of the
