@@ -474,9 +474,11 @@ def box_dims(E):
     return nx, ny, nz
 
 
-def run_batch(probs, iters_per_launch, seed):
+def run_batch(probs, iters_per_launch, seed, one_graph=False):
     """Set up one graph batch, solve it (untimed, then timed), then solve the
-    same handles one after another (host-driven); returns the measurements."""
+    same handles one after another (host-driven); returns the measurements.
+    one_graph: also time the single-graph mode (all problems as branches of one
+    graph; slower on this ROCm, so left out by default)."""
     import numpy as np
     import torch
     import nekrs_amd
@@ -487,7 +489,9 @@ def run_batch(probs, iters_per_launch, seed):
     rng = np.random.default_rng(seed)
     rhs = [torch.from_numpy(rng.uniform(-1, 1, m.Nlocal)).cuda() for m, _ in probs]
     t_mode = {}
-    for mode in ("one_graph", "per_problem"):
+    modes = ("one_graph", "per_problem") if one_graph else ("per_problem",)
+    xb, st0 = None, None
+    for mode in modes:
         t0 = time.perf_counter()
         b.set_mode(mode)
         t_capture = time.perf_counter() - t0
@@ -502,15 +506,15 @@ def run_batch(probs, iters_per_launch, seed):
         info = b.info()
         t_mode[mode + "_launches"] = info["last_launches"]
         t_mode[mode + "_capture_seconds"] = t_capture
-        if mode == "one_graph":
+        if xb is None:
             xb = [t.cpu().numpy() for t in x]
             st0 = st
-        else:
+        else:   # the two modes give the same iterates
             assert [s_["iterations"] for s_ in st] == [s_["iterations"] for s_ in st0]
             assert all(np.array_equal(t.cpu().numpy(), xb[p]) for p, t in enumerate(x))
-    t_batch = min(t_mode["one_graph"], t_mode["per_problem"])
+    t_batch = min(t_mode[m] for m in modes)
     # the same handles, one problem at a time (host loop per iteration)
-    t_seq, seq_it, diff = 0.0, [], 0.0
+    t_seq, seq_it, diff, t_each = 0.0, [], 0.0, []
     for p in range(len(probs)):
         el = b.problem(p)
         r = rhs[p].clone()
@@ -519,7 +523,8 @@ def run_batch(probs, iters_per_launch, seed):
         t0 = time.perf_counter()
         s1 = el.solve(r, xs)
         torch.cuda.synchronize()
-        t_seq += time.perf_counter() - t0
+        t_each.append(time.perf_counter() - t0)
+        t_seq += t_each[-1]
         seq_it.append(s1["iterations"])
         diff = max(diff, float(np.max(np.abs(xs.cpu().numpy() - xb[p]))))
     its = [s["iterations"] for s in st]
@@ -528,6 +533,9 @@ def run_batch(probs, iters_per_launch, seed):
     del b
     return {"problems": len(probs), "setup_seconds": t_setup, "seconds_batched": t_batch,
             "seconds_sequential": t_seq, "speedup_batched_vs_sequential": t_seq / t_batch,
+            # the slowest problem solved alone: a floor for any schedule of the batch
+            "seconds_slowest_problem_alone": max(t_each),
+            "converged": [bool(s["converged"]) for s in st],
             "dof_iterations_per_s_batched": work / t_batch,
             "iterations": its, "iterations_sequential": seq_it,
             "all_converged": all(s["converged"] for s in st),
@@ -575,7 +583,7 @@ def block_velocity(m, seed):
     return out
 
 
-def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
+def batch_cases(count, seed=0, iters_per_launch=4, block_n=32, one_graph=False):
     """The uneven Helmholtz batch and the 3-component velocity Helmholtz solve."""
     import numpy as np
     import nekrs_amd
@@ -591,12 +599,15 @@ def batch_cases(count, seed=0, iters_per_launch=4, block_n=32):
                       dict(BATCH_KW, lambda0=1.0, lambda1=lam, poisson=False)))
         desc.append({"N": N, "E": nx * ny * nz, "box": [nx, ny, nz], "lambda1": lam, "eps": eps})
     log("batch: %d problems, %d elements in all" % (count, sum(d["E"] for d in desc)))
-    out = {"uneven": run_batch(probs, iters_per_launch, seed + 1)}
+    out = {"uneven": run_batch(probs, iters_per_launch, seed + 1, one_graph)}
     out["uneven"]["cases"] = desc
     u = out["uneven"]
-    log("batch: %.3f s batched (one graph %.3f s, graph per problem %.3f s) vs %.3f s one by one "
+    log("batch: slowest problem alone %.3f s; %d of %d converged within the iteration cap" %
+        (u["seconds_slowest_problem_alone"], sum(u["converged"]), len(u["converged"])))
+    log("batch: %.3f s batched (one graph %s s, graph per problem %.3f s) vs %.3f s one by one "
         "(x%.2f), %.2f G dof-iterations/s, its %d..%d" %
-        (u["seconds_batched"], u["modes"]["one_graph"], u["modes"]["per_problem"],
+        (u["seconds_batched"], ("%.3f" % u["modes"]["one_graph"]) if "one_graph" in u["modes"] else "-",
+         u["modes"]["per_problem"],
          u["seconds_sequential"], u["speedup_batched_vs_sequential"],
          u["dof_iterations_per_s_batched"] / 1e9, min(u["iterations"]), max(u["iterations"])))
     del probs
@@ -723,6 +734,8 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--batch-iters-per-launch", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-one-graph", action="store_true",
+                    help="also time the batch's single-graph mode (slower; tested, not timed by default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 512 MiB (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -860,7 +873,8 @@ def main():
     batch = None
     if not args.no_batch:
         log("batched solves ...")
-        batch = batch_cases(args.batch_problems, 0, args.batch_iters_per_launch, n)
+        batch = batch_cases(args.batch_problems, 0, args.batch_iters_per_launch, n,
+                            one_graph=args.batch_one_graph)
 
     cpu = None
     if not args.no_cpu_baseline:
