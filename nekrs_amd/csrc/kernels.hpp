@@ -44,7 +44,8 @@ void launch_geofacs(int N, int E, const double *D, const double *w, const double
 
 // transfer.hip
 template <typename T>
-void launch_coarsen(int Nf, int Nc, int E, const T *R, const T *qf, T *qc, hipStream_t s);
+void launch_coarsen(int Nf, int Nc, int E, const T *R, const T *qf, T *qc, hipStream_t s,
+                    const T *w = nullptr);   // w: qf .* w contracted (qf left unchanged)
 template <typename T>
 void launch_prolongate(int Nf, int Nc, int E, const T *R, const T *qc, T *qf, hipStream_t s);
 

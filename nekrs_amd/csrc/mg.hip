@@ -155,6 +155,46 @@ __global__ void cheby1_kernel(int64_t n, T dC, T rC, const T *__restrict__ SAd, 
   }
 }
 
+// The same update with the Jacobi smoother applied on the fly: SAd = invDiag .*
+// Ad, the product rounded on its own (no contraction into the subtraction), so
+// the result equals a separate invDiag .* Ad pass followed by cheby1_kernel.
+template <typename T>
+__global__ void cheby1_jacobi_kernel(int64_t n, T dC, T rC, const T *__restrict__ invDiag,
+                                     const T *__restrict__ Ad, T *__restrict__ d, T *__restrict__ r,
+                                     T *__restrict__ x)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T sad;
+    {
+#pragma clang fp contract(off)
+      sad = invDiag[i] * Ad[i];
+    }
+    const T dn = d[i];
+    const T rn = r[i] - sad;
+    x[i] = x[i] + dn;
+    d[i] = dC * dn + rC * rn;
+    r[i] = rn;
+  }
+}
+
+// First step of the Jacobi-smoothed Chebyshev: r = invDiag .* res, d = invTheta r
+// (a separate invDiag .* res pass and d = invTheta r, fused; res may be the
+// right-hand side itself when x = 0).
+template <typename T>
+__global__ void jacobi_first_kernel(int64_t n, T invTheta, const T *__restrict__ invDiag,
+                                    const T *__restrict__ res, T *__restrict__ r, T *__restrict__ d)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T ri;
+    {
+#pragma clang fp contract(off)
+      ri = invDiag[i] * res[i];
+    }
+    r[i] = ri;
+    d[i] = invTheta * ri;
+  }
+}
+
 // updateFourthKindChebyshev (kernels/elliptic/updateFourthKindChebyshev.okl:5-25)
 template <typename T>
 __global__ void cheby4_kernel(int64_t n, T beta, const T *__restrict__ Ad, const T *__restrict__ d,
@@ -595,25 +635,38 @@ struct MGImpl final : MGSolver {
     const T sigma = theta / delta;
     T rho_n = T(1.0 / (double)sigma);
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get(), *spare = sTmp.get();
+    // Jacobi smoothing (no Schwarz): S is applied inside the update kernels
+    const bool jac = !l.schwarz;
     if (xIsZero) {
       fill<T>(l.L, T(0), x, s);
-      k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
+      if (!jac) k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
     } else {
       op(l, x, res, s);
       k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
     }
-    smootherS(l, res, spare, s);
-    std::swap(res, spare);
-    k_axpby<T>(l.L, invTheta, res, T(0), d, s);
+    if (jac) {
+      jacobi_first_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, invTheta, l.invDiagA.get(),
+                                                       xIsZero ? rhs : res, spare, d);
+      NRS_LAUNCH_CHECK();
+      std::swap(res, spare);
+    } else {
+      smootherS(l, res, spare, s);
+      std::swap(res, spare);
+      k_axpby<T>(l.L, invTheta, res, T(0), d, s);
+    }
     for (int k = 1; k < deg; ++k) {
       op(l, d, Ad, s);
-      smootherS(l, Ad, spare, s);
-      std::swap(Ad, spare);
       const T rhoSave = rho_n;
       rho_n = T(1.0 / (2.0 * (double)sigma - (double)rho_n));
       const T rC = T(2.0 * (double)rho_n / (double)delta);
       const T dC = rho_n * rhoSave;
-      cheby1_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, dC, rC, Ad, d, res, x);
+      if (jac) {
+        cheby1_jacobi_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, dC, rC, l.invDiagA.get(), Ad, d, res, x);
+      } else {
+        smootherS(l, Ad, spare, s);
+        std::swap(Ad, spare);
+        cheby1_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, dC, rC, Ad, d, res, x);
+      }
       NRS_LAUNCH_CHECK();
     }
     k_axpby<T>(l.L, T(1), d, T(1), x, s);
@@ -677,9 +730,9 @@ struct MGImpl final : MGSolver {
     // residual (ellipticMultiGridLevel.cpp:37-41)
     op(l, l.x, l.res.get(), s);
     k_axpby<T>(l.L, T(1), l.rhs, T(-1), l.res.get(), s);
-    // coarsen (:43-68): res .*= invDegreeFine, contract, gs, mask
-    k_wmul<T>(l.L, l.invDegree.get(), l.res.get(), l.res.get(), s);
-    launch_coarsen<T>(l.N, c.N, l.E, c.R.get(), l.res.get(), c.rhs, s);
+    // coarsen (:43-68): res .*= invDegreeFine (applied as the contraction loads
+    // res; res itself is not needed afterwards), contract, gs, mask
+    launch_coarsen<T>(l.N, c.N, l.E, c.R.get(), l.res.get(), c.rhs, s, l.invDegree.get());
     ogs_gather_scatter<T>(c.ogs, c.rhs, 1, c.L, NRS_ADD, s);
     launch_mask<T>(c.Nmasked, c.maskIds, c.rhs, s);
     vcycle(k + 1, s);
