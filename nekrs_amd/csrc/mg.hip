@@ -141,54 +141,51 @@ namespace {
 constexpr int KB = 256;
 inline unsigned kgrid(int64_t n) { return grid_for(n, KB, 16384); }
 
-// updateChebyshev (kernels/elliptic/updateChebyshev.okl:7-30)
-template <typename T>
-__global__ void cheby1_kernel(int64_t n, T dC, T rC, const T *__restrict__ SAd, T *__restrict__ d,
-                              T *__restrict__ r, T *__restrict__ x)
-{
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const T dn = d[i];
-    const T rn = r[i] - SAd[i];
-    x[i] = x[i] + dn;
-    d[i] = dC * dn + rC * rn;
-    r[i] = rn;
-  }
-}
-
-// The same update with the Jacobi smoother applied on the fly: SAd = invDiag .*
-// Ad, the product rounded on its own (no contraction into the subtraction), so
-// the result equals a separate invDiag .* Ad pass followed by cheby1_kernel.
-template <typename T>
-__global__ void cheby1_jacobi_kernel(int64_t n, T dC, T rC, const T *__restrict__ invDiag,
-                                     const T *__restrict__ Ad, T *__restrict__ d, T *__restrict__ r,
-                                     T *__restrict__ x)
+// updateChebyshev (kernels/elliptic/updateChebyshev.okl:7-30).  JAC: the Jacobi
+// smoother applied on the fly, SAd = w .* Ad with the product rounded on its own
+// (no contraction into the subtraction), equal to a separate w .* Ad pass.
+// LAST: the last step of the smoothing, fused with the x += d that follows it:
+// x = (x + d) + d_new in that order, and d, r (no longer needed) not stored.
+template <typename T, bool JAC, bool LAST>
+__global__ void cheby1_kernel(int64_t n, T dC, T rC, const T *__restrict__ w, const T *__restrict__ Ad,
+                              T *__restrict__ d, T *__restrict__ r, T *__restrict__ x)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     T sad;
-    {
+    if constexpr (JAC) {
 #pragma clang fp contract(off)
-      sad = invDiag[i] * Ad[i];
+      sad = w[i] * Ad[i];
+    } else {
+      sad = Ad[i];
     }
     const T dn = d[i];
     const T rn = r[i] - sad;
-    x[i] = x[i] + dn;
-    d[i] = dC * dn + rC * rn;
-    r[i] = rn;
+    const T x1 = x[i] + dn;
+    const T dnew = dC * dn + rC * rn;
+    if constexpr (LAST) {
+      x[i] = x1 + dnew;
+    } else {
+      x[i] = x1;
+      d[i] = dnew;
+      r[i] = rn;
+    }
   }
 }
 
-// First step of the Jacobi-smoothed Chebyshev: r = invDiag .* res, d = invTheta r
-// (a separate invDiag .* res pass and d = invTheta r, fused; res may be the
-// right-hand side itself when x = 0).
-template <typename T>
-__global__ void jacobi_first_kernel(int64_t n, T invTheta, const T *__restrict__ invDiag,
-                                    const T *__restrict__ res, T *__restrict__ r, T *__restrict__ d)
+// First step of the Jacobi-smoothed Chebyshev: r = w .* t, d = invTheta r with
+// t = rhs - res (SUB, after the operator wrote A x into res) or t = rhs (x = 0);
+// the same roundings as the separate passes.
+template <typename T, bool SUB>
+__global__ void jacobi_first_kernel(int64_t n, T invTheta, const T *__restrict__ w,
+                                    const T *__restrict__ rhs, const T *__restrict__ res,
+                                    T *__restrict__ r, T *__restrict__ d)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T t = SUB ? rhs[i] - res[i] : rhs[i];
     T ri;
     {
 #pragma clang fp contract(off)
-      ri = invDiag[i] * res[i];
+      ri = w[i] * t;
     }
     r[i] = ri;
     d[i] = invTheta * ri;
@@ -196,13 +193,28 @@ __global__ void jacobi_first_kernel(int64_t n, T invTheta, const T *__restrict__
 }
 
 // updateFourthKindChebyshev (kernels/elliptic/updateFourthKindChebyshev.okl:5-25)
-template <typename T>
+// X0: x is zero on entry (not read), and r is read from rin (the right-hand
+// side) and written to r.
+template <typename T, bool X0>
 __global__ void cheby4_kernel(int64_t n, T beta, const T *__restrict__ Ad, const T *__restrict__ d,
-                              T *__restrict__ r, T *__restrict__ x)
+                              const T *__restrict__ rin, T *__restrict__ r, T *__restrict__ x)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] = x[i] + beta * d[i];
-    r[i] = r[i] - Ad[i];
+    x[i] = (X0 ? T(0) : x[i]) + beta * d[i];
+    r[i] = (X0 ? rin[i] : r[i]) - Ad[i];
+  }
+}
+
+// The last step of the 4th-kind smoothing fused with the final update:
+// d' = rC Ad + dC d, x = beta d' + x, as the two axpby passes form them
+// (d' not stored).
+template <typename T>
+__global__ void cheby4_last_kernel(int64_t n, T rC, T dC, T beta, const T *__restrict__ Ad,
+                                   const T *__restrict__ d, T *__restrict__ x)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T dp = rC * Ad[i] + dC * d[i];
+    x[i] = beta * dp + T(1) * x[i];
   }
 }
 
@@ -635,18 +647,23 @@ struct MGImpl final : MGSolver {
     const T sigma = theta / delta;
     T rho_n = T(1.0 / (double)sigma);
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get(), *spare = sTmp.get();
-    // Jacobi smoothing (no Schwarz): S is applied inside the update kernels
+    // Jacobi smoothing (no Schwarz): S is applied inside the update kernels, and
+    // the residual rhs - A x inside the first of them
     const bool jac = !l.schwarz;
     if (xIsZero) {
       fill<T>(l.L, T(0), x, s);
       if (!jac) k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
     } else {
       op(l, x, res, s);
-      k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
+      if (!jac) k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
     }
     if (jac) {
-      jacobi_first_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, invTheta, l.invDiagA.get(),
-                                                       xIsZero ? rhs : res, spare, d);
+      if (xIsZero)
+        jacobi_first_kernel<T, false><<<kgrid(l.L), KB, 0, s>>>(l.L, invTheta, l.invDiagA.get(), rhs,
+                                                                nullptr, spare, d);
+      else
+        jacobi_first_kernel<T, true><<<kgrid(l.L), KB, 0, s>>>(l.L, invTheta, l.invDiagA.get(), rhs, res,
+                                                               spare, d);
       NRS_LAUNCH_CHECK();
       std::swap(res, spare);
     } else {
@@ -660,16 +677,20 @@ struct MGImpl final : MGSolver {
       rho_n = T(1.0 / (2.0 * (double)sigma - (double)rho_n));
       const T rC = T(2.0 * (double)rho_n / (double)delta);
       const T dC = rho_n * rhoSave;
-      if (jac) {
-        cheby1_jacobi_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, dC, rC, l.invDiagA.get(), Ad, d, res, x);
-      } else {
+      const bool last = k + 1 == deg;   // fused with the final x += d
+      if (!jac) {
         smootherS(l, Ad, spare, s);
         std::swap(Ad, spare);
-        cheby1_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, dC, rC, Ad, d, res, x);
       }
+      const T *w = l.invDiagA.get();
+      const unsigned g = kgrid(l.L);
+      if (jac && last) cheby1_kernel<T, true, true><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
+      else if (jac) cheby1_kernel<T, true, false><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
+      else if (last) cheby1_kernel<T, false, true><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
+      else cheby1_kernel<T, false, false><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
       NRS_LAUNCH_CHECK();
     }
-    k_axpby<T>(l.L, T(1), d, T(1), x, s);
+    if (deg == 1) k_axpby<T>(l.L, T(1), d, T(1), x, s);
     launch_mask<T>(l.Nmasked, l.maskIds, x, s);
   }
 
@@ -681,26 +702,39 @@ struct MGImpl final : MGSolver {
     const auto &betas = xIsZero ? l.betasDown : l.betasUp;
     const double rho = l.lmax;
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get();
+    // x = 0 with at least one update step: the first step treats x as zero and
+    // takes the residual from rhs, so neither is written beforehand
+    const bool x0 = xIsZero && deg > 1;
     if (xIsZero) {
-      fill<T>(l.L, T(0), x, s);
-      k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
+      if (!x0) {
+        fill<T>(l.L, T(0), x, s);
+        k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
+      }
     } else {
       op(l, x, res, s);
       k_axpby<T>(l.L, T(1), rhs, T(-1), res, s);
     }
-    smootherS(l, res, Ad, s);
+    smootherS(l, x0 ? rhs : res, Ad, s);
     const T coeff = T(4.0 / (3.0 * rho));
     k_axpby<T>(l.L, coeff, Ad, T(0), d, s);
     for (int k = 1; k < deg; ++k) {
       op(l, d, Ad, s);
-      cheby4_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, betas[k - 1], Ad, d, res, x);
+      if (k == 1 && x0)
+        cheby4_kernel<T, true><<<kgrid(l.L), KB, 0, s>>>(l.L, betas[k - 1], Ad, d, rhs, res, x);
+      else
+        cheby4_kernel<T, false><<<kgrid(l.L), KB, 0, s>>>(l.L, betas[k - 1], Ad, d, res, res, x);
       NRS_LAUNCH_CHECK();
       smootherS(l, res, Ad, s);
       const T dC = T((2.0 * k - 1.0) / (2.0 * k + 3.0));
       const T rC = T((8.0 * k + 4.0) / ((2.0 * k + 3.0) * rho));
-      k_axpby<T>(l.L, rC, Ad, dC, d, s);
+      if (k + 1 == deg) {   // the last step, fused with the final x += beta d
+        cheby4_last_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, rC, dC, betas.back(), Ad, d, x);
+        NRS_LAUNCH_CHECK();
+      } else {
+        k_axpby<T>(l.L, rC, Ad, dC, d, s);
+      }
     }
-    k_axpby<T>(l.L, betas.back(), d, T(1), x, s);
+    if (deg == 1) k_axpby<T>(l.L, betas.back(), d, T(1), x, s);
     launch_mask<T>(l.Nmasked, l.maskIds, x, s);
   }
 
@@ -727,12 +761,11 @@ struct MGImpl final : MGSolver {
     }
     Level<T> &c = *levels[k + 1];
     smooth(l, l.rhs, l.x, true, s);
-    // residual (ellipticMultiGridLevel.cpp:37-41)
+    // residual (ellipticMultiGridLevel.cpp:37-41) and coarsen (:43-68):
+    // invDegreeFine .* (rhs - A x) formed as the contraction loads A x (the
+    // residual itself is not needed afterwards), contract, gs, mask
     op(l, l.x, l.res.get(), s);
-    k_axpby<T>(l.L, T(1), l.rhs, T(-1), l.res.get(), s);
-    // coarsen (:43-68): res .*= invDegreeFine (applied as the contraction loads
-    // res; res itself is not needed afterwards), contract, gs, mask
-    launch_coarsen<T>(l.N, c.N, l.E, c.R.get(), l.res.get(), c.rhs, s, l.invDegree.get());
+    launch_coarsen<T>(l.N, c.N, l.E, c.R.get(), l.res.get(), c.rhs, s, l.invDegree.get(), l.rhs);
     ogs_gather_scatter<T>(c.ogs, c.rhs, 1, c.L, NRS_ADD, s);
     launch_mask<T>(c.Nmasked, c.maskIds, c.rhs, s);
     vcycle(k + 1, s);

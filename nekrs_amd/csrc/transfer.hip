@@ -14,13 +14,15 @@ namespace {
 constexpr int kMaxNq = 10;
 constexpr int kTB = 256;
 
-// out = (A x A x A) (w .* in), A is [NO][NI] given through (a_stride_o,
-// a_stride_i) into M; w = null: no weight.  The weighted input is rounded on
-// its own, as a separate w .* in pass would leave it.
+// out = (A x A x A) (w .* (sub - in)), A is [NO][NI] given through
+// (a_stride_o, a_stride_i) into M; w = null: no weight; sub = null: in itself.
+// The difference and the weighted input are rounded on their own, as separate
+// passes would leave them.
 template <typename T, bool ACCUM>
 __global__ void __launch_bounds__(kTB)
 tensor3_kernel(int E, int NI, int NO, const T *__restrict__ M, int so, int si,
-               const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ w)
+               const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ w,
+               const T *__restrict__ sub)
 {
   __shared__ T s_A[kMaxNq * kMaxNq];
   __shared__ T s_in[kMaxNq * kMaxNq * kMaxNq];
@@ -32,7 +34,12 @@ tensor3_kernel(int E, int NI, int NO, const T *__restrict__ M, int so, int si,
     const int o = t / NI, i = t % NI;
     s_A[o * NI + i] = M[o * so + i * si];
   }
-  if (w) {
+  if (w && sub) {
+    for (int t = threadIdx.x; t < NPI; t += kTB) {
+      const T r = sub[e * NPI + t] - in[e * NPI + t];
+      s_in[t] = w[e * NPI + t] * r;
+    }
+  } else if (w) {
     for (int t = threadIdx.x; t < NPI; t += kTB) s_in[t] = w[e * NPI + t] * in[e * NPI + t];
   } else {
     for (int t = threadIdx.x; t < NPI; t += kTB) s_in[t] = in[e * NPI + t];
@@ -66,13 +73,15 @@ tensor3_kernel(int E, int NI, int NO, const T *__restrict__ M, int so, int si,
 }  // namespace
 
 template <typename T>
-void launch_coarsen(int Nf, int Nc, int E, const T *R, const T *qf, T *qc, hipStream_t s, const T *w)
+void launch_coarsen(int Nf, int Nc, int E, const T *R, const T *qf, T *qc, hipStream_t s, const T *w,
+                    const T *sub)
 {
+  NRS_CHECK(!sub || w, "coarsen: a subtrahend needs the weight");
   NRS_CHECK(Nf + 1 <= kMaxNq && Nc >= 1 && Nc < Nf, "coarsen: need 1 <= Nc < Nf <= 9");
   if (E <= 0) return;
   const int NqF = Nf + 1, NqC = Nc + 1;
   // A[o][i] = R[o][i] = R[o*NqF + i]
-  tensor3_kernel<T, false><<<E, kTB, 0, s>>>(E, NqF, NqC, R, NqF, 1, qf, qc, w);
+  tensor3_kernel<T, false><<<E, kTB, 0, s>>>(E, NqF, NqC, R, NqF, 1, qf, qc, w, sub);
   NRS_LAUNCH_CHECK();
 }
 
@@ -83,14 +92,14 @@ void launch_prolongate(int Nf, int Nc, int E, const T *R, const T *qc, T *qf, hi
   if (E <= 0) return;
   const int NqF = Nf + 1, NqC = Nc + 1;
   // A[o][i] = R^T[o][i] = R[i][o] = R[i*NqF + o]
-  tensor3_kernel<T, true><<<E, kTB, 0, s>>>(E, NqC, NqF, R, 1, NqF, qc, qf, nullptr);
+  tensor3_kernel<T, true><<<E, kTB, 0, s>>>(E, NqC, NqF, R, 1, NqF, qc, qf, nullptr, nullptr);
   NRS_LAUNCH_CHECK();
 }
 
 template void launch_coarsen<double>(int, int, int, const double *, const double *, double *, hipStream_t,
-                                     const double *);
+                                     const double *, const double *);
 template void launch_coarsen<float>(int, int, int, const float *, const float *, float *, hipStream_t,
-                                    const float *);
+                                    const float *, const float *);
 template void launch_prolongate<double>(int, int, int, const double *, const double *, double *, hipStream_t);
 template void launch_prolongate<float>(int, int, int, const float *, const float *, float *, hipStream_t);
 
