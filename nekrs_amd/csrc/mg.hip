@@ -146,7 +146,8 @@ inline unsigned kgrid(int64_t n) { return grid_for(n, KB, 16384); }
 // (no contraction into the subtraction), equal to a separate w .* Ad pass.
 // LAST: the last step of the smoothing, fused with the x += d that follows it:
 // x = (x + d) + d_new in that order, and d, r (no longer needed) not stored.
-template <typename T, bool JAC, bool LAST>
+// X0: x is zero on entry and not read (the first step of a zero start).
+template <typename T, bool JAC, bool LAST, bool X0 = false>
 __global__ void cheby1_kernel(int64_t n, T dC, T rC, const T *__restrict__ w, const T *__restrict__ Ad,
                               T *__restrict__ d, T *__restrict__ r, T *__restrict__ x)
 {
@@ -160,7 +161,7 @@ __global__ void cheby1_kernel(int64_t n, T dC, T rC, const T *__restrict__ w, co
     }
     const T dn = d[i];
     const T rn = r[i] - sad;
-    const T x1 = x[i] + dn;
+    const T x1 = (X0 ? T(0) : x[i]) + dn;
     const T dnew = dC * dn + rC * rn;
     if constexpr (LAST) {
       x[i] = x1 + dnew;
@@ -650,8 +651,10 @@ struct MGImpl final : MGSolver {
     // Jacobi smoothing (no Schwarz): S is applied inside the update kernels, and
     // the residual rhs - A x inside the first of them
     const bool jac = !l.schwarz;
+    // x = 0 with at least one update step: that step takes x as zero (no fill)
+    const bool x0 = xIsZero && deg > 1;
     if (xIsZero) {
-      fill<T>(l.L, T(0), x, s);
+      if (!x0) fill<T>(l.L, T(0), x, s);
       if (!jac) k_axpby<T>(l.L, T(1), rhs, T(0), res, s);   // res = rhs
     } else {
       op(l, x, res, s);
@@ -684,10 +687,17 @@ struct MGImpl final : MGSolver {
       }
       const T *w = l.invDiagA.get();
       const unsigned g = kgrid(l.L);
-      if (jac && last) cheby1_kernel<T, true, true><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
-      else if (jac) cheby1_kernel<T, true, false><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
-      else if (last) cheby1_kernel<T, false, true><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
-      else cheby1_kernel<T, false, false><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);
+      const bool first0 = x0 && k == 1;
+#define NRS_CHEBY1(J_, L_)                                                                          \
+  do {                                                                                            \
+    if (first0) cheby1_kernel<T, J_, L_, true><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);    \
+    else cheby1_kernel<T, J_, L_, false><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);          \
+  } while (0)
+      if (jac && last) NRS_CHEBY1(true, true);
+      else if (jac) NRS_CHEBY1(true, false);
+      else if (last) NRS_CHEBY1(false, true);
+      else NRS_CHEBY1(false, false);
+#undef NRS_CHEBY1
       NRS_LAUNCH_CHECK();
     }
     if (deg == 1) k_axpby<T>(l.L, T(1), d, T(1), x, s);
