@@ -194,28 +194,36 @@ __global__ void jacobi_first_kernel(int64_t n, T invTheta, const T *__restrict__
 }
 
 // updateFourthKindChebyshev (kernels/elliptic/updateFourthKindChebyshev.okl:5-25)
-// X0: x is zero on entry (not read), and r is read from rin (the right-hand
-// side) and written to r.
+// split in two: its residual update r -= A d runs before the smoother (which
+// reads r), and its x += beta d is deferred into the direction update after
+// the smoother, which reads d anyway.  X0: r = rin - A d with rin the
+// right-hand side (a zero start, where r is not yet formed).
 template <typename T, bool X0>
-__global__ void cheby4_kernel(int64_t n, T beta, const T *__restrict__ Ad, const T *__restrict__ d,
-                              const T *__restrict__ rin, T *__restrict__ r, T *__restrict__ x)
+__global__ void cheby4_res_kernel(int64_t n, const T *__restrict__ Ad, const T *__restrict__ rin,
+                                  T *__restrict__ r)
 {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] = (X0 ? T(0) : x[i]) + beta * d[i];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     r[i] = (X0 ? rin[i] : r[i]) - Ad[i];
-  }
 }
 
-// The last step of the 4th-kind smoothing fused with the final update:
-// d' = rC Ad + dC d, x = beta d' + x, as the two axpby passes form them
-// (d' not stored).
-template <typename T>
-__global__ void cheby4_last_kernel(int64_t n, T rC, T dC, T beta, const T *__restrict__ Ad,
-                                   const T *__restrict__ d, T *__restrict__ x)
+// After the smoother: x = x + beta d (the deferred half of the update; X0: x
+// taken as zero, not read), then d = rC SAd + dC d -- or, on the last step
+// (LAST), x = beta_last (rC SAd + dC d) + x with d not stored: the roundings
+// of the separate update and axpby passes, in their order.
+template <typename T, bool X0, bool LAST>
+__global__ void cheby4_dir_kernel(int64_t n, T beta, T rC, T dC, T betaLast, const T *__restrict__ SAd,
+                                  T *__restrict__ d, T *__restrict__ x)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const T dp = rC * Ad[i] + dC * d[i];
-    x[i] = beta * dp + T(1) * x[i];
+    const T dn = d[i];
+    const T x1 = (X0 ? T(0) : x[i]) + beta * dn;
+    const T dp = rC * SAd[i] + dC * dn;
+    if constexpr (LAST) {
+      x[i] = betaLast * dp + T(1) * x1;
+    } else {
+      x[i] = x1;
+      d[i] = dp;
+    }
   }
 }
 
@@ -729,20 +737,22 @@ struct MGImpl final : MGSolver {
     k_axpby<T>(l.L, coeff, Ad, T(0), d, s);
     for (int k = 1; k < deg; ++k) {
       op(l, d, Ad, s);
-      if (k == 1 && x0)
-        cheby4_kernel<T, true><<<kgrid(l.L), KB, 0, s>>>(l.L, betas[k - 1], Ad, d, rhs, res, x);
-      else
-        cheby4_kernel<T, false><<<kgrid(l.L), KB, 0, s>>>(l.L, betas[k - 1], Ad, d, res, res, x);
+      const bool first0 = k == 1 && x0, last = k + 1 == deg;
+      const unsigned g = kgrid(l.L);
+      if (first0) cheby4_res_kernel<T, true><<<g, KB, 0, s>>>(l.L, Ad, rhs, res);
+      else cheby4_res_kernel<T, false><<<g, KB, 0, s>>>(l.L, Ad, res, res);
       NRS_LAUNCH_CHECK();
       smootherS(l, res, Ad, s);
       const T dC = T((2.0 * k - 1.0) / (2.0 * k + 3.0));
       const T rC = T((8.0 * k + 4.0) / ((2.0 * k + 3.0) * rho));
-      if (k + 1 == deg) {   // the last step, fused with the final x += beta d
-        cheby4_last_kernel<T><<<kgrid(l.L), KB, 0, s>>>(l.L, rC, dC, betas.back(), Ad, d, x);
-        NRS_LAUNCH_CHECK();
-      } else {
-        k_axpby<T>(l.L, rC, Ad, dC, d, s);
-      }
+      const T b = betas[k - 1], bl = betas.back();
+      // x += beta_(k-1) d deferred to here, with the direction update (and, on
+      // the last step, the final x += beta_last d)
+      if (first0 && last) cheby4_dir_kernel<T, true, true><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
+      else if (first0) cheby4_dir_kernel<T, true, false><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
+      else if (last) cheby4_dir_kernel<T, false, true><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
+      else cheby4_dir_kernel<T, false, false><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
+      NRS_LAUNCH_CHECK();
     }
     if (deg == 1) k_axpby<T>(l.L, betas.back(), d, T(1), x, s);
     launch_mask<T>(l.Nmasked, l.maskIds, x, s);
