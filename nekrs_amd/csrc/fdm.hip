@@ -370,7 +370,8 @@ template <int NQ> struct FdmCfg {
 template <typename T, int NQ, bool ASM>
 __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
     int E, const T *__restrict__ u, const T *__restrict__ S, const double *__restrict__ lam,
-    double2 helm, const int4 *__restrict__ gmap, T *__restrict__ out, T *__restrict__ gout)
+    double2 helm, const int4 *__restrict__ gmap, T *__restrict__ out, T *__restrict__ gout,
+    const uint32_t *__restrict__ mbits)
 {
   using C = FdmCfg<NQ>;
   constexpr int NL = C::NL, NLP = C::NLP, NP = C::NP, NQ2 = NQ * NQ, TPE = C::TPE;
@@ -505,10 +506,12 @@ __global__ __launch_bounds__(FdmCfg<NQ>::BLOCK) void fdm_kernel(
   const bool ai = a >= 1 && a <= NL - 2, bi = b >= 1 && b <= NL - 2;
   const int64_t eb = (int64_t)e * NP;
   if (ai && bi) {
+    // RAS writes Su itself: its masked nodes (in no gather-scatter row) as zero
+    const uint32_t *mb = (!ASM && mbits) ? mbits + (int64_t)e * ((NP + 31) / 32) : nullptr;
 #pragma unroll
     for (int k = 1; k <= NL - 2; ++k) {
-      const int64_t idx = eb + ((k - 1) * NQ + (a - 1)) * NQ + (b - 1);
-      out[idx] = o[k];
+      const int n = ((k - 1) * NQ + (a - 1)) * NQ + (b - 1);
+      out[eb + n] = (mb && ((mb[n >> 5] >> (n & 31)) & 1u)) ? T(0) : o[k];
     }
   }
   if (!ASM) return;
@@ -537,7 +540,8 @@ template <typename T, int NQ>
 __global__ __launch_bounds__(256) void fdm_post_kernel(int E, const T *__restrict__ box,
                                                        const T *__restrict__ gout,
                                                        const int4 *__restrict__ smap,
-                                                       const T *__restrict__ wts, T *__restrict__ Su)
+                                                       const T *__restrict__ wts, T *__restrict__ Su,
+                                                       const uint32_t *__restrict__ mbits)
 {
   constexpr int NP = NQ * NQ * NQ, NQ2 = NQ * NQ;
   const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -570,8 +574,13 @@ __global__ __launch_bounds__(256) void fdm_post_kernel(int E, const T *__restric
     for (int i = 0; i < NQ; ++i) v[i] += ov(3, i, k);
   v[1] += ov(4, j, k);
   v[NQ - 2] += ov(2, j, k);
+  // masked nodes (in no gather-scatter row) stored as zero
+  const uint32_t *mb = mbits ? mbits + e * ((NP + 31) / 32) : nullptr;
 #pragma unroll
-  for (int i = 0; i < NQ; ++i) Su[base + i] = v[i] * wts[base + i];
+  for (int i = 0; i < NQ; ++i) {
+    const int n = r * NQ + i;
+    Su[base + i] = (mb && ((mb[n >> 5] >> (n & 31)) & 1u)) ? T(0) : v[i] * wts[base + i];
+  }
 }
 
 // overlap counts behind the ASM weights (generate_weights, :798-835 before
@@ -992,23 +1001,23 @@ void launch_fdm_mfma_nq(bool ras, int E, const float *u, const uint16_t *S16, co
 
 template <typename T, int NQ>
 void launch_fdm_nq(bool ras, int E, const T *u, const T *S, const double *lam, double2 helm, const int4 *gmap,
-                   T *out, T *gout, hipStream_t s)
+                   T *out, T *gout, const uint32_t *mbits, hipStream_t s)
 {
   using C = FdmCfg<NQ>;
   const unsigned grid = (unsigned)((E + C::EPB - 1) / C::EPB);
   if (ras)
-    fdm_kernel<T, NQ, false><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, helm, gmap, out, gout);
+    fdm_kernel<T, NQ, false><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, helm, gmap, out, gout, mbits);
   else
-    fdm_kernel<T, NQ, true><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, helm, gmap, out, gout);
+    fdm_kernel<T, NQ, true><<<grid, C::BLOCK, 0, s>>>(E, u, S, lam, helm, gmap, out, gout, mbits);
   NRS_LAUNCH_CHECK();
 }
 
 template <typename T, int NQ>
 void launch_post_nq(int E, const T *box, const T *gout, const int4 *smap, const T *wts, T *Su,
-                    hipStream_t s)
+                    const uint32_t *mbits, hipStream_t s)
 {
   const int64_t rows = (int64_t)E * NQ * NQ;
-  fdm_post_kernel<T, NQ><<<(unsigned)((rows + 255) / 256), 256, 0, s>>>(E, box, gout, smap, wts, Su);
+  fdm_post_kernel<T, NQ><<<(unsigned)((rows + 255) / 256), 256, 0, s>>>(E, box, gout, smap, wts, Su, mbits);
   NRS_LAUNCH_CHECK();
 }
 
@@ -1017,8 +1026,8 @@ void launch_post_nq(int E, const T *box, const T *gout, const int4 *smap, const 
 template <typename T>
 void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<double> &lengths,
                        const int *EToB, const std::vector<FaceLink> &lk, const nrs_ogs *ogs_,
-                       const T *invDegree_, int64_t Nmasked_, const int *maskIds_, bool poisson,
-                       double lambda0, double lambda1, hipStream_t s)
+                       const T *invDegree_, int64_t Nmasked_, const int *maskIds_,
+                       const uint32_t *maskBits_, bool poisson, double lambda0, double lambda1, hipStream_t s)
 {
   N = N_; E = E_; ras = ras_; fdmPrecision = prec;
   NRS_CHECK(N >= 1 && N <= 9, "schwarz: orders 1..9 are built");
@@ -1026,6 +1035,7 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
             "schwarz: BF16/FP16 contractions need the FP32 multigrid");
   Nq = N + 1; Np = Nq * Nq * Nq; nl = N + 3;
   ogs = ogs_; invDegree = invDegree_; Nmasked = Nmasked_; maskIds = maskIds_;
+  maskBits = Nmasked > 0 ? maskBits_ : nullptr;
   std::vector<double> r, wq;
   gll_nodes(N, r, wq);
   const auto D = dmatrix(r);
@@ -1152,7 +1162,7 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
       fdm_mfma_boxes(Nq, fdmPrecision, ras, E, u, S16.get(), lam32.get(), helm32(), gmap.get(), out, gout, epw, s);
       if (!ras) {
 #define NRS_FDM_POST_CASE(NQ_)                                                                 \
-  case NQ_: launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); break;
+  case NQ_: launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, maskBits, s); break;
         switch (Nq) {
           NRS_FDM_POST_CASE(2) NRS_FDM_POST_CASE(3) NRS_FDM_POST_CASE(4) NRS_FDM_POST_CASE(5)
           NRS_FDM_POST_CASE(6) NRS_FDM_POST_CASE(7) NRS_FDM_POST_CASE(8) NRS_FDM_POST_CASE(9)
@@ -1161,14 +1171,15 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
         }
 #undef NRS_FDM_POST_CASE
       }
-      finish(Su, s);
+      finish(Su, !ras && maskBits, s);   // the matrix-core RAS kernel stores no mask
       return;
     }
   }
 #define NRS_FDM_CASE(NQ_)                                                                      \
   case NQ_:                                                                                    \
-    launch_fdm_nq<T, NQ_>(ras, E, u, S.get(), lam.get(), make_double2(helm0, helm1), gmap.get(), out, gout, s);           \
-    if (!ras) launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, s); \
+    launch_fdm_nq<T, NQ_>(ras, E, u, S.get(), lam.get(), make_double2(helm0, helm1), gmap.get(), out, gout,       \
+                          maskBits, s);                                                                        \
+    if (!ras) launch_post_nq<T, NQ_>(E, boxOut.get(), ghostOut.get(), smap.get(), wts.get(), Su, maskBits, s);   \
     break;
   switch (Nq) {
     NRS_FDM_CASE(2) NRS_FDM_CASE(3) NRS_FDM_CASE(4) NRS_FDM_CASE(5) NRS_FDM_CASE(6)
@@ -1176,20 +1187,22 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
     default: throw Error(NRS_ERR_INVALID, "schwarz: unsupported order");
   }
 #undef NRS_FDM_CASE
-  finish(Su, s);
+  finish(Su, maskBits != nullptr, s);
 }
 
 // RAS: Su = gs(invDegree .* restricted) (fusedFDM.c p_restrict, then oogs); the
 // weighted gather-scatter multiplies each member by its weight before the row
 // sum, as the reference's separate multiply does.  ASM: Su = gs(postFDM(...)).
-// Then the mask (smoothSchwarz, :1146).
+// Then the mask (smoothSchwarz, :1146): a pass over the masked ids, unless the
+// kernel that wrote Su stored its masked nodes as zero already (masked: those
+// nodes are in no row of the gather-scatter, so that is the same result).
 template <typename T>
-void Schwarz<T>::finish(T *Su, hipStream_t s)
+void Schwarz<T>::finish(T *Su, bool masked, hipStream_t s)
 {
   const int64_t L = (int64_t)E * Np;
   if (ras) ogs_gather_scatter_weighted<T>(ogs, Su, invDegree, s);
   else ogs_gather_scatter<T>(ogs, Su, 1, L, NRS_ADD, s);
-  launch_mask<T>(Nmasked, maskIds, Su, s);
+  if (!masked) launch_mask<T>(Nmasked, maskIds, Su, s);
 }
 
 // Time the matrix-core box solve with each elements-per-wavefront variant on

@@ -51,6 +51,12 @@ struct Schwarz {
   const nrs_ogs *ogs = nullptr;
   int64_t Nmasked = 0;
   const int *maskIds = nullptr;
+  // [E][ceil(Np/32)] Dirichlet mask bits of the level's nodes: the ASM
+  // overlap kernel and the SIMT box solve store the masked nodes of Su as zero
+  // (null, or RAS on the matrix cores: the mask runs as its own pass over
+  // maskIds).  The masked nodes must be in no row of ogs, as the level's
+  // handles are built.
+  const uint32_t *maskBits = nullptr;
   double helm0 = 1.0, helm1 = 0.0;   // box solve inverts helm0 (lx + ly + lz) + helm1
   float2 helm32() const { return make_float2((float)helm0, (float)helm1); }
 
@@ -60,13 +66,13 @@ struct Schwarz {
   // in the eigenbasis (S^T B S = I per direction).
   void setup(int N, int E, bool ras, int fdmPrecision, const std::vector<double> &lengths,
              const int *EToB, const std::vector<FaceLink> &links, const nrs_ogs *ogs,
-             const T *invDegree, int64_t Nmasked, const int *maskIds, bool poisson,
-             double lambda0, double lambda1, hipStream_t s);
+             const T *invDegree, int64_t Nmasked, const int *maskIds, const uint32_t *maskBits,
+             bool poisson, double lambda0, double lambda1, hipStream_t s);
   // Su = smoothSchwarz(u): FDM solves on the extended boxes, overlap exchange,
   // gather-scatter, mask.  Su must not alias u (the RAS kernel reads the
   // neighbours' u while writing its own Su).
   void apply(const T *u, T *Su, hipStream_t s);
-  void finish(T *Su, hipStream_t s);
+  void finish(T *Su, bool masked, hipStream_t s);
   // choose the matrix-core kernel's elements per wavefront for this level
   // (FP32 vectors with BF16/FP16 operators; otherwise nothing to tune); returns it
   int autotune(hipStream_t s);

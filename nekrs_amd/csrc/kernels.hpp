@@ -46,7 +46,7 @@ void launch_geofacs(int N, int E, const double *D, const double *w, const double
 template <typename T>
 // w: w .* qf contracted; w and sub: w .* (sub - qf) (qf left unchanged in both)
 void launch_coarsen(int Nf, int Nc, int E, const T *R, const T *qf, T *qc, hipStream_t s,
-                    const T *w = nullptr, const T *sub = nullptr);
+                    const T *w = nullptr, const T *sub = nullptr, const uint32_t *cmask = nullptr);
 template <typename T>
 void launch_prolongate(int Nf, int Nc, int E, const T *R, const T *qc, T *qf, hipStream_t s);
 

@@ -783,11 +783,13 @@ struct MGImpl final : MGSolver {
     smooth(l, l.rhs, l.x, true, s);
     // residual (ellipticMultiGridLevel.cpp:37-41) and coarsen (:43-68):
     // invDegreeFine .* (rhs - A x) formed as the contraction loads A x (the
-    // residual itself is not needed afterwards), contract, gs, mask
+    // residual itself is not needed afterwards), contract, gs, mask.  The
+    // coarse level's masked nodes are in no row of its gather-scatter, so the
+    // contraction stores them as zero and the mask needs no pass of its own.
     op(l, l.x, l.res.get(), s);
-    launch_coarsen<T>(l.N, c.N, l.E, c.R.get(), l.res.get(), c.rhs, s, l.invDegree.get(), l.rhs);
+    launch_coarsen<T>(l.N, c.N, l.E, c.R.get(), l.res.get(), c.rhs, s, l.invDegree.get(), l.rhs,
+                      c.Nmasked ? c.maskBits : nullptr);
     ogs_gather_scatter<T>(c.ogs, c.rhs, 1, c.L, NRS_ADD, s);
-    launch_mask<T>(c.Nmasked, c.maskIds, c.rhs, s);
     vcycle(k + 1, s);
     // prolongate (:70-83): x += P xC
     launch_prolongate<T>(l.N, c.N, l.E, c.R.get(), c.x, l.x, s);
@@ -1157,7 +1159,7 @@ struct MGImpl final : MGSolver {
           // Schwarz smoother (pMGLevel::build, ellipticMultiGridSchwarz.cpp:837-1054)
           l.schwarz = std::make_unique<Schwarz<T>>();
           l.schwarz->setup(l.N, l.E, schwarzKind == 2, o.fdmPrecision, lengths, d->EToB, links,
-                           l.ogs, l.invDegree.get(), l.Nmasked, l.maskIds, el->poisson,
+                           l.ogs, l.invDegree.get(), l.Nmasked, l.maskIds, l.maskBits, el->poisson,
                            el->lambda0, el->lambda1, s);
         } else {
           // Jacobi smoother: 1/diag(A), assembled (ellipticUpdateJacobi.cpp:36-61)

@@ -17,12 +17,15 @@ constexpr int kTB = 256;
 // out = (A x A x A) (w .* (sub - in)), A is [NO][NI] given through
 // (a_stride_o, a_stride_i) into M; w = null: no weight; sub = null: in itself.
 // The difference and the weighted input are rounded on their own, as separate
-// passes would leave them.
+// passes would leave them.  omask ([E][ceil(NO^3/32)] bits, optional): output
+// nodes stored as zero, the Dirichlet mask of the coarse level.  Its masked
+// nodes are in no row of that level's gather-scatter, so zeroing them here
+// equals masking after the gather-scatter.
 template <typename T, bool ACCUM>
 __global__ void __launch_bounds__(kTB)
 tensor3_kernel(int E, int NI, int NO, const T *__restrict__ M, int so, int si,
                const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ w,
-               const T *__restrict__ sub)
+               const T *__restrict__ sub, const uint32_t *__restrict__ omask)
 {
   __shared__ T s_A[kMaxNq * kMaxNq];
   __shared__ T s_in[kMaxNq * kMaxNq * kMaxNq];
@@ -66,6 +69,7 @@ tensor3_kernel(int E, int NI, int NO, const T *__restrict__ M, int so, int si,
     const int io = t % NO, jo = (t / NO) % NO, ko = t / (NO * NO);
     T acc = 0;
     for (int k = 0; k < NI; ++k) acc += s_A[ko * NI + k] * s_t2[io + NO * (jo + NO * k)];
+    if (omask && ((omask[e * ((NPO + 31) / 32) + (t >> 5)] >> (t & 31)) & 1u)) acc = T(0);
     if (ACCUM) out[e * NPO + t] += acc;
     else out[e * NPO + t] = acc;
   }
@@ -127,12 +131,12 @@ prolong_8_4_kernel(int E, const T *__restrict__ R, const T *__restrict__ qc, T *
   }
 }
 
-// qc = (A x A x A) (w .* (sub - qf)) with A = R ([NqC][NqF]); w, sub optional as
-// for tensor3_kernel
+// qc = (A x A x A) (w .* (sub - qf)) with A = R ([NqC][NqF]); w, sub and the
+// coarse mask bits cmask ([E][2]) optional as for tensor3_kernel
 template <typename T>
 __global__ void __launch_bounds__(64 * kTW)
 coarsen_8_4_kernel(int E, const T *__restrict__ R, const T *__restrict__ qf, T *__restrict__ qc,
-                   const T *__restrict__ w, const T *__restrict__ sub)
+                   const T *__restrict__ w, const T *__restrict__ sub, const uint32_t *__restrict__ cmask)
 {
   __shared__ T s_f[kTW][512];    // the weighted fine element, [k][j][i]
   __shared__ T s_t1[kTW][256];   // [k][j][io]
@@ -193,6 +197,7 @@ coarsen_8_4_kernel(int E, const T *__restrict__ R, const T *__restrict__ qf, T *
     T acc = 0;
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) acc += R[ko * 8 + kk] * s_t2[wv][io + 4 * (jo + 4 * kk)];
+    if (cmask && ((cmask[e * 2 + (l >> 5)] >> (l & 31)) & 1u)) acc = T(0);
     qc[e * 64 + l] = acc;
   }
 }
@@ -200,19 +205,19 @@ coarsen_8_4_kernel(int E, const T *__restrict__ R, const T *__restrict__ qf, T *
 
 template <typename T>
 void launch_coarsen(int Nf, int Nc, int E, const T *R, const T *qf, T *qc, hipStream_t s, const T *w,
-                    const T *sub)
+                    const T *sub, const uint32_t *cmask)
 {
   NRS_CHECK(!sub || w, "coarsen: a subtrahend needs the weight");
   NRS_CHECK(Nf + 1 <= kMaxNq && Nc >= 1 && Nc < Nf, "coarsen: need 1 <= Nc < Nf <= 9");
   if (E <= 0) return;
   const int NqF = Nf + 1, NqC = Nc + 1;
   if (Nf == 7 && Nc == 3) {
-    coarsen_8_4_kernel<T><<<(unsigned)((E + kTW - 1) / kTW), 64 * kTW, 0, s>>>(E, R, qf, qc, w, sub);
+    coarsen_8_4_kernel<T><<<(unsigned)((E + kTW - 1) / kTW), 64 * kTW, 0, s>>>(E, R, qf, qc, w, sub, cmask);
     NRS_LAUNCH_CHECK();
     return;
   }
   // A[o][i] = R[o][i] = R[o*NqF + i]
-  tensor3_kernel<T, false><<<E, kTB, 0, s>>>(E, NqF, NqC, R, NqF, 1, qf, qc, w, sub);
+  tensor3_kernel<T, false><<<E, kTB, 0, s>>>(E, NqF, NqC, R, NqF, 1, qf, qc, w, sub, cmask);
   NRS_LAUNCH_CHECK();
 }
 
@@ -228,14 +233,14 @@ void launch_prolongate(int Nf, int Nc, int E, const T *R, const T *qc, T *qf, hi
     return;
   }
   // A[o][i] = R^T[o][i] = R[i][o] = R[i*NqF + o]
-  tensor3_kernel<T, true><<<E, kTB, 0, s>>>(E, NqC, NqF, R, 1, NqF, qc, qf, nullptr, nullptr);
+  tensor3_kernel<T, true><<<E, kTB, 0, s>>>(E, NqC, NqF, R, 1, NqF, qc, qf, nullptr, nullptr, nullptr);
   NRS_LAUNCH_CHECK();
 }
 
 template void launch_coarsen<double>(int, int, int, const double *, const double *, double *, hipStream_t,
-                                     const double *, const double *);
+                                     const double *, const double *, const uint32_t *);
 template void launch_coarsen<float>(int, int, int, const float *, const float *, float *, hipStream_t,
-                                    const float *, const float *);
+                                    const float *, const float *, const uint32_t *);
 template void launch_prolongate<double>(int, int, int, const double *, const double *, double *, hipStream_t);
 template void launch_prolongate<float>(int, int, int, const float *, const float *, float *, hipStream_t);
 
