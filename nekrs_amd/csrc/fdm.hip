@@ -1036,6 +1036,16 @@ void Schwarz<T>::setup(int N_, int E_, bool ras_, int prec, const std::vector<do
   Nq = N + 1; Np = Nq * Nq * Nq; nl = N + 3;
   ogs = ogs_; invDegree = invDegree_; Nmasked = Nmasked_; maskIds = maskIds_;
   maskBits = Nmasked > 0 ? maskBits_ : nullptr;
+  if (maskBits) {
+    // the kernels that write Su, and the weighted gather-scatter, store the
+    // masked nodes as zero in place of a mask pass: none may be in a row of ogs
+    std::vector<int> ids((size_t)Nmasked);
+    NRS_HIP(hipMemcpy(ids.data(), maskIds, (size_t)Nmasked * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> isMasked((size_t)E * Np, 0);
+    for (int id : ids) isMasked[id] = 1;
+    for (const std::vector<int> *h : {&ogs->h2, &ogs->h4, &ogs->h8, &ogs->hxi})
+      for (int id : *h) NRS_CHECK(!isMasked[id], "schwarz: a masked node is in a row of the gather-scatter");
+  }
   std::vector<double> r, wq;
   gll_nodes(N, r, wq);
   const auto D = dmatrix(r);
@@ -1193,15 +1203,21 @@ void Schwarz<T>::apply(const T *u, T *Su, hipStream_t s)
 // RAS: Su = gs(invDegree .* restricted) (fusedFDM.c p_restrict, then oogs); the
 // weighted gather-scatter multiplies each member by its weight before the row
 // sum, as the reference's separate multiply does.  ASM: Su = gs(postFDM(...)).
-// Then the mask (smoothSchwarz, :1146): a pass over the masked ids, unless the
-// kernel that wrote Su stored its masked nodes as zero already (masked: those
-// nodes are in no row of the gather-scatter, so that is the same result).
+// Then the mask (smoothSchwarz, :1146), unless the kernel that wrote Su stored
+// its masked nodes as zero already (masked); RAS stores them as zero inside the
+// weighted gather-scatter's launch.  Either is the same result as a pass after
+// it: the masked nodes are in no row of the gather-scatter (checked at setup).
 template <typename T>
 void Schwarz<T>::finish(T *Su, bool masked, hipStream_t s)
 {
   const int64_t L = (int64_t)E * Np;
-  if (ras) ogs_gather_scatter_weighted<T>(ogs, Su, invDegree, s);
-  else ogs_gather_scatter<T>(ogs, Su, 1, L, NRS_ADD, s);
+  if (ras) {
+    const bool inGs = !masked && maskBits && ogs->Nrows > 0;
+    ogs_gather_scatter_weighted<T>(ogs, Su, invDegree, s, inGs ? maskIds : nullptr, inGs ? Nmasked : 0);
+    if (!masked && !inGs) launch_mask<T>(Nmasked, maskIds, Su, s);
+    return;
+  }
+  ogs_gather_scatter<T>(ogs, Su, 1, L, NRS_ADD, s);
   if (!masked) launch_mask<T>(Nmasked, maskIds, Su, s);
 }
 

@@ -122,15 +122,22 @@ __device__ __forceinline__ void gs_fixed_row(const int *__restrict__ idx, T *__r
 // threads as one list measured 3-6 % slower).  Blocks are dealt round-robin to
 // the 8 XCDs, so block b runs tile (b mod 8) * per + b / 8: each XCD walks a
 // contiguous run of tiles and the lines that neighbouring tiles share stay in
-// its L2 as well.  WEIGHTED: v <- gs_add(w .* v) (nvec = 1, NV = 1).  NV: the
+// its L2 as well.  WEIGHTED: v <- gs_add(w .* v) (nvec = 1, NV = 1), and the nz
+// nodes zids (in no row) stored as zero, dealt evenly over the blocks.  NV: the
 // vector count when fixed at compile time (1 or 3), 0 for any nvec.
 template <typename T, int OP, bool WEIGHTED, int NV>
 __global__ void __launch_bounds__(kGsBlock)
 gs_kernel(int t0, int nt, int per, const int *__restrict__ toff, int tstride,
           const int *__restrict__ pairs, const int *__restrict__ quads, const int *__restrict__ octs,
           const int *__restrict__ xoff, const int *__restrict__ xids, const T *__restrict__ w,
-          T *__restrict__ v, int nvec, int64_t stride)
+          T *__restrict__ v, int nvec, int64_t stride, const int *__restrict__ zids, int64_t nz)
 {
+  if constexpr (WEIGHTED) {
+    if (nz > 0) {
+      const int64_t b = blockIdx.x, nb = gridDim.x, z1 = nz * (b + 1) / nb;
+      for (int64_t i = nz * b / nb + threadIdx.x; i < z1; i += kGsBlock) v[zids[i]] = T(0);
+    }
+  }
   const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
   if (t >= nt) return;
   const int *tb = toff + t0 + t;
@@ -164,7 +171,7 @@ gs_kernel(int t0, int nt, int per, const int *__restrict__ toff, int tstride,
 // Launches tiles [t0, t1) of the handle.
 template <typename T, int OP, bool WEIGHTED>
 void launch_gs(const nrs_ogs *o, int t0, int t1, const T *w, T *v, int nvec, int64_t stride,
-               hipStream_t s)
+               hipStream_t s, const int *zids = nullptr, int64_t nz = 0)
 {
   const int nt = t1 - t0;
   if (nt <= 0) return;
@@ -172,7 +179,7 @@ void launch_gs(const nrs_ogs *o, int t0, int t1, const T *w, T *v, int nvec, int
   auto go = [&](auto kernel) {
     kernel<<<8 * per, kGsBlock, 0, s>>>(t0, nt, per, o->tOff.get(), o->nTiles + 1, o->pairs.get(),
                                         o->quads.get(), o->octs.get(), o->xoff.get(), o->xids.get(), w, v,
-                                        nvec, stride);
+                                        nvec, stride, zids, nz);
   };
   if (nvec == 1) go(gs_kernel<T, OP, WEIGHTED, 1>);
   else if (nvec == 3 && !WEIGHTED) go(gs_kernel<T, OP, WEIGHTED, 3>);
@@ -233,14 +240,18 @@ void ogs_layout(nrs_ogs *o, int64_t npt)
 }  // namespace
 
 template <typename T>
-void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s)
+void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s, const int *zeroIds,
+                                 int64_t nZero)
 {
   NRS_CHECK(o != nullptr, "ogs: null handle");
+  NRS_CHECK(nZero == 0 || (zeroIds && o->Nrows > 0), "ogs: ids to zero need a handle with rows");
   if (o->Nrows == 0) return;
-  launch_gs<T, NRS_ADD, true>(o, 0, o->nTiles, w, v, 1, 0, s);
+  launch_gs<T, NRS_ADD, true>(o, 0, o->nTiles, w, v, 1, 0, s, zeroIds, nZero);
 }
-template void ogs_gather_scatter_weighted<double>(const nrs_ogs *, double *, const double *, hipStream_t);
-template void ogs_gather_scatter_weighted<float>(const nrs_ogs *, float *, const float *, hipStream_t);
+template void ogs_gather_scatter_weighted<double>(const nrs_ogs *, double *, const double *, hipStream_t,
+                                                  const int *, int64_t);
+template void ogs_gather_scatter_weighted<float>(const nrs_ogs *, float *, const float *, hipStream_t,
+                                                 const int *, int64_t);
 
 template <typename T>
 void ogs_gather_scatter(const nrs_ogs *o, T *v, int nvec, int64_t stride, int op, hipStream_t s)

@@ -47,9 +47,12 @@ void ogs_gather_scatter(const nrs_ogs *o, T *v, int nvec, int64_t stride, int op
 nrs_ogs *ogs_setup(int64_t N, const long long *ids);
 // v <- gs_add(w .* v) for one vector, w = the handle's invDegree in T: each
 // member is scaled before its row sum; nodes outside any row are left as they
-// are (their weight is 1).
+// are (their weight is 1), except the nZero nodes zeroIds, stored as zero in
+// the same launch (a mask; they must be in no row of the handle, which needs
+// rows when nZero > 0).
 template <typename T>
-void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s);
+void ogs_gather_scatter_weighted(const nrs_ogs *o, T *v, const T *w, hipStream_t s,
+                                 const int *zeroIds = nullptr, int64_t nZero = 0);
 // Group rows by chunk (chunk = last member / nodesPerChunk, at most nChunks - 1),
 // re-tiling so that tiles nest in chunks.  Reorders rows inside buckets only:
 // every gather-scatter result is unchanged.
