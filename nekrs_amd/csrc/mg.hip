@@ -141,15 +141,26 @@ namespace {
 constexpr int KB = 256;
 inline unsigned kgrid(int64_t n) { return grid_for(n, KB, 16384); }
 
+// Dirichlet mask bit of entry i of a level's E-vector (mb: [E][ceil(np/32)]
+// bits, np nodes per element; the vectors stay below 2^32 entries)
+__device__ __forceinline__ bool masked_at(const uint32_t *__restrict__ mb, int64_t i, int np)
+{
+  const uint32_t iu = (uint32_t)i, e = iu / (uint32_t)np, nn = iu - e * (uint32_t)np;
+  return (mb[(int64_t)e * ((np + 31) >> 5) + (nn >> 5)] >> (nn & 31)) & 1u;
+}
+
 // updateChebyshev (kernels/elliptic/updateChebyshev.okl:7-30).  JAC: the Jacobi
 // smoother applied on the fly, SAd = w .* Ad with the product rounded on its own
 // (no contraction into the subtraction), equal to a separate w .* Ad pass.
 // LAST: the last step of the smoothing, fused with the x += d that follows it:
 // x = (x + d) + d_new in that order, and d, r (no longer needed) not stored.
 // X0: x is zero on entry and not read (the first step of a zero start).
+// mb (LAST, optional): the level's mask bits, x stored as zero where set (the
+// mask that follows the smoothing), np nodes per element.
 template <typename T, bool JAC, bool LAST, bool X0 = false>
 __global__ void cheby1_kernel(int64_t n, T dC, T rC, const T *__restrict__ w, const T *__restrict__ Ad,
-                              T *__restrict__ d, T *__restrict__ r, T *__restrict__ x)
+                              T *__restrict__ d, T *__restrict__ r, T *__restrict__ x,
+                              const uint32_t *__restrict__ mb = nullptr, int np = 0)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     T sad;
@@ -164,7 +175,7 @@ __global__ void cheby1_kernel(int64_t n, T dC, T rC, const T *__restrict__ w, co
     const T x1 = (X0 ? T(0) : x[i]) + dn;
     const T dnew = dC * dn + rC * rn;
     if constexpr (LAST) {
-      x[i] = x1 + dnew;
+      x[i] = (mb && masked_at(mb, i, np)) ? T(0) : x1 + dnew;
     } else {
       x[i] = x1;
       d[i] = dnew;
@@ -212,14 +223,15 @@ __global__ void cheby4_res_kernel(int64_t n, const T *__restrict__ Ad, const T *
 // of the separate update and axpby passes, in their order.
 template <typename T, bool X0, bool LAST>
 __global__ void cheby4_dir_kernel(int64_t n, T beta, T rC, T dC, T betaLast, const T *__restrict__ SAd,
-                                  T *__restrict__ d, T *__restrict__ x)
+                                  T *__restrict__ d, T *__restrict__ x, const uint32_t *__restrict__ mb = nullptr,
+                                  int np = 0)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const T dn = d[i];
     const T x1 = (X0 ? T(0) : x[i]) + beta * dn;
     const T dp = rC * SAd[i] + dC * dn;
     if constexpr (LAST) {
-      x[i] = betaLast * dp + T(1) * x1;
+      x[i] = (mb && masked_at(mb, i, np)) ? T(0) : betaLast * dp + T(1) * x1;
     } else {
       x[i] = x1;
       d[i] = dp;
@@ -646,10 +658,18 @@ struct MGImpl final : MGSolver {
   // smoothChebyshev (ellipticMultiGridLevel.cpp:144-210).  The reference
   // smooths res and Ad in place; here S writes to a spare vector whose role is
   // swapped with the input's.  Scalars are formed in FP64 and rounded to T.
+  // mask bits for the last Chebyshev update to store x's masked nodes as zero
+  // (null: no masked node, or a level too long for the kernels' 32-bit index)
+  static const uint32_t *last_mask(const Level<T> &l)
+  {
+    return (l.Nmasked > 0 && l.maskBits && l.L <= (int64_t)UINT32_MAX) ? l.maskBits : nullptr;
+  }
+
   void smooth_cheby1(Level<T> &l, const T *rhs, T *x, bool xIsZero, hipStream_t s)
   {
     const int deg = xIsZero ? l.degDown : l.degUp;
     if (deg == 0) return;
+    const uint32_t *lm = last_mask(l);
     const T theta = T(0.5 * (l.lmax + l.lmin));
     const T delta = T(0.5 * (l.lmax - l.lmin));
     const T invTheta = T(1.0 / (double)theta);
@@ -698,8 +718,8 @@ struct MGImpl final : MGSolver {
       const bool first0 = x0 && k == 1;
 #define NRS_CHEBY1(J_, L_)                                                                          \
   do {                                                                                            \
-    if (first0) cheby1_kernel<T, J_, L_, true><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);    \
-    else cheby1_kernel<T, J_, L_, false><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x);          \
+    if (first0) cheby1_kernel<T, J_, L_, true><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x, lm, l.Np); \
+    else cheby1_kernel<T, J_, L_, false><<<g, KB, 0, s>>>(l.L, dC, rC, w, Ad, d, res, x, lm, l.Np);       \
   } while (0)
       if (jac && last) NRS_CHEBY1(true, true);
       else if (jac) NRS_CHEBY1(true, false);
@@ -709,7 +729,7 @@ struct MGImpl final : MGSolver {
       NRS_LAUNCH_CHECK();
     }
     if (deg == 1) k_axpby<T>(l.L, T(1), d, T(1), x, s);
-    launch_mask<T>(l.Nmasked, l.maskIds, x, s);
+    if (deg == 1 || !lm) launch_mask<T>(l.Nmasked, l.maskIds, x, s);   // else in the last update
   }
 
   // smoothFourthKindChebyshev (ellipticMultiGridLevel.cpp:212-270)
@@ -719,6 +739,7 @@ struct MGImpl final : MGSolver {
     if (deg == 0) return;
     const auto &betas = xIsZero ? l.betasDown : l.betasUp;
     const double rho = l.lmax;
+    const uint32_t *lm = last_mask(l);
     T *res = sRes.get(), *Ad = sAd.get(), *d = sD.get();
     // x = 0 with at least one update step: the first step treats x as zero and
     // takes the residual from rhs, so neither is written beforehand
@@ -748,14 +769,14 @@ struct MGImpl final : MGSolver {
       const T b = betas[k - 1], bl = betas.back();
       // x += beta_(k-1) d deferred to here, with the direction update (and, on
       // the last step, the final x += beta_last d)
-      if (first0 && last) cheby4_dir_kernel<T, true, true><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
+      if (first0 && last) cheby4_dir_kernel<T, true, true><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x, lm, l.Np);
       else if (first0) cheby4_dir_kernel<T, true, false><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
-      else if (last) cheby4_dir_kernel<T, false, true><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
+      else if (last) cheby4_dir_kernel<T, false, true><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x, lm, l.Np);
       else cheby4_dir_kernel<T, false, false><<<g, KB, 0, s>>>(l.L, b, rC, dC, bl, Ad, d, x);
       NRS_LAUNCH_CHECK();
     }
     if (deg == 1) k_axpby<T>(l.L, betas.back(), d, T(1), x, s);
-    launch_mask<T>(l.Nmasked, l.maskIds, x, s);
+    if (deg == 1 || !lm) launch_mask<T>(l.Nmasked, l.maskIds, x, s);   // else in the last update
   }
 
   // pMGLevel::smooth (ellipticMultiGridLevel.cpp:85-103): pure ASM / RAS
